@@ -1,0 +1,221 @@
+/*
+ * bem_hip.h -- C ABI of the MI355X Galerkin BEM hot path (libbem_hip.so).
+ *
+ * Drop-in for the boundary-integral-operator assembly and dense solve of
+ * GeCao/taichi-BEM.  The reference has no FFI: its boundary is a Python class
+ * API over Taichi fields.  Each entry point below names the reference
+ * function (file:line) whose work it replaces; the Python layer in
+ * taichi-bem_amd/ keeps the reference's class and method names and calls these
+ * through ctypes.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (HBM), except where a comment
+ *     says host.  The library never allocates or frees caller buffers; its own
+ *     scratch lives in the context and is released by bem_destroy.
+ *   - Matrices are row-major with leading dimension `ld` (elements).  Complex
+ *     values are interleaved (re, im): the layout of a Taichi
+ *     Vector.field(2) and of torch complex64/complex128.
+ *   - Every call is ordered on `stream` (a hipStream_t, NULL = default
+ *     stream); results are visible after a stream synchronisation.
+ *   - Return value: BEM_OK (0) or an error code; bem_last_error() gives the
+ *     message of the calling thread's last failure.  The library never exits
+ *     the process (compare src/managers/BEM_manager.py:833-835).
+ */
+#ifndef BEM_HIP_H
+#define BEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the Python layer maps them to exception types) ---- */
+enum {
+    BEM_OK = 0,
+    BEM_ERR_INVALID = 1,      /* RuntimeError   (BEM_manager.py:41,50,193)   */
+    BEM_ERR_UNSUPPORTED = 2,  /* NotImplementedError (utils/functions.py:80-83) */
+    BEM_ERR_HIP = 3,          /* HIP runtime failure                           */
+    BEM_ERR_SINGULAR = 4      /* exact zero pivot in the LU                     */
+};
+
+/* ---- operators (build's numbering) ---- */
+enum { BEM_OP_V = 0, BEM_OP_K = 1, BEM_OP_KADJ = 2, BEM_OP_W = 3 };
+
+/* ---- KernelType, src/BEM_functions/utils/types.py:32-35 ---- */
+enum { BEM_LAPLACE = 0, BEM_HELMHOLTZ = 1, BEM_HELMHOLTZ_TRANSMISSION = 2 };
+
+/* ---- PanelsRelation, src/BEM_functions/utils/types.py:41-45 ---- */
+enum { BEM_SEPARATE = 0, BEM_COINCIDE = 1, BEM_COMMON_VERTEX = 2, BEM_COMMON_EDGE = 3 };
+
+/* ---- element types of caller buffers ---- */
+enum { BEM_F64 = 0, BEM_C128 = 1, BEM_F32 = 2, BEM_C64 = 3 };
+
+/* ---- arithmetic of the regular (well-separated) pairs ---- */
+enum {
+    BEM_PREC_FP64 = 0,  /* everything in FP64 (the parity preset)            */
+    BEM_PREC_FP32 = 1   /* regular pairs in FP32 with FP32 accumulation and
+                           FP64 panel-centroid differences; near field FP64  */
+};
+
+/* ---- quadrature variants; 0 reproduces the reference rules exactly ---- */
+enum {
+    BEM_FLAG_ROTATE_NEAR = 1,  /* put the shared vertex/edge at the slots the
+                                  Sauter-Schwab rules expect (the reference
+                                  uses stored order, single_layer_3d.py:133-141) */
+    BEM_FLAG_SYM_COINCIDE = 2  /* coincident rule: f(y,x) with swapped bases
+                                  instead of f(x,y) twice (single_layer_3d.py:
+                                  198-206) */
+};
+
+typedef struct bem_context bem_context;
+
+/* Operator descriptor.  `kernel`, `k` and `sqrt_n` mean what the reference's
+ * BEMManager._kernel_type/_k/sqrt_n mean (BEM_manager.py:31-62): the wave
+ * number used in G is k for HELMHOLTZ and k*sqrt_n for
+ * HELMHOLTZ_TRANSMISSION; the hypersingular k^2 term always uses
+ * k^2 sqrt_n^2 (hypersingular_layer_3d.py:171). */
+typedef struct {
+    int32_t op;          /* BEM_OP_* */
+    int32_t kernel;      /* BEM_LAPLACE / BEM_HELMHOLTZ / ..._TRANSMISSION */
+    double k;
+    double sqrt_n;
+    int32_t deg_test;    /* 0 = P0 panel dofs, 1 = P1 vertex dofs */
+    int32_t deg_trial;
+    int32_t precision;   /* BEM_PREC_* */
+    int32_t dtype;       /* element type of the output / vectors: BEM_* */
+    int32_t flags;       /* BEM_FLAG_* */
+    int32_t reserved;
+} bem_op_desc;
+
+typedef struct {
+    int32_t iterations;      /* refinement steps taken */
+    int32_t converged;       /* 1 if ||r|| <= tol * ||b|| */
+    double residual_norm0;   /* ||b - A x0|| (FP64) */
+    double residual_norm;    /* final ||b - A x|| (FP64) */
+    double b_norm;
+} bem_refine_stats;
+
+/* ---- lifetime -------------------------------------------------------- */
+
+/* Creates a context on `device`.  Replaces the field allocations of
+ * BEMManager.initialization, src/managers/BEM_manager.py:82-195. */
+int bem_create(bem_context** ctx, int device);
+
+/* Releases everything the context owns.  Replaces BEMManager.kill,
+ * src/managers/BEM_manager.py:1075-1108 (and the layers' kill(), e.g.
+ * single_layer_3d.py:42-52). */
+int bem_destroy(bem_context* ctx);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* bem_last_error(void);
+
+/* Library ABI version. */
+int bem_version(void);
+
+/* ---- mesh and tables ------------------------------------------------- */
+
+/* Uploads a triangle mesh (device pointers: vertices [nv*3] f64, panels
+ * [nf*3] i32, 0-based) and builds on the device: panel areas and normals,
+ * vertex areas and normals (MeshManager.initialization,
+ * src/managers/mesh_manager.py:67-164, geometry at :114-139), the GaussQR
+ * tables (BEM_manager.py:166-173, utils/functions.py:5-85) and the
+ * near-field pair list of the reference relation classes
+ * (mesh_manager.py:538-573).  gauss_qr must be 1..8 or 12, else
+ * BEM_ERR_UNSUPPORTED. */
+int bem_set_mesh(bem_context* ctx, const double* vertices, const int32_t* panels,
+                 int32_t nv, int32_t nf, int32_t gauss_qr, void* stream);
+
+/* Copies the device geometry out (any pointer may be NULL):
+ * MeshManager.get_panel_areas/get_panel_normals/get_vert_areas/
+ * get_vert_normals, mesh_manager.py:455-469. */
+int bem_get_geometry(bem_context* ctx, double* area, double* normal,
+                     double* vert_area, double* vert_normal, void* stream);
+
+/* Host array counts[4]: ordered panel pairs per PanelsRelation. */
+int bem_nearfield_counts(bem_context* ctx, int64_t counts[4]);
+
+/* ---- operator assembly ---------------------------------------------- */
+
+/* out[test dof, trial dof] += alpha * Op, dense (rows = nf for P0 test or nv
+ * for P1 test; ld = row stride of out).  Replaces the forward() kernels:
+ *   V  SingleLayer3d.forward         single_layer_3d.py:371-433
+ *   K  DoubleLayer3d.forward         double_layer_3d.py:376-438
+ *   K' AdjDoubleLayer3d.forward      adj_double_layer_3d.py:378-383 (a no-op
+ *      in the reference, which uses K^T at BEM_manager.py:619-622; here K'
+ *      can be assembled directly)
+ *   W  HypersingularLayer3d.forward  hypersingular_layer_3d.py:437-500
+ * together with the block placement of BEMManager.matA_add_M
+ * (BEM_manager.py:608-628): pass `out` offset to the block and alpha = the
+ * block's multiplier.  Real dtypes (BEM_F64/BEM_F32) take alpha_re only and
+ * require the Laplace kernel. */
+int bem_assemble(bem_context* ctx, const bem_op_desc* op, void* out, int64_t ld,
+                 double alpha_re, double alpha_im, void* stream);
+
+/* Matrix-free y[test dof] += alpha * Op . xhat, xhat = conj(x) if conj_x.
+ * Replaces apply_V_dot_N_boundary (single_layer_3d.py:435-498),
+ * apply_K_dot_D_boundary (double_layer_3d.py:440-501),
+ * apply_K_dot_N_boundary (adj_double_layer_3d.py:443-506) and
+ * apply_W_dot_D_boundary (hypersingular_layer_3d.py:502-565); the reference
+ * always conjugates complex data (conj_x = 1). */
+int bem_apply(bem_context* ctx, const bem_op_desc* op, const void* x, int32_t conj_x,
+              void* y, double alpha_re, double alpha_im, void* stream);
+
+/* out[test dof, trial dof] += alpha * M (or out[trial, test] when transpose
+ * = 1), M_{s,t} = int phi_s phi_t with the single-panel GaussQR^2 rule.
+ * Replaces IdentityLayer3d.matA_add_global_Identity_matrix
+ * (identity_layer_3d.py:148-196), which adds M at (N+i, D+v) and its
+ * transpose at (D+v, N+i): call twice with offset pointers. */
+int bem_mass_assemble(bem_context* ctx, int32_t deg_test, int32_t deg_trial, int32_t dtype,
+                      void* out, int64_t ld, double alpha, int32_t transpose, void* stream);
+
+/* y[test dof] += alpha * M . xhat.  Replaces
+ * RHSConstructor3d.multiply_identity_to_vector (rhs_constructor_3d.py:256-359). */
+int bem_mass_apply(bem_context* ctx, int32_t deg_test, int32_t deg_trial, int32_t dtype,
+                   const void* x, int32_t conj_x, void* y, double alpha, void* stream);
+
+/* ---- dense solve ----------------------------------------------------- */
+
+/* In-place LU with partial (row) pivoting of the row-major n x n matrix A
+ * (dtype BEM_F64/BEM_C128/BEM_F32/BEM_C64); ipiv (device, n int32) gets the
+ * 0-based LAPACK-style row interchanges.  Trailing updates run on MFMA.
+ * Together with bem_lu_solve it replaces torch.linalg.solve in
+ * BEMManager.solve (BEM_manager.py:1032-1073, the call at :1056). */
+int bem_lu_factor(bem_context* ctx, void* A, int64_t ld, int32_t n, int32_t dtype,
+                  int32_t* ipiv, void* stream);
+
+/* Solves A x = b in place of b (n entries of dtype) from bem_lu_factor. */
+int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
+                 const int32_t* ipiv, void* b, void* stream);
+
+/* r = b - A x in FP64/complex128 (A of dtype BEM_F64/BEM_C128; x, b, r of
+ * the matching FP64 type). */
+int bem_residual(bem_context* ctx, const void* A, int64_t ld, int32_t n, int32_t dtype,
+                 const void* x, const void* b, void* r, void* stream);
+
+/* Mixed-precision solve: x (FP64 type) from the FP32 LU of A (dtype_lu
+ * BEM_F32/BEM_C64) with FP64 iterative refinement against A64 (BEM_F64 or
+ * BEM_C128, matching).  Stops when ||b - A64 x|| <= tol ||b|| or after
+ * max_iter steps.  stats (host) may be NULL.  Synchronises the stream. */
+int bem_refine_solve(bem_context* ctx, const void* A64, int64_t lda, const void* LU,
+                     int64_t ldlu, int32_t n, int32_t dtype_lu, const int32_t* ipiv,
+                     const void* b, void* x, int32_t max_iter, double tol,
+                     bem_refine_stats* stats, void* stream);
+
+/* dst = (dtype_dst) src for a rows x cols block (precision conversion of a
+ * matrix or vector; complex <-> complex and real <-> real only). */
+int bem_convert(const void* src, int32_t dtype_src, int64_t ld_src, void* dst,
+                int32_t dtype_dst, int64_t ld_dst, int32_t rows, int32_t cols, void* stream);
+
+/* dst[r, c] += alpha * src[r, c] (or src[c, r] when transpose = 1) for a
+ * rows x cols block of dst; src and dst of the same dtype.  Replaces the
+ * block placement loops of BEMManager.matA_add_M (BEM_manager.py:608-628),
+ * including the -K^T block at :619-622. */
+int bem_matrix_axpy(const void* src, int64_t ld_src, void* dst, int64_t ld_dst, int32_t dtype,
+                    int32_t rows, int32_t cols, double alpha_re, double alpha_im, int32_t transpose,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEM_HIP_H */

@@ -1,0 +1,278 @@
+"""ctypes binding of libbem_hip.so (the C ABI in include/bem_hip.h).
+
+Loads the in-tree library built by ``build()`` (csrc/Makefile).  There is no
+fallback: if the library is missing or cannot be loaded, every call raises.
+Status codes map to the exception types the reference raises for the same
+conditions (RuntimeError for invalid configuration, BEM_manager.py:41,193;
+NotImplementedError for an unsupported GaussQR, utils/functions.py:80-83).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbem_hip.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+BEM_OK, BEM_ERR_INVALID, BEM_ERR_UNSUPPORTED, BEM_ERR_HIP, BEM_ERR_SINGULAR = 0, 1, 2, 3, 4
+OP_V, OP_K, OP_KADJ, OP_W = 0, 1, 2, 3
+F64, C128, F32, C64 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32 = 0, 1
+FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE = 1, 2
+
+# Every symbol include/bem_hip.h declares.
+EXPORTS = (
+    "bem_create", "bem_destroy", "bem_last_error", "bem_version", "bem_set_mesh",
+    "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_apply",
+    "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
+    "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy",
+)
+
+
+class BemOpDesc(ctypes.Structure):
+    _fields_ = [
+        ("op", ctypes.c_int32), ("kernel", ctypes.c_int32),
+        ("k", ctypes.c_double), ("sqrt_n", ctypes.c_double),
+        ("deg_test", ctypes.c_int32), ("deg_trial", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("dtype", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+class BemRefineStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+        ("residual_norm0", ctypes.c_double), ("residual_norm", ctypes.c_double),
+        ("b_norm", ctypes.c_double),
+    ]
+
+
+class BEMError(RuntimeError):
+    pass
+
+
+class SingularMatrixError(BEMError):
+    pass
+
+
+def build(jobs: int = 4) -> str:
+    """Compiles csrc/ into libbem_hip.so for gfx950 (always recompiles
+    stale objects; make decides)."""
+    subprocess.run(["make", "-s", "-C", CSRC, f"-j{jobs}"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BEMError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        sig = {
+            "bem_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int]),
+            "bem_destroy": (ctypes.c_int, [vp]),
+            "bem_last_error": (ctypes.c_char_p, []),
+            "bem_version": (ctypes.c_int, []),
+            "bem_set_mesh": (ctypes.c_int, [vp, vp, vp, i32, i32, i32, vp]),
+            "bem_get_geometry": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+            "bem_nearfield_counts": (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+            "bem_assemble": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i64, dbl, dbl, vp]),
+            "bem_apply": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i32, vp, dbl, dbl, vp]),
+            "bem_mass_assemble": (ctypes.c_int, [vp, i32, i32, i32, vp, i64, dbl, i32, vp]),
+            "bem_mass_apply": (ctypes.c_int, [vp, i32, i32, i32, vp, i32, vp, dbl, vp]),
+            "bem_lu_factor": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp]),
+            "bem_lu_solve": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
+            "bem_residual": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp, vp]),
+            "bem_refine_solve": (ctypes.c_int, [vp, vp, i64, vp, i64, i32, i32, vp, vp, vp, i32, dbl,
+                                                ctypes.POINTER(BemRefineStats), vp]),
+            "bem_convert": (ctypes.c_int, [vp, i32, i64, vp, i32, i64, i32, i32, vp]),
+            "bem_matrix_axpy": (ctypes.c_int, [vp, i64, vp, i64, i32, i32, i32, dbl, dbl, i32, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc == BEM_OK:
+        return
+    msg = lib().bem_last_error().decode(errors="replace")
+    if rc == BEM_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == BEM_ERR_SINGULAR:
+        raise SingularMatrixError(msg)
+    raise BEMError(msg)
+
+
+def _ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def current_stream() -> int:
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+def dtype_code(t) -> int:
+    import torch
+
+    m = {torch.float64: F64, torch.complex128: C128, torch.float32: F32, torch.complex64: C64}
+    if t.dtype not in m:
+        raise BEMError(f"unsupported dtype {t.dtype}")
+    return m[t.dtype]
+
+
+def _require_cuda(*ts) -> None:
+    for t in ts:
+        if t is not None and not t.is_cuda:
+            raise BEMError("the BEM library takes device (HIP) tensors only")
+
+
+class Context:
+    """Owns one bem_context (device scratch, mesh tables, near-field list)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().bem_create(ctypes.byref(h), int(device)))
+        self._h = h
+        self.device = int(device)
+        self.nv = self.nf = 0
+        self.gauss_qr = 0
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().bem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_mesh(self, vertices, panels, gauss_qr: int) -> None:
+        """vertices [nv,3] float64 and panels [nf,3] int32, device tensors."""
+        import torch
+
+        v = vertices.contiguous().to(torch.float64)
+        p = panels.contiguous().to(torch.int32)
+        _require_cuda(v, p)
+        check(lib().bem_set_mesh(self._h, v.data_ptr(), p.data_ptr(), int(v.shape[0]), int(p.shape[0]),
+                                 int(gauss_qr), current_stream()))
+        self.nv, self.nf, self.gauss_qr = int(v.shape[0]), int(p.shape[0]), int(gauss_qr)
+        self._verts, self._panels = v, p
+
+    def geometry(self):
+        import torch
+
+        dev = torch.device("cuda", self.device)
+        area = torch.empty(self.nf, dtype=torch.float64, device=dev)
+        normal = torch.empty(self.nf, 3, dtype=torch.float64, device=dev)
+        va = torch.empty(self.nv, dtype=torch.float64, device=dev)
+        vn = torch.empty(self.nv, 3, dtype=torch.float64, device=dev)
+        check(lib().bem_get_geometry(self._h, area.data_ptr(), normal.data_ptr(), va.data_ptr(),
+                                     vn.data_ptr(), current_stream()))
+        return area, normal, va, vn
+
+    def nearfield_counts(self) -> list[int]:
+        c = (ctypes.c_int64 * 4)()
+        check(lib().bem_nearfield_counts(self._h, c))
+        return [int(x) for x in c]
+
+    @staticmethod
+    def op_desc(op: int, kernel: int, k: float, sqrt_n: float, deg_test: int, deg_trial: int,
+                precision: int, dtype: int, flags: int = 0) -> BemOpDesc:
+        return BemOpDesc(int(op), int(kernel), float(k), float(sqrt_n), int(deg_test), int(deg_trial),
+                         int(precision), int(dtype), int(flags), 0)
+
+    def assemble(self, desc: BemOpDesc, out, alpha: complex = 1.0) -> None:
+        """out (a 2-D device tensor or a view of a block with unit column
+        stride) += alpha * Op."""
+        _require_cuda(out)
+        if out.stride(-1) != 1:
+            raise BEMError("output block needs unit column stride")
+        desc.dtype = dtype_code(out)
+        a = complex(alpha)
+        check(lib().bem_assemble(self._h, ctypes.byref(desc), out.data_ptr(), int(out.stride(0)),
+                                 a.real, a.imag, current_stream()))
+
+    def apply(self, desc: BemOpDesc, x, y, alpha: complex = 1.0, conj_x: bool = True) -> None:
+        _require_cuda(x, y)
+        if x.dtype != y.dtype:
+            raise BEMError("x and y must share a dtype")
+        desc.dtype = dtype_code(y)
+        a = complex(alpha)
+        check(lib().bem_apply(self._h, ctypes.byref(desc), x.data_ptr(), int(bool(conj_x)), y.data_ptr(),
+                              a.real, a.imag, current_stream()))
+
+    def mass_assemble(self, deg_test: int, deg_trial: int, out, alpha: float = 1.0,
+                      transpose: bool = False) -> None:
+        _require_cuda(out)
+        check(lib().bem_mass_assemble(self._h, int(deg_test), int(deg_trial), dtype_code(out),
+                                      out.data_ptr(), int(out.stride(0)), float(alpha), int(bool(transpose)),
+                                      current_stream()))
+
+    def mass_apply(self, deg_test: int, deg_trial: int, x, y, alpha: float = 1.0, conj_x: bool = True) -> None:
+        _require_cuda(x, y)
+        check(lib().bem_mass_apply(self._h, int(deg_test), int(deg_trial), dtype_code(y), x.data_ptr(),
+                                   int(bool(conj_x)), y.data_ptr(), float(alpha), current_stream()))
+
+    def lu_factor(self, A, ipiv) -> None:
+        _require_cuda(A, ipiv)
+        n = int(A.shape[0])
+        check(lib().bem_lu_factor(self._h, A.data_ptr(), int(A.stride(0)), n, dtype_code(A), ipiv.data_ptr(),
+                                  current_stream()))
+
+    def lu_solve(self, LU, ipiv, b) -> None:
+        _require_cuda(LU, ipiv, b)
+        check(lib().bem_lu_solve(self._h, LU.data_ptr(), int(LU.stride(0)), int(LU.shape[0]), dtype_code(LU),
+                                 ipiv.data_ptr(), b.data_ptr(), current_stream()))
+
+    def residual(self, A, x, b, r) -> None:
+        _require_cuda(A, x, b, r)
+        check(lib().bem_residual(self._h, A.data_ptr(), int(A.stride(0)), int(A.shape[0]), dtype_code(A),
+                                 x.data_ptr(), b.data_ptr(), r.data_ptr(), current_stream()))
+
+    def refine_solve(self, A64, LU, ipiv, b, x, max_iter: int = 30, tol: float = 1e-14) -> BemRefineStats:
+        _require_cuda(A64, LU, ipiv, b, x)
+        st = BemRefineStats()
+        check(lib().bem_refine_solve(self._h, A64.data_ptr(), int(A64.stride(0)), LU.data_ptr(),
+                                     int(LU.stride(0)), int(A64.shape[0]), dtype_code(LU), ipiv.data_ptr(),
+                                     b.data_ptr(), x.data_ptr(), int(max_iter), float(tol), ctypes.byref(st),
+                                     current_stream()))
+        return st
+
+
+def convert(src, dst) -> None:
+    """dst[...] = src with precision conversion on the device (2-D or 1-D)."""
+    _require_cuda(src, dst)
+    s2 = src if src.dim() == 2 else src.view(1, -1)
+    d2 = dst if dst.dim() == 2 else dst.view(1, -1)
+    check(lib().bem_convert(s2.data_ptr(), dtype_code(src), int(s2.stride(0)), d2.data_ptr(), dtype_code(dst),
+                            int(d2.stride(0)), int(s2.shape[0]), int(s2.shape[1]), current_stream()))
+
+
+def matrix_axpy(src, dst, alpha: complex = 1.0, transpose: bool = False) -> None:
+    """dst += alpha * src (or src^T): 2-D device blocks of one dtype."""
+    _require_cuda(src, dst)
+    if src.dtype != dst.dtype:
+        raise BEMError("axpy needs one dtype")
+    a = complex(alpha)
+    rows, cols = int(dst.shape[0]), int(dst.shape[1])
+    check(lib().bem_matrix_axpy(src.data_ptr(), int(src.stride(0)), dst.data_ptr(), int(dst.stride(0)),
+                                dtype_code(dst), rows, cols, a.real, a.imag, int(bool(transpose)),
+                                current_stream()))

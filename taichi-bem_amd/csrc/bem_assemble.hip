@@ -1,0 +1,718 @@
+// bem_assemble.hip -- Galerkin operator assembly and matrix-free products.
+//
+// Work split (the reference runs one thread per test panel over all trial
+// panels, basis pairs and Gauss points serially, single_layer_3d.py:385-433):
+//
+//   far field  : regular (well-separated) pairs with the tensor rule.  A
+//                block owns one test panel i (wave-uniform: its quadrature
+//                points come through scalar loads) and 256 trial panels j
+//                (one per lane, coalesced SoA loads).  Every lane evaluates
+//                the kernel once per point pair and accumulates all basis
+//                pairs from that single evaluation.  Pairs that share a
+//                vertex are skipped here.
+//   near field : coincident / edge / vertex pairs from the near list, one
+//                wave per pair, lanes striding over the Sauter-Schwab points
+//                of the pair's relation, FP64 always, wave-reduced sums.
+//
+// Output: out[row * ld + col] += alpha * value with row/col the test/trial
+// dof (panel for P0, vertex for P1).  P1 rows or columns collect from several
+// panels and use atomics; P0 x P0 entries have a single writer.
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+
+namespace {
+
+struct OpParams {
+    int nf, nv, q2, flags;
+    const int32_t* __restrict__ tris;
+    const double* __restrict__ verts;
+    const double* __restrict__ area;
+    const double* __restrict__ normal;
+    const double* __restrict__ curl;
+    const double* __restrict__ cen;
+    const double* __restrict__ pu;
+    const double* __restrict__ pl;
+    const float* __restrict__ pu32;
+    const float* __restrict__ pl32;
+    const double* __restrict__ phi;
+    const float* __restrict__ phi32;
+    const NearPair* __restrict__ near;
+    const int32_t* __restrict__ near_row;
+    const NearPt* __restrict__ rule;
+    int32_t rule_off[4], rule_len[4];
+    int64_t n_near;
+    double kap;   // wave number inside G
+    double k2n;   // k^2 sqrt_n^2 of the hypersingular term
+    // output / vectors
+    void* out;
+    int64_t ld;
+    int dtype;
+    double ar, ai;
+    const void* x;
+    int conj_x;
+    int dtype_x;
+    void* y;
+    double* near_buf;
+};
+
+template <typename T> struct Ptr;
+template <> struct Ptr<double> {
+    static __device__ __forceinline__ const double* pu(const OpParams& p) { return p.pu; }
+    static __device__ __forceinline__ const double* pl(const OpParams& p) { return p.pl; }
+    static __device__ __forceinline__ const double* phi(const OpParams& p) { return p.phi; }
+};
+template <> struct Ptr<float> {
+    static __device__ __forceinline__ const float* pu(const OpParams& p) { return p.pu32; }
+    static __device__ __forceinline__ const float* pl(const OpParams& p) { return p.pl32; }
+    static __device__ __forceinline__ const float* phi(const OpParams& p) { return p.phi32; }
+};
+
+// dtype-generic accumulate into an output element.
+__device__ __forceinline__ void add_elem(void* out, int dtype, int64_t idx, double vr, double vi,
+                                         bool atomic) {
+    switch (dtype) {
+        case BEM_F64: {
+            double* o = (double*)out + idx;
+            if (atomic) atomicAdd(o, vr); else *o += vr;
+        } break;
+        case BEM_C128: {
+            double* o = (double*)out + 2 * idx;
+            if (atomic) { atomicAdd(o, vr); atomicAdd(o + 1, vi); }
+            else { o[0] += vr; o[1] += vi; }
+        } break;
+        case BEM_F32: {
+            float* o = (float*)out + idx;
+            if (atomic) atomicAdd(o, (float)vr); else *o += (float)vr;
+        } break;
+        default: {
+            float* o = (float*)out + 2 * idx;
+            if (atomic) { atomicAdd(o, (float)vr); atomicAdd(o + 1, (float)vi); }
+            else { o[0] += (float)vr; o[1] += (float)vi; }
+        } break;
+    }
+}
+
+__device__ __forceinline__ void load_elem(const void* x, int dtype, int64_t idx, int conj,
+                                          double& xr, double& xi) {
+    switch (dtype) {
+        case BEM_F64: xr = ((const double*)x)[idx]; xi = 0.0; break;
+        case BEM_C128: xr = ((const double*)x)[2 * idx]; xi = ((const double*)x)[2 * idx + 1]; break;
+        case BEM_F32: xr = ((const float*)x)[idx]; xi = 0.0; break;
+        default: xr = ((const float*)x)[2 * idx]; xi = ((const float*)x)[2 * idx + 1]; break;
+    }
+    if (conj) xi = -xi;
+}
+
+__device__ __forceinline__ bool shares_vertex(const int32_t* t, int i, int j) {
+    int a0 = t[3 * i], a1 = t[3 * i + 1], a2 = t[3 * i + 2];
+    int b0 = t[3 * j], b1 = t[3 * j + 1], b2 = t[3 * j + 2];
+    return a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
+           a2 == b1 || a2 == b2;
+}
+
+// ---------------------------------------------------------------------------
+// Regular pair (i test, j trial): full basis block in FP64 (values already
+// multiplied by 1/(4 pi) and, for W, combined with the curl term).
+//   V  G                          (single_layer_3d.py:155-177)
+//   K  dG/dn_y(x, y)              (double_layer_3d.py:158-180)
+//   K' dG/dn_y with x<->y, n_x    (adj_double_layer_3d.py:160-182)
+//   W  G [curl.curl - k^2 n phi phi n_x.n_y]   (hypersingular_layer_3d.py:186-211)
+// ---------------------------------------------------------------------------
+template <typename R, int OP, bool HELM, int DT, int DR>
+__device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double (&vr)[3][3],
+                                         double (&vi)[3][3]) {
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    const int q2 = p.q2;
+    const int nf = p.nf;
+    const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+    const R* __restrict__ PL = Ptr<R>::pl(p);
+    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    // FP32: d = (c_i - c_j) + (o_a - o_b); FP64: absolute coordinates.
+    R dcx = 0, dcy = 0, dcz = 0;
+    if constexpr (sizeof(R) == 4) {
+        dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
+        dcy = (R)(p.cen[3 * i + 1] - p.cen[3 * j + 1]);
+        dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
+    }
+    R nnx = 0, nny = 0, nnz = 0;
+    if constexpr (OP == BEM_OP_K) {
+        nnx = (R)p.normal[3 * j]; nny = (R)p.normal[3 * j + 1]; nnz = (R)p.normal[3 * j + 2];
+    } else if constexpr (OP == BEM_OP_KADJ) {
+        // (y - x).n_x = -(x - y).n_x
+        nnx = -(R)p.normal[3 * i]; nny = -(R)p.normal[3 * i + 1]; nnz = -(R)p.normal[3 * i + 2];
+    }
+    const R kap = (R)p.kap;
+    R ar[NS][NT], ai[NS][NT];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) { ar[s][t] = 0; ai[s][t] = 0; }
+    R s0r = 0, s0i = 0;  // W: sum of w G (curl term)
+
+    for (int b = 0; b < q2; ++b) {
+        const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
+        const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
+        const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
+        const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
+        R pr[NS], pi[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) { pr[s] = 0; pi[s] = 0; }
+        for (int a = 0; a < q2; ++a) {
+            const R dx = PU[4 * a + 0] + ex;
+            const R dy = PU[4 * a + 1] + ey;
+            const R dz = PU[4 * a + 2] + ez;
+            const R wa = PU[4 * a + 3];
+            const R r2 = dx * dx + dy * dy + dz * dz;
+            const R rinv = Math<R>::rsqrt(r2);
+            R fr, fi;
+            if constexpr (OP == BEM_OP_V || OP == BEM_OP_W) {
+                const R g = wa * rinv;
+                if constexpr (HELM) {
+                    R sn, cs;
+                    Math<R>::sincos(kap * (r2 * rinv), &sn, &cs);
+                    fr = g * cs; fi = g * sn;
+                } else { fr = g; fi = 0; }
+            } else {
+                const R dn = dx * nnx + dy * nny + dz * nnz;
+                const R base = wa * dn * (rinv * rinv * rinv);
+                if constexpr (HELM) {
+                    const R kr = kap * (r2 * rinv);
+                    R sn, cs;
+                    Math<R>::sincos(kr, &sn, &cs);
+                    fr = base * (cs + kr * sn); fi = base * (sn - kr * cs);
+                } else { fr = base; fi = 0; }
+            }
+            if constexpr (DT) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    const R ph = PHI[3 * a + s];
+                    pr[s] += fr * ph; pi[s] += fi * ph;
+                }
+            } else { pr[0] += fr; pi[0] += fi; }
+        }
+        if constexpr (OP == BEM_OP_W) {
+            // sum_a w_a G: the P1 bases sum to one, so it is the sum of the
+            // basis-weighted partials
+            R sr = 0, si = 0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) { sr += pr[s]; si += pi[s]; }
+            s0r += wb * sr;
+            s0i += wb * si;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const R wt = DR ? wb * PHI[3 * b + t] : wb;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) { ar[s][t] += pr[s] * wt; ai[s][t] += pi[s] * wt; }
+        }
+    }
+    if constexpr (OP == BEM_OP_W) {
+        const double nxy = p.normal[3 * i] * p.normal[3 * j] + p.normal[3 * i + 1] * p.normal[3 * j + 1] +
+                           p.normal[3 * i + 2] * p.normal[3 * j + 2];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                double cc = 0.0;
+                if (DT && DR)
+                    for (int d = 0; d < 3; ++d) cc += p.curl[9 * i + 3 * s + d] * p.curl[9 * j + 3 * t + d];
+                vr[s][t] = kInv4Pi * (cc * (double)s0r - p.k2n * nxy * (double)ar[s][t]);
+                vi[s][t] = kInv4Pi * (cc * (double)s0i - p.k2n * nxy * (double)ai[s][t]);
+            }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                vr[s][t] = kInv4Pi * (double)ar[s][t];
+                vi[s][t] = kInv4Pi * (double)ai[s][t];
+            }
+    }
+}
+
+template <typename R, int OP, bool HELM, int DT, int DR>
+__global__ __launch_bounds__(256) void far_assemble_kernel(OpParams p, int njb) {
+    const int i = blockIdx.x / njb;
+    const int j = (blockIdx.x % njb) * 256 + threadIdx.x;
+    if (j >= p.nf) return;
+    if (shares_vertex(p.tris, i, j)) return;  // near field
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    double vr[3][3], vi[3][3];
+    far_pair<R, OP, HELM, DT, DR>(p, i, j, vr, vi);
+    const bool atomic = DT || DR;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int64_t row = DT ? p.tris[3 * i + s] : i;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int64_t col = DR ? p.tris[3 * j + t] : j;
+            add_elem(p.out, p.dtype, row * p.ld + col, p.ar * vr[s][t] - p.ai * vi[s][t],
+                     p.ar * vi[s][t] + p.ai * vr[s][t], atomic);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near pair: Sauter-Schwab rule of the pair's relation in FP64, lanes
+// striding over the rule points, then a wave reduction.  The coincident rule
+// evaluates its second term f(y, x) with the same bases (reference,
+// single_layer_3d.py:198-206) unless BEM_FLAG_SYM_COINCIDE.
+// ---------------------------------------------------------------------------
+template <int OP, bool HELM, int DT, int DR>
+__device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np, int lane,
+                                          double (&vr)[3][3], double (&vi)[3][3]) {
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    const int i = np.i, j = np.j, rel = np.rel;
+    int px[3], py[3], invx[3], invy[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        px[m] = (np.perm >> (2 * m)) & 3;
+        py[m] = (np.perm >> (8 + 2 * m)) & 3;
+    }
+    // position of stored slot s in the (possibly rotated) parametrisation
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) {
+        invx[sl] = px[0] == sl ? 0 : (px[1] == sl ? 1 : 2);
+        invy[sl] = py[0] == sl ? 0 : (py[1] == sl ? 1 : 2);
+    }
+    double X[3][3], Y[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            X[m][d] = p.verts[3 * p.tris[3 * i + px[m]] + d];
+            Y[m][d] = p.verts[3 * p.tris[3 * j + py[m]] + d];
+        }
+    const double nx0 = p.normal[3 * i], nx1 = p.normal[3 * i + 1], nx2 = p.normal[3 * i + 2];
+    const double ny0 = p.normal[3 * j], ny1 = p.normal[3 * j + 1], ny2 = p.normal[3 * j + 2];
+    const double w4a = (p.area[i] * 2.0) * (p.area[j] * 2.0);
+    const double kap = p.kap;
+    const bool twice = rel == BEM_COINCIDE;
+    const bool sym = (p.flags & BEM_FLAG_SYM_COINCIDE) != 0;
+    const NearPt* __restrict__ rule = p.rule + p.rule_off[rel];
+    const int len = p.rule_len[rel];
+
+    double ar[NS][NT], ai[NS][NT];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) { ar[s][t] = 0.0; ai[s][t] = 0.0; }
+    double s0r = 0.0, s0i = 0.0;
+
+    for (int e = lane; e < len; e += 64) {
+        const NearPt q = rule[e];
+        double x[3], y[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x[d] = (1.0 - q.r1x) * X[0][d] + (q.r1x - q.r2x) * X[1][d] + q.r2x * X[2][d];
+            y[d] = (1.0 - q.r1y) * Y[0][d] + (q.r1y - q.r2y) * Y[1][d] + q.r2y * Y[2][d];
+        }
+        const double w = q.w * w4a;
+        for (int pass = 0; pass <= (int)twice; ++pass) {
+            // kernel argument order: pass 1 evaluates f(y, x)
+            const double sg = pass ? -1.0 : 1.0;
+            const double dx = sg * (x[0] - y[0]), dy = sg * (x[1] - y[1]), dz = sg * (x[2] - y[2]);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double rinv = rsqrt_f64(r2);
+            double fr, fi;
+            if constexpr (OP == BEM_OP_V || OP == BEM_OP_W) {
+                if constexpr (HELM) {
+                    double sn, cs;
+                    sincos_f64(kap * (r2 * rinv), &sn, &cs);
+                    fr = rinv * cs; fi = rinv * sn;
+                } else { fr = rinv; fi = 0.0; }
+            } else {
+                const double dn = (OP == BEM_OP_K) ? (dx * ny0 + dy * ny1 + dz * ny2)
+                                                   : -(dx * nx0 + dy * nx1 + dz * nx2);
+                const double base = dn * (rinv * rinv * rinv);
+                if constexpr (HELM) {
+                    const double kr = kap * (r2 * rinv);
+                    double sn, cs;
+                    sincos_f64(kr, &sn, &cs);
+                    fr = base * (cs + kr * sn); fi = base * (sn - kr * cs);
+                } else { fr = base; fi = 0.0; }
+            }
+            fr *= w; fi *= w;
+            const bool swap_b = pass && sym;
+            const double bx1 = swap_b ? q.r1y : q.r1x, bx2 = swap_b ? q.r2y : q.r2x;
+            const double by1 = swap_b ? q.r1x : q.r1y, by2 = swap_b ? q.r2x : q.r2y;
+            if constexpr (OP == BEM_OP_W) { s0r += fr; s0i += fi; }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const double phx = DT ? shape(bx1, bx2, invx[s]) : 1.0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const double ph = phx * (DR ? shape(by1, by2, invy[t]) : 1.0);
+                    ar[s][t] += fr * ph; ai[s][t] += fi * ph;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) { ar[s][t] = wave_sum(ar[s][t]); ai[s][t] = wave_sum(ai[s][t]); }
+    if constexpr (OP == BEM_OP_W) {
+        s0r = wave_sum(s0r); s0i = wave_sum(s0i);
+        const double nxy = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                double cc = 0.0;
+                if (DT && DR)
+                    for (int d = 0; d < 3; ++d) cc += p.curl[9 * i + 3 * s + d] * p.curl[9 * j + 3 * t + d];
+                vr[s][t] = kInv4Pi * (cc * s0r - p.k2n * nxy * ar[s][t]);
+                vi[s][t] = kInv4Pi * (cc * s0i - p.k2n * nxy * ai[s][t]);
+            }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) { vr[s][t] = kInv4Pi * ar[s][t]; vi[s][t] = kInv4Pi * ai[s][t]; }
+    }
+}
+
+template <int OP, bool HELM, int DT, int DR>
+__global__ __launch_bounds__(256) void near_assemble_kernel(OpParams p) {
+    const int lane = threadIdx.x & 63;
+    const int64_t pid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pid >= p.n_near) return;
+    const NearPair np = p.near[pid];
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    double vr[3][3], vi[3][3];
+    near_pair<OP, HELM, DT, DR>(p, np, lane, vr, vi);
+    if (lane != 0) return;
+    const bool atomic = DT || DR;
+    for (int s = 0; s < NS; ++s) {
+        const int64_t row = DT ? p.tris[3 * np.i + s] : np.i;
+        for (int t = 0; t < NT; ++t) {
+            const int64_t col = DR ? p.tris[3 * np.j + t] : np.j;
+            add_elem(p.out, p.dtype, row * p.ld + col, p.ar * vr[s][t] - p.ai * vi[s][t],
+                     p.ar * vi[s][t] + p.ai * vr[s][t], atomic);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free products y[test] += alpha Op xhat.
+// ---------------------------------------------------------------------------
+template <typename R, int OP, bool HELM, int DT, int DR>
+__global__ __launch_bounds__(256) void far_apply_kernel(OpParams p) {
+    const int i = blockIdx.x;
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    double yr[NS], yi[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) { yr[s] = 0.0; yi[s] = 0.0; }
+    for (int j = threadIdx.x; j < p.nf; j += 256) {
+        if (shares_vertex(p.tris, i, j)) continue;
+        double vr[3][3], vi[3][3];
+        far_pair<R, OP, HELM, DT, DR>(p, i, j, vr, vi);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            double xr, xi;
+            load_elem(p.x, p.dtype_x, DR ? p.tris[3 * j + t] : j, p.conj_x, xr, xi);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                yr[s] += vr[s][t] * xr - vi[s][t] * xi;
+                yi[s] += vr[s][t] * xi + vi[s][t] * xr;
+            }
+        }
+    }
+    __shared__ double red[4][6];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = wave_sum(yr[s]), b = wave_sum(yi[s]);
+        if (lane == 0) { red[wid][2 * s] = a; red[wid][2 * s + 1] = b; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int s = 0; s < NS; ++s) {
+            double a = red[0][2 * s] + red[1][2 * s] + red[2][2 * s] + red[3][2 * s];
+            double b = red[0][2 * s + 1] + red[1][2 * s + 1] + red[2][2 * s + 1] + red[3][2 * s + 1];
+            const int64_t row = DT ? p.tris[3 * i + s] : i;
+            add_elem(p.y, p.dtype_x, row, p.ar * a - p.ai * b, p.ar * b + p.ai * a, DT != 0);
+        }
+    }
+}
+
+template <int OP, bool HELM, int DT, int DR>
+__global__ __launch_bounds__(256) void near_apply_kernel(OpParams p) {
+    const int lane = threadIdx.x & 63;
+    const int64_t pid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pid >= p.n_near) return;
+    const NearPair np = p.near[pid];
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    double vr[3][3], vi[3][3];
+    near_pair<OP, HELM, DT, DR>(p, np, lane, vr, vi);
+    if (lane != 0) return;
+    for (int s = 0; s < NS; ++s) {
+        double yr = 0.0, yi = 0.0;
+        for (int t = 0; t < NT; ++t) {
+            double xr, xi;
+            load_elem(p.x, p.dtype_x, DR ? p.tris[3 * np.j + t] : np.j, p.conj_x, xr, xi);
+            yr += vr[s][t] * xr - vi[s][t] * xi;
+            yi += vr[s][t] * xi + vi[s][t] * xr;
+        }
+        p.near_buf[pid * 6 + 2 * s] = yr;
+        p.near_buf[pid * 6 + 2 * s + 1] = yi;
+    }
+}
+
+// Deterministic per-row sum of the near contributions (P0 test rows), or an
+// atomic scatter to vertex rows (P1 test).
+__global__ void near_reduce_kernel(OpParams p, int deg_test) {
+    if (!deg_test) {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= p.nf) return;
+        double a = 0.0, b = 0.0;
+        for (int e = p.near_row[i]; e < p.near_row[i + 1]; ++e) {
+            a += p.near_buf[(int64_t)e * 6];
+            b += p.near_buf[(int64_t)e * 6 + 1];
+        }
+        add_elem(p.y, p.dtype_x, i, p.ar * a - p.ai * b, p.ar * b + p.ai * a, false);
+    } else {
+        const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= p.n_near) return;
+        const int i = p.near[e].i;
+        for (int s = 0; s < 3; ++s) {
+            double a = p.near_buf[e * 6 + 2 * s], b = p.near_buf[e * 6 + 2 * s + 1];
+            add_elem(p.y, p.dtype_x, p.tris[3 * i + s], p.ar * a - p.ai * b, p.ar * b + p.ai * a, true);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mass matrix, single-panel rule (identity_layer_3d.py:117-146).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mass_block(const OpParams& p, const double* mw, int i, int dt, int dr,
+                                           double (&m)[3][3]) {
+    for (int s = 0; s < 3; ++s)
+        for (int t = 0; t < 3; ++t) m[s][t] = 0.0;
+    const double a2 = p.area[i] * 2.0;
+    for (int e = 0; e < p.q2; ++e) {
+        const double w = mw[e] * a2;
+        for (int s = 0; s < (dt ? 3 : 1); ++s) {
+            const double ps = dt ? p.phi[3 * e + s] : 1.0;
+            for (int t = 0; t < (dr ? 3 : 1); ++t) m[s][t] += ps * (dr ? p.phi[3 * e + t] : 1.0) * w;
+        }
+    }
+}
+
+__global__ void mass_assemble_kernel(OpParams p, const double* mw, int dt, int dr, int transpose) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nf) return;
+    double m[3][3];
+    mass_block(p, mw, i, dt, dr, m);
+    const bool atomic = dt && dr;
+    for (int s = 0; s < (dt ? 3 : 1); ++s)
+        for (int t = 0; t < (dr ? 3 : 1); ++t) {
+            int64_t row = dt ? p.tris[3 * i + s] : i, col = dr ? p.tris[3 * i + t] : i;
+            if (transpose) { int64_t tmp = row; row = col; col = tmp; }
+            add_elem(p.out, p.dtype, row * p.ld + col, p.ar * m[s][t], 0.0, atomic);
+        }
+}
+
+__global__ void mass_apply_kernel(OpParams p, const double* mw, int dt, int dr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nf) return;
+    double m[3][3];
+    mass_block(p, mw, i, dt, dr, m);
+    for (int s = 0; s < (dt ? 3 : 1); ++s) {
+        double yr = 0.0, yi = 0.0;
+        for (int t = 0; t < (dr ? 3 : 1); ++t) {
+            double xr, xi;
+            load_elem(p.x, p.dtype_x, dr ? p.tris[3 * i + t] : i, p.conj_x, xr, xi);
+            yr += m[s][t] * xr; yi += m[s][t] * xi;
+        }
+        add_elem(p.y, p.dtype_x, dt ? p.tris[3 * i + s] : i, p.ar * yr, p.ar * yi, dt != 0);
+    }
+}
+
+OpParams make_params(Context* ctx, const bem_op_desc* d) {
+    Mesh& m = ctx->mesh;
+    OpParams p;
+    p.nf = m.nf; p.nv = m.nv; p.q2 = m.q2; p.flags = d ? d->flags : 0;
+    p.tris = m.tris; p.verts = m.verts; p.area = m.area; p.normal = m.normal; p.curl = m.curl;
+    p.cen = m.cen; p.pu = m.pu; p.pl = m.pl; p.pu32 = m.pu32; p.pl32 = m.pl32;
+    p.phi = m.phi; p.phi32 = m.phi32; p.near = m.near; p.near_row = m.near_row; p.rule = m.rule;
+    for (int r = 0; r < 4; ++r) { p.rule_off[r] = m.rule_off[r]; p.rule_len[r] = m.rule_len[r]; }
+    p.n_near = m.n_near;
+    p.kap = 0.0; p.k2n = 0.0;
+    if (d) {
+        p.kap = d->kernel == BEM_HELMHOLTZ_TRANSMISSION ? d->k * d->sqrt_n : d->k;
+        p.k2n = d->k * d->k * d->sqrt_n * d->sqrt_n;
+    }
+    p.out = nullptr; p.ld = 0; p.dtype = BEM_C128; p.ar = 1.0; p.ai = 0.0;
+    p.x = nullptr; p.conj_x = 0; p.dtype_x = BEM_C128; p.y = nullptr; p.near_buf = m.near_buf;
+    return p;
+}
+
+// ------------------------- dispatch -----------------------------------------
+using FarFn = void (*)(OpParams, int);
+using NearFn = void (*)(OpParams);
+
+template <typename R, int OP, bool HELM>
+FarFn far_assemble_fn(int dt, int dr) {
+    if (dt == 0 && dr == 0) return far_assemble_kernel<R, OP, HELM, 0, 0>;
+    if (dt == 0 && dr == 1) return far_assemble_kernel<R, OP, HELM, 0, 1>;
+    if (dt == 1 && dr == 0) return far_assemble_kernel<R, OP, HELM, 1, 0>;
+    return far_assemble_kernel<R, OP, HELM, 1, 1>;
+}
+template <typename R, int OP, bool HELM>
+NearFn far_apply_fn(int dt, int dr) {
+    if (dt == 0 && dr == 0) return far_apply_kernel<R, OP, HELM, 0, 0>;
+    if (dt == 0 && dr == 1) return far_apply_kernel<R, OP, HELM, 0, 1>;
+    if (dt == 1 && dr == 0) return far_apply_kernel<R, OP, HELM, 1, 0>;
+    return far_apply_kernel<R, OP, HELM, 1, 1>;
+}
+template <int OP, bool HELM>
+NearFn near_fn(int dt, int dr, bool apply) {
+    if (apply) {
+        if (dt == 0 && dr == 0) return near_apply_kernel<OP, HELM, 0, 0>;
+        if (dt == 0 && dr == 1) return near_apply_kernel<OP, HELM, 0, 1>;
+        if (dt == 1 && dr == 0) return near_apply_kernel<OP, HELM, 1, 0>;
+        return near_apply_kernel<OP, HELM, 1, 1>;
+    }
+    if (dt == 0 && dr == 0) return near_assemble_kernel<OP, HELM, 0, 0>;
+    if (dt == 0 && dr == 1) return near_assemble_kernel<OP, HELM, 0, 1>;
+    if (dt == 1 && dr == 0) return near_assemble_kernel<OP, HELM, 1, 0>;
+    return near_assemble_kernel<OP, HELM, 1, 1>;
+}
+
+template <typename R, int OP>
+FarFn far_assemble_k(bool helm, int dt, int dr) {
+    return helm ? far_assemble_fn<R, OP, true>(dt, dr) : far_assemble_fn<R, OP, false>(dt, dr);
+}
+template <typename R>
+FarFn far_assemble_any(int op, bool helm, int dt, int dr) {
+    switch (op) {
+        case BEM_OP_V: return far_assemble_k<R, BEM_OP_V>(helm, dt, dr);
+        case BEM_OP_K: return far_assemble_k<R, BEM_OP_K>(helm, dt, dr);
+        case BEM_OP_KADJ: return far_assemble_k<R, BEM_OP_KADJ>(helm, dt, dr);
+        default: return far_assemble_k<R, BEM_OP_W>(helm, dt, dr);
+    }
+}
+template <typename R, int OP>
+NearFn far_apply_k(bool helm, int dt, int dr) {
+    return helm ? far_apply_fn<R, OP, true>(dt, dr) : far_apply_fn<R, OP, false>(dt, dr);
+}
+template <typename R>
+NearFn far_apply_any(int op, bool helm, int dt, int dr) {
+    switch (op) {
+        case BEM_OP_V: return far_apply_k<R, BEM_OP_V>(helm, dt, dr);
+        case BEM_OP_K: return far_apply_k<R, BEM_OP_K>(helm, dt, dr);
+        case BEM_OP_KADJ: return far_apply_k<R, BEM_OP_KADJ>(helm, dt, dr);
+        default: return far_apply_k<R, BEM_OP_W>(helm, dt, dr);
+    }
+}
+template <int OP>
+NearFn near_k(bool helm, int dt, int dr, bool apply) {
+    return helm ? near_fn<OP, true>(dt, dr, apply) : near_fn<OP, false>(dt, dr, apply);
+}
+NearFn near_any(int op, bool helm, int dt, int dr, bool apply) {
+    switch (op) {
+        case BEM_OP_V: return near_k<BEM_OP_V>(helm, dt, dr, apply);
+        case BEM_OP_K: return near_k<BEM_OP_K>(helm, dt, dr, apply);
+        case BEM_OP_KADJ: return near_k<BEM_OP_KADJ>(helm, dt, dr, apply);
+        default: return near_k<BEM_OP_W>(helm, dt, dr, apply);
+    }
+}
+
+int validate(Context* ctx, const bem_op_desc* d) {
+    if (!ctx->mesh.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
+    if (!d) return set_error(BEM_ERR_INVALID, "null operator descriptor");
+    if (d->op < BEM_OP_V || d->op > BEM_OP_W) return set_error(BEM_ERR_INVALID, "unknown operator");
+    if (d->kernel < BEM_LAPLACE || d->kernel > BEM_HELMHOLTZ_TRANSMISSION)
+        return set_error(BEM_ERR_INVALID, "Kernel Type only support Laplace and Helmholtz for now");
+    if ((d->deg_test | d->deg_trial) & ~1) return set_error(BEM_ERR_INVALID, "basis degree must be 0 or 1");
+    if (d->dtype < BEM_F64 || d->dtype > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    if (d->kernel != BEM_LAPLACE && !dtype_complex(d->dtype))
+        return set_error(BEM_ERR_INVALID, "Helmholtz operators need a complex dtype");
+    if (d->precision != BEM_PREC_FP64 && d->precision != BEM_PREC_FP32)
+        return set_error(BEM_ERR_INVALID, "unknown precision");
+    return BEM_OK;
+}
+
+}  // namespace
+
+int assemble_op(Context* ctx, const bem_op_desc* d, void* out, int64_t ld, double ar, double ai,
+                hipStream_t s) {
+    int rc = validate(ctx, d);
+    if (rc) return rc;
+    if (!out) return set_error(BEM_ERR_INVALID, "null output");
+    rc = mesh_build_near(ctx, d->flags, s);
+    if (rc) return rc;
+    OpParams p = make_params(ctx, d);
+    p.out = out; p.ld = ld; p.dtype = d->dtype; p.ar = ar; p.ai = ai;
+    const bool helm = d->kernel != BEM_LAPLACE;
+    const int nf = p.nf, njb = (nf + 255) / 256;
+    FarFn far = d->precision == BEM_PREC_FP32 ? far_assemble_any<float>(d->op, helm, d->deg_test, d->deg_trial)
+                                              : far_assemble_any<double>(d->op, helm, d->deg_test, d->deg_trial);
+    hipLaunchKernelGGL(far, dim3((unsigned)((int64_t)nf * njb)), dim3(256), 0, s, p, njb);
+    rc = check_hip(hipGetLastError(), "far-field assembly launch");
+    if (rc) return rc;
+    if (p.n_near > 0) {
+        NearFn nearf = near_any(d->op, helm, d->deg_test, d->deg_trial, false);
+        hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
+        rc = check_hip(hipGetLastError(), "near-field assembly launch");
+    }
+    return rc;
+}
+
+int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void* y, double ar,
+             double ai, hipStream_t s) {
+    int rc = validate(ctx, d);
+    if (rc) return rc;
+    if (!x || !y) return set_error(BEM_ERR_INVALID, "null vector");
+    rc = mesh_build_near(ctx, d->flags, s);
+    if (rc) return rc;
+    OpParams p = make_params(ctx, d);
+    p.x = x; p.conj_x = conj_x; p.dtype_x = d->dtype; p.y = y; p.ar = ar; p.ai = ai;
+    const bool helm = d->kernel != BEM_LAPLACE;
+    NearFn far = d->precision == BEM_PREC_FP32 ? far_apply_any<float>(d->op, helm, d->deg_test, d->deg_trial)
+                                               : far_apply_any<double>(d->op, helm, d->deg_test, d->deg_trial);
+    hipLaunchKernelGGL(far, dim3(p.nf), dim3(256), 0, s, p);
+    rc = check_hip(hipGetLastError(), "far-field apply launch");
+    if (rc) return rc;
+    if (p.n_near > 0) {
+        NearFn nearf = near_any(d->op, helm, d->deg_test, d->deg_trial, true);
+        hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
+        const int64_t nth = d->deg_test ? p.n_near : p.nf;
+        hipLaunchKernelGGL(near_reduce_kernel, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, p,
+                           d->deg_test);
+        rc = check_hip(hipGetLastError(), "near-field apply launch");
+    }
+    return rc;
+}
+
+int mass_assemble(Context* ctx, int deg_test, int deg_trial, int dtype, void* out, int64_t ld,
+                  double alpha, int transpose, hipStream_t s) {
+    if (!ctx->mesh.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
+    if (!out) return set_error(BEM_ERR_INVALID, "null output");
+    if ((deg_test | deg_trial) & ~1) return set_error(BEM_ERR_INVALID, "basis degree must be 0 or 1");
+    if (dtype < BEM_F64 || dtype > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    OpParams p = make_params(ctx, nullptr);
+    p.out = out; p.ld = ld; p.dtype = dtype; p.ar = alpha;
+    hipLaunchKernelGGL(mass_assemble_kernel, dim3((p.nf + 255) / 256), dim3(256), 0, s, p,
+                       ctx->mesh.mass_w, deg_test, deg_trial, transpose);
+    return check_hip(hipGetLastError(), "mass assembly launch");
+}
+
+int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void* x, int conj_x,
+               void* y, double alpha, hipStream_t s) {
+    if (!ctx->mesh.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
+    if (!x || !y) return set_error(BEM_ERR_INVALID, "null vector");
+    if ((deg_test | deg_trial) & ~1) return set_error(BEM_ERR_INVALID, "basis degree must be 0 or 1");
+    if (dtype < BEM_F64 || dtype > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    OpParams p = make_params(ctx, nullptr);
+    p.x = x; p.conj_x = conj_x; p.dtype_x = dtype; p.y = y; p.ar = alpha;
+    hipLaunchKernelGGL(mass_apply_kernel, dim3((p.nf + 255) / 256), dim3(256), 0, s, p,
+                       ctx->mesh.mass_w, deg_test, deg_trial);
+    return check_hip(hipGetLastError(), "mass apply launch");
+}
+
+}  // namespace bem

@@ -1,0 +1,188 @@
+// bem_capi.hip -- the extern "C" entry points declared in include/bem_hip.h.
+#include <string>
+
+#include "bem_internal.h"
+
+struct bem_context {
+    bem::Context impl;
+};
+
+namespace bem {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return BEM_OK;
+    return set_error(BEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace bem
+
+using bem::check_hip;
+using bem::set_error;
+
+namespace {
+
+// Runs `fn` with the context's device current; restores the caller's device.
+template <typename F>
+int with_device(bem_context* ctx, F fn) {
+    if (!ctx) return set_error(BEM_ERR_INVALID, "null context");
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess) prev = ctx->impl.device;
+    if (prev != ctx->impl.device) {
+        int rc = check_hip(hipSetDevice(ctx->impl.device), "hipSetDevice");
+        if (rc) return rc;
+    }
+    int rc = fn();
+    if (prev != ctx->impl.device) hipSetDevice(prev);
+    return rc;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+int bem_version(void) { return 1; }
+
+const char* bem_last_error(void) { return bem::g_last_error.c_str(); }
+
+int bem_create(bem_context** out, int device) {
+    if (!out) return set_error(BEM_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    int count = 0;
+    int rc = check_hip(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (rc) return rc;
+    if (device < 0 || device >= count) return set_error(BEM_ERR_INVALID, "device index out of range");
+    bem_context* ctx = new bem_context();
+    ctx->impl.device = device;
+    rc = with_device(ctx, [&]() -> int {
+        return check_hip(hipMalloc((void**)&ctx->impl.counters, 64 * sizeof(int)), "counters");
+    });
+    if (rc) { delete ctx; return rc; }
+    ctx->impl.counters_len = 64;
+    *out = ctx;
+    return BEM_OK;
+}
+
+int bem_destroy(bem_context* ctx) {
+    if (!ctx) return BEM_OK;
+    with_device(ctx, [&]() -> int {
+        hipDeviceSynchronize();
+        bem::mesh_free(&ctx->impl.mesh);
+        if (ctx->impl.work) hipFree(ctx->impl.work);
+        if (ctx->impl.counters) hipFree(ctx->impl.counters);
+        return BEM_OK;
+    });
+    delete ctx;
+    return BEM_OK;
+}
+
+int bem_set_mesh(bem_context* ctx, const double* vertices, const int32_t* panels, int32_t nv, int32_t nf,
+                 int32_t gauss_qr, void* stream) {
+    if (!vertices || !panels) return set_error(BEM_ERR_INVALID, "null mesh arrays");
+    return with_device(ctx, [&]() -> int {
+        return bem::mesh_build(&ctx->impl, vertices, panels, nv, nf, gauss_qr, as_stream(stream));
+    });
+}
+
+int bem_get_geometry(bem_context* ctx, double* area, double* normal, double* vert_area, double* vert_normal,
+                     void* stream) {
+    return with_device(ctx, [&]() -> int {
+        const bem::Mesh& m = ctx->impl.mesh;
+        if (!m.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
+        hipStream_t s = as_stream(stream);
+        int rc = 0;
+        if (area) rc |= check_hip(hipMemcpyAsync(area, m.area, sizeof(double) * m.nf, hipMemcpyDeviceToDevice, s), "area");
+        if (normal) rc |= check_hip(hipMemcpyAsync(normal, m.normal, sizeof(double) * m.nf * 3, hipMemcpyDeviceToDevice, s), "normal");
+        if (vert_area) rc |= check_hip(hipMemcpyAsync(vert_area, m.vert_area, sizeof(double) * m.nv, hipMemcpyDeviceToDevice, s), "vert_area");
+        if (vert_normal) rc |= check_hip(hipMemcpyAsync(vert_normal, m.vert_normal, sizeof(double) * m.nv * 3, hipMemcpyDeviceToDevice, s), "vert_normal");
+        return rc ? BEM_ERR_HIP : BEM_OK;
+    });
+}
+
+int bem_nearfield_counts(bem_context* ctx, int64_t counts[4]) {
+    if (!counts) return set_error(BEM_ERR_INVALID, "null counts");
+    return with_device(ctx, [&]() -> int {
+        const bem::Mesh& m = ctx->impl.mesh;
+        if (!m.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
+        for (int i = 0; i < 4; ++i) counts[i] = m.near_counts[i];
+        return BEM_OK;
+    });
+}
+
+int bem_assemble(bem_context* ctx, const bem_op_desc* op, void* out, int64_t ld, double alpha_re,
+                 double alpha_im, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::assemble_op(&ctx->impl, op, out, ld, alpha_re, alpha_im, as_stream(stream));
+    });
+}
+
+int bem_apply(bem_context* ctx, const bem_op_desc* op, const void* x, int32_t conj_x, void* y,
+              double alpha_re, double alpha_im, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::apply_op(&ctx->impl, op, x, conj_x, y, alpha_re, alpha_im, as_stream(stream));
+    });
+}
+
+int bem_mass_assemble(bem_context* ctx, int32_t deg_test, int32_t deg_trial, int32_t dtype, void* out,
+                      int64_t ld, double alpha, int32_t transpose, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::mass_assemble(&ctx->impl, deg_test, deg_trial, dtype, out, ld, alpha, transpose,
+                                  as_stream(stream));
+    });
+}
+
+int bem_mass_apply(bem_context* ctx, int32_t deg_test, int32_t deg_trial, int32_t dtype, const void* x,
+                   int32_t conj_x, void* y, double alpha, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::mass_apply(&ctx->impl, deg_test, deg_trial, dtype, x, conj_x, y, alpha, as_stream(stream));
+    });
+}
+
+int bem_lu_factor(bem_context* ctx, void* A, int64_t ld, int32_t n, int32_t dtype, int32_t* ipiv,
+                  void* stream) {
+    return with_device(ctx, [&]() -> int { return bem::lu_factor(&ctx->impl, A, ld, n, dtype, ipiv, as_stream(stream)); });
+}
+
+int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype, const int32_t* ipiv,
+                 void* b, void* stream) {
+    return with_device(ctx, [&]() -> int { return bem::lu_solve(&ctx->impl, LU, ld, n, dtype, ipiv, b, as_stream(stream)); });
+}
+
+int bem_residual(bem_context* ctx, const void* A, int64_t ld, int32_t n, int32_t dtype, const void* x,
+                 const void* b, void* r, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::gemv_residual(&ctx->impl, A, ld, n, dtype, x, b, r, as_stream(stream));
+    });
+}
+
+int bem_refine_solve(bem_context* ctx, const void* A64, int64_t lda, const void* LU, int64_t ldlu, int32_t n,
+                     int32_t dtype_lu, const int32_t* ipiv, const void* b, void* x, int32_t max_iter, double tol,
+                     bem_refine_stats* stats, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::refine_solve(&ctx->impl, A64, lda, LU, ldlu, n, dtype_lu, ipiv, b, x, max_iter, tol, stats,
+                                 as_stream(stream));
+    });
+}
+
+int bem_convert(const void* src, int32_t dtype_src, int64_t ld_src, void* dst, int32_t dtype_dst, int64_t ld_dst,
+                int32_t rows, int32_t cols, void* stream) {
+    return bem::convert_matrix(src, dtype_src, ld_src, dst, dtype_dst, ld_dst, rows, cols, as_stream(stream));
+}
+
+int bem_matrix_axpy(const void* src, int64_t ld_src, void* dst, int64_t ld_dst, int32_t dtype, int32_t rows,
+                    int32_t cols, double alpha_re, double alpha_im, int32_t transpose, void* stream) {
+    return bem::matrix_axpy(src, ld_src, dst, ld_dst, dtype, rows, cols, alpha_re, alpha_im, transpose,
+                            as_stream(stream));
+}
+
+}  // extern "C"

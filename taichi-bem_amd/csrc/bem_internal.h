@@ -1,0 +1,109 @@
+// bem_internal.h -- context layout and internal launchers of libbem_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bem_hip.h"
+
+namespace bem {
+
+// One near-field pair: test panel i, trial panel j, relation, and the vertex
+// slot order used for each panel (2 bits per slot: px = s0 | s1<<2 | s2<<4,
+// py the same at bit 8).  Identity order reproduces the reference.
+struct NearPair {
+    int32_t i, j, rel, perm;
+};
+
+// Point of a near-field rule: parametric coordinates on both panels and the
+// product of the four Gauss weights with the region Jacobian.
+struct NearPt {
+    double r1x, r2x, r1y, r2y, w;
+};
+
+struct Mesh {
+    int nv = 0, nf = 0, q = 0, q2 = 0;
+    double* verts = nullptr;      // [nv*3]
+    int32_t* tris = nullptr;      // [nf*3]
+    double* area = nullptr;       // [nf]
+    double* normal = nullptr;     // [nf*3]
+    double* vert_area = nullptr;  // [nv]
+    double* vert_normal = nullptr;// [nv*3]
+    double* curl = nullptr;       // [nf][3 slots][3]  n x grad(phi_s)
+    // Regular-rule points of every panel (r1, r2) = (xi, xi*eta), weight
+    // w_xi w_eta xi (2A).  FP64 absolute coordinates:
+    double* pu = nullptr;         // AoS [nf][q2][4] (x, y, z, w): uniform (scalar) loads
+    double* pl = nullptr;         // SoA [q2][4][nf]: per-lane coalesced loads
+    // FP32 tables: offsets from the FP64 panel centroid.
+    double* cen = nullptr;        // [nf*3]
+    float* pu32 = nullptr;        // [nf][q2][4]
+    float* pl32 = nullptr;        // [q2][4][nf]
+    double* phi = nullptr;        // [q2][3] P1 shape values at the regular points
+    float* phi32 = nullptr;       // [q2][3]
+    double* mass_w = nullptr;     // [q2] w_xi w_eta xi (single-panel rule, no area)
+    // Near field
+    NearPair* near = nullptr;     // sorted by i then j
+    int32_t* near_row = nullptr;  // CSR row pointer over test panels [nf+1]
+    int64_t n_near = 0;
+    int64_t near_counts[4] = {0, 0, 0, 0};
+    int near_flags = -1;          // flags the near list was built for
+    NearPt* rule = nullptr;       // concatenated near rules
+    int32_t rule_off[4] = {0, 0, 0, 0};  // offset per relation (index 0 unused)
+    int32_t rule_len[4] = {0, 0, 0, 0};
+    std::vector<int32_t> h_tris;  // host copy for near-list construction
+    double* near_buf = nullptr;   // scratch for matrix-free near contributions
+    int64_t near_buf_len = 0;
+};
+
+struct Context {
+    int device = 0;
+    Mesh mesh;
+    // LU / solver scratch
+    void* work = nullptr;
+    size_t work_bytes = 0;
+    int* counters = nullptr;      // grid-barrier / pivot scratch
+    size_t counters_len = 0;
+};
+
+// error helpers (bem_capi.hip)
+int set_error(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+// geometry / tables (bem_geometry.hip)
+int mesh_build(Context* ctx, const double* d_verts, const int32_t* d_tris, int nv, int nf, int q,
+               hipStream_t s);
+int mesh_build_near(Context* ctx, int flags, hipStream_t s);
+void mesh_free(Mesh* m);
+
+// assembly (bem_assemble.hip)
+int assemble_op(Context* ctx, const bem_op_desc* d, void* out, int64_t ld, double ar, double ai,
+                hipStream_t s);
+int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void* y, double ar,
+             double ai, hipStream_t s);
+int mass_assemble(Context* ctx, int deg_test, int deg_trial, int dtype, void* out, int64_t ld,
+                  double alpha, int transpose, hipStream_t s);
+int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void* x, int conj_x,
+               void* y, double alpha, hipStream_t s);
+
+// dense linear algebra (bem_linalg.hip)
+int lu_factor(Context* ctx, void* A, int64_t ld, int n, int dtype, int32_t* ipiv, hipStream_t s);
+int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv,
+             void* b, hipStream_t s);
+int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, const void* x,
+                  const void* b, void* r, hipStream_t s);
+int refine_solve(Context* ctx, const void* A64, int64_t lda, const void* LU, int64_t ldlu,
+                 int n, int dtype_lu, const int32_t* ipiv, const void* b, void* x, int max_iter,
+                 double tol, bem_refine_stats* st, hipStream_t s);
+int convert_matrix(const void* src, int dtype_src, int64_t lds, void* dst, int dtype_dst,
+                   int64_t ldd, int rows, int cols, hipStream_t s);
+
+int matrix_axpy(const void* src, int64_t lds, void* dst, int64_t ldd, int dtype, int rows, int cols,
+                double ar, double ai, int transpose, hipStream_t s);
+
+size_t dtype_size(int dtype);
+bool dtype_complex(int dtype);
+
+}  // namespace bem

@@ -1,0 +1,746 @@
+// bem_linalg.hip -- dense LU with partial pivoting, triangular solves,
+// FP64 residuals and mixed-precision iterative refinement.
+//
+// Replaces torch.linalg.solve in BEMManager.solve
+// (src/managers/BEM_manager.py:1032-1073).  Matrices are row-major (the
+// layout of the reference's Taichi field _mat_A and of torch), complex
+// interleaved.  Right-looking blocked LU, panel width NB:
+//   1. panel: per column, pivot search (|re|+|im|, first maximum, as LAPACK
+//      izamax) and row swap inside the panel, then scale + rank-1 update
+//   2. the panel's row interchanges applied to the other columns
+//   3. U12 = L11^{-1} A12 (unit lower triangular solve)
+//   4. A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32),
+//      complex products as four real products
+#include <math.h>
+
+#include <vector>
+
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+
+size_t dtype_size(int dtype) {
+    switch (dtype) {
+        case BEM_F64: return 8;
+        case BEM_C128: return 16;
+        case BEM_F32: return 4;
+        default: return 8;
+    }
+}
+bool dtype_complex(int dtype) { return dtype == BEM_C128 || dtype == BEM_C64; }
+
+namespace {
+
+constexpr int NB = 64;  // panel width
+
+template <typename T>
+struct cx {
+    T re, im;
+};
+
+template <typename T, bool C>
+struct Acc {
+    // element access of a row-major matrix of T (C: interleaved complex)
+    static __device__ __forceinline__ cx<T> ld(const T* a, int64_t idx) {
+        if constexpr (C) return {a[2 * idx], a[2 * idx + 1]};
+        else return {a[idx], T(0)};
+    }
+    static __device__ __forceinline__ void st(T* a, int64_t idx, cx<T> v) {
+        if constexpr (C) { a[2 * idx] = v.re; a[2 * idx + 1] = v.im; }
+        else a[idx] = v.re;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
+    // Smith's algorithm
+    if (fabs((double)b.re) >= fabs((double)b.im)) {
+        T r = b.im / b.re, d = b.re + b.im * r;
+        return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+    }
+    T r = b.re / b.im, d = b.im + b.re * r;
+    return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+// ---------------------------------------------------------------------------
+// Panel, column c: pivot search over rows [c, n) and swap rows c <-> p inside
+// the panel columns [k, kend).  One block of 1024 threads.
+// ---------------------------------------------------------------------------
+template <typename T, bool C>
+__global__ __launch_bounds__(1024) void pivot_kernel(T* A, int64_t ld, int n, int c, int k, int kend,
+                                                     int32_t* ipiv, int* info) {
+    __shared__ double sv[1024];
+    __shared__ int si[1024];
+    double best = -1.0;
+    int bi = n;
+    for (int r = c + threadIdx.x; r < n; r += 1024) {
+        cx<T> v = Acc<T, C>::ld(A, (int64_t)r * ld + c);
+        double m = fabs((double)v.re) + fabs((double)v.im);
+        if (m > best) { best = m; bi = r; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            double ov = sv[threadIdx.x + off];
+            int oi = si[threadIdx.x + off];
+            if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = ov; si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = si[0];
+    if (threadIdx.x == 0) {
+        ipiv[c] = p;
+        if (sv[0] == 0.0 && *info == 0) *info = c + 1;
+    }
+    if (p != c) {
+        for (int col = k + threadIdx.x; col < kend; col += 1024) {
+            cx<T> a = Acc<T, C>::ld(A, (int64_t)c * ld + col);
+            cx<T> b = Acc<T, C>::ld(A, (int64_t)p * ld + col);
+            Acc<T, C>::st(A, (int64_t)c * ld + col, b);
+            Acc<T, C>::st(A, (int64_t)p * ld + col, a);
+        }
+    }
+}
+
+// Panel, column c: l_r = A[r][c] / A[c][c]; A[r][c+1:kend] -= l_r A[c][c+1:kend]
+// for rows r > c.  blockDim = (64 columns, 4 rows).
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t ld, int n, int c, int kend) {
+    const int r = c + 1 + blockIdx.x * 4 + threadIdx.y;
+    const int col = c + threadIdx.x;  // c .. c+63
+    __shared__ cx<T> piv_row[64];
+    if (threadIdx.y == 0 && col < kend) piv_row[threadIdx.x] = Acc<T, C>::ld(A, (int64_t)c * ld + col);
+    __syncthreads();
+    if (r >= n) return;
+    const cx<T> diag = piv_row[0];
+    const cx<T> arc = Acc<T, C>::ld(A, (int64_t)r * ld + c);
+    cx<T> l = (diag.re == T(0) && diag.im == T(0)) ? arc : cdiv(arc, diag);
+    if (col >= kend) return;
+    if (threadIdx.x == 0) {
+        Acc<T, C>::st(A, (int64_t)r * ld + c, l);
+    } else {
+        cx<T> a = Acc<T, C>::ld(A, (int64_t)r * ld + col);
+        cx<T> u = piv_row[threadIdx.x];
+        cx<T> lu = cmul(l, u);
+        a.re -= lu.re; a.im -= lu.im;
+        Acc<T, C>::st(A, (int64_t)r * ld + col, a);
+    }
+}
+
+// Row interchanges of the panel [k, kend) applied to columns outside it.
+template <typename T, bool C>
+__global__ void laswp_kernel(T* A, int64_t ld, int n, int k, int kend, const int32_t* ipiv) {
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n || (col >= k && col < kend)) return;
+    for (int c = k; c < kend; ++c) {
+        int p = ipiv[c];
+        if (p != c) {
+            cx<T> a = Acc<T, C>::ld(A, (int64_t)c * ld + col);
+            cx<T> b = Acc<T, C>::ld(A, (int64_t)p * ld + col);
+            Acc<T, C>::st(A, (int64_t)c * ld + col, b);
+            Acc<T, C>::st(A, (int64_t)p * ld + col, a);
+        }
+    }
+}
+
+// U12 = L11^{-1} A12 for columns [kend, n): one thread per column, forward
+// substitution down the nb rows of the panel, L11 staged in LDS.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void trsm_kernel(T* A, int64_t ld, int n, int k, int kend) {
+    __shared__ cx<T> L[NB][NB];
+    const int nb = kend - k;
+    for (int e = threadIdx.x; e < nb * nb; e += 256) {
+        int r = e / nb, c = e % nb;
+        L[r][c] = Acc<T, C>::ld(A, (int64_t)(k + r) * ld + (k + c));
+    }
+    __syncthreads();
+    const int col = kend + blockIdx.x * 256 + threadIdx.x;
+    if (col >= n) return;
+    cx<T> x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+        if (r < nb) x[r] = Acc<T, C>::ld(A, (int64_t)(k + r) * ld + col);
+#pragma unroll
+    for (int r = 1; r < NB; ++r) {
+        if (r < nb) {
+            cx<T> s = x[r];
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < r) {
+                    cx<T> lc = cmul(L[r][c], x[c]);
+                    s.re -= lc.re; s.im -= lc.im;
+                }
+            x[r] = s;
+        }
+    }
+#pragma unroll
+    for (int r = 1; r < NB; ++r)
+        if (r < nb) Acc<T, C>::st(A, (int64_t)(k + r) * ld + col, x[r]);
+}
+
+// ---------------------------------------------------------------------------
+// C[M x N] -= A[M x K] B[K x N] on MFMA.  Block tile 64 x 64, four waves in a
+// 2 x 2 arrangement, each wave 32 x 32 = 2 x 2 MFMA 16x16 blocks.  K is
+// staged through LDS in chunks of 16.  MFMA 16x16x4 operand layout: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; result register r of lane l is
+// D[4*(l/16)+r][l%16].
+// ---------------------------------------------------------------------------
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int GT = 64;   // block tile
+constexpr int KC = 16;   // K chunk
+
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void gemm_sub_kernel(T* Cm, int64_t ldc, const T* Am, int64_t lda,
+                                                       const T* Bm, int64_t ldb, int M, int N, int K) {
+    using V4 = typename Mfma<T>::v4;
+    __shared__ T sAr[GT][KC + 1], sAi[C ? GT : 1][KC + 1];
+    __shared__ T sBr[KC][GT + 1], sBi[C ? KC : 1][GT + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bm = blockIdx.y * GT, bn = blockIdx.x * GT;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    V4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accR[a][b] = V4{0, 0, 0, 0};
+            accI[a][b] = V4{0, 0, 0, 0};
+        }
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        // stage A (GT x KC) and B (KC x GT); zero outside the matrix
+        for (int e = tid; e < GT * KC; e += 256) {
+            int r = e / KC, c = e % KC;
+            int gr = bm + r, gc = k0 + c;
+            cx<T> v = {T(0), T(0)};
+            if (gr < M && gc < K) v = Acc<T, C>::ld(Am, (int64_t)gr * lda + gc);
+            sAr[r][c] = v.re;
+            if constexpr (C) sAi[r][c] = v.im;
+        }
+        for (int e = tid; e < KC * GT; e += 256) {
+            int r = e / GT, c = e % GT;
+            int gr = k0 + r, gc = bn + c;
+            cx<T> v = {T(0), T(0)};
+            if (gr < K && gc < N) v = Acc<T, C>::ld(Bm, (int64_t)gr * ldb + gc);
+            sBr[r][c] = v.re;
+            if constexpr (C) sBi[r][c] = v.im;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            const int ka = kk + (lane >> 4);
+            T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                ar[a] = sAr[wm + 16 * a + (lane & 15)][ka];
+                if constexpr (C) ai[a] = sAi[wm + 16 * a + (lane & 15)][ka];
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                br[b] = sBr[ka][wn + 16 * b + (lane & 15)];
+                if constexpr (C) bi[b] = sBi[ka][wn + 16 * b + (lane & 15)];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accR[a][b] = Mfma<T>::op(ar[a], br[b], accR[a][b]);
+                    if constexpr (C) {
+                        accR[a][b] = Mfma<T>::op(-ai[a], bi[b], accR[a][b]);
+                        accI[a][b] = Mfma<T>::op(ar[a], bi[b], accI[a][b]);
+                        accI[a][b] = Mfma<T>::op(ai[a], br[b], accI[a][b]);
+                    }
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = bm + wm + 16 * a + 4 * (lane >> 4) + r;
+                const int gc = bn + wn + 16 * b + (lane & 15);
+                if (gr < M && gc < N) {
+                    cx<T> v = Acc<T, C>::ld(Cm, (int64_t)gr * ldc + gc);
+                    v.re -= accR[a][b][r];
+                    if constexpr (C) v.im -= accI[a][b][r];
+                    Acc<T, C>::st(Cm, (int64_t)gr * ldc + gc, v);
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with one right-hand side, blocked by TB rows.
+// ---------------------------------------------------------------------------
+constexpr int TB = 64;
+
+// x[blk] solve with the TB x TB diagonal block (lower unit or upper): the
+// block is staged in LDS first so the sequential sweep touches LDS only.
+template <typename T, bool C, bool LOWER>
+__global__ __launch_bounds__(64) void trsv_diag_kernel(const T* A, int64_t ld, int n, int r0, double* x) {
+    __shared__ double xs[TB][2];
+    __shared__ T blk[TB][TB][C ? 2 : 1];
+    const int nb = min(TB, n - r0);
+    const int t = threadIdx.x;
+    for (int r = 0; r < nb; ++r)
+        if (t < nb) {
+            cx<T> v = Acc<T, C>::ld(A, (int64_t)(r0 + r) * ld + (r0 + t));
+            blk[r][t][0] = v.re;
+            if constexpr (C) blk[r][t][C ? 1 : 0] = v.im;
+        }
+    if (t < nb) { xs[t][0] = x[2 * (r0 + t)]; xs[t][1] = x[2 * (r0 + t) + 1]; }
+    __syncthreads();
+    auto ldb = [&](int r, int c) -> cx<double> {
+        return {(double)blk[r][c][0], C ? (double)blk[r][c][C ? 1 : 0] : 0.0};
+    };
+    if (LOWER) {
+        for (int r = 1; r < nb; ++r) {
+            double sr = 0.0, si = 0.0;
+            if (t < r) {
+                cx<double> l = ldb(r, t);
+                sr = l.re * xs[t][0] - l.im * xs[t][1];
+                si = l.re * xs[t][1] + l.im * xs[t][0];
+            }
+            sr = wave_sum(sr); si = wave_sum(si);
+            if (t == 0) { xs[r][0] -= sr; xs[r][1] -= si; }
+            __syncthreads();
+        }
+    } else {
+        for (int r = nb - 1; r >= 0; --r) {
+            double sr = 0.0, si = 0.0;
+            if (t > r && t < nb) {
+                cx<double> u = ldb(r, t);
+                sr = u.re * xs[t][0] - u.im * xs[t][1];
+                si = u.re * xs[t][1] + u.im * xs[t][0];
+            }
+            sr = wave_sum(sr); si = wave_sum(si);
+            if (t == 0) {
+                cx<double> num = {xs[r][0] - sr, xs[r][1] - si};
+                cx<double> q = cdiv(num, ldb(r, r));
+                xs[r][0] = q.re; xs[r][1] = q.im;
+            }
+            __syncthreads();
+        }
+    }
+    if (t < nb) { x[2 * (r0 + t)] = xs[t][0]; x[2 * (r0 + t) + 1] = xs[t][1]; }
+}
+
+// x[rows] -= A[rows, c0:c0+TB] x[c0:c0+TB]: one wave per row.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void trsv_update_kernel(const T* A, int64_t ld, int row_begin,
+                                                          int row_end, int c0, int nb, double* x) {
+    const int lane = threadIdx.x & 63;
+    const int r = row_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= row_end) return;
+    double sr = 0.0, si = 0.0;
+    if (lane < nb) {
+        cx<T> a = Acc<T, C>::ld(A, (int64_t)r * ld + c0 + lane);
+        double xr = x[2 * (c0 + lane)], xi = x[2 * (c0 + lane) + 1];
+        sr = (double)a.re * xr - (double)a.im * xi;
+        si = (double)a.re * xi + (double)a.im * xr;
+    }
+    sr = wave_sum(sr); si = wave_sum(si);
+    if (lane == 0) { x[2 * r] -= sr; x[2 * r + 1] -= si; }
+}
+
+// r = b - A x in FP64 (A of T/C, x/b/r as complex128 work vectors): one wave
+// per row.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void residual_kernel(const T* A, int64_t ld, int n, const double* x,
+                                                       const double* b, double* r) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    double sr = 0.0, si = 0.0;
+    for (int c = lane; c < n; c += 64) {
+        cx<T> a = Acc<T, C>::ld(A, (int64_t)row * ld + c);
+        double xr = x[2 * c], xi = x[2 * c + 1];
+        sr += (double)a.re * xr - (double)a.im * xi;
+        si += (double)a.re * xi + (double)a.im * xr;
+    }
+    sr = wave_sum(sr); si = wave_sum(si);
+    if (lane == 0) { r[2 * row] = b[2 * row] - sr; r[2 * row + 1] = b[2 * row + 1] - si; }
+}
+
+// gather y[i] = x[perm[i]] (complex128 work vectors)
+__global__ void permute_kernel(const double* x, const int32_t* perm, int n, double* y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[2 * i] = x[2 * perm[i]];
+    y[2 * i + 1] = x[2 * perm[i] + 1];
+}
+
+// complex128 work vector <-> caller vector of dtype
+__global__ void vec_to_work(const void* v, int dtype, int n, double* w) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double re, im;
+    switch (dtype) {
+        case BEM_F64: re = ((const double*)v)[i]; im = 0.0; break;
+        case BEM_C128: re = ((const double*)v)[2 * i]; im = ((const double*)v)[2 * i + 1]; break;
+        case BEM_F32: re = ((const float*)v)[i]; im = 0.0; break;
+        default: re = ((const float*)v)[2 * i]; im = ((const float*)v)[2 * i + 1]; break;
+    }
+    w[2 * i] = re; w[2 * i + 1] = im;
+}
+__global__ void work_to_vec(const double* w, int n, void* v, int dtype, int accumulate) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double re = w[2 * i], im = w[2 * i + 1];
+    switch (dtype) {
+        case BEM_F64: { double* o = (double*)v + i; *o = accumulate ? *o + re : re; } break;
+        case BEM_C128: {
+            double* o = (double*)v + 2 * i;
+            if (accumulate) { o[0] += re; o[1] += im; } else { o[0] = re; o[1] = im; }
+        } break;
+        case BEM_F32: { float* o = (float*)v + i; *o = (float)(accumulate ? *o + re : re); } break;
+        default: {
+            float* o = (float*)v + 2 * i;
+            if (accumulate) { o[0] += (float)re; o[1] += (float)im; }
+            else { o[0] = (float)re; o[1] = (float)im; }
+        } break;
+    }
+}
+
+// block sums of |w_i|^2 (complex128 work vector)
+__global__ void norm2_kernel(const double* w, int n, double* out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += w[2 * i] * w[2 * i] + w[2 * i + 1] * w[2 * i + 1];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+}
+
+template <typename TS, bool CS, typename TD, bool CD>
+__global__ void convert_kernel(const TS* src, int64_t lds, TD* dst, int64_t ldd, int rows, int cols) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    int r = (int)(e / cols), c = (int)(e % cols);
+    cx<TS> v = Acc<TS, CS>::ld(src, (int64_t)r * lds + c);
+    Acc<TD, CD>::st(dst, (int64_t)r * ldd + c, cx<TD>{(TD)v.re, (TD)v.im});
+}
+
+// dst += alpha * op(src), op = identity or transpose; transposed reads go
+// through a 32 x 32 LDS tile so both sides stay coalesced.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void axpy_kernel(const T* src, int64_t lds, T* dst, int64_t ldd, int rows,
+                                                   int cols, double ar, double ai, int transpose) {
+    __shared__ cx<T> tile[32][33];
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    if (transpose) {
+        // dst[r][c] += alpha src[c][r]; load src rows c0.., cols r0..
+        for (int yy = ty; yy < 32; yy += 8) {
+            int sr = c0 + yy, sc = r0 + tx;
+            if (sr < cols && sc < rows) tile[yy][tx] = Acc<T, C>::ld(src, (int64_t)sr * lds + sc);
+        }
+        __syncthreads();
+        for (int yy = ty; yy < 32; yy += 8) {
+            int r = r0 + yy, c = c0 + tx;
+            if (r < rows && c < cols) {
+                cx<T> v = tile[tx][yy];
+                cx<T> d = Acc<T, C>::ld(dst, (int64_t)r * ldd + c);
+                d.re += (T)(ar * v.re - ai * v.im);
+                d.im += (T)(ar * v.im + ai * v.re);
+                Acc<T, C>::st(dst, (int64_t)r * ldd + c, d);
+            }
+        }
+    } else {
+        for (int yy = ty; yy < 32; yy += 8) {
+            int r = r0 + yy, c = c0 + tx;
+            if (r < rows && c < cols) {
+                cx<T> v = Acc<T, C>::ld(src, (int64_t)r * lds + c);
+                cx<T> d = Acc<T, C>::ld(dst, (int64_t)r * ldd + c);
+                d.re += (T)(ar * v.re - ai * v.im);
+                d.im += (T)(ar * v.im + ai * v.re);
+                Acc<T, C>::st(dst, (int64_t)r * ldd + c, d);
+            }
+        }
+    }
+}
+
+template <typename T, bool C>
+int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_t s) {
+    int* info = ctx->counters;
+    hipMemsetAsync(info, 0, sizeof(int), s);
+    for (int k = 0; k < n; k += NB) {
+        const int kend = min(n, k + NB);
+        for (int c = k; c < kend; ++c) {
+            hipLaunchKernelGGL((pivot_kernel<T, C>), dim3(1), dim3(1024), 0, s, A, ld, n, c, k, kend, ipiv, info);
+            if (c + 1 < n)
+                hipLaunchKernelGGL((panel_update_kernel<T, C>), dim3((n - c - 1 + 3) / 4), dim3(64, 4), 0, s,
+                                   A, ld, n, c, kend);
+        }
+        hipLaunchKernelGGL((laswp_kernel<T, C>), dim3((n + 255) / 256), dim3(256), 0, s, A, ld, n, k, kend, ipiv);
+        if (kend < n) {
+            hipLaunchKernelGGL((trsm_kernel<T, C>), dim3((n - kend + 255) / 256), dim3(256), 0, s, A, ld, n, k, kend);
+            const int M = n - kend, N = n - kend, K = kend - k;
+            dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+            hipLaunchKernelGGL((gemm_sub_kernel<T, C>), grid, dim3(256), 0, s,
+                               A + (C ? 2 : 1) * ((int64_t)kend * ld + kend), ld,
+                               A + (C ? 2 : 1) * ((int64_t)kend * ld + k), ld,
+                               A + (C ? 2 : 1) * ((int64_t)k * ld + kend), ld, M, N, K);
+        }
+        int rc = check_hip(hipGetLastError(), "LU step");
+        if (rc) return rc;
+    }
+    int h_info = 0;
+    int rc = check_hip(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, s), "LU info");
+    if (rc) return rc;
+    rc = check_hip(hipStreamSynchronize(s), "LU sync");
+    if (rc) return rc;
+    if (h_info) return set_error(BEM_ERR_SINGULAR, "exactly singular matrix: zero pivot in column " + std::to_string(h_info - 1));
+    return BEM_OK;
+}
+
+// Solve on the complex128 work vector w (already permuted).
+template <typename T, bool C>
+void tri_solve_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
+    for (int r0 = 0; r0 < n; r0 += TB) {
+        const int nb = min(TB, n - r0);
+        hipLaunchKernelGGL((trsv_diag_kernel<T, C, true>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
+        if (r0 + nb < n)
+            hipLaunchKernelGGL((trsv_update_kernel<T, C>), dim3((n - r0 - nb + 3) / 4), dim3(256), 0, s, LU, ld,
+                               r0 + nb, n, r0, nb, w);
+    }
+    const int last = ((n - 1) / TB) * TB;
+    for (int r0 = last; r0 >= 0; r0 -= TB) {
+        const int nb = min(TB, n - r0);
+        hipLaunchKernelGGL((trsv_diag_kernel<T, C, false>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
+        if (r0 > 0)
+            hipLaunchKernelGGL((trsv_update_kernel<T, C>), dim3((r0 + 3) / 4), dim3(256), 0, s, LU, ld, 0, r0,
+                               r0, nb, w);
+    }
+}
+
+void tri_solve(const void* LU, int64_t ld, int n, int dtype, double* w, hipStream_t s) {
+    switch (dtype) {
+        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, s); break;
+        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w, s); break;
+        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w, s); break;
+        default: tri_solve_t<float, true>((const float*)LU, ld, n, w, s); break;
+    }
+}
+
+int ensure_work(Context* ctx, size_t bytes) {
+    if (ctx->work_bytes >= bytes) return BEM_OK;
+    if (ctx->work) hipFree(ctx->work);
+    ctx->work = nullptr;
+    ctx->work_bytes = 0;
+    int rc = check_hip(hipMalloc(&ctx->work, bytes), "solver workspace");
+    if (rc) return rc;
+    ctx->work_bytes = bytes;
+    return BEM_OK;
+}
+
+// LAPACK-style interchanges -> gather permutation (host; ipiv is n ints).
+int make_perm(const int32_t* d_ipiv, int n, int32_t* d_perm, hipStream_t s) {
+    std::vector<int32_t> ip(n), perm(n);
+    int rc = check_hip(hipMemcpyAsync(ip.data(), d_ipiv, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s), "ipiv");
+    if (rc) return rc;
+    rc = check_hip(hipStreamSynchronize(s), "ipiv sync");
+    if (rc) return rc;
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int c = 0; c < n; ++c) {
+        int p = ip[c];
+        if (p < 0 || p >= n) return set_error(BEM_ERR_INVALID, "invalid pivot index");
+        int t = perm[c]; perm[c] = perm[p]; perm[p] = t;
+    }
+    rc = check_hip(hipMemcpyAsync(d_perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s), "perm");
+    if (rc) return rc;
+    return check_hip(hipStreamSynchronize(s), "perm sync");
+}
+
+bool valid_dtype(int d) { return d >= BEM_F64 && d <= BEM_C64; }
+
+void residual_any(const void* A, int64_t ld, int n, int dtype, const double* x, const double* b, double* r,
+                  hipStream_t s) {
+    dim3 g((n + 3) / 4);
+    switch (dtype) {
+        case BEM_F64: hipLaunchKernelGGL((residual_kernel<double, false>), g, dim3(256), 0, s, (const double*)A, ld, n, x, b, r); break;
+        case BEM_C128: hipLaunchKernelGGL((residual_kernel<double, true>), g, dim3(256), 0, s, (const double*)A, ld, n, x, b, r); break;
+        case BEM_F32: hipLaunchKernelGGL((residual_kernel<float, false>), g, dim3(256), 0, s, (const float*)A, ld, n, x, b, r); break;
+        default: hipLaunchKernelGGL((residual_kernel<float, true>), g, dim3(256), 0, s, (const float*)A, ld, n, x, b, r); break;
+    }
+}
+
+}  // namespace
+
+int lu_factor(Context* ctx, void* A, int64_t ld, int n, int dtype, int32_t* ipiv, hipStream_t s) {
+    if (!A || !ipiv) return set_error(BEM_ERR_INVALID, "null matrix or pivot array");
+    if (n <= 0 || ld < n) return set_error(BEM_ERR_INVALID, "invalid matrix size");
+    if (!valid_dtype(dtype)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    switch (dtype) {
+        case BEM_F64: return lu_factor_t<double, false>(ctx, (double*)A, ld, n, ipiv, s);
+        case BEM_C128: return lu_factor_t<double, true>(ctx, (double*)A, ld, n, ipiv, s);
+        case BEM_F32: return lu_factor_t<float, false>(ctx, (float*)A, ld, n, ipiv, s);
+        default: return lu_factor_t<float, true>(ctx, (float*)A, ld, n, ipiv, s);
+    }
+}
+
+int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv, void* b,
+             hipStream_t s) {
+    if (!LU || !ipiv || !b) return set_error(BEM_ERR_INVALID, "null argument");
+    if (n <= 0 || ld < n) return set_error(BEM_ERR_INVALID, "invalid matrix size");
+    if (!valid_dtype(dtype)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    int rc = ensure_work(ctx, (size_t)n * 16 * 2 + (size_t)n * 4);
+    if (rc) return rc;
+    double* w0 = (double*)ctx->work;
+    double* w1 = w0 + 2 * (size_t)n;
+    int32_t* perm = (int32_t*)(w1 + 2 * (size_t)n);
+    rc = make_perm(ipiv, n, perm, s);
+    if (rc) return rc;
+    dim3 g((n + 255) / 256);
+    hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, (const void*)b, dtype, n, w0);
+    hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)w0, (const int32_t*)perm, n, w1);
+    tri_solve(LU, ld, n, dtype, w1, s);
+    hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
+    return check_hip(hipGetLastError(), "LU solve");
+}
+
+int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, const void* x,
+                  const void* b, void* r, hipStream_t s) {
+    if (!A || !x || !b || !r) return set_error(BEM_ERR_INVALID, "null argument");
+    if (dtype_a != BEM_F64 && dtype_a != BEM_C128)
+        return set_error(BEM_ERR_INVALID, "residual needs an FP64 matrix");
+    int rc = ensure_work(ctx, (size_t)n * 16 * 3 + (size_t)n * 4);
+    if (rc) return rc;
+    double* wx = (double*)ctx->work;
+    double* wb = wx + 2 * (size_t)n;
+    double* wr = wb + 2 * (size_t)n;
+    dim3 g((n + 255) / 256);
+    hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, x, dtype_a, n, wx);
+    hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, b, dtype_a, n, wb);
+    residual_any(A, ld, n, dtype_a, wx, wb, wr, s);
+    hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)wr, n, r, dtype_a, 0);
+    return check_hip(hipGetLastError(), "residual");
+}
+
+int refine_solve(Context* ctx, const void* A64, int64_t lda, const void* LU, int64_t ldlu, int n,
+                 int dtype_lu, const int32_t* ipiv, const void* b, void* x, int max_iter, double tol,
+                 bem_refine_stats* st, hipStream_t s) {
+    if (!A64 || !LU || !ipiv || !b || !x) return set_error(BEM_ERR_INVALID, "null argument");
+    if (dtype_lu != BEM_F32 && dtype_lu != BEM_C64 && dtype_lu != BEM_F64 && dtype_lu != BEM_C128)
+        return set_error(BEM_ERR_INVALID, "unknown LU dtype");
+    const int dtype64 = dtype_complex(dtype_lu) ? BEM_C128 : BEM_F64;
+    // work: x, b, r, d (complex128 each), perm, norm accumulator
+    int rc = ensure_work(ctx, (size_t)n * 16 * 4 + (size_t)n * 4 + 64);
+    if (rc) return rc;
+    double* wx = (double*)ctx->work;
+    double* wb = wx + 2 * (size_t)n;
+    double* wr = wb + 2 * (size_t)n;
+    double* wd = wr + 2 * (size_t)n;
+    int32_t* perm = (int32_t*)(wd + 2 * (size_t)n);
+    double* nrm = (double*)(((uintptr_t)(perm + n) + 63) & ~(uintptr_t)63);
+    rc = make_perm(ipiv, n, perm, s);
+    if (rc) return rc;
+    dim3 g((n + 255) / 256);
+    auto norm = [&](const double* w, double* out) -> int {
+        hipMemsetAsync(nrm, 0, sizeof(double), s);
+        hipLaunchKernelGGL(norm2_kernel, dim3(min(1024, (n + 255) / 256)), dim3(256), 0, s, w, n, nrm);
+        int r2 = check_hip(hipMemcpyAsync(out, nrm, sizeof(double), hipMemcpyDeviceToHost, s), "norm");
+        if (r2) return r2;
+        r2 = check_hip(hipStreamSynchronize(s), "norm sync");
+        *out = sqrt(*out);
+        return r2;
+    };
+    hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, b, dtype64, n, wb);
+    double bn = 0.0;
+    if ((rc = norm(wb, &bn))) return rc;
+    // x0 = LU^{-1} b
+    hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)wb, (const int32_t*)perm, n, wx);
+    tri_solve(LU, ldlu, n, dtype_lu, wx, s);
+    double r0n = -1.0, rn = 0.0;
+    int it = 0, conv = 0;
+    for (;;) {
+        residual_any(A64, lda, n, dtype64, wx, wb, wr, s);
+        if ((rc = norm(wr, &rn))) return rc;
+        if (r0n < 0.0) r0n = rn;
+        if (rn <= tol * bn) { conv = 1; break; }
+        if (it >= max_iter) break;
+        hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)wr, (const int32_t*)perm, n, wd);
+        tri_solve(LU, ldlu, n, dtype_lu, wd, s);
+        hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)wd, n, (void*)wx, BEM_C128, 1);
+        ++it;
+    }
+    hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)wx, n, x, dtype64, 0);
+    rc = check_hip(hipGetLastError(), "refinement");
+    if (rc) return rc;
+    rc = check_hip(hipStreamSynchronize(s), "refinement sync");
+    if (st) {
+        st->iterations = it; st->converged = conv; st->residual_norm0 = r0n;
+        st->residual_norm = rn; st->b_norm = bn;
+    }
+    return rc;
+}
+
+int matrix_axpy(const void* src, int64_t lds, void* dst, int64_t ldd, int dtype, int rows, int cols,
+                double ar, double ai, int transpose, hipStream_t s) {
+    if (!src || !dst) return set_error(BEM_ERR_INVALID, "null argument");
+    if (!valid_dtype(dtype)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    if (rows <= 0 || cols <= 0) return BEM_OK;
+    dim3 g((cols + 31) / 32, (rows + 31) / 32);
+    switch (dtype) {
+        case BEM_F64: hipLaunchKernelGGL((axpy_kernel<double, false>), g, dim3(256), 0, s, (const double*)src, lds, (double*)dst, ldd, rows, cols, ar, ai, transpose); break;
+        case BEM_C128: hipLaunchKernelGGL((axpy_kernel<double, true>), g, dim3(256), 0, s, (const double*)src, lds, (double*)dst, ldd, rows, cols, ar, ai, transpose); break;
+        case BEM_F32: hipLaunchKernelGGL((axpy_kernel<float, false>), g, dim3(256), 0, s, (const float*)src, lds, (float*)dst, ldd, rows, cols, ar, ai, transpose); break;
+        default: hipLaunchKernelGGL((axpy_kernel<float, true>), g, dim3(256), 0, s, (const float*)src, lds, (float*)dst, ldd, rows, cols, ar, ai, transpose); break;
+    }
+    return check_hip(hipGetLastError(), "matrix axpy");
+}
+
+int convert_matrix(const void* src, int ds, int64_t lds, void* dst, int dd, int64_t ldd, int rows, int cols,
+                   hipStream_t s) {
+    if (!src || !dst) return set_error(BEM_ERR_INVALID, "null argument");
+    if (!valid_dtype(ds) || !valid_dtype(dd)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    if (dtype_complex(ds) != dtype_complex(dd))
+        return set_error(BEM_ERR_INVALID, "conversion between real and complex types");
+    int64_t tot = (int64_t)rows * cols;
+    if (tot <= 0) return BEM_OK;
+    dim3 g((unsigned)((tot + 255) / 256));
+    if (ds == BEM_C128 && dd == BEM_C64)
+        hipLaunchKernelGGL((convert_kernel<double, true, float, true>), g, dim3(256), 0, s, (const double*)src, lds, (float*)dst, ldd, rows, cols);
+    else if (ds == BEM_C64 && dd == BEM_C128)
+        hipLaunchKernelGGL((convert_kernel<float, true, double, true>), g, dim3(256), 0, s, (const float*)src, lds, (double*)dst, ldd, rows, cols);
+    else if (ds == BEM_F64 && dd == BEM_F32)
+        hipLaunchKernelGGL((convert_kernel<double, false, float, false>), g, dim3(256), 0, s, (const double*)src, lds, (float*)dst, ldd, rows, cols);
+    else if (ds == BEM_F32 && dd == BEM_F64)
+        hipLaunchKernelGGL((convert_kernel<float, false, double, false>), g, dim3(256), 0, s, (const float*)src, lds, (double*)dst, ldd, rows, cols);
+    else if (ds == BEM_C128)
+        hipLaunchKernelGGL((convert_kernel<double, true, double, true>), g, dim3(256), 0, s, (const double*)src, lds, (double*)dst, ldd, rows, cols);
+    else if (ds == BEM_C64)
+        hipLaunchKernelGGL((convert_kernel<float, true, float, true>), g, dim3(256), 0, s, (const float*)src, lds, (float*)dst, ldd, rows, cols);
+    else if (ds == BEM_F64)
+        hipLaunchKernelGGL((convert_kernel<double, false, double, false>), g, dim3(256), 0, s, (const double*)src, lds, (double*)dst, ldd, rows, cols);
+    else
+        hipLaunchKernelGGL((convert_kernel<float, false, float, false>), g, dim3(256), 0, s, (const float*)src, lds, (float*)dst, ldd, rows, cols);
+    return check_hip(hipGetLastError(), "convert");
+}
+
+}  // namespace bem

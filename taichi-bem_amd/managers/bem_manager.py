@@ -1,0 +1,339 @@
+"""BEM manager (src/managers/BEM_manager.py): orchestrates operator
+assembly, block placement, right-hand side and the dense solve.  Method names,
+argument order, enum values and the block layout (Dirichlet unknowns first,
+Neumann unknowns at offset M_Dirichlet, BEM_manager.py:104-111) follow the
+reference; the work runs in libbem_hip.so."""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .. import _lib
+from ..bem_functions import (AdjDoubleLayer3d, DoubleLayer3d, HypersingularLayer3d, IdentityLayer3d,
+                             RHSConstructor3d, SingleLayer3d)
+from ..bem_functions.utils import (AssembleType, KernelType, ScopeType, VertAttachType,
+                                   get_gaussion_integration_points_and_weights)
+
+_PRECISION = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32}
+_QUADRATURE = {"reference": 0, "corrected": _lib.FLAG_ROTATE_NEAR | _lib.FLAG_SYM_COINCIDE}
+
+
+class BEMManager:
+    def __init__(self, core_manager):
+        self._core_manager = core_manager
+        p = core_manager._simulation_parameters
+        self._simulation_parameters = p
+        self._is_transmission = core_manager._is_transmission
+        kernel = p["kernel"]
+        if kernel == "Laplace":
+            self._kernel_type, self._n = int(KernelType.LAPLACE), 1
+        elif kernel == "Helmholtz":
+            self._kernel_type, self._n = int(KernelType.HELMHOLTZ), 2
+        elif kernel == "Helmholtz_Transmission":
+            self._kernel_type, self._n = int(KernelType.HELMHOLTZ_TRANSMISSION), 2
+        else:
+            raise RuntimeError("Kernel Type only support Laplace and Helmholtz for now")
+        self._k = 0.0
+        if self._kernel_type != int(KernelType.LAPLACE):
+            self._k = float(p["k"])
+        self._sqrt_ni = 1.0
+        self._sqrt_no = 1.0
+        if self._kernel_type == int(KernelType.HELMHOLTZ_TRANSMISSION):
+            self._sqrt_ni = float(math.sqrt(p["n_i"]))
+            self._sqrt_no = float(math.sqrt(p["n_o"]))
+        self._dim = p.get("dim", 3)
+        self._GaussQR = p["GaussQR"]
+        # the reference raises NotImplementedError for unsupported GaussQR
+        get_gaussion_integration_points_and_weights(self._GaussQR)
+        self._scope_type = int(ScopeType.NOT_DEFINED)
+        if p.get("scope") == "Interior":
+            self._scope_type = int(ScopeType.INTERIOR)
+        elif p.get("scope") == "Exterior":
+            self._scope_type = int(ScopeType.EXTERIOR)
+        prec = p.get("precision", "fp64")
+        if prec not in _PRECISION:
+            raise RuntimeError(f"precision must be one of {sorted(_PRECISION)}")
+        self._precision = _PRECISION[prec]
+        self._rhs_precision = self._precision
+        quad = p.get("quadrature", "reference")
+        if quad not in _QUADRATURE:
+            raise RuntimeError(f"quadrature must be one of {sorted(_QUADRATURE)}")
+        self._quadrature_flags = _QUADRATURE[quad]
+        if self._n == 1:
+            self._dtype = torch.float64 if prec == "fp64" else torch.float32
+        else:
+            self._dtype = torch.complex128 if prec == "fp64" else torch.complex64
+        self.num_of_vertices = 0
+        self.num_of_panels = 0
+        self._L1_inv_H = None
+        self._L2_inv_H = None
+        self.initialized = False
+
+    @property
+    def ctx(self) -> _lib.Context:
+        return self._core_manager.ctx
+
+    @property
+    def device(self):
+        return self._core_manager.device
+
+    def new_matrix(self, rows: int, cols: int) -> torch.Tensor:
+        return torch.zeros(rows, cols, dtype=self._dtype, device=self.device)
+
+    def new_vector(self, n: int) -> torch.Tensor:
+        return torch.zeros(n, dtype=self._dtype, device=self.device)
+
+    def initialization(self, analytical_function_Dirichlet, analytical_function_Neumann):
+        mm = self._core_manager._mesh_manager
+        if not mm.initialized:
+            raise RuntimeError("The initialization of Mesh Manager should always before the BEM Manager")
+        self.num_of_vertices = mm.get_num_of_vertices()
+        self.num_of_panels = mm.get_num_of_panels()
+        self.N_Neumann = mm.get_N_Neumann()
+        self.shape_func_degree_Neumann = mm.get_shape_func_degree_Neumann()
+        self.M_Dirichlet = mm.get_M_Dirichlet()
+        self.shape_func_degree_Dirichlet = mm.get_shape_func_degree_Dirichlet()
+        self.num_of_panels_Neumann = mm.get_num_of_panels_Neumann()
+        self.num_of_panels_Dirichlet = mm.get_num_of_panels_Dirichlet()
+        self._Dirichlet_offset_i = 0
+        self._Dirichlet_offset_j = 0
+        self._Neumann_offset_i = self.M_Dirichlet
+        self._Neumann_offset_j = self.M_Dirichlet
+        self.analytical_function_Dirichlet = analytical_function_Dirichlet
+        self.analytical_function_Neumann = analytical_function_Neumann
+        n = self.N_Neumann + self.M_Dirichlet
+        self.raw_solved = self.new_vector(n)
+        self.raw_analytical_solved = self.new_vector(n)
+        self._mat_A = self.new_matrix(n, n)
+        self.rhs = self.new_vector(2 * n if self._is_transmission else n)
+        self.solved = self.new_vector(self.num_of_vertices)
+        self.analytical_solved = self.new_vector(self.num_of_vertices)
+        self.identity_layer = IdentityLayer3d(self)
+        self.single_layer = SingleLayer3d(self)
+        self.double_layer = DoubleLayer3d(self)
+        self.adj_double_layer = AdjDoubleLayer3d(self)
+        self.hypersingular_layer = HypersingularLayer3d(self)
+        self.rhs_constructor = RHSConstructor3d(self)
+        self.initialized = True
+
+    # -- getters (BEM_manager.py:197-403) -----------------------------------------
+    def get_GaussQR(self):
+        return self._GaussQR
+
+    def get_num_of_vertices(self):
+        return self.num_of_vertices
+
+    def get_num_of_panels(self):
+        return self.num_of_panels
+
+    def get_M_Dirichlet(self):
+        return self.M_Dirichlet
+
+    def get_N_Neumann(self):
+        return self.N_Neumann
+
+    def get_num_of_panels_Neumann(self):
+        return self.num_of_panels_Neumann
+
+    def get_num_of_panels_Dirichlet(self):
+        return self.num_of_panels_Dirichlet
+
+    def get_shape_func_degree_Neumann(self):
+        return self.shape_func_degree_Neumann
+
+    def get_shape_func_degree_Dirichlet(self):
+        return self.shape_func_degree_Dirichlet
+
+    def get_mat_A(self):
+        return self._mat_A
+
+    def get_Dirichlet_offset_i(self):
+        return self._Dirichlet_offset_i
+
+    def get_Dirichlet_offset_j(self):
+        return self._Dirichlet_offset_j
+
+    def get_Neumann_offset_i(self):
+        return self._Neumann_offset_i
+
+    def get_Neumann_offset_j(self):
+        return self._Neumann_offset_j
+
+    def get_scope_type(self):
+        return self._scope_type
+
+    def set_scope_type(self, scope_type):
+        self._scope_type = scope_type
+
+    # -- assembly -------------------------------------------------------------------
+    def matrix_layer_init(self):
+        for m in (self.single_layer._Vmat, self.double_layer._Kmat, self.hypersingular_layer._Wmat):
+            if m is not None:
+                m.zero_()
+
+    def matrix_layer_forward(self, k: float, sqrt_n: float = 1.0, scope_type: int = int(ScopeType.NOT_DEFINED)):
+        """BEM_manager.py:829-844; an undefined scope raises instead of exit()."""
+        if scope_type == int(ScopeType.NOT_DEFINED):
+            raise RuntimeError("Did not indicate a correct scope type!")
+        self.set_scope_type(scope_type=scope_type)
+        self.matrix_layer_init()
+        self.single_layer.forward(scope_type, k, sqrt_n)
+        self.double_layer.forward(scope_type, k, sqrt_n)
+        self.adj_double_layer.forward(scope_type, k, sqrt_n)
+        self.hypersingular_layer.forward(scope_type, k, sqrt_n)
+
+    def rhs_layer_forward(self, scope_type, k: float, sqrt_n: float = 1.0):
+        self.rhs_constructor.forward(scope_type=scope_type, k=k, sqrt_n=sqrt_n)
+
+    def _block(self, r0: int, c0: int, rows: int, cols: int):
+        return self._mat_A[r0: r0 + rows, c0: c0 + cols]
+
+    def matA_add_M(self, multiplier: float):
+        """A += -m V at (N,N), +m K at (N,D), -m K^T at (D,N), -m W at (D,D)
+        (BEM_manager.py:608-628)."""
+        N, M = self.N_Neumann, self.M_Dirichlet
+        if N > 0:
+            _lib.matrix_axpy(self.single_layer._Vmat, self._block(self._Neumann_offset_i, self._Neumann_offset_j, N, N),
+                             -multiplier)
+        if N * M > 0:
+            K = self.double_layer._Kmat
+            _lib.matrix_axpy(K, self._block(self._Neumann_offset_i, self._Dirichlet_offset_j, N, M), multiplier)
+            _lib.matrix_axpy(K, self._block(self._Dirichlet_offset_i, self._Neumann_offset_j, M, N), -multiplier,
+                             transpose=True)
+        if M > 0:
+            _lib.matrix_axpy(self.hypersingular_layer._Wmat,
+                             self._block(self._Dirichlet_offset_i, self._Dirichlet_offset_j, M, M), -multiplier)
+
+    def assemble_matA(self, assemble_type: int, multiplier: float):
+        """BEM_manager.py:630-645."""
+        if assemble_type == int(AssembleType.ADD_M):
+            self.matA_add_M(multiplier)
+        elif assemble_type == int(AssembleType.ADD_HALF_IDENTITY):
+            self.identity_layer.matA_add_global_Identity_matrix(0.5 * multiplier)
+        elif assemble_type == int(AssembleType.ADD_P_MINUS):
+            self.identity_layer.matA_add_global_Identity_matrix(0.5 * multiplier)
+            self.matA_add_M(-multiplier)
+        elif assemble_type == int(AssembleType.ADD_P_PLUS):
+            self.identity_layer.matA_add_global_Identity_matrix(0.5 * multiplier)
+            self.matA_add_M(multiplier)
+
+    def assemble_rhs(self):
+        """BEM_manager.py:647-662: transmission rhs = A conj(f) over the first
+        N+M rows, otherwise a copy of the constructor's vector."""
+        n = self.N_Neumann + self.M_Dirichlet
+        self.rhs.zero_()
+        if self._is_transmission:
+            f = self.rhs_constructor.get_f_boundary()
+            A = self._mat_A
+            A64 = A if A.dtype in (torch.complex128, torch.float64) else A.to(
+                torch.complex128 if A.is_complex() else torch.float64)
+            x = f.conj().to(A64.dtype).contiguous()
+            zero = torch.zeros_like(x)
+            r = torch.empty_like(x)
+            self.ctx.residual(A64, x, zero, r)  # r = 0 - A x
+            self.rhs[:n] = (-r).to(self.rhs.dtype)
+        else:
+            self.rhs.copy_(self.rhs_constructor.get_rhs_vec())
+
+    # -- solve ------------------------------------------------------------------------
+    def solve(self):
+        """BEM_manager.py:1032-1073 with the HIP LU.  For transmission the
+        reference solves the stacked [A; P+_i] system by least squares; this
+        build solves the square block system A x = rhs[:n] (documented
+        difference)."""
+        n = self.N_Neumann + self.M_Dirichlet
+        LU = self._mat_A.clone()
+        ipiv = torch.empty(n, dtype=torch.int32, device=self.device)
+        self.ctx.lu_factor(LU, ipiv)
+        b = self.rhs[:n].clone()
+        self.ctx.lu_solve(LU, ipiv, b)
+        self.raw_solved.copy_(b)
+        self._LU, self._ipiv = LU, ipiv
+        return self.raw_solved
+
+    def run(self):
+        """BEM_manager.py:990-1030 (without the GUI colouring)."""
+        self._mat_A.zero_()
+        if self._is_transmission:
+            self.matrix_layer_forward(k=self._k, sqrt_n=self._sqrt_no, scope_type=int(ScopeType.EXTERIOR))
+            self.assemble_matA(assemble_type=int(AssembleType.ADD_P_MINUS), multiplier=1.0)
+            self.rhs_constructor.set_boundaries(scope_type=int(ScopeType.INTERIOR), sqrt_ni=self._sqrt_ni,
+                                                sqrt_no=self._sqrt_no)
+            self.assemble_rhs()
+            self.matrix_layer_forward(k=self._k, sqrt_n=self._sqrt_ni, scope_type=int(ScopeType.INTERIOR))
+            self.assemble_matA(assemble_type=int(AssembleType.ADD_P_PLUS), multiplier=1.0)
+        else:
+            self.matrix_layer_forward(k=self._k, scope_type=self._scope_type)
+            self.assemble_matA(assemble_type=int(AssembleType.ADD_M), multiplier=-1.0)
+            self.rhs_constructor.set_boundaries(scope_type=self._scope_type, sqrt_ni=self._sqrt_ni,
+                                                sqrt_no=self._sqrt_no)
+            self.rhs_layer_forward(k=self._k, scope_type=self._scope_type)
+            self.assemble_rhs()
+        self.solve()
+        self.compute_analytical_raw_solved()
+        self.splat_u_from_raw_solved_to_vertices()
+
+    # -- post-processing (BEM_manager.py:664-801) ---------------------------------------
+    def compute_analytical_raw_solved(self):
+        mm = self._core_manager._mesh_manager
+        rc = self.rhs_constructor
+        self.raw_analytical_solved.zero_()
+        if self._is_transmission:
+            D, N = self._Dirichlet_offset_j, self._Neumann_offset_j
+            self.raw_analytical_solved[D: D + self.M_Dirichlet] = rc._Dirichlet_boundary[: self.M_Dirichlet]
+            self.raw_analytical_solved[N: N + self.N_Neumann] = rc._Neumann_boundary[: self.N_Neumann]
+            return
+        if self.N_Neumann > 0:
+            off = self._Neumann_offset_j
+            if self.shape_func_degree_Neumann == 0:
+                v = self.analytical_function_Neumann(mm.panel_centroids(), mm.get_panel_normals())
+            else:
+                v = self.analytical_function_Neumann(mm.vertices, mm.get_vert_normals())
+            self.raw_analytical_solved[off: off + self.N_Neumann] = rc._as_field(v)
+        if self.M_Dirichlet > 0:
+            off = self._Dirichlet_offset_j
+            if self.shape_func_degree_Dirichlet == 0:
+                v = self.analytical_function_Dirichlet(mm.panel_centroids())
+            else:
+                v = self.analytical_function_Dirichlet(mm.vertices)
+            self.raw_analytical_solved[off: off + self.M_Dirichlet] = rc._as_field(v)
+
+    def splat_u_from_raw_solved_to_vertices(self):
+        """Area-weighted panel -> vertex average for P0, identity for P1."""
+        mm = self._core_manager._mesh_manager
+        self.solved.zero_()
+        self.analytical_solved.zero_()
+        tris = mm.panels.long()
+        show_dirichlet = self._is_transmission == 0 and self.N_Neumann == 0
+        if not show_dirichlet and self.N_Neumann > 0:
+            off, q = self._Neumann_offset_j, self.shape_func_degree_Neumann
+        else:
+            off, q = self._Dirichlet_offset_j, self.shape_func_degree_Dirichlet
+        n = self.N_Neumann if not show_dirichlet and self.N_Neumann > 0 else self.M_Dirichlet
+        sol = self.raw_solved[off: off + n]
+        ana = self.raw_analytical_solved[off: off + n]
+        if q == 0:
+            w = (mm.get_panel_areas() / 3.0).to(self.solved.real.dtype)
+            va = mm.get_vert_areas().to(self.solved.real.dtype)
+            for s in range(3):
+                self.solved.index_add_(0, tris[:, s], sol * w / va[tris[:, s]])
+                self.analytical_solved.index_add_(0, tris[:, s], ana * w / va[tris[:, s]])
+        else:
+            self.solved.copy_(sol)
+            self.analytical_solved.copy_(ana)
+
+    def kill(self):
+        self.initialized = False
+        self.num_of_vertices = 0
+        self.num_of_panels = 0
+        self.solved = None
+        self.analytical_solved = None
+        self.raw_solved = None
+        self.raw_analytical_solved = None
+        self._mat_A = None
+        self.rhs = None
+        for layer in ("single_layer", "double_layer", "adj_double_layer", "hypersingular_layer",
+                      "identity_layer", "rhs_constructor"):
+            if hasattr(self, layer):
+                getattr(self, layer).kill()

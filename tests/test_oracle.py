@@ -1,0 +1,298 @@
+"""Pins the CPU FP64 oracle against what the mathematics fixes, independent
+of the oracle's own formulas: Gauss-Legendre nodes, icosphere relation
+counts, exact mass integrals, the Gauss solid-angle identity, the curl of a
+constant, sphere eigenvalues of V/K/W, and an independent closed-form
+triangle potential for the singular (Sauter-Schwab) rules."""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+# ---------------------------------------------------------------- Gauss rule
+@pytest.mark.parametrize("q", [1, 2, 3, 4, 5, 6, 7, 8, 12])
+def test_gauss_matches_legendre_roots(oracle, q):
+    p, w = oracle.gauss01(q)
+    xp, xw = np.polynomial.legendre.leggauss(q)
+    assert np.abs(p - (xp + 1) / 2).max() < 1e-15
+    assert np.abs(w - xw / 2).max() < 1e-15
+    # exact for monomials of degree 2q-1 on [0, 1]
+    for d in range(2 * q):
+        assert abs(np.dot(w, p ** d) - 1.0 / (d + 1)) < 1e-14
+
+
+@pytest.mark.parametrize("q", [0, 9, 10, 11, 13])
+def test_gauss_unsupported(oracle, q):
+    with pytest.raises(NotImplementedError):
+        oracle.gauss01(q)
+
+
+# ---------------------------------------------------------------- relations
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_relation_counts_icosphere(bem, oracle, level):
+    v, f = bem.meshes.icosphere(level)
+    c = oracle.relation_counts(f)
+    nf = len(f)
+    deg = np.bincount(f.ravel(), minlength=len(v))
+    assert c[1] == nf                      # coincident
+    assert c[3] == 3 * nf                  # each panel has 3 edge neighbours
+    assert c[2] == int(np.sum(deg * (deg - 3)))  # vertex-only neighbours
+    assert c.sum() == nf * nf
+
+
+def test_relation_two_triangle_fixtures(oracle):
+    t = np.array([[0, 1, 2], [0, 1, 2], [1, 0, 3], [0, 1, 3], [2, 4, 5], [6, 7, 8], [4, 2, 0]], dtype=np.int32)
+    assert oracle.relation(t, 0, 0) == oracle.COINCIDE
+    assert oracle.relation(t, 0, 2) == oracle.COMMON_EDGE   # reversed shared edge
+    assert oracle.relation(t, 0, 3) == oracle.COMMON_EDGE   # same-direction shared edge
+    assert oracle.relation(t, 0, 4) == oracle.COMMON_VERTEX
+    assert oracle.relation(t, 0, 5) == oracle.SEPARATE
+    assert oracle.relation(t, 0, 6) == oracle.COMMON_EDGE   # edge (2,0) vs (0,4)->(4,2)->(2,0)? shares 0 and 2
+    assert oracle.relation(t, 0, 1) == oracle.COMMON_EDGE   # identical vertex lists, different panels
+
+
+# ---------------------------------------------------------------- geometry / mass
+def test_geometry_closed_forms(bem, oracle):
+    v = np.array([[0, 0, 0], [2, 0, 0], [0, 3, 0], [0, 0, 1]], dtype=float)
+    f = np.array([[0, 1, 2], [0, 3, 1]], dtype=np.int32)
+    area, normal, va, vn = oracle.geometry(v, f)
+    assert np.allclose(area, [3.0, 1.0])
+    assert np.allclose(normal[0], [0, 0, 1]) and np.allclose(normal[1], [0, 1, 0])
+    assert np.allclose(va, [4 / 3, 4 / 3, 1.0, 1 / 3])
+
+
+def test_mass_exact(bem, oracle):
+    v, f = bem.meshes.perturbed_icosphere(1, seed=3)
+    area = oracle.geometry(v, f)[0]
+    M01 = oracle.mass(5, 0, 1, v, f)
+    for i in range(len(f)):
+        assert np.allclose(M01[i, f[i]], area[i] / 3, rtol=0, atol=1e-15)
+    M11 = oracle.mass(5, 1, 1, v, f)
+    ref = np.zeros_like(M11)
+    for i, t in enumerate(f):
+        for a in range(3):
+            for b in range(3):
+                ref[t[a], t[b]] += area[i] * (2.0 if a == b else 1.0) / 12.0
+    assert rel(M11, ref) < 1e-14
+    M00 = oracle.mass(5, 0, 0, v, f)
+    assert rel(np.diag(M00), area) < 1e-15
+
+
+# ---------------------------------------------------------------- identities
+@pytest.fixture(scope="module")
+def l2(bem, oracle):
+    v, f = bem.meshes.icosphere(2)
+    return v, f, oracle.geometry(v, f)
+
+
+def test_double_layer_gauss_identity(oracle, l2):
+    """sum_v K[i, v] = int_{T_i} int_S dG/dn_y = -A_i/2 (solid angle)."""
+    v, f, geom = l2
+    K = oracle.assemble(oracle.make_op(oracle.OP_K, oracle.LAPLACE, q=5, deg_trial=1), v, f, geom=geom).real
+    area = geom[0]
+    assert np.abs(K.sum(1) / (-0.5 * area) - 1).max() < 0.02
+    # with the singularity-adapted rules the identity holds much better
+    Kc = oracle.assemble(oracle.make_op(oracle.OP_K, oracle.LAPLACE, q=5, deg_trial=1, flags=3), v, f,
+                         geom=geom).real
+    assert np.abs(Kc.sum(1) / (-0.5 * area) - 1).max() < 2e-3
+
+
+def test_adjoint_double_layer_is_transpose(oracle, l2):
+    """K' (P1 test x P0 trial) equals K^T for regular pairs (same integrand
+    with x and y exchanged); coincident pairs are zero in both."""
+    v, f, geom = l2
+    for kern, k in [(oracle.LAPLACE, 0.0), (oracle.HELMHOLTZ, 2.0)]:
+        K = oracle.assemble(oracle.make_op(oracle.OP_K, kern, k, q=4, deg_trial=1, flags=3), v, f, geom=geom)
+        Ka = oracle.assemble(oracle.make_op(oracle.OP_KADJ, kern, k, q=4, deg_test=1, flags=3), v, f, geom=geom)
+        assert rel(Ka, K.T) < 5e-3
+
+
+def test_hypersingular_constant_nullspace(oracle, l2):
+    v, f, geom = l2
+    W = oracle.assemble(oracle.make_op(oracle.OP_W, oracle.LAPLACE, q=4, deg_test=1, deg_trial=1), v, f,
+                        geom=geom).real
+    assert np.abs(W.sum(1)).max() < 1e-13 * np.abs(W).max()
+    assert np.abs(W - W.T).max() < 0.05 * np.abs(W).max()
+
+
+def _ylm_vertex(v, l, m):
+    theta = np.arccos(np.clip(v[:, 2], -1, 1))
+    phi = np.arctan2(v[:, 1], v[:, 0])
+    from importlib import import_module
+    return import_module("taichi-bem_amd").meshes.real_sph_harm(l, m, theta, phi)
+
+
+def test_sphere_eigenvalues_laplace(bem, oracle):
+    """Unit sphere: V Y_l = Y_l/(2l+1), K Y_l = -Y_l/(2(2l+1)),
+    W Y_l = l(l+1)/(2l+1) Y_l (Rayleigh quotients of the Galerkin matrices
+    with P1 interpolants, corrected near-field rules; O(h^2) at L3)."""
+    v, f = bem.meshes.icosphere(3)
+    geom = oracle.geometry(v, f)
+    M = oracle.mass(4, 1, 1, v, f, geom=geom)
+    V = oracle.assemble(oracle.make_op(oracle.OP_V, oracle.LAPLACE, q=4, deg_test=1, deg_trial=1, flags=3),
+                        v, f, geom=geom).real
+    K = oracle.assemble(oracle.make_op(oracle.OP_KADJ, oracle.LAPLACE, q=4, deg_test=1, deg_trial=1, flags=3),
+                        v, f, geom=geom).real
+    W = oracle.assemble(oracle.make_op(oracle.OP_W, oracle.LAPLACE, q=4, deg_test=1, deg_trial=1, flags=3),
+                        v, f, geom=geom).real
+    for l in (1, 2, 3):
+        y = _ylm_vertex(v, l, 1)
+        den = y @ M @ y
+        assert abs((y @ V @ y) / den - 1 / (2 * l + 1)) < 0.03 / (2 * l + 1)
+        assert abs((y @ K.T @ y) / den + 1 / (2 * (2 * l + 1))) < 0.05 / (2 * l + 1)
+        assert abs((y @ W @ y) / den - l * (l + 1) / (2 * l + 1)) < 0.05 * l * (l + 1) / (2 * l + 1)
+
+
+def test_sphere_eigenvalues_helmholtz_single_layer(bem, oracle):
+    """V Y_l = i k j_l(k) h_l(k) Y_l on the unit sphere (k = 2)."""
+    v, f = bem.meshes.icosphere(3)
+    geom = oracle.geometry(v, f)
+    k = 2.0
+    M = oracle.mass(4, 1, 1, v, f, geom=geom)
+    V = oracle.assemble(oracle.make_op(oracle.OP_V, oracle.HELMHOLTZ, k, q=4, deg_test=1, deg_trial=1, flags=3),
+                        v, f, geom=geom)
+    for l in (0, 1, 2):
+        y = _ylm_vertex(v, l, 0)
+        lam = (y @ V @ y) / (y @ M @ y)
+        h = special.spherical_jn(l, k) + 1j * special.spherical_yn(l, k)
+        exact = 1j * k * special.spherical_jn(l, k) * h
+        assert abs(lam - exact) < 0.03 * abs(exact)
+
+
+# --------------------------------------------- singular rules vs closed form
+def _triangle_potential(x, p):
+    """int_T 1/|x - y| dy for a flat triangle p[3,3] and any point x
+    (closed form of the potential of a uniform triangle: edge log terms plus
+    the solid-angle term of the out-of-plane distance)."""
+    n = np.cross(p[1] - p[0], p[2] - p[0])
+    n /= np.linalg.norm(n)
+    w = np.dot(x - p[0], n)
+    xp = x - w * n
+    total = 0.0
+    for i in range(3):
+        a, b = p[i], p[(i + 1) % 3]
+        L = np.linalg.norm(b - a)
+        s = (b - a) / L
+        m = np.cross(s, n)       # outward in-plane edge normal for CCW p
+        t0 = np.dot(a - xp, m)   # signed distance of the edge line
+        sm, sp = np.dot(a - xp, s), np.dot(b - xp, s)
+        Rm, Rp = np.linalg.norm(x - a), np.linalg.norm(x - b)
+        R0 = math.sqrt(t0 * t0 + w * w)
+        if R0 > 1e-14:
+            total += t0 * math.log((Rp + sp) / (Rm + sm)) if (Rm + sm) > 1e-300 else 0.0
+        if abs(w) > 1e-14:
+            total -= abs(w) * (math.atan2(t0 * sp, R0 * R0 + abs(w) * Rp) - math.atan2(t0 * sm, R0 * R0 + abs(w) * Rm))
+    return total
+
+
+def _outer(p, fn, order=40):
+    """int_T fn(x) dx with a collapsed Gauss rule of `order`^2 points."""
+    g, w = np.polynomial.legendre.leggauss(order)
+    g = (g + 1) / 2
+    w = w / 2
+    area2 = np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+    s = 0.0
+    for a in range(order):
+        for b in range(order):
+            r1, r2 = g[a], g[b] * g[a]
+            x = (1 - r1) * p[0] + (r1 - r2) * p[1] + r2 * p[2]
+            s += w[a] * w[b] * r1 * area2 * fn(x)
+    return s
+
+
+def _pair_mesh():
+    v = np.array([[0.0, 0, 0], [1.0, 0, 0], [0.2, 0.9, 0], [0.6, -0.8, 0.3], [-0.9, -0.2, 0.4]])
+    f = np.array([[0, 1, 2], [1, 0, 3], [0, 4, 3]], dtype=np.int32)
+    return v, f
+
+
+@pytest.mark.parametrize("j,rel_expected", [(0, 1), (1, 3), (2, 2)])
+def test_singular_rules_match_closed_form(oracle, j, rel_expected):
+    """Laplace V on coincident / edge / vertex pairs with the adapted rules
+    (rotation + symmetric coincident) at GaussQR 12 against the closed-form
+    inner potential integrated over the test panel."""
+    v, f = _pair_mesh()
+    assert oracle.relation(f, 0, j) == rel_expected
+    op = oracle.make_op(oracle.OP_V, oracle.LAPLACE, q=12, flags=3)
+    val = oracle.pair(op, v, f, 0, j)[0, 0].real
+    py = v[f[j]]
+    ref = _outer(v[f[0]], lambda x: _triangle_potential(x, py)) / (4 * math.pi)
+    assert abs(val - ref) < 2e-6 * abs(ref)
+
+
+def test_reference_rules_equal_adapted_when_order_matches(oracle):
+    """With the shared vertex already at slot 0 of both panels the
+    reference's stored-order vertex rule is the adapted rule."""
+    v, f = _pair_mesh()
+    op0 = oracle.make_op(oracle.OP_V, oracle.HELMHOLTZ, 3.0, q=5, flags=0)
+    op1 = oracle.make_op(oracle.OP_V, oracle.HELMHOLTZ, 3.0, q=5, flags=1)
+    a = oracle.pair(op0, v, f, 0, 2)
+    b = oracle.pair(op1, v, f, 0, 2)
+    assert abs(a - b).max() == 0.0
+    # but not for the edge pair, whose shared edge runs 0->1 vs 1->0
+    c = oracle.pair(op0, v, f, 0, 1)
+    d = oracle.pair(op1, v, f, 0, 1)
+    assert abs(c - d).max() > 1e-6 * abs(d).max()
+
+
+def test_helmholtz_smooth_part(oracle):
+    """V_k - V_0 has the smooth kernel (e^{ikr}-1)/(4 pi r): a plain high-order
+    product rule reproduces the oracle's adapted coincident value."""
+    v, f = _pair_mesh()
+    k = 2.5
+    vk = oracle.pair(oracle.make_op(oracle.OP_V, oracle.HELMHOLTZ, k, q=12, flags=3), v, f, 0, 0)[0, 0]
+    v0 = oracle.pair(oracle.make_op(oracle.OP_V, oracle.LAPLACE, 0.0, q=12, flags=3), v, f, 0, 0)[0, 0]
+    p = v[f[0]]
+
+    def inner(x):
+        def fn(y):
+            r = np.linalg.norm(x - y)
+            return (np.expm1(1j * k * r) / r) if r > 1e-12 else 1j * k
+        return _outer(p, fn, order=14)
+
+    ref = _outer(p, inner, order=14) / (4 * math.pi)
+    # sin(kr)/r is analytic in r^2, so the product rule is spectrally accurate
+    # for the imaginary part; (cos kr - 1)/r has an |r| kink on the diagonal
+    # and converges only algebraically.
+    assert abs((vk - v0).imag - ref.imag) < 1e-10 * abs(ref.imag)
+    assert abs((vk - v0).real - ref.real) < 1e-3 * abs(ref.real)
+
+
+def test_coincident_quirk_is_exact_for_p0(oracle):
+    """The reference coincident rule (f(x,y) twice) equals the symmetric one
+    for P0 x P0 V, and differs for P1 x P1 pairs of different bases."""
+    v, f = _pair_mesh()
+    a = oracle.pair(oracle.make_op(oracle.OP_V, oracle.LAPLACE, q=6, flags=0), v, f, 0, 0)
+    b = oracle.pair(oracle.make_op(oracle.OP_V, oracle.LAPLACE, q=6, flags=2), v, f, 0, 0)
+    assert abs(a - b).max() < 1e-14 * abs(b).max()
+    a1 = oracle.pair(oracle.make_op(oracle.OP_V, oracle.LAPLACE, q=6, deg_test=1, deg_trial=1), v, f, 0, 0)
+    b1 = oracle.pair(oracle.make_op(oracle.OP_V, oracle.LAPLACE, q=6, deg_test=1, deg_trial=1, flags=2), v, f, 0, 0)
+    assert abs(np.diag(a1) - np.diag(b1)).max() < 1e-13 * abs(b1).max()
+    assert abs(a1 - b1).max() > 1e-3 * abs(b1).max()
+
+
+def test_config1_quadrature_presets(bem, oracle):
+    """Config 1 shape at L1/L2: the reference's stored-order rules do not
+    converge to the analytic trace; the adapted rules do (O(h^2))."""
+    x0 = np.array([2.0, 0.5, 0.3])
+    errs = {}
+    for flags in (0, 3):
+        for L in (1, 2):
+            v, f = bem.meshes.icosphere(L)
+            geom = oracle.geometry(v, f)
+            V = oracle.assemble(oracle.make_op(oracle.OP_V, oracle.LAPLACE, q=5, flags=flags), v, f, geom=geom).real
+            g = 1 / np.linalg.norm(v - x0, axis=1)
+            rhs = oracle.rhs_dirichlet(oracle.LAPLACE, 0.0, 5, v, f, g.astype(complex), True, 1.0, flags=flags,
+                                       geom=geom).real
+            t = np.linalg.solve(V, rhs)
+            c = v[f].mean(1)
+            d = c - x0
+            texact = -np.einsum("ij,ij->i", d, geom[1]) / np.linalg.norm(d, axis=1) ** 3
+            errs[(flags, L)] = rel(t, texact)
+    assert errs[(3, 2)] < errs[(3, 1)] / 2.5
+    assert errs[(0, 2)] > 0.15
