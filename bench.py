@@ -1,0 +1,225 @@
+"""Benchmark of the MI355X BEM hot path on the mixed-precision Helmholtz
+workload: interior Dirichlet problem, k = 3, seeded perturbed icosphere level
+5 (20,480 panels, 10,242 vertices), plane-wave data, GaussQR 5, P0 Neumann
+unknowns and P1 Dirichlet data.
+
+One step = the FP64 time-to-solution pipeline on resident inputs: assemble V
+(c128, far + near field), the right-hand side 1/2 M conj(g) + K conj(g)
+(matrix-free), LU factorisation and solve.  `value` is BIO entries/s of the
+V assembly (F^2 entries / device time of its launches).  The mixed-precision
+pipelines (FP32 regular pairs + FP64 near field in c64, c64 LU, FP64
+refinement) are timed after the steps and reported beside it with their error
+against the FP64 solution.
+
+Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_FP32_TFLOPS = 157.3
+FLOPS_PER_EVAL_V_HELM = 15  # algorithmic flops per quadrature-point pair, V Helmholtz
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--k", type=float, default=3.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design note only)"}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    bem = importlib.import_module("taichi-bem_amd")
+    L = bem._lib
+    dev = torch.device("cuda", 0)
+    k = args.k
+    v, f = bem.meshes.perturbed_icosphere(args.level, seed=0)
+    d = bem.meshes.seeded_direction(2)
+    nv, nf = len(v), len(f)
+    log(f"mesh level {args.level}: nv={nv} nf={nf}")
+    ctx = bem.Context(0)
+    ctx.set_mesh(torch.tensor(v, device=dev), torch.tensor(f, device=dev), 5)
+    counts = ctx.nearfield_counts()
+    g = torch.exp(1j * k * (torch.tensor(v, device=dev) @ torch.tensor(d, device=dev))).to(torch.complex128)
+
+    def desc(op, dt, dr, prec, parts=0):
+        return bem.Context.op_desc(op, 1, k, 1.0, dt, dr, prec, 0, 0, parts)  # kernel 1 = Helmholtz
+
+    A64 = torch.zeros(nf, nf, dtype=torch.complex128, device=dev)
+    LU64 = torch.empty_like(A64)
+    b64 = torch.zeros(nf, dtype=torch.complex128, device=dev)
+    ipiv = torch.empty(nf, dtype=torch.int32, device=dev)
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def fp64_step():
+        e = [ev() for _ in range(7)]
+        e[0].record()
+        A64.zero_()
+        ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_FAR), A64)
+        e[1].record()
+        ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_NEAR), A64)
+        e[2].record()
+        b64.zero_()
+        ctx.mass_apply(0, 1, g, b64, 0.5, conj_x=True)
+        ctx.apply(desc(L.OP_K, 0, 1, L.PREC_FP64), g, b64, 1.0, conj_x=True)
+        e[3].record()
+        LU64.copy_(A64)
+        ctx.lu_factor(LU64, ipiv)
+        e[4].record()
+        x = b64.clone()
+        ctx.lu_solve(LU64, ipiv, x)
+        e[5].record()
+        torch.cuda.synchronize()
+        t = [e[i].elapsed_time(e[i + 1]) for i in range(5)]
+        return x, {"far_ms": t[0], "near_ms": t[1], "rhs_ms": t[2], "lu_ms": t[3], "solve_ms": t[4],
+                   "total_ms": e[0].elapsed_time(e[5])}
+
+    for w in range(args.warmup):
+        log(f"warmup {w}")
+        fp64_step()
+    torch.cuda.synchronize()
+    steps = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        x64, tm = fp64_step()
+        steps.append(tm)
+        log(f"step {s}: " + ", ".join(f"{kk}={vv:.1f}" for kk, vv in tm.items()))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    mean = {kk: float(np.mean([s[kk] for s in steps])) for kk in steps[0]}
+    asm_ms = mean["far_ms"] + mean["near_ms"]
+    entries = nf * nf
+    value = entries / (asm_ms * 1e-3)
+
+    # roofline of the dominant assembly kernel (far field, FP64 Helmholtz V)
+    far_pairs = counts[0]
+    far_flops = far_pairs * 625 * FLOPS_PER_EVAL_V_HELM
+    achieved = far_flops / (mean["far_ms"] * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": "far_assemble_kernel<double,V,Helmholtz,P0,P0>",
+                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "flops_per_launch": far_flops, "launch_ms": round(mean["far_ms"], 3)}
+
+    result = {
+        "metric": "BIO entries/s assembled; time-to-solution at 1e-8 rel err, mixed vs FP64 GPU",
+        "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": mean["total_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic (seeded perturbed icosphere, plane wave)",
+        "config": {"workload": f"Helmholtz interior Dirichlet k={k}, perturbed icosphere L{args.level} "
+                               f"({nf} panels, {nv} vertices), GaussQR 5, P0/P1",
+                   "panels": nf, "vertices": nv, "near_pairs": counts[1:], "far_pairs": far_pairs},
+        "stages_ms_fp64": {kk: round(vv, 3) for kk, vv in mean.items()},
+        "roofline": roofline,
+        "wall_s_steps": round(wall, 3),
+    }
+
+    if not args.no_mixed:
+        tts = {"fp64_ms": round(mean["total_ms"], 3)}
+        x64n = torch.linalg.vector_norm(x64).item()
+        # mixed (fast): FP32 regular pairs + FP64 near field into c64, c64 LU,
+        # refinement against that c64 matrix with FP64 residuals
+        A32 = torch.zeros(nf, nf, dtype=torch.complex64, device=dev)
+        LU32 = torch.empty_like(A32)
+        b = torch.zeros(nf, dtype=torch.complex128, device=dev)
+        x = torch.zeros(nf, dtype=torch.complex128, device=dev)
+        for rep in range(2):
+            e = [ev() for _ in range(6)]
+            e[0].record()
+            A32.zero_()
+            ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP32, L.PART_FAR), A32)
+            e[1].record()
+            ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_NEAR), A32)
+            b.zero_()
+            ctx.mass_apply(0, 1, g, b, 0.5, conj_x=True)
+            ctx.apply(desc(L.OP_K, 0, 1, L.PREC_FP32), g, b, 1.0, conj_x=True)
+            e[2].record()
+            LU32.copy_(A32)
+            ctx.lu_factor(LU32, ipiv)
+            e[3].record()
+            st = ctx.refine_solve(A32, LU32, ipiv, b, x, max_iter=30, tol=1e-13)
+            e[4].record()
+            torch.cuda.synchronize()
+        err = (torch.linalg.vector_norm(x - x64).item()) / x64n
+        tts["mixed_fp32asm"] = {"ms": round(e[0].elapsed_time(e[4]), 3), "far_fp32_ms": round(e[0].elapsed_time(e[1]), 3),
+                                "assembly_ms": round(e[0].elapsed_time(e[2]), 3), "lu_c64_ms": round(e[2].elapsed_time(e[3]), 3),
+                                "refine_ms": round(e[3].elapsed_time(e[4]), 3), "refine_iters": st.iterations,
+                                "rel_err_vs_fp64": err,
+                                "assembly_entries_per_s": entries / (e[0].elapsed_time(e[2]) * 1e-3)}
+        # mixed (accurate): FP64 assembly, c64 LU, FP64 refinement against A64
+        for rep in range(2):
+            e = [ev() for _ in range(5)]
+            e[0].record()
+            A64.zero_()
+            ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64), A64)
+            b64.zero_()
+            ctx.mass_apply(0, 1, g, b64, 0.5, conj_x=True)
+            ctx.apply(desc(L.OP_K, 0, 1, L.PREC_FP64), g, b64, 1.0, conj_x=True)
+            e[1].record()
+            bem.convert(A64, LU32)
+            ctx.lu_factor(LU32, ipiv)
+            e[2].record()
+            st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=1e-14)
+            e[3].record()
+            torch.cuda.synchronize()
+        err2 = (torch.linalg.vector_norm(x - x64).item()) / x64n
+        tts["mixed_fp64asm"] = {"ms": round(e[0].elapsed_time(e[3]), 3), "assembly_ms": round(e[0].elapsed_time(e[1]), 3),
+                                "lu_c64_ms": round(e[1].elapsed_time(e[2]), 3), "refine_ms": round(e[2].elapsed_time(e[3]), 3),
+                                "refine_iters": st.iterations, "converged": st.converged,
+                                "rel_err_vs_fp64": err2}
+        tts["speedup_mixed_fp32asm"] = round(mean["total_ms"] / tts["mixed_fp32asm"]["ms"], 3)
+        tts["speedup_mixed_fp64asm"] = round(mean["total_ms"] / tts["mixed_fp64asm"]["ms"], 3)
+        result["time_to_solution"] = tts
+        del A32, LU32
+        log("mixed: " + json.dumps(tts))
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        op = O.make_op(O.OP_V, O.HELMHOLTZ, k, 1.0, 5, 0, 0, 0)
+        geom = O.geometry(v, f)
+        rows = 8
+        t = time.perf_counter()
+        O.assemble_rows(op, v, f, 0, rows, nthreads=threads, geom=geom)
+        dt = time.perf_counter() - t
+        rows2 = max(rows, min(nf, int(rows * args.cpu_seconds / max(dt, 1e-3))))
+        t = time.perf_counter()
+        O.assemble_rows(op, v, f, 0, rows2, nthreads=threads, geom=geom)
+        dt2 = time.perf_counter() - t
+        result["cpu_baseline"] = {"value": rows2 * nf / dt2, "unit": "entries/s", "cores": threads, "kind": "port",
+                                  "sample": f"V Helmholtz FP64 rows 0..{rows2 - 1} of {nf} (all {nf} columns, "
+                                            f"GaussQR 5), {dt2:.1f} s, C/OpenMP oracle"}
+        result["cpu_baseline"]["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
+        log(f"cpu baseline: {rows2} rows in {dt2:.1f}s")
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -85,8 +85,11 @@ typedef struct {
     int32_t precision;   /* BEM_PREC_* */
     int32_t dtype;       /* element type of the output / vectors: BEM_* */
     int32_t flags;       /* BEM_FLAG_* */
-    int32_t reserved;
+    int32_t parts;       /* BEM_PART_* mask of the launches to run; 0 = all */
 } bem_op_desc;
+
+/* ---- parts of an assembly/apply call (to time them separately) ---- */
+enum { BEM_PART_FAR = 1, BEM_PART_NEAR = 2 };
 
 typedef struct {
     int32_t iterations;      /* refinement steps taken */
@@ -193,14 +196,15 @@ int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_
 int bem_residual(bem_context* ctx, const void* A, int64_t ld, int32_t n, int32_t dtype,
                  const void* x, const void* b, void* r, void* stream);
 
-/* Mixed-precision solve: x (FP64 type) from the FP32 LU of A (dtype_lu
- * BEM_F32/BEM_C64) with FP64 iterative refinement against A64 (BEM_F64 or
- * BEM_C128, matching).  Stops when ||b - A64 x|| <= tol ||b|| or after
- * max_iter steps.  stats (host) may be NULL.  Synchronises the stream. */
-int bem_refine_solve(bem_context* ctx, const void* A64, int64_t lda, const void* LU,
-                     int64_t ldlu, int32_t n, int32_t dtype_lu, const int32_t* ipiv,
-                     const void* b, void* x, int32_t max_iter, double tol,
-                     bem_refine_stats* stats, void* stream);
+/* Mixed-precision solve: x (FP64 type) from the LU of A (dtype_lu, usually
+ * BEM_F32/BEM_C64) with iterative refinement whose residuals b - A x are
+ * accumulated in FP64 against the matrix A (dtype_a: FP64 or FP32 storage,
+ * same real/complex kind as the LU).  Stops when ||b - A x|| <= tol ||b|| or
+ * after max_iter steps.  stats (host) may be NULL.  Synchronises the stream. */
+int bem_refine_solve(bem_context* ctx, const void* A, int64_t lda, int32_t dtype_a,
+                     const void* LU, int64_t ldlu, int32_t n, int32_t dtype_lu,
+                     const int32_t* ipiv, const void* b, void* x, int32_t max_iter,
+                     double tol, bem_refine_stats* stats, void* stream);
 
 /* dst = (dtype_dst) src for a rows x cols block (precision conversion of a
  * matrix or vector; complex <-> complex and real <-> real only). */

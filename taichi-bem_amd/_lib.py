@@ -21,6 +21,7 @@ OP_V, OP_K, OP_KADJ, OP_W = 0, 1, 2, 3
 F64, C128, F32, C64 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE = 1, 2
+PART_FAR, PART_NEAR = 1, 2
 
 # Every symbol include/bem_hip.h declares.
 EXPORTS = (
@@ -37,7 +38,7 @@ class BemOpDesc(ctypes.Structure):
         ("k", ctypes.c_double), ("sqrt_n", ctypes.c_double),
         ("deg_test", ctypes.c_int32), ("deg_trial", ctypes.c_int32),
         ("precision", ctypes.c_int32), ("dtype", ctypes.c_int32),
-        ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("parts", ctypes.c_int32),
     ]
 
 
@@ -89,7 +90,7 @@ def lib() -> ctypes.CDLL:
             "bem_lu_factor": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp]),
             "bem_lu_solve": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
             "bem_residual": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp, vp]),
-            "bem_refine_solve": (ctypes.c_int, [vp, vp, i64, vp, i64, i32, i32, vp, vp, vp, i32, dbl,
+            "bem_refine_solve": (ctypes.c_int, [vp, vp, i64, i32, vp, i64, i32, i32, vp, vp, vp, i32, dbl,
                                                 ctypes.POINTER(BemRefineStats), vp]),
             "bem_convert": (ctypes.c_int, [vp, i32, i64, vp, i32, i64, i32, i32, vp]),
             "bem_matrix_axpy": (ctypes.c_int, [vp, i64, vp, i64, i32, i32, i32, dbl, dbl, i32, vp]),
@@ -195,9 +196,9 @@ class Context:
 
     @staticmethod
     def op_desc(op: int, kernel: int, k: float, sqrt_n: float, deg_test: int, deg_trial: int,
-                precision: int, dtype: int, flags: int = 0) -> BemOpDesc:
+                precision: int, dtype: int, flags: int = 0, parts: int = 0) -> BemOpDesc:
         return BemOpDesc(int(op), int(kernel), float(k), float(sqrt_n), int(deg_test), int(deg_trial),
-                         int(precision), int(dtype), int(flags), 0)
+                         int(precision), int(dtype), int(flags), int(parts))
 
     def assemble(self, desc: BemOpDesc, out, alpha: complex = 1.0) -> None:
         """out (a 2-D device tensor or a view of a block with unit column
@@ -248,9 +249,11 @@ class Context:
                                  x.data_ptr(), b.data_ptr(), r.data_ptr(), current_stream()))
 
     def refine_solve(self, A64, LU, ipiv, b, x, max_iter: int = 30, tol: float = 1e-14) -> BemRefineStats:
+        """x from the LU (any precision) refined with FP64-accumulated
+        residuals against A64 (FP64 or FP32 storage)."""
         _require_cuda(A64, LU, ipiv, b, x)
         st = BemRefineStats()
-        check(lib().bem_refine_solve(self._h, A64.data_ptr(), int(A64.stride(0)), LU.data_ptr(),
+        check(lib().bem_refine_solve(self._h, A64.data_ptr(), int(A64.stride(0)), dtype_code(A64), LU.data_ptr(),
                                      int(LU.stride(0)), int(A64.shape[0]), dtype_code(LU), ipiv.data_ptr(),
                                      b.data_ptr(), x.data_ptr(), int(max_iter), float(tol), ctypes.byref(st),
                                      current_stream()))

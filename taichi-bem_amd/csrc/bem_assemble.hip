@@ -630,6 +630,7 @@ int validate(Context* ctx, const bem_op_desc* d) {
         return set_error(BEM_ERR_INVALID, "Kernel Type only support Laplace and Helmholtz for now");
     if ((d->deg_test | d->deg_trial) & ~1) return set_error(BEM_ERR_INVALID, "basis degree must be 0 or 1");
     if (d->dtype < BEM_F64 || d->dtype > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    if (d->parts & ~(BEM_PART_FAR | BEM_PART_NEAR)) return set_error(BEM_ERR_INVALID, "unknown parts mask");
     if (d->kernel != BEM_LAPLACE && !dtype_complex(d->dtype))
         return set_error(BEM_ERR_INVALID, "Helmholtz operators need a complex dtype");
     if (d->precision != BEM_PREC_FP64 && d->precision != BEM_PREC_FP32)
@@ -650,12 +651,15 @@ int assemble_op(Context* ctx, const bem_op_desc* d, void* out, int64_t ld, doubl
     p.out = out; p.ld = ld; p.dtype = d->dtype; p.ar = ar; p.ai = ai;
     const bool helm = d->kernel != BEM_LAPLACE;
     const int nf = p.nf, njb = (nf + 255) / 256;
-    FarFn far = d->precision == BEM_PREC_FP32 ? far_assemble_any<float>(d->op, helm, d->deg_test, d->deg_trial)
-                                              : far_assemble_any<double>(d->op, helm, d->deg_test, d->deg_trial);
-    hipLaunchKernelGGL(far, dim3((unsigned)((int64_t)nf * njb)), dim3(256), 0, s, p, njb);
-    rc = check_hip(hipGetLastError(), "far-field assembly launch");
-    if (rc) return rc;
-    if (p.n_near > 0) {
+    const int parts = d->parts ? d->parts : (BEM_PART_FAR | BEM_PART_NEAR);
+    if (parts & BEM_PART_FAR) {
+        FarFn far = d->precision == BEM_PREC_FP32 ? far_assemble_any<float>(d->op, helm, d->deg_test, d->deg_trial)
+                                                  : far_assemble_any<double>(d->op, helm, d->deg_test, d->deg_trial);
+        hipLaunchKernelGGL(far, dim3((unsigned)((int64_t)nf * njb)), dim3(256), 0, s, p, njb);
+        rc = check_hip(hipGetLastError(), "far-field assembly launch");
+        if (rc) return rc;
+    }
+    if (p.n_near > 0 && (parts & BEM_PART_NEAR)) {
         NearFn nearf = near_any(d->op, helm, d->deg_test, d->deg_trial, false);
         hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
         rc = check_hip(hipGetLastError(), "near-field assembly launch");
@@ -673,12 +677,15 @@ int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void
     OpParams p = make_params(ctx, d);
     p.x = x; p.conj_x = conj_x; p.dtype_x = d->dtype; p.y = y; p.ar = ar; p.ai = ai;
     const bool helm = d->kernel != BEM_LAPLACE;
-    NearFn far = d->precision == BEM_PREC_FP32 ? far_apply_any<float>(d->op, helm, d->deg_test, d->deg_trial)
-                                               : far_apply_any<double>(d->op, helm, d->deg_test, d->deg_trial);
-    hipLaunchKernelGGL(far, dim3(p.nf), dim3(256), 0, s, p);
-    rc = check_hip(hipGetLastError(), "far-field apply launch");
-    if (rc) return rc;
-    if (p.n_near > 0) {
+    const int parts = d->parts ? d->parts : (BEM_PART_FAR | BEM_PART_NEAR);
+    if (parts & BEM_PART_FAR) {
+        NearFn far = d->precision == BEM_PREC_FP32 ? far_apply_any<float>(d->op, helm, d->deg_test, d->deg_trial)
+                                                   : far_apply_any<double>(d->op, helm, d->deg_test, d->deg_trial);
+        hipLaunchKernelGGL(far, dim3(p.nf), dim3(256), 0, s, p);
+        rc = check_hip(hipGetLastError(), "far-field apply launch");
+        if (rc) return rc;
+    }
+    if (p.n_near > 0 && (parts & BEM_PART_NEAR)) {
         NearFn nearf = near_any(d->op, helm, d->deg_test, d->deg_trial, true);
         hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
         const int64_t nth = d->deg_test ? p.n_near : p.nf;

@@ -165,12 +165,12 @@ int bem_residual(bem_context* ctx, const void* A, int64_t ld, int32_t n, int32_t
     });
 }
 
-int bem_refine_solve(bem_context* ctx, const void* A64, int64_t lda, const void* LU, int64_t ldlu, int32_t n,
-                     int32_t dtype_lu, const int32_t* ipiv, const void* b, void* x, int32_t max_iter, double tol,
-                     bem_refine_stats* stats, void* stream) {
+int bem_refine_solve(bem_context* ctx, const void* A, int64_t lda, int32_t dtype_a, const void* LU, int64_t ldlu,
+                     int32_t n, int32_t dtype_lu, const int32_t* ipiv, const void* b, void* x, int32_t max_iter,
+                     double tol, bem_refine_stats* stats, void* stream) {
     return with_device(ctx, [&]() -> int {
-        return bem::refine_solve(&ctx->impl, A64, lda, LU, ldlu, n, dtype_lu, ipiv, b, x, max_iter, tol, stats,
-                                 as_stream(stream));
+        return bem::refine_solve(&ctx->impl, A, lda, dtype_a, LU, ldlu, n, dtype_lu, ipiv, b, x, max_iter, tol,
+                                 stats, as_stream(stream));
     });
 }
 

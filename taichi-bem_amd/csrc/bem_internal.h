@@ -94,7 +94,7 @@ int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const i
              void* b, hipStream_t s);
 int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, const void* x,
                   const void* b, void* r, hipStream_t s);
-int refine_solve(Context* ctx, const void* A64, int64_t lda, const void* LU, int64_t ldlu,
+int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const void* LU, int64_t ldlu,
                  int n, int dtype_lu, const int32_t* ipiv, const void* b, void* x, int max_iter,
                  double tol, bem_refine_stats* st, hipStream_t s);
 int convert_matrix(const void* src, int dtype_src, int64_t lds, void* dst, int dtype_dst,

@@ -191,8 +191,9 @@ __global__ __launch_bounds__(256) void trsm_kernel(T* A, int64_t ld, int n, int 
 // C[M x N] -= A[M x K] B[K x N] on MFMA.  Block tile 64 x 64, four waves in a
 // 2 x 2 arrangement, each wave 32 x 32 = 2 x 2 MFMA 16x16 blocks.  K is
 // staged through LDS in chunks of 16.  MFMA 16x16x4 operand layout: lane l
-// supplies A[l%16][l/16] and B[l/16][l%16]; result register r of lane l is
-// D[4*(l/16)+r][l%16].
+// supplies A[l%16][l/16] and B[l/16][l%16].  Result register r of lane l
+// holds column l%16 of row 4*(l/16)+r for v_mfma_f32_16x16x4f32 but of row
+// l/16+4*r for v_mfma_f64_16x16x4f64 (both measured on gfx950).
 // ---------------------------------------------------------------------------
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
@@ -200,12 +201,14 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int GT = 64;   // block tile
@@ -281,7 +284,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(T* Cm, int64_t ldc, const
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gr = bm + wm + 16 * a + 4 * (lane >> 4) + r;
+                const int gr = bm + wm + 16 * a + Mfma<T>::row(lane, r);
                 const int gc = bn + wn + 16 * b + (lane & 15);
                 if (gr < M && gc < N) {
                     cx<T> v = Acc<T, C>::ld(Cm, (int64_t)gr * ldc + gc);
@@ -642,15 +645,17 @@ int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, c
     return check_hip(hipGetLastError(), "residual");
 }
 
-int refine_solve(Context* ctx, const void* A64, int64_t lda, const void* LU, int64_t ldlu, int n,
+int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const void* LU, int64_t ldlu, int n,
                  int dtype_lu, const int32_t* ipiv, const void* b, void* x, int max_iter, double tol,
                  bem_refine_stats* st, hipStream_t s) {
-    if (!A64 || !LU || !ipiv || !b || !x) return set_error(BEM_ERR_INVALID, "null argument");
-    if (dtype_lu != BEM_F32 && dtype_lu != BEM_C64 && dtype_lu != BEM_F64 && dtype_lu != BEM_C128)
-        return set_error(BEM_ERR_INVALID, "unknown LU dtype");
+    if (!A || !LU || !ipiv || !b || !x) return set_error(BEM_ERR_INVALID, "null argument");
+    if (!valid_dtype(dtype_lu) || !valid_dtype(dtype_a)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    if (dtype_complex(dtype_lu) != dtype_complex(dtype_a))
+        return set_error(BEM_ERR_INVALID, "matrix and LU must both be real or both complex");
     const int dtype64 = dtype_complex(dtype_lu) ? BEM_C128 : BEM_F64;
-    // work: x, b, r, d (complex128 each), perm, norm accumulator
-    int rc = ensure_work(ctx, (size_t)n * 16 * 4 + (size_t)n * 4 + 64);
+    // work: x, b, r, d (complex128 each), perm, then a 64-byte aligned norm
+    // accumulator (up to 63 bytes of padding + 8 bytes)
+    int rc = ensure_work(ctx, (size_t)n * 16 * 4 + (size_t)n * 4 + 128);
     if (rc) return rc;
     double* wx = (double*)ctx->work;
     double* wb = wx + 2 * (size_t)n;
@@ -679,7 +684,7 @@ int refine_solve(Context* ctx, const void* A64, int64_t lda, const void* LU, int
     double r0n = -1.0, rn = 0.0;
     int it = 0, conv = 0;
     for (;;) {
-        residual_any(A64, lda, n, dtype64, wx, wb, wr, s);
+        residual_any(A, lda, n, dtype_a, wx, wb, wr, s);
         if ((rc = norm(wr, &rn))) return rc;
         if (r0n < 0.0) r0n = rn;
         if (rn <= tol * bn) { conv = 1; break; }
