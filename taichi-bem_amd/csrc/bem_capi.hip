@@ -80,6 +80,7 @@ int bem_destroy(bem_context* ctx) {
         bem::mesh_free(&ctx->impl.mesh);
         if (ctx->impl.work) hipFree(ctx->impl.work);
         if (ctx->impl.counters) hipFree(ctx->impl.counters);
+        if (ctx->impl.lu_scratch) hipFree(ctx->impl.lu_scratch);
         return BEM_OK;
     });
     delete ctx;

@@ -64,8 +64,10 @@ struct Context {
     // LU / solver scratch
     void* work = nullptr;
     size_t work_bytes = 0;
-    int* counters = nullptr;      // grid-barrier / pivot scratch
+    int* counters = nullptr;      // small per-context flags
     size_t counters_len = 0;
+    void* lu_scratch = nullptr;   // cooperative-panel candidates, barrier words, inv(L11)
+    size_t lu_scratch_bytes = 0;
 };
 
 // error helpers (bem_capi.hip)

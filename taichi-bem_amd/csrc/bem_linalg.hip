@@ -13,6 +13,7 @@
 //      complex products as four real products
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "bem_device.h"
@@ -67,75 +68,6 @@ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
     return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
 
-// ---------------------------------------------------------------------------
-// Panel, column c: pivot search over rows [c, n) and swap rows c <-> p inside
-// the panel columns [k, kend).  One block of 1024 threads.
-// ---------------------------------------------------------------------------
-template <typename T, bool C>
-__global__ __launch_bounds__(1024) void pivot_kernel(T* A, int64_t ld, int n, int c, int k, int kend,
-                                                     int32_t* ipiv, int* info) {
-    __shared__ double sv[1024];
-    __shared__ int si[1024];
-    double best = -1.0;
-    int bi = n;
-    for (int r = c + threadIdx.x; r < n; r += 1024) {
-        cx<T> v = Acc<T, C>::ld(A, (int64_t)r * ld + c);
-        double m = fabs((double)v.re) + fabs((double)v.im);
-        if (m > best) { best = m; bi = r; }
-    }
-    sv[threadIdx.x] = best;
-    si[threadIdx.x] = bi;
-    __syncthreads();
-    for (int off = 512; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            double ov = sv[threadIdx.x + off];
-            int oi = si[threadIdx.x + off];
-            if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
-                sv[threadIdx.x] = ov; si[threadIdx.x] = oi;
-            }
-        }
-        __syncthreads();
-    }
-    const int p = si[0];
-    if (threadIdx.x == 0) {
-        ipiv[c] = p;
-        if (sv[0] == 0.0 && *info == 0) *info = c + 1;
-    }
-    if (p != c) {
-        for (int col = k + threadIdx.x; col < kend; col += 1024) {
-            cx<T> a = Acc<T, C>::ld(A, (int64_t)c * ld + col);
-            cx<T> b = Acc<T, C>::ld(A, (int64_t)p * ld + col);
-            Acc<T, C>::st(A, (int64_t)c * ld + col, b);
-            Acc<T, C>::st(A, (int64_t)p * ld + col, a);
-        }
-    }
-}
-
-// Panel, column c: l_r = A[r][c] / A[c][c]; A[r][c+1:kend] -= l_r A[c][c+1:kend]
-// for rows r > c.  blockDim = (64 columns, 4 rows).
-template <typename T, bool C>
-__global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t ld, int n, int c, int kend) {
-    const int r = c + 1 + blockIdx.x * 4 + threadIdx.y;
-    const int col = c + threadIdx.x;  // c .. c+63
-    __shared__ cx<T> piv_row[64];
-    if (threadIdx.y == 0 && col < kend) piv_row[threadIdx.x] = Acc<T, C>::ld(A, (int64_t)c * ld + col);
-    __syncthreads();
-    if (r >= n) return;
-    const cx<T> diag = piv_row[0];
-    const cx<T> arc = Acc<T, C>::ld(A, (int64_t)r * ld + c);
-    cx<T> l = (diag.re == T(0) && diag.im == T(0)) ? arc : cdiv(arc, diag);
-    if (col >= kend) return;
-    if (threadIdx.x == 0) {
-        Acc<T, C>::st(A, (int64_t)r * ld + c, l);
-    } else {
-        cx<T> a = Acc<T, C>::ld(A, (int64_t)r * ld + col);
-        cx<T> u = piv_row[threadIdx.x];
-        cx<T> lu = cmul(l, u);
-        a.re -= lu.re; a.im -= lu.im;
-        Acc<T, C>::st(A, (int64_t)r * ld + col, a);
-    }
-}
-
 // Row interchanges of the panel [k, kend) applied to columns outside it.
 template <typename T, bool C>
 __global__ void laswp_kernel(T* A, int64_t ld, int n, int k, int kend, const int32_t* ipiv) {
@@ -152,48 +84,156 @@ __global__ void laswp_kernel(T* A, int64_t ld, int n, int k, int kend, const int
     }
 }
 
-// U12 = L11^{-1} A12 for columns [kend, n): one thread per column, forward
-// substitution down the nb rows of the panel, L11 staged in LDS.
+// ---------------------------------------------------------------------------
+// Panel column step c (one launch per column of the NB-wide panel [k, kend)):
+//   - rows r > c-1 get the rank-1 update of column c-1 (multiplier
+//     l_r = A[r][c-1] / A[c-1][c-1], the pivot row c-1 being in place);
+//   - the updated column c is searched for the pivot (largest |re|+|im|,
+//     first row on ties: LAPACK's rule) with a per-block partial result;
+//   - the last block to finish (threadfence + counter) reduces the partials,
+//     writes ipiv[c] and swaps rows c and p inside the panel columns.
+// Wave lanes map to panel columns so row segments are read coalesced; no
+// block ever waits for another.  With c == kend only the update runs.
+// ---------------------------------------------------------------------------
+constexpr int RPB = 64;  // rows per block (16 per wave)
+
+// Coherent (agent-scope) load of a value another workgroup published.
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename T, bool C>
-__global__ __launch_bounds__(256) void trsm_kernel(T* A, int64_t ld, int n, int k, int kend) {
-    __shared__ cx<T> L[NB][NB];
-    const int nb = kend - k;
-    for (int e = threadIdx.x; e < nb * nb; e += 256) {
-        int r = e / nb, c = e % nb;
-        L[r][c] = Acc<T, C>::ld(A, (int64_t)(k + r) * ld + (k + c));
+__global__ __launch_bounds__(256) void col_step_kernel(T* A, int64_t ld, int n, int k, int kend, int c,
+                                                       double* pval, int* pidx, unsigned* counter,
+                                                       int32_t* ipiv, int* info) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = k + lane;  // panel column of this lane
+    const bool search = c < kend;
+    const int rbeg = search ? c : kend;  // first row touched (rows >= c; row c-1 is the pivot row)
+    __shared__ double wv[4];
+    __shared__ int wi[4];
+    __shared__ bool s_last;
+    cx<T> u = {T(0), T(0)}, diag = {T(1), T(0)};
+    const bool upd = c > k;
+    if (upd) {
+        if (col >= c - 1 && col < kend) u = Acc<T, C>::ld(A, (int64_t)(c - 1) * ld + col);
+        diag.re = __shfl(u.re, c - 1 - k, 64);
+        diag.im = __shfl(u.im, c - 1 - k, 64);
     }
-    __syncthreads();
-    const int col = kend + blockIdx.x * 256 + threadIdx.x;
-    if (col >= n) return;
-    cx<T> x[NB];
-#pragma unroll
-    for (int r = 0; r < NB; ++r)
-        if (r < nb) x[r] = Acc<T, C>::ld(A, (int64_t)(k + r) * ld + col);
-#pragma unroll
-    for (int r = 1; r < NB; ++r) {
-        if (r < nb) {
-            cx<T> s = x[r];
-#pragma unroll
-            for (int c = 0; c < NB; ++c)
-                if (c < r) {
-                    cx<T> lc = cmul(L[r][c], x[c]);
-                    s.re -= lc.re; s.im -= lc.im;
-                }
-            x[r] = s;
+    const bool dzero = diag.re == T(0) && diag.im == T(0);
+    double best = -1.0;
+    int brow = 0x7fffffff;
+    const int r0 = rbeg + blockIdx.x * RPB + wave * (RPB / 4);
+    for (int q = 0; q < RPB / 4; ++q) {
+        const int r = r0 + q;
+        if (r >= n) break;
+        cx<T> x = {T(0), T(0)};
+        const bool mine = col >= (upd ? c - 1 : c) && col < kend;
+        if (mine) x = Acc<T, C>::ld(A, (int64_t)r * ld + col);
+        if (upd) {
+            cx<T> l = {__shfl(x.re, c - 1 - k, 64), __shfl(x.im, c - 1 - k, 64)};
+            if (!dzero) l = cdiv(l, diag);
+            if (col == c - 1) x = l;
+            else if (col >= c && col < kend) {
+                cx<T> lu = cmul(l, u);
+                x.re -= lu.re; x.im -= lu.im;
+            }
+            if (mine) Acc<T, C>::st(A, (int64_t)r * ld + col, x);
+        }
+        if (search && col == c) {
+            double m = fabs((double)x.re) + fabs((double)x.im);
+            if (m > best) { best = m; brow = r; }  // rows visited in increasing order
         }
     }
-#pragma unroll
-    for (int r = 1; r < NB; ++r)
-        if (r < nb) Acc<T, C>::st(A, (int64_t)(k + r) * ld + col, x[r]);
+    if (!search) return;
+    if (col == c) { wv[wave] = best; wi[wave] = brow; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wv[0];
+        int bi = wi[0];
+        for (int q = 1; q < 4; ++q)
+            if (wv[q] > b || (wv[q] == b && wi[q] < bi)) { b = wv[q]; bi = wi[q]; }
+        pval[blockIdx.x] = b;
+        pidx[blockIdx.x] = bi;
+        __threadfence();
+        const unsigned done = atomicAdd(counter, 1u);
+        s_last = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    __shared__ double rv[256];
+    __shared__ int ri[256];
+    double b = -1.0;
+    int bi = 0x7fffffff;
+    for (int g = threadIdx.x; g < (int)gridDim.x; g += 256) {
+        double v = ld_agent(&pval[g]);
+        int i2 = ld_agent(&pidx[g]);
+        if (v > b || (v == b && i2 < bi)) { b = v; bi = i2; }
+    }
+    rv[threadIdx.x] = b;
+    ri[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            double ov = rv[threadIdx.x + off];
+            int oi = ri[threadIdx.x + off];
+            if (ov > rv[threadIdx.x] || (ov == rv[threadIdx.x] && oi < ri[threadIdx.x])) {
+                rv[threadIdx.x] = ov; ri[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = ri[0];
+    if (threadIdx.x == 0) {
+        ipiv[c] = p;
+        if (rv[0] == 0.0 && *info == 0) *info = c + 1;
+        *counter = 0;
+    }
+    if (p != c && threadIdx.x < kend - k) {
+        const int cc = k + threadIdx.x;
+        cx<T> a = Acc<T, C>::ld(A, (int64_t)c * ld + cc);
+        cx<T> bb = Acc<T, C>::ld(A, (int64_t)p * ld + cc);
+        Acc<T, C>::st(A, (int64_t)c * ld + cc, bb);
+        Acc<T, C>::st(A, (int64_t)p * ld + cc, a);
+    }
+}
+
+// S_L = I - inv(L11) for the unit lower w x w block at (k, k), so that
+// U12 = inv(L11) A12 = A12 - S_L A12 runs on the MFMA GEMM.  One wave; lane t
+// computes column t right-looking (independent updates down the column).
+template <typename T, bool C>
+__global__ __launch_bounds__(64) void linv_kernel(const T* A, int64_t ld, int k, int w, T* SL) {
+    __shared__ cx<T> M[NB][NB + 1];
+    __shared__ cx<T> X[NB][NB + 1];
+    const int t = threadIdx.x;
+    for (int r = 0; r < NB; ++r)
+        M[r][t] = (r < w && t < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + t) : cx<T>{T(r == t ? 1 : 0), T(0)};
+    for (int i = 0; i < NB; ++i) X[i][t] = {T(i == t ? 1 : 0), T(0)};
+    __syncthreads();
+    for (int j = t; j < NB; ++j) {
+        const cx<T> xj = X[j][t];
+        for (int i = j + 1; i < NB; ++i) {
+            cx<T> lx = cmul(M[i][j], xj);
+            X[i][t].re -= lx.re; X[i][t].im -= lx.im;
+        }
+    }
+    for (int i = 0; i < NB; ++i) {
+        cx<T> v = {T(i == t ? 1 : 0) - X[i][t].re, -X[i][t].im};
+        Acc<T, C>::st(SL, (int64_t)i * NB + t, v);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// C[M x N] -= A[M x K] B[K x N] on MFMA.  Block tile 64 x 64, four waves in a
-// 2 x 2 arrangement, each wave 32 x 32 = 2 x 2 MFMA 16x16 blocks.  K is
-// staged through LDS in chunks of 16.  MFMA 16x16x4 operand layout: lane l
-// supplies A[l%16][l/16] and B[l/16][l%16].  Result register r of lane l
-// holds column l%16 of row 4*(l/16)+r for v_mfma_f32_16x16x4f32 but of row
-// l/16+4*r for v_mfma_f64_16x16x4f64 (both measured on gfx950).
+// C[M x N] -= A[M x K] B[K x N] on MFMA.  Block tile 64 x 64, four waves in
+// a 2 x 2 arrangement, each wave 32 x 32 = 2 x 2 MFMA 16x16 blocks, K staged
+// through LDS in chunks of KC.  MFMA 16x16x4 operand layout: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16].  Result register r of lane l holds column
+// l%16 of row 4*(l/16)+r for v_mfma_f32_16x16x4f32 but of row l/16+4*r for
+// v_mfma_f64_16x16x4f64 (both measured on gfx950).  Complex products use four
+// real MFMAs.  C may alias B when M <= 64 and each column tile is owned by
+// one block (all of a block's B reads precede its C writes).
 // ---------------------------------------------------------------------------
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
@@ -212,27 +252,28 @@ template <> struct Mfma<float> {
 };
 
 constexpr int GT = 64;   // block tile
+constexpr int WT = 32;   // wave tile
 constexpr int KC = 16;   // K chunk
 
 template <typename T, bool C>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(T* Cm, int64_t ldc, const T* Am, int64_t lda,
                                                        const T* Bm, int64_t ldb, int M, int N, int K) {
     using V4 = typename Mfma<T>::v4;
+    constexpr int NBLK = WT / 16;
     __shared__ T sAr[GT][KC + 1], sAi[C ? GT : 1][KC + 1];
     __shared__ T sBr[KC][GT + 1], sBi[C ? KC : 1][GT + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int bm = blockIdx.y * GT, bn = blockIdx.x * GT;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    V4 accR[2][2], accI[2][2];
+    const int wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
+    V4 accR[NBLK][NBLK], accI[C ? NBLK : 1][C ? NBLK : 1];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NBLK; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < NBLK; ++b) {
             accR[a][b] = V4{0, 0, 0, 0};
-            accI[a][b] = V4{0, 0, 0, 0};
+            if constexpr (C) accI[a][b] = V4{0, 0, 0, 0};
         }
     for (int k0 = 0; k0 < K; k0 += KC) {
-        // stage A (GT x KC) and B (KC x GT); zero outside the matrix
         for (int e = tid; e < GT * KC; e += 256) {
             int r = e / KC, c = e % KC;
             int gr = bm + r, gc = k0 + c;
@@ -253,21 +294,21 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(T* Cm, int64_t ldc, const
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
             const int ka = kk + (lane >> 4);
-            T ar[2], ai[2], br[2], bi[2];
+            T ar[NBLK], ai[NBLK], br[NBLK], bi[NBLK];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < NBLK; ++a) {
                 ar[a] = sAr[wm + 16 * a + (lane & 15)][ka];
                 if constexpr (C) ai[a] = sAi[wm + 16 * a + (lane & 15)][ka];
             }
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
+            for (int b = 0; b < NBLK; ++b) {
                 br[b] = sBr[ka][wn + 16 * b + (lane & 15)];
                 if constexpr (C) bi[b] = sBi[ka][wn + 16 * b + (lane & 15)];
             }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < NBLK; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < NBLK; ++b) {
                     accR[a][b] = Mfma<T>::op(ar[a], br[b], accR[a][b]);
                     if constexpr (C) {
                         accR[a][b] = Mfma<T>::op(-ai[a], bi[b], accR[a][b]);
@@ -279,9 +320,9 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(T* Cm, int64_t ldc, const
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NBLK; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NBLK; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gr = bm + wm + 16 * a + Mfma<T>::row(lane, r);
@@ -490,31 +531,57 @@ __global__ __launch_bounds__(256) void axpy_kernel(const T* src, int64_t lds, T*
 
 template <typename T, bool C>
 int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_t s) {
-    int* info = ctx->counters;
-    hipMemsetAsync(info, 0, sizeof(int), s);
+    constexpr int E = C ? 2 : 1;
+    const size_t esz = sizeof(cx<T>);
+    const int gmax = (n + RPB - 1) / RPB;
+    // scratch: counter + info, per-block partials, S_L (NB x NB)
+    const size_t need = 512 + (size_t)gmax * (sizeof(double) + sizeof(int)) + NB * NB * esz + 1024;
+    int rc;
+    if (ctx->lu_scratch_bytes < need) {
+        if (ctx->lu_scratch) hipFree(ctx->lu_scratch);
+        ctx->lu_scratch = nullptr;
+        ctx->lu_scratch_bytes = 0;
+        rc = check_hip(hipMalloc(&ctx->lu_scratch, need), "LU scratch");
+        if (rc) return rc;
+        ctx->lu_scratch_bytes = need;
+    }
+    char* base = (char*)ctx->lu_scratch;
+    unsigned* counter = (unsigned*)base;
+    int* info = (int*)(base + 64);
+    double* pval = (double*)(base + 512);
+    int* pidx = (int*)(pval + gmax);
+    T* SL = (T*)(((uintptr_t)(pidx + gmax) + 255) & ~(uintptr_t)255);
+    rc = check_hip(hipMemsetAsync(base, 0, 128, s), "LU scratch reset");
+    if (rc) return rc;
     for (int k = 0; k < n; k += NB) {
-        const int kend = min(n, k + NB);
-        for (int c = k; c < kend; ++c) {
-            hipLaunchKernelGGL((pivot_kernel<T, C>), dim3(1), dim3(1024), 0, s, A, ld, n, c, k, kend, ipiv, info);
-            if (c + 1 < n)
-                hipLaunchKernelGGL((panel_update_kernel<T, C>), dim3((n - c - 1 + 3) / 4), dim3(64, 4), 0, s,
-                                   A, ld, n, c, kend);
+        const int kend = min(n, k + NB), w = kend - k;
+        for (int c = k; c <= kend; ++c) {
+            const int rows = n - (c < kend ? c : kend);
+            if (rows <= 0) break;
+            const int G = (rows + RPB - 1) / RPB;
+            hipLaunchKernelGGL((col_step_kernel<T, C>), dim3(G), dim3(256), 0, s, A, ld, n, k, kend, c, pval, pidx,
+                               counter, ipiv, info);
         }
         hipLaunchKernelGGL((laswp_kernel<T, C>), dim3((n + 255) / 256), dim3(256), 0, s, A, ld, n, k, kend, ipiv);
         if (kend < n) {
-            hipLaunchKernelGGL((trsm_kernel<T, C>), dim3((n - kend + 255) / 256), dim3(256), 0, s, A, ld, n, k, kend);
-            const int M = n - kend, N = n - kend, K = kend - k;
-            dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+            const int N = n - kend;
+            hipLaunchKernelGGL((linv_kernel<T, C>), dim3(1), dim3(64), 0, s, (const T*)A, ld, k, w, SL);
+            // U12 = A12 - S_L A12 (in place: one block row, each block owns its columns)
+            hipLaunchKernelGGL((gemm_sub_kernel<T, C>), dim3((N + GT - 1) / GT, 1), dim3(256), 0, s,
+                               A + E * ((int64_t)k * ld + kend), ld, (const T*)SL, (int64_t)NB,
+                               (const T*)(A + E * ((int64_t)k * ld + kend)), ld, w, N, w);
+            // A22 -= L21 U12
+            dim3 grid((N + GT - 1) / GT, (N + GT - 1) / GT);
             hipLaunchKernelGGL((gemm_sub_kernel<T, C>), grid, dim3(256), 0, s,
-                               A + (C ? 2 : 1) * ((int64_t)kend * ld + kend), ld,
-                               A + (C ? 2 : 1) * ((int64_t)kend * ld + k), ld,
-                               A + (C ? 2 : 1) * ((int64_t)k * ld + kend), ld, M, N, K);
+                               A + E * ((int64_t)kend * ld + kend), ld,
+                               (const T*)(A + E * ((int64_t)kend * ld + k)), ld,
+                               (const T*)(A + E * ((int64_t)k * ld + kend)), ld, N, N, w);
         }
-        int rc = check_hip(hipGetLastError(), "LU step");
+        rc = check_hip(hipGetLastError(), "LU step");
         if (rc) return rc;
     }
     int h_info = 0;
-    int rc = check_hip(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, s), "LU info");
+    rc = check_hip(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, s), "LU info");
     if (rc) return rc;
     rc = check_hip(hipStreamSynchronize(s), "LU sync");
     if (rc) return rc;
