@@ -4,9 +4,10 @@ workload: interior Dirichlet problem, k = 3, seeded perturbed icosphere level
 unknowns and P1 Dirichlet data.
 
 One step = the FP64 time-to-solution pipeline on resident inputs: assemble V
-(c128, far + near field), the right-hand side 1/2 M conj(g) + K conj(g)
-(matrix-free), LU factorisation and solve.  `value` is BIO entries/s of the
-V assembly (F^2 entries / device time of its launches).  The mixed-precision
+(c128, far + near field) and the right-hand side 1/2 M conj(g) + K conj(g)
+(matrix-free, fused with V), LU factorisation and solve.  `value` is BIO
+entries/s of that assembly: (F^2 V entries + F*Nv K entries) / device time of
+its launches.  The mixed-precision
 pipelines (FP32 regular pairs + FP64 near field in c64, c64 LU, FP64
 refinement) are timed after the steps and reported beside it with their error
 against the FP64 solution.
@@ -27,7 +28,11 @@ sys.path.insert(0, REPO)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3
-FLOPS_PER_EVAL_V_HELM = 15  # algorithmic flops per quadrature-point pair, V Helmholtz
+# Algorithmic flops per quadrature-point pair of the fused far-field kernel
+# (add/mul = 1, FMA = 2; rsqrt and sin/cos are transcendentals, not counted):
+# d = x - y 3, r^2 5, r 1, kr 1, w/r 1, V accumulate 4, (x-y).n 5,
+# w (x-y).n / r^3 3, (cos + kr sin, sin - kr cos) 4, K accumulate 4.
+FLOPS_PER_EVAL_FUSED = 31
 
 
 def log(msg: str) -> None:
@@ -76,16 +81,16 @@ def main() -> int:
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
 
     def fp64_step():
+        # A = V and b = 1/2 M conj(g) + K conj(g): far field in one fused pass
+        # (V and K conj(g) share each point-pair evaluation), then the near field
         e = [ev() for _ in range(7)]
         e[0].record()
         A64.zero_()
-        ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_FAR), A64)
-        e[1].record()
-        ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_NEAR), A64)
-        e[2].record()
         b64.zero_()
-        ctx.mass_apply(0, 1, g, b64, 0.5, conj_x=True)
-        ctx.apply(desc(L.OP_K, 0, 1, L.PREC_FP64), g, b64, 1.0, conj_x=True)
+        ctx.assemble_dirichlet(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_FAR), A64, g, b64, 1.0, 1.0, 0.5)
+        e[1].record()
+        ctx.assemble_dirichlet(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_NEAR), A64, g, b64, 1.0, 1.0, 0.5)
+        e[2].record()
         e[3].record()
         LU64.copy_(A64)
         ctx.lu_factor(LU64, ipiv)
@@ -95,7 +100,7 @@ def main() -> int:
         e[5].record()
         torch.cuda.synchronize()
         t = [e[i].elapsed_time(e[i + 1]) for i in range(5)]
-        return x, {"far_ms": t[0], "near_ms": t[1], "rhs_ms": t[2], "lu_ms": t[3], "solve_ms": t[4],
+        return x, {"far_ms": t[0], "near_ms": t[1], "rhs_extra_ms": t[2], "lu_ms": t[3], "solve_ms": t[4],
                    "total_ms": e[0].elapsed_time(e[5])}
 
     for w in range(args.warmup):
@@ -112,14 +117,16 @@ def main() -> int:
     wall = time.perf_counter() - t0
     mean = {kk: float(np.mean([s[kk] for s in steps])) for kk in steps[0]}
     asm_ms = mean["far_ms"] + mean["near_ms"]
-    entries = nf * nf
+    # entries produced per assembly: V (F x F, into the matrix) and K (F x Nv,
+    # contracted with conj(g) on the fly for the right-hand side)
+    entries = nf * nf + nf * nv
     value = entries / (asm_ms * 1e-3)
 
     # roofline of the dominant assembly kernel (far field, FP64 Helmholtz V)
     far_pairs = counts[0]
-    far_flops = far_pairs * 625 * FLOPS_PER_EVAL_V_HELM
+    far_flops = far_pairs * 625 * FLOPS_PER_EVAL_FUSED
     achieved = far_flops / (mean["far_ms"] * 1e-3) / 1e12
-    roofline = {"bound": "valu", "kernel": "far_assemble_kernel<double,V,Helmholtz,P0,P0>",
+    roofline = {"bound": "valu", "kernel": "far_dirichlet_kernel<double,Helmholtz> (V + K conj(g))",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
                 "flops_per_launch": far_flops, "launch_ms": round(mean["far_ms"], 3)}
@@ -150,12 +157,10 @@ def main() -> int:
             e = [ev() for _ in range(6)]
             e[0].record()
             A32.zero_()
-            ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP32, L.PART_FAR), A32)
-            e[1].record()
-            ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_NEAR), A32)
             b.zero_()
-            ctx.mass_apply(0, 1, g, b, 0.5, conj_x=True)
-            ctx.apply(desc(L.OP_K, 0, 1, L.PREC_FP32), g, b, 1.0, conj_x=True)
+            ctx.assemble_dirichlet(desc(L.OP_V, 0, 0, L.PREC_FP32, L.PART_FAR), A32, g, b, 1.0, 1.0, 0.5)
+            e[1].record()
+            ctx.assemble_dirichlet(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_NEAR), A32, g, b, 1.0, 1.0, 0.5)
             e[2].record()
             LU32.copy_(A32)
             ctx.lu_factor(LU32, ipiv)
@@ -174,10 +179,8 @@ def main() -> int:
             e = [ev() for _ in range(5)]
             e[0].record()
             A64.zero_()
-            ctx.assemble(desc(L.OP_V, 0, 0, L.PREC_FP64), A64)
             b64.zero_()
-            ctx.mass_apply(0, 1, g, b64, 0.5, conj_x=True)
-            ctx.apply(desc(L.OP_K, 0, 1, L.PREC_FP64), g, b64, 1.0, conj_x=True)
+            ctx.assemble_dirichlet(desc(L.OP_V, 0, 0, L.PREC_FP64), A64, g, b64, 1.0, 1.0, 0.5)
             e[1].record()
             bem.convert(A64, LU32)
             ctx.lu_factor(LU32, ipiv)
@@ -201,19 +204,28 @@ def main() -> int:
         import oracle as O
 
         threads = min(16, os.cpu_count() or 1)
-        op = O.make_op(O.OP_V, O.HELMHOLTZ, k, 1.0, 5, 0, 0, 0)
+        opv = O.make_op(O.OP_V, O.HELMHOLTZ, k, 1.0, 5, 0, 0, 0)
+        opk = O.make_op(O.OP_K, O.HELMHOLTZ, k, 1.0, 5, 0, 1, 0)
         geom = O.geometry(v, f)
-        rows = 8
+        gh = g.cpu().numpy()
+
+        def cpu_rows(r):
+            O.assemble_rows(opv, v, f, 0, r, nthreads=threads, geom=geom)
+            O.apply_rows(opk, v, f, gh, 0, r, True, 1.0, nthreads=threads, geom=geom)
+
+        rows = 4
         t = time.perf_counter()
-        O.assemble_rows(op, v, f, 0, rows, nthreads=threads, geom=geom)
+        cpu_rows(rows)
         dt = time.perf_counter() - t
         rows2 = max(rows, min(nf, int(rows * args.cpu_seconds / max(dt, 1e-3))))
         t = time.perf_counter()
-        O.assemble_rows(op, v, f, 0, rows2, nthreads=threads, geom=geom)
+        cpu_rows(rows2)
         dt2 = time.perf_counter() - t
-        result["cpu_baseline"] = {"value": rows2 * nf / dt2, "unit": "entries/s", "cores": threads, "kind": "port",
-                                  "sample": f"V Helmholtz FP64 rows 0..{rows2 - 1} of {nf} (all {nf} columns, "
-                                            f"GaussQR 5), {dt2:.1f} s, C/OpenMP oracle"}
+        result["cpu_baseline"] = {"value": rows2 * (nf + nv) / dt2, "unit": "entries/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": f"test panels 0..{rows2 - 1} of {nf}: V Helmholtz row (all {nf} "
+                                            f"columns) + K conj(g) row ({nv} vertex columns), FP64, GaussQR 5, "
+                                            f"{dt2:.1f} s, C/OpenMP oracle"}
         result["cpu_baseline"]["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
         log(f"cpu baseline: {rows2} rows in {dt2:.1f}s")
 

@@ -155,6 +155,20 @@ int bem_nearfield_counts(bem_context* ctx, int64_t counts[4]);
 int bem_assemble(bem_context* ctx, const bem_op_desc* op, void* out, int64_t ld,
                  double alpha_re, double alpha_im, void* stream);
 
+/* Interior/exterior Dirichlet system of the P0 Neumann unknowns with P1
+ * Dirichlet data g, in one far-field pass: A += alpha V, and
+ * rhs += alpha_k K ghat + alpha_m M ghat (ghat = conj(g) if conj_g; rhs and g
+ * of dtype_rhs).  The reference builds the same pieces with
+ * SingleLayer3d.forward + matA_add_M (BEM_manager.py:1006-1008) and
+ * RHSConstructor3d.forward (rhs_constructor_3d.py:374-386: alpha_m = +-1/2,
+ * alpha_k = 1); here V and K ghat share every point-pair evaluation.
+ * `op` is the P0 x P0 single-layer descriptor (kernel, k, precision, flags;
+ * its dtype is that of A). */
+int bem_assemble_dirichlet(bem_context* ctx, const bem_op_desc* op, void* A, int64_t ld,
+                           double alpha_re, double alpha_im, const void* g, int32_t conj_g,
+                           void* rhs, int32_t dtype_rhs, double alpha_k, double alpha_m,
+                           void* stream);
+
 /* Matrix-free y[test dof] += alpha * Op . xhat, xhat = conj(x) if conj_x.
  * Replaces apply_V_dot_N_boundary (single_layer_3d.py:435-498),
  * apply_K_dot_D_boundary (double_layer_3d.py:440-501),

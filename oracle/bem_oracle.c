@@ -428,6 +428,14 @@ int bo_apply(const bo_op *op, int nv, int nf, const double *verts,
              const int32_t *tris, const double *area, const double *normal,
              const double *x, int conj_x, double alpha, double *y, int nthreads)
 {
+    return bo_apply_rows(op, nv, nf, verts, tris, area, normal, x, conj_x, alpha, 0, nf, y, nthreads);
+}
+
+int bo_apply_rows(const bo_op *op, int nv, int nf, const double *verts,
+                  const int32_t *tris, const double *area, const double *normal,
+                  const double *x, int conj_x, double alpha, int row_begin, int row_end,
+                  double *y, int nthreads)
+{
     double gp[BO_QMAX], gw[BO_QMAX];
     if (bo_gauss01(op->q, gp, gw)) return -1;
     const int ns = op->deg_test ? 3 : 1, nt = op->deg_trial ? 3 : 1;
@@ -435,7 +443,7 @@ int bo_apply(const bo_op *op, int nv, int nf, const double *verts,
     set_threads(nthreads);
     int i;
 #pragma omp parallel for schedule(dynamic)
-    for (i = 0; i < nf; ++i) {
+    for (i = row_begin; i < row_end; ++i) {
         double blk[18], acc[6] = {0, 0, 0, 0, 0, 0};
         for (int j = 0; j < nf; ++j) {
             int rel = bo_relation(tris, i, j);

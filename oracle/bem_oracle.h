@@ -105,6 +105,12 @@ int bo_apply(const bo_op *op, int nv, int nf, const double *verts,
              const int32_t *tris, const double *area, const double *normal,
              const double *x, int conj_x, double alpha, double *y, int nthreads);
 
+/* bo_apply restricted to test panels [row_begin, row_end) (CPU baseline samples). */
+int bo_apply_rows(const bo_op *op, int nv, int nf, const double *verts,
+                  const int32_t *tris, const double *area, const double *normal,
+                  const double *x, int conj_x, double alpha, int row_begin, int row_end,
+                  double *y, int nthreads);
+
 /* Mass matrix with the single-panel Q^2 Duffy rule.  out[rows x cols]
  * complex, += alpha * M.  identity_layer_3d.py:46-146 and
  * rhs_constructor_3d.py:203-254. */

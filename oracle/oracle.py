@@ -157,6 +157,21 @@ def apply(op: BoOp, verts, tris, x: np.ndarray, conj_x: bool = True, alpha: floa
     return y[:, 0] + 1j * y[:, 1]
 
 
+def apply_rows(op: BoOp, verts, tris, x: np.ndarray, row_begin: int, row_end: int, conj_x: bool = True,
+               alpha: float = 1.0, nthreads: int = 0, geom=None) -> np.ndarray:
+    """Matrix-free product for test panels [row_begin, row_end) only (P0 test)."""
+    v, t = _mesh(verts, tris)
+    area, normal = (geom or geometry(v, t))[:2]
+    rows = len(v) if op.deg_test else len(t)
+    xx = np.ascontiguousarray(np.stack([np.real(x), np.imag(x)], -1), dtype=np.float64)
+    y = np.zeros((rows, 2))
+    lib().bo_apply_rows(ctypes.byref(op), len(v), len(t), _dp(v), _ip(t),
+                        _dp(np.ascontiguousarray(area)), _dp(np.ascontiguousarray(normal)),
+                        _dp(xx), int(bool(conj_x)), ctypes.c_double(alpha), int(row_begin), int(row_end),
+                        _dp(y), int(nthreads))
+    return y[:, 0] + 1j * y[:, 1]
+
+
 def mass(q: int, deg_test: int, deg_trial: int, verts, tris, alpha: float = 1.0, geom=None) -> np.ndarray:
     v, t = _mesh(verts, tris)
     area = np.ascontiguousarray((geom or geometry(v, t))[0])

@@ -26,7 +26,7 @@ PART_FAR, PART_NEAR = 1, 2
 # Every symbol include/bem_hip.h declares.
 EXPORTS = (
     "bem_create", "bem_destroy", "bem_last_error", "bem_version", "bem_set_mesh",
-    "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_apply",
+    "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_assemble_dirichlet", "bem_apply",
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy",
 )
@@ -85,6 +85,8 @@ def lib() -> ctypes.CDLL:
             "bem_nearfield_counts": (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
             "bem_assemble": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i64, dbl, dbl, vp]),
             "bem_apply": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i32, vp, dbl, dbl, vp]),
+            "bem_assemble_dirichlet": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i64, dbl, dbl, vp, i32, vp,
+                                                      i32, dbl, dbl, vp]),
             "bem_mass_assemble": (ctypes.c_int, [vp, i32, i32, i32, vp, i64, dbl, i32, vp]),
             "bem_mass_apply": (ctypes.c_int, [vp, i32, i32, i32, vp, i32, vp, dbl, vp]),
             "bem_lu_factor": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp]),
@@ -210,6 +212,21 @@ class Context:
         a = complex(alpha)
         check(lib().bem_assemble(self._h, ctypes.byref(desc), out.data_ptr(), int(out.stride(0)),
                                  a.real, a.imag, current_stream()))
+
+    def assemble_dirichlet(self, desc: BemOpDesc, A, g, rhs, alpha: complex = 1.0, alpha_k: float = 1.0,
+                           alpha_m: float = 0.5, conj_g: bool = True) -> None:
+        """A += alpha V and rhs += alpha_k K ghat + alpha_m M ghat in one far-field
+        pass (P0 Neumann unknowns, P1 Dirichlet data g)."""
+        _require_cuda(A, g, rhs)
+        if A.stride(-1) != 1:
+            raise BEMError("matrix needs unit column stride")
+        if g.dtype != rhs.dtype:
+            raise BEMError("g and rhs must share a dtype")
+        desc.dtype = dtype_code(A)
+        a = complex(alpha)
+        check(lib().bem_assemble_dirichlet(self._h, ctypes.byref(desc), A.data_ptr(), int(A.stride(0)), a.real,
+                                           a.imag, g.data_ptr(), int(bool(conj_g)), rhs.data_ptr(),
+                                           dtype_code(rhs), float(alpha_k), float(alpha_m), current_stream()))
 
     def apply(self, desc: BemOpDesc, x, y, alpha: complex = 1.0, conj_x: bool = True) -> None:
         _require_cuda(x, y)

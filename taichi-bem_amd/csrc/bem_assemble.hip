@@ -551,6 +551,105 @@ OpParams make_params(Context* ctx, const bem_op_desc* d) {
     return p;
 }
 
+// ---------------------------------------------------------------------------
+// Dirichlet system in one far-field pass (P0 test panels): V (P0 trial) into
+// the matrix and the matrix-free K conj(g) (P1 trial) for the right-hand side
+// share every point-pair evaluation of r, 1/r and exp(i kap r).  The K part
+// is reduced over the block's 256 trial panels and written as one partial per
+// (test panel, trial block); a second kernel sums the partials in order.
+// ---------------------------------------------------------------------------
+template <typename R, bool HELM>
+__global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb, double* kpart) {
+    const int i = blockIdx.x / njb;
+    const int jb = blockIdx.x % njb;
+    const int j = jb * 256 + threadIdx.x;
+    double kgr = 0.0, kgi = 0.0;
+    if (j < p.nf && !shares_vertex(p.tris, i, j)) {
+        const int q2 = p.q2, nf = p.nf;
+        const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+        const R* __restrict__ PL = Ptr<R>::pl(p);
+        const R* __restrict__ PHI = Ptr<R>::phi(p);
+        R dcx = 0, dcy = 0, dcz = 0;
+        if constexpr (sizeof(R) == 4) {
+            dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
+            dcy = (R)(p.cen[3 * i + 1] - p.cen[3 * j + 1]);
+            dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
+        }
+        const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
+        // trial data at the three vertices of panel j
+        R gr[3], gi[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            double xr, xi;
+            load_elem(p.x, p.dtype_x, p.tris[3 * j + t], p.conj_x, xr, xi);
+            gr[t] = (R)xr; gi[t] = (R)xi;
+        }
+        const R kap = (R)p.kap;
+        R vr = 0, vi = 0, kr_ = 0, ki_ = 0;
+        for (int b = 0; b < q2; ++b) {
+            const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
+            const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
+            const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
+            const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
+            R pvr = 0, pvi = 0, pkr = 0, pki = 0;
+            for (int a = 0; a < q2; ++a) {
+                const R dx = PU[4 * a + 0] + ex;
+                const R dy = PU[4 * a + 1] + ey;
+                const R dz = PU[4 * a + 2] + ez;
+                const R wa = PU[4 * a + 3];
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                const R rinv = Math<R>::rsqrt(r2);
+                const R g = wa * rinv;
+                const R base = g * (dx * nnx + dy * nny + dz * nnz) * (rinv * rinv);
+                if constexpr (HELM) {
+                    const R kr = kap * (r2 * rinv);
+                    R sn, cs;
+                    Math<R>::sincos(kr, &sn, &cs);
+                    pvr += g * cs; pvi += g * sn;
+                    pkr += base * (cs + kr * sn); pki += base * (sn - kr * cs);
+                } else {
+                    pvr += g;
+                    pkr += base;
+                }
+            }
+            vr += pvr * wb; vi += pvi * wb;
+            // interpolated trial data at point b
+            const R ph0 = PHI[3 * b], ph1 = PHI[3 * b + 1], ph2 = PHI[3 * b + 2];
+            const R xr = ph0 * gr[0] + ph1 * gr[1] + ph2 * gr[2];
+            const R xi = ph0 * gi[0] + ph1 * gi[1] + ph2 * gi[2];
+            const R wr = pkr * wb, wi = pki * wb;
+            kr_ += wr * xr - wi * xi;
+            ki_ += wr * xi + wi * xr;
+        }
+        const double Vr = kInv4Pi * (double)vr, Vi = kInv4Pi * (double)vi;
+        add_elem(p.out, p.dtype, (int64_t)i * p.ld + j, p.ar * Vr - p.ai * Vi, p.ar * Vi + p.ai * Vr, false);
+        kgr = kInv4Pi * (double)kr_;
+        kgi = kInv4Pi * (double)ki_;
+    }
+    __shared__ double red[4][2];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    kgr = wave_sum(kgr);
+    kgi = wave_sum(kgi);
+    if (lane == 0) { red[wid][0] = kgr; red[wid][1] = kgi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        kpart[((int64_t)i * njb + jb) * 2] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+        kpart[((int64_t)i * njb + jb) * 2 + 1] = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    }
+}
+
+// rhs[i] += alpha * sum_jb kpart[i][jb] (fixed order).
+__global__ void kpart_reduce_kernel(const double* kpart, int nf, int njb, void* y, int dtype, double ar, double ai) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < njb; ++q) {
+        a += kpart[((int64_t)i * njb + q) * 2];
+        b += kpart[((int64_t)i * njb + q) * 2 + 1];
+    }
+    add_elem(y, dtype, i, ar * a - ai * b, ar * b + ai * a, false);
+}
+
 // ------------------------- dispatch -----------------------------------------
 using FarFn = void (*)(OpParams, int);
 using NearFn = void (*)(OpParams);
@@ -692,6 +791,62 @@ int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void
         hipLaunchKernelGGL(near_reduce_kernel, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, p,
                            d->deg_test);
         rc = check_hip(hipGetLastError(), "near-field apply launch");
+    }
+    return rc;
+}
+
+int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, double ar, double ai,
+                       const void* g, int conj_g, void* rhs, int dtype_rhs, double alpha_k, double alpha_m,
+                       hipStream_t s) {
+    int rc = validate(ctx, d);
+    if (rc) return rc;
+    if (!A || !g || !rhs) return set_error(BEM_ERR_INVALID, "null argument");
+    if (d->op != BEM_OP_V || d->deg_test != 0 || d->deg_trial != 0)
+        return set_error(BEM_ERR_INVALID, "the Dirichlet system takes the P0 single-layer descriptor");
+    if (dtype_rhs < BEM_F64 || dtype_rhs > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown rhs dtype");
+    if (dtype_complex(d->dtype) != dtype_complex(dtype_rhs))
+        return set_error(BEM_ERR_INVALID, "matrix and right-hand side must both be real or both complex");
+    rc = mesh_build_near(ctx, d->flags, s);
+    if (rc) return rc;
+    Mesh& m = ctx->mesh;
+    const int nf = m.nf, njb = (nf + 255) / 256;
+    const size_t kbytes = (size_t)nf * njb * 2 * sizeof(double);
+    if (m.kpart_bytes < kbytes) {
+        if (m.kpart) hipFree(m.kpart);
+        m.kpart = nullptr;
+        m.kpart_bytes = 0;
+        rc = check_hip(hipMalloc((void**)&m.kpart, kbytes), "K partials");
+        if (rc) return rc;
+        m.kpart_bytes = kbytes;
+    }
+    OpParams p = make_params(ctx, d);
+    p.out = A; p.ld = ld; p.dtype = d->dtype; p.ar = ar; p.ai = ai;
+    p.x = g; p.conj_x = conj_g; p.dtype_x = dtype_rhs; p.y = rhs;
+    const bool helm = d->kernel != BEM_LAPLACE;
+    const int parts = d->parts ? d->parts : (BEM_PART_FAR | BEM_PART_NEAR);
+    if (parts & BEM_PART_FAR) {
+        auto fk = d->precision == BEM_PREC_FP32 ? (helm ? far_dirichlet_kernel<float, true> : far_dirichlet_kernel<float, false>)
+                                                : (helm ? far_dirichlet_kernel<double, true> : far_dirichlet_kernel<double, false>);
+        hipLaunchKernelGGL(fk, dim3((unsigned)((int64_t)nf * njb)), dim3(256), 0, s, p, njb, m.kpart);
+        hipLaunchKernelGGL(kpart_reduce_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, (const double*)m.kpart, nf,
+                           njb, rhs, dtype_rhs, alpha_k, 0.0);
+        rc = check_hip(hipGetLastError(), "far-field Dirichlet system launch");
+        if (rc) return rc;
+    }
+    if (parts & BEM_PART_NEAR) {
+        // near-field V into the matrix, near-field K conj(g) into the rhs,
+        // and alpha_m M conj(g)
+        bem_op_desc dk = *d;
+        dk.op = BEM_OP_K; dk.deg_trial = 1; dk.dtype = dtype_rhs; dk.parts = BEM_PART_NEAR;
+        if (p.n_near > 0) {
+            NearFn nv = near_any(BEM_OP_V, helm, 0, 0, false);
+            hipLaunchKernelGGL(nv, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
+        }
+        rc = check_hip(hipGetLastError(), "near-field V launch");
+        if (rc) return rc;
+        rc = apply_op(ctx, &dk, g, conj_g, rhs, alpha_k, 0.0, s);
+        if (rc) return rc;
+        if (alpha_m != 0.0) rc = mass_apply(ctx, 0, 1, dtype_rhs, g, conj_g, rhs, alpha_m, s);
     }
     return rc;
 }

@@ -134,6 +134,15 @@ int bem_apply(bem_context* ctx, const bem_op_desc* op, const void* x, int32_t co
     });
 }
 
+int bem_assemble_dirichlet(bem_context* ctx, const bem_op_desc* op, void* A, int64_t ld, double alpha_re,
+                           double alpha_im, const void* g, int32_t conj_g, void* rhs, int32_t dtype_rhs,
+                           double alpha_k, double alpha_m, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::assemble_dirichlet(&ctx->impl, op, A, ld, alpha_re, alpha_im, g, conj_g, rhs, dtype_rhs,
+                                       alpha_k, alpha_m, as_stream(stream));
+    });
+}
+
 int bem_mass_assemble(bem_context* ctx, int32_t deg_test, int32_t deg_trial, int32_t dtype, void* out,
                       int64_t ld, double alpha, int32_t transpose, void* stream) {
     return with_device(ctx, [&]() -> int {

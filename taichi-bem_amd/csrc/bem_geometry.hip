@@ -238,7 +238,7 @@ int dev_alloc(T** p, size_t n) {
 void mesh_free(Mesh* m) {
     void* ptrs[] = {m->verts, m->tris, m->area, m->normal, m->vert_area, m->vert_normal, m->curl,
                     m->pu, m->pl, m->cen, m->pu32, m->pl32, m->phi, m->phi32, m->mass_w,
-                    m->near, m->near_row, m->rule, m->near_buf};
+                    m->near, m->near_row, m->rule, m->near_buf, m->kpart};
     for (void* p : ptrs)
         if (p) hipFree(p);
     *m = Mesh();

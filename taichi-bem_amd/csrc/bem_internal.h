@@ -56,6 +56,8 @@ struct Mesh {
     std::vector<int32_t> h_tris;  // host copy for near-list construction
     double* near_buf = nullptr;   // scratch for matrix-free near contributions
     int64_t near_buf_len = 0;
+    double* kpart = nullptr;      // per (test panel, trial block) partial sums of K conj(g)
+    size_t kpart_bytes = 0;
 };
 
 struct Context {
@@ -87,6 +89,9 @@ int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void
              double ai, hipStream_t s);
 int mass_assemble(Context* ctx, int deg_test, int deg_trial, int dtype, void* out, int64_t ld,
                   double alpha, int transpose, hipStream_t s);
+int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, double ar, double ai,
+                       const void* g, int conj_g, void* rhs, int dtype_rhs, double alpha_k, double alpha_m,
+                       hipStream_t s);
 int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void* x, int conj_x,
                void* y, double alpha, hipStream_t s);
 

@@ -124,25 +124,32 @@ __global__ __launch_bounds__(256) void col_step_kernel(T* A, int64_t ld, int n, 
     const bool dzero = diag.re == T(0) && diag.im == T(0);
     double best = -1.0;
     int brow = 0x7fffffff;
-    const int r0 = rbeg + blockIdx.x * RPB + wave * (RPB / 4);
-    for (int q = 0; q < RPB / 4; ++q) {
+    constexpr int RW = RPB / 4;  // rows per wave
+    const int r0 = rbeg + blockIdx.x * RPB + wave * RW;
+    const bool mine = col >= (upd ? c - 1 : c) && col < kend;
+    // all loads first (independent, in flight together), then update + store
+    cx<T> x[RW];
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+        x[q] = {T(0), T(0)};
+        if (mine && r0 + q < n) x[q] = Acc<T, C>::ld(A, (int64_t)(r0 + q) * ld + col);
+    }
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
         const int r = r0 + q;
         if (r >= n) break;
-        cx<T> x = {T(0), T(0)};
-        const bool mine = col >= (upd ? c - 1 : c) && col < kend;
-        if (mine) x = Acc<T, C>::ld(A, (int64_t)r * ld + col);
         if (upd) {
-            cx<T> l = {__shfl(x.re, c - 1 - k, 64), __shfl(x.im, c - 1 - k, 64)};
+            cx<T> l = {__shfl(x[q].re, c - 1 - k, 64), __shfl(x[q].im, c - 1 - k, 64)};
             if (!dzero) l = cdiv(l, diag);
-            if (col == c - 1) x = l;
+            if (col == c - 1) x[q] = l;
             else if (col >= c && col < kend) {
                 cx<T> lu = cmul(l, u);
-                x.re -= lu.re; x.im -= lu.im;
+                x[q].re -= lu.re; x[q].im -= lu.im;
             }
-            if (mine) Acc<T, C>::st(A, (int64_t)r * ld + col, x);
+            if (mine) Acc<T, C>::st(A, (int64_t)r * ld + col, x[q]);
         }
         if (search && col == c) {
-            double m = fabs((double)x.re) + fabs((double)x.im);
+            double m = fabs((double)x[q].re) + fabs((double)x[q].im);
             if (m > best) { best = m; brow = r; }  // rows visited in increasing order
         }
     }
@@ -201,25 +208,30 @@ __global__ __launch_bounds__(256) void col_step_kernel(T* A, int64_t ld, int n, 
 }
 
 // S_L = I - inv(L11) for the unit lower w x w block at (k, k), so that
-// U12 = inv(L11) A12 = A12 - S_L A12 runs on the MFMA GEMM.  One wave; lane t
-// computes column t right-looking (independent updates down the column).
+// U12 = inv(L11) A12 = A12 - S_L A12 runs on the MFMA GEMM.  Column t of the
+// inverse is computed right-looking by the four threads (t, g), thread g
+// updating rows i = g (mod 4): once X[j][t] is final it is eliminated from
+// the rows below with independent updates.
 template <typename T, bool C>
-__global__ __launch_bounds__(64) void linv_kernel(const T* A, int64_t ld, int k, int w, T* SL) {
+__global__ __launch_bounds__(256) void linv_kernel(const T* A, int64_t ld, int k, int w, T* SL) {
     __shared__ cx<T> M[NB][NB + 1];
     __shared__ cx<T> X[NB][NB + 1];
-    const int t = threadIdx.x;
-    for (int r = 0; r < NB; ++r)
+    const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
+    for (int r = g; r < NB; r += 4) {
         M[r][t] = (r < w && t < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + t) : cx<T>{T(r == t ? 1 : 0), T(0)};
-    for (int i = 0; i < NB; ++i) X[i][t] = {T(i == t ? 1 : 0), T(0)};
-    __syncthreads();
-    for (int j = t; j < NB; ++j) {
-        const cx<T> xj = X[j][t];
-        for (int i = j + 1; i < NB; ++i) {
-            cx<T> lx = cmul(M[i][j], xj);
-            X[i][t].re -= lx.re; X[i][t].im -= lx.im;
-        }
+        X[r][t] = {T(r == t ? 1 : 0), T(0)};
     }
-    for (int i = 0; i < NB; ++i) {
+    __syncthreads();
+    for (int j = 0; j < NB - 1; ++j) {
+        const cx<T> xj = X[j][t];
+        if (j >= t)
+            for (int i = j + 1 + ((g - j - 1) & 3); i < NB; i += 4) {
+                cx<T> lx = cmul(M[i][j], xj);
+                X[i][t].re -= lx.re; X[i][t].im -= lx.im;
+            }
+        __syncthreads();
+    }
+    for (int i = g; i < NB; i += 4) {
         cx<T> v = {T(i == t ? 1 : 0) - X[i][t].re, -X[i][t].im};
         Acc<T, C>::st(SL, (int64_t)i * NB + t, v);
     }
@@ -565,7 +577,7 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
         hipLaunchKernelGGL((laswp_kernel<T, C>), dim3((n + 255) / 256), dim3(256), 0, s, A, ld, n, k, kend, ipiv);
         if (kend < n) {
             const int N = n - kend;
-            hipLaunchKernelGGL((linv_kernel<T, C>), dim3(1), dim3(64), 0, s, (const T*)A, ld, k, w, SL);
+            hipLaunchKernelGGL((linv_kernel<T, C>), dim3(1), dim3(256), 0, s, (const T*)A, ld, k, w, SL);
             // U12 = A12 - S_L A12 (in place: one block row, each block owns its columns)
             hipLaunchKernelGGL((gemm_sub_kernel<T, C>), dim3((N + GT - 1) / GT, 1), dim3(256), 0, s,
                                A + E * ((int64_t)k * ld + kend), ld, (const T*)SL, (int64_t)NB,

@@ -349,3 +349,27 @@ def test_transmission_matrix(bem, oracle):
     Mp[:nv, nv:] = Mm.T
     A = Mp + calderon(np.sqrt(ni)) - calderon(np.sqrt(no))
     assert rel(to_np(bm.get_mat_A()), A) <= TOL64
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL64), (1, 5e-6)])
+@pytest.mark.parametrize("kernel,k", [(0, 0.0), (1, 3.0)])
+def test_assemble_dirichlet_fused(bem, oracle, meshes, kernel, k, prec, tol):
+    """One-pass V + (1/2 M + K) conj(g) equals the separate oracle pieces."""
+    v, f = meshes["p1"]
+    ctx = make_ctx(bem, v, f)
+    n = len(f)
+    cplx = kernel != 0
+    dt = (torch.complex128 if prec == 0 else torch.complex64) if cplx else (torch.float64 if prec == 0 else torch.float32)
+    A = torch.zeros(n, n, dtype=dt, device=DEV)
+    rng = np.random.default_rng(2)
+    g = rng.standard_normal(len(v)) + (1j * rng.standard_normal(len(v)) if cplx else 0)
+    gt = torch.tensor(g, device=DEV).to(torch.complex128 if cplx else torch.float64)
+    rhs = torch.zeros(n, dtype=gt.dtype, device=DEV)
+    desc = bem.Context.op_desc(0, kernel, k, 1.0, 0, 0, prec, 0, 0)
+    ctx.assemble_dirichlet(desc, A, gt, rhs, alpha=1.0, alpha_k=1.0, alpha_m=0.5, conj_g=True)
+    V = oracle.assemble(oracle.make_op(0, kernel, k, 1.0, 5, 0, 0, 0), v, f)
+    r = oracle.rhs_dirichlet(kernel, k, 5, v, f, np.asarray(g, dtype=complex), True, 1.0)
+    if not cplx:
+        V, r = V.real, r.real
+    assert rel(to_np(A).astype(V.dtype), V) <= tol
+    assert rel(to_np(rhs), r) <= tol
