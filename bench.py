@@ -213,7 +213,7 @@ def main() -> int:
             O.assemble_rows(opv, v, f, 0, r, nthreads=threads, geom=geom)
             O.apply_rows(opk, v, f, gh, 0, r, True, 1.0, nthreads=threads, geom=geom)
 
-        rows = 4
+        rows = 2 * threads  # calibration sample: two test panels per thread
         t = time.perf_counter()
         cpu_rows(rows)
         dt = time.perf_counter() - t
