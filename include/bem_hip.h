@@ -63,9 +63,14 @@ enum {
     BEM_FLAG_ROTATE_NEAR = 1,  /* put the shared vertex/edge at the slots the
                                   Sauter-Schwab rules expect (the reference
                                   uses stored order, single_layer_3d.py:133-141) */
-    BEM_FLAG_SYM_COINCIDE = 2  /* coincident rule: f(y,x) with swapped bases
+    BEM_FLAG_SYM_COINCIDE = 2, /* coincident rule: f(y,x) with swapped bases
                                   instead of f(x,y) twice (single_layer_3d.py:
                                   198-206) */
+    BEM_FLAG_DK = 4            /* bem_apply only: the product with dOp/dk, the
+                                  derivative of every entry with respect to the
+                                  wave number k (kap = k or k*sqrt_n); FP64.
+                                  No reference counterpart (the reference has
+                                  no derivatives) */
 };
 
 typedef struct bem_context bem_context;
@@ -205,6 +210,12 @@ int bem_lu_factor(bem_context* ctx, void* A, int64_t ld, int32_t n, int32_t dtyp
 int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
                  const int32_t* ipiv, void* b, void* stream);
 
+/* Solves A^T x = b (transpose, not conjugate transpose) in place of b from
+ * the same bem_lu_factor output: the adjoint solve of the gradient below.
+ * No reference counterpart (the reference has no derivatives). */
+int bem_lu_solve_transpose(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
+                           const int32_t* ipiv, void* b, void* stream);
+
 /* r = b - A x in FP64/complex128 (A of dtype BEM_F64/BEM_C128; x, b, r of
  * the matching FP64 type). */
 int bem_residual(bem_context* ctx, const void* A, int64_t ld, int32_t n, int32_t dtype,
@@ -219,6 +230,29 @@ int bem_refine_solve(bem_context* ctx, const void* A, int64_t lda, int32_t dtype
                      const void* LU, int64_t ldlu, int32_t n, int32_t dtype_lu,
                      const int32_t* ipiv, const void* b, void* x, int32_t max_iter,
                      double tol, bem_refine_stats* stats, void* stream);
+
+/* ---- derivatives (adjoint) ----------------------------------------- */
+
+/* Adjoint contraction of an operator's derivative, without forming it:
+ *   grad_vertices[3 v + d] += Re( alpha sum_{s,t} lam_s u_t dOp_st/dX_{v,d} )
+ *   *grad_k                += Re( alpha sum_{s,t} lam_s u_t dOp_st/dk )
+ * over every regular and near-field pair, re-evaluating the quadrature of
+ * the same discrete operator bem_assemble builds (flags included).  op: V
+ * (P0 x P0) or K (P0 test x P1 trial), FP64, dtype BEM_C128 (lam, u complex128
+ * of nf and nf|nv entries); grad_vertices [nv*3] f64; grad_k (one f64) may be
+ * NULL and is untouched for Laplace.  Accumulation uses atomics (the order of
+ * the sums is not fixed).  No reference counterpart: it differentiates
+ * SingleLayer3d/DoubleLayer3d (single_layer_3d.py:155-177,
+ * double_layer_3d.py:158-180) for the shape gradient of the Dirichlet
+ * problem of BEMManager.run (BEM_manager.py:990-1011). */
+int bem_shape_gradient(bem_context* ctx, const bem_op_desc* op, const void* lam, const void* u,
+                       double alpha_re, double alpha_im, double* grad_vertices, double* grad_k,
+                       void* stream);
+
+/* grad_vertices[3 v + d] += sum_i coef_i dA_i/dX_{v,d} (panel areas A_i of
+ * mesh_manager.py:114-124; coef f64 [nf]).  The derivative of the P0 mass
+ * and of area-weighted misfits. */
+int bem_area_gradient(bem_context* ctx, const double* coef, double* grad_vertices, void* stream);
 
 /* dst = (dtype_dst) src for a rows x cols block (precision conversion of a
  * matrix or vector; complex <-> complex and real <-> real only). */

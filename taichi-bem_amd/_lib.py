@@ -20,7 +20,7 @@ BEM_OK, BEM_ERR_INVALID, BEM_ERR_UNSUPPORTED, BEM_ERR_HIP, BEM_ERR_SINGULAR = 0,
 OP_V, OP_K, OP_KADJ, OP_W = 0, 1, 2, 3
 F64, C128, F32, C64 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
-FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE = 1, 2
+FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE, FLAG_DK = 1, 2, 4
 PART_FAR, PART_NEAR = 1, 2
 
 # Every symbol include/bem_hip.h declares.
@@ -28,7 +28,8 @@ EXPORTS = (
     "bem_create", "bem_destroy", "bem_last_error", "bem_version", "bem_set_mesh",
     "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_assemble_dirichlet", "bem_apply",
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
-    "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy",
+    "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
+    "bem_shape_gradient", "bem_area_gradient",
 )
 
 
@@ -91,6 +92,9 @@ def lib() -> ctypes.CDLL:
             "bem_mass_apply": (ctypes.c_int, [vp, i32, i32, i32, vp, i32, vp, dbl, vp]),
             "bem_lu_factor": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp]),
             "bem_lu_solve": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
+            "bem_lu_solve_transpose": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
+            "bem_shape_gradient": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, vp, dbl, dbl, vp, vp, vp]),
+            "bem_area_gradient": (ctypes.c_int, [vp, vp, vp, vp]),
             "bem_residual": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp, vp]),
             "bem_refine_solve": (ctypes.c_int, [vp, vp, i64, i32, vp, i64, i32, i32, vp, vp, vp, i32, dbl,
                                                 ctypes.POINTER(BemRefineStats), vp]),
@@ -259,6 +263,32 @@ class Context:
         _require_cuda(LU, ipiv, b)
         check(lib().bem_lu_solve(self._h, LU.data_ptr(), int(LU.stride(0)), int(LU.shape[0]), dtype_code(LU),
                                  ipiv.data_ptr(), b.data_ptr(), current_stream()))
+
+    def lu_solve_transpose(self, LU, ipiv, b) -> None:
+        """b <- A^-T b (plain transpose) from the factors of A."""
+        _require_cuda(LU, ipiv, b)
+        check(lib().bem_lu_solve_transpose(self._h, LU.data_ptr(), int(LU.stride(0)), int(LU.shape[0]),
+                                           dtype_code(LU), ipiv.data_ptr(), b.data_ptr(), current_stream()))
+
+    def shape_gradient(self, desc: BemOpDesc, lam, u, grad, grad_k=None, alpha: complex = 1.0) -> None:
+        """grad [nv,3] += Re(alpha sum lam_s u_t dOp_st/dX); grad_k [1] +=
+        Re(alpha sum lam_s u_t dOp_st/dk).  lam, u complex128 device vectors."""
+        import torch
+
+        _require_cuda(lam, u, grad, grad_k)
+        if lam.dtype != torch.complex128 or u.dtype != torch.complex128:
+            raise BEMError("shape gradient coefficients must be complex128")
+        if grad.dtype != torch.float64 or not grad.is_contiguous():
+            raise BEMError("grad must be a contiguous float64 tensor")
+        desc.dtype = C128
+        a = complex(alpha)
+        check(lib().bem_shape_gradient(self._h, ctypes.byref(desc), lam.data_ptr(), u.data_ptr(), a.real, a.imag,
+                                       grad.data_ptr(), _ptr(grad_k), current_stream()))
+
+    def area_gradient(self, coef, grad) -> None:
+        """grad [nv,3] += sum_i coef_i dA_i/dX (coef float64 [nf])."""
+        _require_cuda(coef, grad)
+        check(lib().bem_area_gradient(self._h, coef.contiguous().data_ptr(), grad.data_ptr(), current_stream()))
 
     def residual(self, A, x, b, r) -> None:
         _require_cuda(A, x, b, r)

@@ -1,0 +1,576 @@
+// bem_adjoint.hip -- adjoint contractions of the Galerkin operators with
+// respect to the vertex positions and the wave number.
+//
+// For coefficient vectors lam (test dofs) and u (trial dofs) the kernels add
+//
+//     grad[v][d] += Re( alpha sum_{s,t} lam_s u_t dA_st / dX_{v,d} )
+//     grad_k     += Re( alpha sum_{s,t} lam_s u_t dA_st / dk )
+//
+// without forming dA: every quadrature point pair of every panel pair is
+// evaluated again and its derivative contracted on the spot.  The discrete
+// operator differentiated is exactly the one bem_assemble builds (same
+// tensor rule for regular pairs, same Sauter-Schwab tables, vertex-slot
+// permutations and coincident passes for the near field), so the result is
+// the exact derivative of the discrete functional.
+//
+// Supported: V (P0 x P0) and K (P0 test x P1 trial) -- the operators of the
+// Dirichlet problem V t = (M/2 + K) g (BEM_manager.py:990-1011).  Entry
+// derivatives restated from the integrands of single_layer_3d.py:155-177
+// (G) and double_layer_3d.py:158-180 (dG/dn_y):
+//
+//   V_ij = sum w_a w_b (2A_i)(2A_j) G(x_a - y_b)
+//   K_iv = sum w_a w_b (2A_i) phi_v(y_b) (x_a - y_b).N_j H(|x_a - y_b|)
+//
+// with N_j = (Y1-Y0) x (Y2-Y0) = 2A_j n_j, 4 pi G = exp(i kap r)/r,
+// 4 pi H = (1 - i kap r) exp(i kap r)/r^3, and x_a = sum_m phi_m(a) X_m.
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+
+namespace {
+
+struct AdjParams {
+    int nf, nv, q2, flags;
+    const int32_t* __restrict__ tris;
+    const double* __restrict__ verts;
+    const double* __restrict__ area;
+    const double* __restrict__ normal;
+    const double* __restrict__ pu;    // [nf][q2][4] absolute points, weight w (2A)
+    const double* __restrict__ pl;    // [q2][4][nf]
+    const double* __restrict__ phi;   // [q2][3]
+    const NearPair* __restrict__ near;
+    const NearPt* __restrict__ rule;
+    int32_t rule_off[4], rule_len[4];
+    int64_t n_near;
+    double kap;    // wave number inside G
+    double dkap;   // d kap / d k
+    const double* __restrict__ lam;  // complex128 [test dofs]
+    const double* __restrict__ u;    // complex128 [trial dofs]
+    double ar, ai;                   // alpha / (4 pi) folded in on the host
+    double* grad;                    // [nv*3]
+    double* gk;                      // [1] or null
+};
+
+// d(2A)/dX_m / (2A) for the stored slots of panel with vertices P[3][3]:
+// 2A = |N|, N = (P1-P0) x (P2-P0): d|N|/dP1 = E2 x n, d|N|/dP2 = n x E1.
+__device__ __forceinline__ void dlog_area(const double (&P)[3][3], double (&g)[3][3]) {
+    double e1[3], e2[3];
+    for (int d = 0; d < 3; ++d) { e1[d] = P[1][d] - P[0][d]; e2[d] = P[2][d] - P[0][d]; }
+    double N[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double l2 = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
+    const double il2 = 1.0 / l2;  // d|N|/dP / |N| = (E x N) / |N|^2
+    double a[3] = {(e2[1] * N[2] - e2[2] * N[1]) * il2, (e2[2] * N[0] - e2[0] * N[2]) * il2,
+                   (e2[0] * N[1] - e2[1] * N[0]) * il2};
+    double b[3] = {(N[1] * e1[2] - N[2] * e1[1]) * il2, (N[2] * e1[0] - N[0] * e1[2]) * il2,
+                   (N[0] * e1[1] - N[1] * e1[0]) * il2};
+    for (int d = 0; d < 3; ++d) { g[1][d] = a[d]; g[2][d] = b[d]; g[0][d] = -a[d] - b[d]; }
+}
+
+__device__ __forceinline__ void load_panel(const AdjParams& p, int i, double (&P)[3][3]) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) P[m][d] = p.verts[3 * p.tris[3 * i + m] + d];
+}
+
+__device__ __forceinline__ bool shares_vertex_adj(const int32_t* t, int i, int j) {
+    int a0 = t[3 * i], a1 = t[3 * i + 1], a2 = t[3 * i + 2];
+    int b0 = t[3 * j], b1 = t[3 * j + 1], b2 = t[3 * j + 2];
+    return a0 == b0 || a0 == b1 || a0 == b2 || a1 == b0 || a1 == b1 || a1 == b2 || a2 == b0 ||
+           a2 == b1 || a2 == b2;
+}
+
+// complex helpers on (re, im) pairs
+__device__ __forceinline__ void cmac(double& zr, double& zi, double ar, double ai, double br, double bi) {
+    zr = __builtin_fma(ar, br, __builtin_fma(-ai, bi, zr));
+    zi = __builtin_fma(ar, bi, __builtin_fma(ai, br, zi));
+}
+__device__ __forceinline__ double cre(double ar, double ai, double br, double bi) { return ar * br - ai * bi; }
+
+// Kernel pieces at separation d (|d| = r), all times 4 pi:
+//   V: val = e/r, grd = e (i kap r - 1)/r^3 (so grad_d G = grd d), dk = i e
+//   K: h = (1 - i kap r) e/r^3, hp = e (kap^2 r^2 - 3 + 3 i kap r)/r^5,
+//      dk(h) = kap e / r
+// with e = exp(i kap r).
+template <bool HELM>
+__device__ __forceinline__ void trig(double kap, double r, double& cs, double& sn) {
+    if constexpr (HELM) sincos_f64(kap * r, &sn, &cs);
+    else { cs = 1.0; sn = 0.0; }
+}
+
+// ---------------------------------------------------------------------------
+// Regular pairs.  Block: 256 trial panels (one per lane) x TI test panels
+// (wave-uniform, scalar loads of the test points).  The trial-vertex
+// gradient stays in registers over the TI test panels; the test-vertex
+// gradient is reduced over the block per test panel.
+// ---------------------------------------------------------------------------
+constexpr int TI = 8;
+
+template <int OP, bool HELM>
+__global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) {
+    const int ib = blockIdx.x / njb, jb = blockIdx.x % njb;
+    const int j = jb * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool jvalid = j < p.nf;
+    const int jj = jvalid ? j : 0;
+    const int q2 = p.q2, nf = p.nf;
+    const double* __restrict__ PL = p.pl;
+    const double* __restrict__ PHI = p.phi;
+    const double kap = p.kap;
+
+    double Y[3][3];
+    load_panel(p, jj, Y);
+    double dlj[3][3];
+    dlog_area(Y, dlj);
+    const double a2j = 2.0 * p.area[jj];
+    const double nj[3] = {p.normal[3 * jj], p.normal[3 * jj + 1], p.normal[3 * jj + 2]};
+    // trial coefficients: u_j (V) or u at the three vertices (K)
+    double ur[3], ui[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int idx = OP == BEM_OP_V ? jj : p.tris[3 * jj + t];
+        ur[t] = p.u[2 * idx]; ui[t] = p.u[2 * idx + 1];
+    }
+    double gy[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gy[m][d] = 0.0;
+    double gk_lane = 0.0;
+
+    __shared__ double red[4][10];
+    const int i_end = min(nf, (ib + 1) * TI);
+    for (int i = ib * TI; i < i_end; ++i) {
+        double gx[3][3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) gx[m][d] = 0.0;
+        double gkp = 0.0;
+        if (jvalid && !shares_vertex_adj(p.tris, i, j)) {
+            const double* __restrict__ PU = p.pu + (int64_t)i * q2 * 4;
+            // accumulators over the pair (complex): test-slot gradient EX,
+            // trial-slot gradient EY, value S0, k-derivative SK, and for K the
+            // N_j-derivative vector T
+            double EXr[3][3], EXi[3][3], EYr[3][3], EYi[3][3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) { EXr[m][d] = EXi[m][d] = EYr[m][d] = EYi[m][d] = 0.0; }
+            double S0r = 0.0, S0i = 0.0, SKr = 0.0, SKi = 0.0;
+            double Tr[3] = {0.0, 0.0, 0.0}, Ti[3] = {0.0, 0.0, 0.0};
+            for (int b = 0; b < q2; ++b) {
+                const double yx = PL[((int64_t)b * 4 + 0) * nf + j];
+                const double yy = PL[((int64_t)b * 4 + 1) * nf + j];
+                const double yz = PL[((int64_t)b * 4 + 2) * nf + j];
+                const double wb = PL[((int64_t)b * 4 + 3) * nf + j];
+                double Pr[3][3], Pi[3][3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) { Pr[m][d] = 0.0; Pi[m][d] = 0.0; }
+                double s0r = 0.0, s0i = 0.0, skr = 0.0, ski = 0.0;
+                double tr[3] = {0.0, 0.0, 0.0}, ti[3] = {0.0, 0.0, 0.0};
+                for (int a = 0; a < q2; ++a) {
+                    const double dv[3] = {PU[4 * a + 0] - yx, PU[4 * a + 1] - yy, PU[4 * a + 2] - yz};
+                    const double wa = PU[4 * a + 3];
+                    const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+                    const double rinv = rsqrt_f64(r2);
+                    const double r = r2 * rinv;
+                    double cs, sn;
+                    trig<HELM>(kap, r, cs, sn);
+                    const double kr = kap * r;
+                    const double ri3 = rinv * rinv * rinv;
+                    double gdr[3], gdi[3];  // wa * grad_d of the integrand
+                    if constexpr (OP == BEM_OP_V) {
+                        s0r += wa * cs * rinv; s0i += wa * sn * rinv;
+                        if constexpr (HELM) { skr -= wa * sn; ski += wa * cs; }
+                        const double hr = wa * (-cs - kr * sn) * ri3, hi = wa * (kr * cs - sn) * ri3;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) { gdr[d] = hr * dv[d]; gdi[d] = hi * dv[d]; }
+                    } else {
+                        const double dn = dv[0] * nj[0] + dv[1] * nj[1] + dv[2] * nj[2];
+                        const double hr = (cs + kr * sn) * ri3, hi = (sn - kr * cs) * ri3;
+                        const double ri5 = ri3 * rinv * rinv;
+                        const double A = kr * kr - 3.0, B = 3.0 * kr;
+                        const double hpr = (cs * A - sn * B) * ri5, hpi = (sn * A + cs * B) * ri5;
+                        s0r += wa * dn * hr; s0i += wa * dn * hi;
+                        if constexpr (HELM) {
+                            const double f = wa * dn * kap * rinv;
+                            skr += f * cs; ski += f * sn;
+                        }
+                        const double whr = wa * hr, whi = wa * hi;
+                        const double wpr = wa * dn * hpr, wpi = wa * dn * hpi;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            gdr[d] = whr * nj[d] + wpr * dv[d];
+                            gdi[d] = whi * nj[d] + wpi * dv[d];
+                            tr[d] += whr * dv[d]; ti[d] += whi * dv[d];
+                        }
+                    }
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        const double ph = PHI[3 * a + m];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) { Pr[m][d] += ph * gdr[d]; Pi[m][d] += ph * gdi[d]; }
+                    }
+                }
+                // trial-point factor: wb (V) or wb * u interpolated at y_b (K)
+                double cr = wb, ci = 0.0;
+                if constexpr (OP == BEM_OP_K) {
+                    const double f0 = PHI[3 * b], f1 = PHI[3 * b + 1], f2 = PHI[3 * b + 2];
+                    cr = wb * (f0 * ur[0] + f1 * ur[1] + f2 * ur[2]);
+                    ci = wb * (f0 * ui[0] + f1 * ui[1] + f2 * ui[2]);
+                }
+                double Qr[3], Qi[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    Qr[d] = Pr[0][d] + Pr[1][d] + Pr[2][d];
+                    Qi[d] = Pi[0][d] + Pi[1][d] + Pi[2][d];
+                }
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const double ph = PHI[3 * b + m];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        cmac(EXr[m][d], EXi[m][d], cr, ci, Pr[m][d], Pi[m][d]);
+                        cmac(EYr[m][d], EYi[m][d], -cr * ph, -ci * ph, Qr[d], Qi[d]);
+                    }
+                }
+                cmac(S0r, S0i, cr, ci, s0r, s0i);
+                cmac(SKr, SKi, cr, ci, skr, ski);
+                if constexpr (OP == BEM_OP_K) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) cmac(Tr[d], Ti[d], cr, ci, tr[d], ti[d]);
+                }
+            }
+            // contraction coefficient c = alpha lam_i (u_j for V)
+            double lr = p.lam[2 * i], li = p.lam[2 * i + 1];
+            if constexpr (OP == BEM_OP_V) {
+                const double t = lr * ur[0] - li * ui[0];
+                li = lr * ui[0] + li * ur[0];
+                lr = t;
+            }
+            const double cr = p.ar * lr - p.ai * li, ci = p.ar * li + p.ai * lr;
+            double X[3][3];
+            load_panel(p, i, X);
+            double dli[3][3];
+            dlog_area(X, dli);
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    gx[m][d] = cre(cr, ci, S0r * dli[m][d] + EXr[m][d], S0i * dli[m][d] + EXi[m][d]);
+                    double yr = EYr[m][d], yi = EYi[m][d];
+                    if constexpr (OP == BEM_OP_V) { yr += S0r * dlj[m][d]; yi += S0i * dlj[m][d]; }
+                    gy[m][d] += cre(cr, ci, yr, yi);
+                }
+            if constexpr (OP == BEM_OP_K) {
+                // d(d.N_j)/dY: slot 1 gets E2 x T~, slot 2 gets T~ x E1 (T~ = T / 2A_j)
+                double e1[3], e2[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) { e1[d] = Y[1][d] - Y[0][d]; e2[d] = Y[2][d] - Y[0][d]; }
+                double tr_[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) tr_[d] = cre(cr, ci, Tr[d], Ti[d]) / a2j;
+                const double c1[3] = {e2[1] * tr_[2] - e2[2] * tr_[1], e2[2] * tr_[0] - e2[0] * tr_[2],
+                                      e2[0] * tr_[1] - e2[1] * tr_[0]};
+                const double c2[3] = {tr_[1] * e1[2] - tr_[2] * e1[1], tr_[2] * e1[0] - tr_[0] * e1[2],
+                                      tr_[0] * e1[1] - tr_[1] * e1[0]};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    gy[1][d] += c1[d];
+                    gy[2][d] += c2[d];
+                    gy[0][d] -= c1[d] + c2[d];
+                }
+            }
+            gkp = cre(cr, ci, SKr, SKi) * p.dkap;
+        }
+        // test-vertex gradient of panel i: block reduction, 9 atomics
+        gk_lane += gkp;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double s = wave_sum(gx[m][d]);
+                if (lane == 0) red[wid][3 * m + d] = s;
+            }
+        __syncthreads();
+        if (threadIdx.x < 9) {
+            const double s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+            const int m = threadIdx.x / 3, d = threadIdx.x % 3;
+            if (s != 0.0) atomicAdd(p.grad + 3 * p.tris[3 * i + m] + d, s);
+        }
+        __syncthreads();
+    }
+    if (jvalid) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (gy[m][d] != 0.0) atomicAdd(p.grad + 3 * p.tris[3 * j + m] + d, gy[m][d]);
+    }
+    if (p.gk) {
+        const double s = wave_sum(gk_lane);
+        if (lane == 0 && s != 0.0) atomicAdd(p.gk, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near pairs: one wave per pair over the Sauter-Schwab points of its
+// relation, in the (possibly rotated) slot order of the pair, with the same
+// coincident passes as the assembly (f(y, x) in the second pass, bases
+// swapped under BEM_FLAG_SYM_COINCIDE).
+// ---------------------------------------------------------------------------
+template <int OP, bool HELM>
+__global__ __launch_bounds__(256) void adjoint_near_kernel(AdjParams p) {
+    const int lane = threadIdx.x & 63;
+    const int64_t pid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pid >= p.n_near) return;
+    const NearPair np = p.near[pid];
+    const int i = np.i, j = np.j, rel = np.rel;
+    int px[3], py[3], invy[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        px[m] = (np.perm >> (2 * m)) & 3;
+        py[m] = (np.perm >> (8 + 2 * m)) & 3;
+    }
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) invy[sl] = py[0] == sl ? 0 : (py[1] == sl ? 1 : 2);
+    double X[3][3], Y[3][3];  // rotated slot order
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            X[m][d] = p.verts[3 * p.tris[3 * i + px[m]] + d];
+            Y[m][d] = p.verts[3 * p.tris[3 * j + py[m]] + d];
+        }
+    const double nj[3] = {p.normal[3 * j], p.normal[3 * j + 1], p.normal[3 * j + 2]};
+    const double w4a = (p.area[i] * 2.0) * (p.area[j] * 2.0);
+    const double kap = p.kap;
+    const bool twice = rel == BEM_COINCIDE;
+    const bool sym = (p.flags & BEM_FLAG_SYM_COINCIDE) != 0;
+    const NearPt* __restrict__ rule = p.rule + p.rule_off[rel];
+    const int len = p.rule_len[rel];
+    // trial coefficients at stored slots
+    double ur[3], ui[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int idx = OP == BEM_OP_V ? j : p.tris[3 * j + t];
+        ur[t] = p.u[2 * idx]; ui[t] = p.u[2 * idx + 1];
+    }
+
+    double EXr[3][3], EXi[3][3], EYr[3][3], EYi[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { EXr[m][d] = EXi[m][d] = EYr[m][d] = EYi[m][d] = 0.0; }
+    double S0r = 0.0, S0i = 0.0, SKr = 0.0, SKi = 0.0;
+    double Tr[3] = {0.0, 0.0, 0.0}, Ti[3] = {0.0, 0.0, 0.0};
+
+    for (int e = lane; e < len; e += 64) {
+        const NearPt q = rule[e];
+        const double al[3] = {1.0 - q.r1x, q.r1x - q.r2x, q.r2x};
+        const double be[3] = {1.0 - q.r1y, q.r1y - q.r2y, q.r2y};
+        double x[3], y[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x[d] = al[0] * X[0][d] + al[1] * X[1][d] + al[2] * X[2][d];
+            y[d] = be[0] * Y[0][d] + be[1] * Y[1][d] + be[2] * Y[2][d];
+        }
+        const double w = q.w * w4a;
+        for (int pass = 0; pass <= (int)twice; ++pass) {
+            const double sg = pass ? -1.0 : 1.0;
+            const double dv[3] = {sg * (x[0] - y[0]), sg * (x[1] - y[1]), sg * (x[2] - y[2])};
+            const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+            const double rinv = rsqrt_f64(r2);
+            const double r = r2 * rinv;
+            double cs, sn;
+            trig<HELM>(kap, r, cs, sn);
+            const double kr = kap * r;
+            const double ri3 = rinv * rinv * rinv;
+            // point factor: w (V) or w * u at the trial basis point (K)
+            double cr = w, ci = 0.0;
+            if constexpr (OP == BEM_OP_K) {
+                const bool swap_b = pass && sym;
+                const double by1 = swap_b ? q.r1x : q.r1y, by2 = swap_b ? q.r2x : q.r2y;
+                double sr = 0.0, si = 0.0;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const double ph = shape(by1, by2, invy[t]);
+                    sr += ph * ur[t]; si += ph * ui[t];
+                }
+                cr = w * sr; ci = w * si;
+            }
+            double gdr[3], gdi[3];
+            if constexpr (OP == BEM_OP_V) {
+                cmac(S0r, S0i, cr, ci, cs * rinv, sn * rinv);
+                if constexpr (HELM) cmac(SKr, SKi, cr, ci, -sn, cs);
+                double hr = (-cs - kr * sn) * ri3, hi = (kr * cs - sn) * ri3;
+                const double t = cr * hr - ci * hi;
+                hi = cr * hi + ci * hr; hr = t;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) { gdr[d] = hr * dv[d]; gdi[d] = hi * dv[d]; }
+            } else {
+                const double dn = dv[0] * nj[0] + dv[1] * nj[1] + dv[2] * nj[2];
+                double hr = (cs + kr * sn) * ri3, hi = (sn - kr * cs) * ri3;
+                const double ri5 = ri3 * rinv * rinv;
+                const double A = kr * kr - 3.0, B = 3.0 * kr;
+                double hpr = (cs * A - sn * B) * ri5, hpi = (sn * A + cs * B) * ri5;
+                cmac(S0r, S0i, cr, ci, dn * hr, dn * hi);
+                if constexpr (HELM) cmac(SKr, SKi, cr, ci, dn * kap * rinv * cs, dn * kap * rinv * sn);
+                double t = cr * hr - ci * hi;
+                hi = cr * hi + ci * hr; hr = t;
+                t = cr * hpr - ci * hpi;
+                hpi = (cr * hpi + ci * hpr) * dn; hpr = t * dn;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    gdr[d] = hr * nj[d] + hpr * dv[d];
+                    gdi[d] = hi * nj[d] + hpi * dv[d];
+                    Tr[d] += hr * dv[d]; Ti[d] += hi * dv[d];
+                }
+            }
+            // d = sg (x - y): dd/dX_m = sg al_m, dd/dY_m = -sg be_m
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const double fa = sg * al[m], fb = -sg * be[m];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    EXr[m][d] += fa * gdr[d]; EXi[m][d] += fa * gdi[d];
+                    EYr[m][d] += fb * gdr[d]; EYi[m][d] += fb * gdi[d];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            EXr[m][d] = wave_sum(EXr[m][d]); EXi[m][d] = wave_sum(EXi[m][d]);
+            EYr[m][d] = wave_sum(EYr[m][d]); EYi[m][d] = wave_sum(EYi[m][d]);
+        }
+    S0r = wave_sum(S0r); S0i = wave_sum(S0i);
+    SKr = wave_sum(SKr); SKi = wave_sum(SKi);
+    if constexpr (OP == BEM_OP_K) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { Tr[d] = wave_sum(Tr[d]); Ti[d] = wave_sum(Ti[d]); }
+    }
+    if (lane != 0) return;
+    double lr = p.lam[2 * i], li = p.lam[2 * i + 1];
+    if constexpr (OP == BEM_OP_V) {
+        const double t = lr * ur[0] - li * ui[0];
+        li = lr * ui[0] + li * ur[0];
+        lr = t;
+    }
+    const double cr = p.ar * lr - p.ai * li, ci = p.ar * li + p.ai * lr;
+    // rotated-slot point derivatives
+    for (int m = 0; m < 3; ++m)
+        for (int d = 0; d < 3; ++d) {
+            atomicAdd(p.grad + 3 * p.tris[3 * i + px[m]] + d, cre(cr, ci, EXr[m][d], EXi[m][d]));
+            atomicAdd(p.grad + 3 * p.tris[3 * j + py[m]] + d, cre(cr, ci, EYr[m][d], EYi[m][d]));
+        }
+    // area (and, for K, N_j) derivatives in stored slot order
+    double Xs[3][3], Ys[3][3], dli[3][3], dlj[3][3];
+    load_panel(p, i, Xs);
+    load_panel(p, j, Ys);
+    dlog_area(Xs, dli);
+    dlog_area(Ys, dlj);
+    for (int m = 0; m < 3; ++m)
+        for (int d = 0; d < 3; ++d) {
+            atomicAdd(p.grad + 3 * p.tris[3 * i + m] + d, cre(cr, ci, S0r, S0i) * dli[m][d]);
+            if constexpr (OP == BEM_OP_V)
+                atomicAdd(p.grad + 3 * p.tris[3 * j + m] + d, cre(cr, ci, S0r, S0i) * dlj[m][d]);
+        }
+    if constexpr (OP == BEM_OP_K) {
+        const double a2j = 2.0 * p.area[j];
+        double e1[3], e2[3], t[3];
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = Ys[1][d] - Ys[0][d]; e2[d] = Ys[2][d] - Ys[0][d];
+            t[d] = cre(cr, ci, Tr[d], Ti[d]) / a2j;
+        }
+        const double c1[3] = {e2[1] * t[2] - e2[2] * t[1], e2[2] * t[0] - e2[0] * t[2], e2[0] * t[1] - e2[1] * t[0]};
+        const double c2[3] = {t[1] * e1[2] - t[2] * e1[1], t[2] * e1[0] - t[0] * e1[2], t[0] * e1[1] - t[1] * e1[0]};
+        for (int d = 0; d < 3; ++d) {
+            atomicAdd(p.grad + 3 * p.tris[3 * j + 1] + d, c1[d]);
+            atomicAdd(p.grad + 3 * p.tris[3 * j + 2] + d, c2[d]);
+            atomicAdd(p.grad + 3 * p.tris[3 * j + 0] + d, -(c1[d] + c2[d]));
+        }
+    }
+    if (p.gk) atomicAdd(p.gk, cre(cr, ci, SKr, SKi) * p.dkap);
+}
+
+// grad += sum_i coef_i dA_i/dX, one thread per panel.
+__global__ void area_grad_kernel(int nf, const int32_t* __restrict__ tris, const double* __restrict__ verts,
+                                 const double* __restrict__ area, const double* __restrict__ coef, double* grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const double c = coef[i];
+    if (c == 0.0) return;
+    double P[3][3], g[3][3];
+    for (int m = 0; m < 3; ++m)
+        for (int d = 0; d < 3; ++d) P[m][d] = verts[3 * tris[3 * i + m] + d];
+    dlog_area(P, g);
+    const double s = c * area[i];  // dA = A dlog(2A)
+    for (int m = 0; m < 3; ++m)
+        for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * tris[3 * i + m] + d, s * g[m][d]);
+}
+
+template <int OP>
+void launch_adjoint(const AdjParams& p, bool helm, int njb, hipStream_t s) {
+    const int nib = (p.nf + TI - 1) / TI;
+    const dim3 gf((unsigned)((int64_t)nib * njb));
+    if (helm) hipLaunchKernelGGL((adjoint_far_kernel<OP, true>), gf, dim3(256), 0, s, p, njb);
+    else hipLaunchKernelGGL((adjoint_far_kernel<OP, false>), gf, dim3(256), 0, s, p, njb);
+    if (p.n_near > 0) {
+        const dim3 gn((unsigned)((p.n_near + 3) / 4));
+        if (helm) hipLaunchKernelGGL((adjoint_near_kernel<OP, true>), gn, dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((adjoint_near_kernel<OP, false>), gn, dim3(256), 0, s, p);
+    }
+}
+
+}  // namespace
+
+int shape_gradient(Context* ctx, const bem_op_desc* d, const void* lam, const void* u, double ar, double ai,
+                   double* grad, double* grad_k, hipStream_t s) {
+    Mesh& m = ctx->mesh;
+    if (!m.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
+    if (!d || !lam || !u || !grad) return set_error(BEM_ERR_INVALID, "null argument");
+    if (d->kernel < BEM_LAPLACE || d->kernel > BEM_HELMHOLTZ_TRANSMISSION)
+        return set_error(BEM_ERR_INVALID, "Kernel Type only support Laplace and Helmholtz for now");
+    const bool v_op = d->op == BEM_OP_V && d->deg_test == 0 && d->deg_trial == 0;
+    const bool k_op = d->op == BEM_OP_K && d->deg_test == 0 && d->deg_trial == 1;
+    if (!v_op && !k_op)
+        return set_error(BEM_ERR_UNSUPPORTED, "shape gradient: V (P0 x P0) and K (P0 x P1) only");
+    if (d->dtype != BEM_C128) return set_error(BEM_ERR_INVALID, "shape gradient: coefficients must be complex128");
+    if (d->precision != BEM_PREC_FP64) return set_error(BEM_ERR_UNSUPPORTED, "shape gradient runs in FP64 only");
+    int rc = mesh_build_near(ctx, d->flags, s);
+    if (rc) return rc;
+    AdjParams p;
+    p.nf = m.nf; p.nv = m.nv; p.q2 = m.q2; p.flags = d->flags;
+    p.tris = m.tris; p.verts = m.verts; p.area = m.area; p.normal = m.normal;
+    p.pu = m.pu; p.pl = m.pl; p.phi = m.phi; p.near = m.near; p.rule = m.rule;
+    for (int r = 0; r < 4; ++r) { p.rule_off[r] = m.rule_off[r]; p.rule_len[r] = m.rule_len[r]; }
+    p.n_near = m.n_near;
+    const bool helm = d->kernel != BEM_LAPLACE;
+    p.kap = d->kernel == BEM_HELMHOLTZ_TRANSMISSION ? d->k * d->sqrt_n : (helm ? d->k : 0.0);
+    p.dkap = d->kernel == BEM_HELMHOLTZ_TRANSMISSION ? d->sqrt_n : 1.0;
+    p.lam = (const double*)lam; p.u = (const double*)u;
+    p.ar = ar * kInv4Pi; p.ai = ai * kInv4Pi;
+    p.grad = grad; p.gk = helm ? grad_k : nullptr;
+    const int njb = (m.nf + 255) / 256;
+    if (v_op) launch_adjoint<BEM_OP_V>(p, helm, njb, s);
+    else launch_adjoint<BEM_OP_K>(p, helm, njb, s);
+    return check_hip(hipGetLastError(), "shape gradient launch");
+}
+
+int area_gradient(Context* ctx, const double* coef, double* grad, hipStream_t s) {
+    Mesh& m = ctx->mesh;
+    if (!m.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
+    if (!coef || !grad) return set_error(BEM_ERR_INVALID, "null argument");
+    hipLaunchKernelGGL(area_grad_kernel, dim3((m.nf + 255) / 256), dim3(256), 0, s, m.nf, (const int32_t*)m.tris,
+                       (const double*)m.verts, (const double*)m.area, coef, grad);
+    return check_hip(hipGetLastError(), "area gradient launch");
+}
+
+}  // namespace bem

@@ -45,6 +45,8 @@ struct OpParams {
     int64_t n_near;
     double kap;   // wave number inside G
     double k2n;   // k^2 sqrt_n^2 of the hypersingular term
+    double dkap;  // d kap / d k        (BEM_FLAG_DK products)
+    double dk2n;  // d k2n / d k = 2 k sqrt_n^2
     // output / vectors
     void* out;
     int64_t ld;
@@ -119,8 +121,12 @@ __device__ __forceinline__ bool shares_vertex(const int32_t* t, int i, int j) {
 //   K  dG/dn_y(x, y)              (double_layer_3d.py:158-180)
 //   K' dG/dn_y with x<->y, n_x    (adj_double_layer_3d.py:160-182)
 //   W  G [curl.curl - k^2 n phi phi n_x.n_y]   (hypersingular_layer_3d.py:186-211)
+// DK = true gives the derivative of the entries with respect to k instead:
+//   dG/dk = dkap i exp(i kap r) / (4 pi),
+//   d(dG/dn_y)/dk = dkap kap (x-y).n_y exp(i kap r) / (4 pi r),
+//   W: dG/dk curl.curl - (dk2n G + k2n dG/dk) phi phi n_x.n_y.
 // ---------------------------------------------------------------------------
-template <typename R, int OP, bool HELM, int DT, int DR>
+template <typename R, int OP, bool HELM, int DT, int DR, bool DK = false>
 __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double (&vr)[3][3],
                                          double (&vi)[3][3]) {
     constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
@@ -144,6 +150,7 @@ __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double
         nnx = -(R)p.normal[3 * i]; nny = -(R)p.normal[3 * i + 1]; nnz = -(R)p.normal[3 * i + 2];
     }
     const R kap = (R)p.kap;
+    const R dkap = (R)p.dkap, k2n = (R)p.k2n, dk2n = (R)p.dk2n;
     R ar[NS][NT], ai[NS][NT];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
@@ -159,6 +166,7 @@ __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double
         R pr[NS], pi[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) { pr[s] = 0; pi[s] = 0; }
+        R p1r = 0, p1i = 0;  // W with DK: sum of w dG/dk
         for (int a = 0; a < q2; ++a) {
             const R dx = PU[4 * a + 0] + ex;
             const R dy = PU[4 * a + 1] + ey;
@@ -167,7 +175,23 @@ __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double
             const R r2 = dx * dx + dy * dy + dz * dz;
             const R rinv = Math<R>::rsqrt(r2);
             R fr, fi;
-            if constexpr (OP == BEM_OP_V || OP == BEM_OP_W) {
+            if constexpr (DK && (OP == BEM_OP_V || OP == BEM_OP_W)) {
+                R sn, cs;
+                Math<R>::sincos(kap * (r2 * rinv), &sn, &cs);
+                const R f1r = -wa * sn * dkap, f1i = wa * cs * dkap;
+                if constexpr (OP == BEM_OP_W) {
+                    const R g = wa * rinv;
+                    fr = dk2n * g * cs + k2n * f1r;
+                    fi = dk2n * g * sn + k2n * f1i;
+                    p1r += f1r; p1i += f1i;
+                } else { fr = f1r; fi = f1i; }
+            } else if constexpr (DK) {
+                const R dn = dx * nnx + dy * nny + dz * nnz;
+                R sn, cs;
+                Math<R>::sincos(kap * (r2 * rinv), &sn, &cs);
+                const R base = wa * dn * rinv * kap * dkap;
+                fr = base * cs; fi = base * sn;
+            } else if constexpr (OP == BEM_OP_V || OP == BEM_OP_W) {
                 const R g = wa * rinv;
                 if constexpr (HELM) {
                     R sn, cs;
@@ -192,7 +216,10 @@ __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double
                 }
             } else { pr[0] += fr; pi[0] += fi; }
         }
-        if constexpr (OP == BEM_OP_W) {
+        if constexpr (OP == BEM_OP_W && DK) {
+            s0r += wb * p1r;
+            s0i += wb * p1i;
+        } else if constexpr (OP == BEM_OP_W) {
             // sum_a w_a G: the P1 bases sum to one, so it is the sum of the
             // basis-weighted partials
             R sr = 0, si = 0;
@@ -211,6 +238,7 @@ __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double
     if constexpr (OP == BEM_OP_W) {
         const double nxy = p.normal[3 * i] * p.normal[3 * j] + p.normal[3 * i + 1] * p.normal[3 * j + 1] +
                            p.normal[3 * i + 2] * p.normal[3 * j + 2];
+        const double kf = DK ? 1.0 : p.k2n;  // DK: the k^2 factor is inside the sums
 #pragma unroll
         for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -218,8 +246,8 @@ __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double
                 double cc = 0.0;
                 if (DT && DR)
                     for (int d = 0; d < 3; ++d) cc += p.curl[9 * i + 3 * s + d] * p.curl[9 * j + 3 * t + d];
-                vr[s][t] = kInv4Pi * (cc * (double)s0r - p.k2n * nxy * (double)ar[s][t]);
-                vi[s][t] = kInv4Pi * (cc * (double)s0i - p.k2n * nxy * (double)ai[s][t]);
+                vr[s][t] = kInv4Pi * (cc * (double)s0r - kf * nxy * (double)ar[s][t]);
+                vi[s][t] = kInv4Pi * (cc * (double)s0i - kf * nxy * (double)ai[s][t]);
             }
     } else {
 #pragma unroll
@@ -260,7 +288,7 @@ __global__ __launch_bounds__(256) void far_assemble_kernel(OpParams p, int njb) 
 // evaluates its second term f(y, x) with the same bases (reference,
 // single_layer_3d.py:198-206) unless BEM_FLAG_SYM_COINCIDE.
 // ---------------------------------------------------------------------------
-template <int OP, bool HELM, int DT, int DR>
+template <int OP, bool HELM, int DT, int DR, bool DK = false>
 __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np, int lane,
                                           double (&vr)[3][3], double (&vi)[3][3]) {
     constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
@@ -316,8 +344,23 @@ __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np,
             const double dx = sg * (x[0] - y[0]), dy = sg * (x[1] - y[1]), dz = sg * (x[2] - y[2]);
             const double r2 = dx * dx + dy * dy + dz * dz;
             const double rinv = rsqrt_f64(r2);
-            double fr, fi;
-            if constexpr (OP == BEM_OP_V || OP == BEM_OP_W) {
+            double fr, fi, f1r = 0.0, f1i = 0.0;
+            if constexpr (DK && (OP == BEM_OP_V || OP == BEM_OP_W)) {
+                double sn, cs;
+                sincos_f64(kap * (r2 * rinv), &sn, &cs);
+                f1r = -sn * p.dkap; f1i = cs * p.dkap;
+                if constexpr (OP == BEM_OP_W) {
+                    fr = p.dk2n * rinv * cs + p.k2n * f1r;
+                    fi = p.dk2n * rinv * sn + p.k2n * f1i;
+                } else { fr = f1r; fi = f1i; }
+            } else if constexpr (DK) {
+                const double dn = (OP == BEM_OP_K) ? (dx * ny0 + dy * ny1 + dz * ny2)
+                                                   : -(dx * nx0 + dy * nx1 + dz * nx2);
+                double sn, cs;
+                sincos_f64(kap * (r2 * rinv), &sn, &cs);
+                const double base = dn * rinv * kap * p.dkap;
+                fr = base * cs; fi = base * sn;
+            } else if constexpr (OP == BEM_OP_V || OP == BEM_OP_W) {
                 if constexpr (HELM) {
                     double sn, cs;
                     sincos_f64(kap * (r2 * rinv), &sn, &cs);
@@ -338,7 +381,8 @@ __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np,
             const bool swap_b = pass && sym;
             const double bx1 = swap_b ? q.r1y : q.r1x, bx2 = swap_b ? q.r2y : q.r2x;
             const double by1 = swap_b ? q.r1x : q.r1y, by2 = swap_b ? q.r2x : q.r2y;
-            if constexpr (OP == BEM_OP_W) { s0r += fr; s0i += fi; }
+            if constexpr (OP == BEM_OP_W && DK) { s0r += f1r * w; s0i += f1i * w; }
+            else if constexpr (OP == BEM_OP_W) { s0r += fr; s0i += fi; }
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 const double phx = DT ? shape(bx1, bx2, invx[s]) : 1.0;
@@ -357,6 +401,7 @@ __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np,
     if constexpr (OP == BEM_OP_W) {
         s0r = wave_sum(s0r); s0i = wave_sum(s0i);
         const double nxy = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+        const double kf = DK ? 1.0 : p.k2n;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -364,8 +409,8 @@ __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np,
                 double cc = 0.0;
                 if (DT && DR)
                     for (int d = 0; d < 3; ++d) cc += p.curl[9 * i + 3 * s + d] * p.curl[9 * j + 3 * t + d];
-                vr[s][t] = kInv4Pi * (cc * s0r - p.k2n * nxy * ar[s][t]);
-                vi[s][t] = kInv4Pi * (cc * s0i - p.k2n * nxy * ai[s][t]);
+                vr[s][t] = kInv4Pi * (cc * s0r - kf * nxy * ar[s][t]);
+                vi[s][t] = kInv4Pi * (cc * s0i - kf * nxy * ai[s][t]);
             }
     } else {
 #pragma unroll
@@ -399,7 +444,7 @@ __global__ __launch_bounds__(256) void near_assemble_kernel(OpParams p) {
 // ---------------------------------------------------------------------------
 // Matrix-free products y[test] += alpha Op xhat.
 // ---------------------------------------------------------------------------
-template <typename R, int OP, bool HELM, int DT, int DR>
+template <typename R, int OP, bool HELM, int DT, int DR, bool DK = false>
 __global__ __launch_bounds__(256) void far_apply_kernel(OpParams p) {
     const int i = blockIdx.x;
     constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
@@ -409,7 +454,7 @@ __global__ __launch_bounds__(256) void far_apply_kernel(OpParams p) {
     for (int j = threadIdx.x; j < p.nf; j += 256) {
         if (shares_vertex(p.tris, i, j)) continue;
         double vr[3][3], vi[3][3];
-        far_pair<R, OP, HELM, DT, DR>(p, i, j, vr, vi);
+        far_pair<R, OP, HELM, DT, DR, DK>(p, i, j, vr, vi);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             double xr, xi;
@@ -439,7 +484,7 @@ __global__ __launch_bounds__(256) void far_apply_kernel(OpParams p) {
     }
 }
 
-template <int OP, bool HELM, int DT, int DR>
+template <int OP, bool HELM, int DT, int DR, bool DK = false>
 __global__ __launch_bounds__(256) void near_apply_kernel(OpParams p) {
     const int lane = threadIdx.x & 63;
     const int64_t pid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -447,7 +492,7 @@ __global__ __launch_bounds__(256) void near_apply_kernel(OpParams p) {
     const NearPair np = p.near[pid];
     constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
     double vr[3][3], vi[3][3];
-    near_pair<OP, HELM, DT, DR>(p, np, lane, vr, vi);
+    near_pair<OP, HELM, DT, DR, DK>(p, np, lane, vr, vi);
     if (lane != 0) return;
     for (int s = 0; s < NS; ++s) {
         double yr = 0.0, yi = 0.0;
@@ -541,10 +586,13 @@ OpParams make_params(Context* ctx, const bem_op_desc* d) {
     p.phi = m.phi; p.phi32 = m.phi32; p.near = m.near; p.near_row = m.near_row; p.rule = m.rule;
     for (int r = 0; r < 4; ++r) { p.rule_off[r] = m.rule_off[r]; p.rule_len[r] = m.rule_len[r]; }
     p.n_near = m.n_near;
-    p.kap = 0.0; p.k2n = 0.0;
+    p.kap = 0.0; p.k2n = 0.0; p.dkap = 0.0; p.dk2n = 0.0;
     if (d) {
-        p.kap = d->kernel == BEM_HELMHOLTZ_TRANSMISSION ? d->k * d->sqrt_n : d->k;
+        const bool tr = d->kernel == BEM_HELMHOLTZ_TRANSMISSION;
+        p.kap = tr ? d->k * d->sqrt_n : d->k;
         p.k2n = d->k * d->k * d->sqrt_n * d->sqrt_n;
+        p.dkap = tr ? d->sqrt_n : 1.0;
+        p.dk2n = 2.0 * d->k * d->sqrt_n * d->sqrt_n;
     }
     p.out = nullptr; p.ld = 0; p.dtype = BEM_C128; p.ar = 1.0; p.ai = 0.0;
     p.x = nullptr; p.conj_x = 0; p.dtype_x = BEM_C128; p.y = nullptr; p.near_buf = m.near_buf;
@@ -721,6 +769,37 @@ NearFn near_any(int op, bool helm, int dt, int dr, bool apply) {
     }
 }
 
+template <int OP>
+NearFn far_apply_dk_fn(int dt, int dr) {
+    if (dt == 0 && dr == 0) return far_apply_kernel<double, OP, true, 0, 0, true>;
+    if (dt == 0 && dr == 1) return far_apply_kernel<double, OP, true, 0, 1, true>;
+    if (dt == 1 && dr == 0) return far_apply_kernel<double, OP, true, 1, 0, true>;
+    return far_apply_kernel<double, OP, true, 1, 1, true>;
+}
+template <int OP>
+NearFn near_apply_dk_fn(int dt, int dr) {
+    if (dt == 0 && dr == 0) return near_apply_kernel<OP, true, 0, 0, true>;
+    if (dt == 0 && dr == 1) return near_apply_kernel<OP, true, 0, 1, true>;
+    if (dt == 1 && dr == 0) return near_apply_kernel<OP, true, 1, 0, true>;
+    return near_apply_kernel<OP, true, 1, 1, true>;
+}
+NearFn far_apply_dk_any(int op, int dt, int dr) {
+    switch (op) {
+        case BEM_OP_V: return far_apply_dk_fn<BEM_OP_V>(dt, dr);
+        case BEM_OP_K: return far_apply_dk_fn<BEM_OP_K>(dt, dr);
+        case BEM_OP_KADJ: return far_apply_dk_fn<BEM_OP_KADJ>(dt, dr);
+        default: return far_apply_dk_fn<BEM_OP_W>(dt, dr);
+    }
+}
+NearFn near_apply_dk_any(int op, int dt, int dr) {
+    switch (op) {
+        case BEM_OP_V: return near_apply_dk_fn<BEM_OP_V>(dt, dr);
+        case BEM_OP_K: return near_apply_dk_fn<BEM_OP_K>(dt, dr);
+        case BEM_OP_KADJ: return near_apply_dk_fn<BEM_OP_KADJ>(dt, dr);
+        default: return near_apply_dk_fn<BEM_OP_W>(dt, dr);
+    }
+}
+
 int validate(Context* ctx, const bem_op_desc* d) {
     if (!ctx->mesh.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
     if (!d) return set_error(BEM_ERR_INVALID, "null operator descriptor");
@@ -734,6 +813,8 @@ int validate(Context* ctx, const bem_op_desc* d) {
         return set_error(BEM_ERR_INVALID, "Helmholtz operators need a complex dtype");
     if (d->precision != BEM_PREC_FP64 && d->precision != BEM_PREC_FP32)
         return set_error(BEM_ERR_INVALID, "unknown precision");
+    if (d->flags & ~(BEM_FLAG_ROTATE_NEAR | BEM_FLAG_SYM_COINCIDE | BEM_FLAG_DK))
+        return set_error(BEM_ERR_INVALID, "unknown flags");
     return BEM_OK;
 }
 
@@ -744,6 +825,8 @@ int assemble_op(Context* ctx, const bem_op_desc* d, void* out, int64_t ld, doubl
     int rc = validate(ctx, d);
     if (rc) return rc;
     if (!out) return set_error(BEM_ERR_INVALID, "null output");
+    if (d->flags & BEM_FLAG_DK)
+        return set_error(BEM_ERR_UNSUPPORTED, "k-derivative operators are matrix-free only (bem_apply)");
     rc = mesh_build_near(ctx, d->flags, s);
     if (rc) return rc;
     OpParams p = make_params(ctx, d);
@@ -771,21 +854,27 @@ int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void
     int rc = validate(ctx, d);
     if (rc) return rc;
     if (!x || !y) return set_error(BEM_ERR_INVALID, "null vector");
+    const bool dk = (d->flags & BEM_FLAG_DK) != 0;
+    const bool helm = d->kernel != BEM_LAPLACE;
+    if (dk && d->precision != BEM_PREC_FP64)
+        return set_error(BEM_ERR_UNSUPPORTED, "k-derivative products run in FP64 only");
+    if (dk && !helm) return BEM_OK;  // the Laplace kernel does not depend on k
     rc = mesh_build_near(ctx, d->flags, s);
     if (rc) return rc;
     OpParams p = make_params(ctx, d);
     p.x = x; p.conj_x = conj_x; p.dtype_x = d->dtype; p.y = y; p.ar = ar; p.ai = ai;
-    const bool helm = d->kernel != BEM_LAPLACE;
     const int parts = d->parts ? d->parts : (BEM_PART_FAR | BEM_PART_NEAR);
     if (parts & BEM_PART_FAR) {
-        NearFn far = d->precision == BEM_PREC_FP32 ? far_apply_any<float>(d->op, helm, d->deg_test, d->deg_trial)
-                                                   : far_apply_any<double>(d->op, helm, d->deg_test, d->deg_trial);
+        NearFn far = dk ? far_apply_dk_any(d->op, d->deg_test, d->deg_trial)
+                        : d->precision == BEM_PREC_FP32 ? far_apply_any<float>(d->op, helm, d->deg_test, d->deg_trial)
+                                                        : far_apply_any<double>(d->op, helm, d->deg_test, d->deg_trial);
         hipLaunchKernelGGL(far, dim3(p.nf), dim3(256), 0, s, p);
         rc = check_hip(hipGetLastError(), "far-field apply launch");
         if (rc) return rc;
     }
     if (p.n_near > 0 && (parts & BEM_PART_NEAR)) {
-        NearFn nearf = near_any(d->op, helm, d->deg_test, d->deg_trial, true);
+        NearFn nearf = dk ? near_apply_dk_any(d->op, d->deg_test, d->deg_trial)
+                          : near_any(d->op, helm, d->deg_test, d->deg_trial, true);
         hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
         const int64_t nth = d->deg_test ? p.n_near : p.nf;
         hipLaunchKernelGGL(near_reduce_kernel, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, p,
@@ -803,6 +892,7 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
     if (!A || !g || !rhs) return set_error(BEM_ERR_INVALID, "null argument");
     if (d->op != BEM_OP_V || d->deg_test != 0 || d->deg_trial != 0)
         return set_error(BEM_ERR_INVALID, "the Dirichlet system takes the P0 single-layer descriptor");
+    if (d->flags & BEM_FLAG_DK) return set_error(BEM_ERR_UNSUPPORTED, "k-derivative: use bem_apply");
     if (dtype_rhs < BEM_F64 || dtype_rhs > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown rhs dtype");
     if (dtype_complex(d->dtype) != dtype_complex(dtype_rhs))
         return set_error(BEM_ERR_INVALID, "matrix and right-hand side must both be real or both complex");

@@ -168,6 +168,27 @@ int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_
     return with_device(ctx, [&]() -> int { return bem::lu_solve(&ctx->impl, LU, ld, n, dtype, ipiv, b, as_stream(stream)); });
 }
 
+int bem_lu_solve_transpose(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
+                           const int32_t* ipiv, void* b, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::lu_solve_transpose(&ctx->impl, LU, ld, n, dtype, ipiv, b, as_stream(stream));
+    });
+}
+
+int bem_shape_gradient(bem_context* ctx, const bem_op_desc* op, const void* lam, const void* u, double alpha_re,
+                       double alpha_im, double* grad_vertices, double* grad_k, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::shape_gradient(&ctx->impl, op, lam, u, alpha_re, alpha_im, grad_vertices, grad_k,
+                                   as_stream(stream));
+    });
+}
+
+int bem_area_gradient(bem_context* ctx, const double* coef, double* grad_vertices, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::area_gradient(&ctx->impl, coef, grad_vertices, as_stream(stream));
+    });
+}
+
 int bem_residual(bem_context* ctx, const void* A, int64_t ld, int32_t n, int32_t dtype, const void* x,
                  const void* b, void* r, void* stream) {
     return with_device(ctx, [&]() -> int {

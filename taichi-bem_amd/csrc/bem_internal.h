@@ -95,10 +95,17 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
 int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void* x, int conj_x,
                void* y, double alpha, hipStream_t s);
 
+// adjoint contractions (bem_adjoint.hip)
+int shape_gradient(Context* ctx, const bem_op_desc* d, const void* lam, const void* u, double ar, double ai,
+                   double* grad, double* grad_k, hipStream_t s);
+int area_gradient(Context* ctx, const double* coef, double* grad, hipStream_t s);
+
 // dense linear algebra (bem_linalg.hip)
 int lu_factor(Context* ctx, void* A, int64_t ld, int n, int dtype, int32_t* ipiv, hipStream_t s);
 int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv,
              void* b, hipStream_t s);
+int lu_solve_transpose(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv,
+                       void* b, hipStream_t s);
 int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, const void* x,
                   const void* b, void* r, hipStream_t s);
 int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const void* LU, int64_t ldlu,

@@ -353,17 +353,21 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(T* Cm, int64_t ldc, const
 // ---------------------------------------------------------------------------
 constexpr int TB = 64;
 
-// x[blk] solve with the TB x TB diagonal block (lower unit or upper): the
-// block is staged in LDS first so the sequential sweep touches LDS only.
-template <typename T, bool C, bool LOWER>
+// x[blk] solve with the TB x TB diagonal block: the block is staged in LDS
+// first so the sequential sweep touches LDS only.  Without TRANS the lower
+// factor is L (unit) and the upper U; with TRANS they are U^T (lower,
+// non-unit) and L^T (upper, unit), read from the same storage.
+template <typename T, bool C, bool LOWER, bool TRANS = false>
 __global__ __launch_bounds__(64) void trsv_diag_kernel(const T* A, int64_t ld, int n, int r0, double* x) {
+    constexpr bool UNIT = LOWER != TRANS;
     __shared__ double xs[TB][2];
     __shared__ T blk[TB][TB][C ? 2 : 1];
     const int nb = min(TB, n - r0);
     const int t = threadIdx.x;
     for (int r = 0; r < nb; ++r)
         if (t < nb) {
-            cx<T> v = Acc<T, C>::ld(A, (int64_t)(r0 + r) * ld + (r0 + t));
+            cx<T> v = TRANS ? Acc<T, C>::ld(A, (int64_t)(r0 + t) * ld + (r0 + r))
+                            : Acc<T, C>::ld(A, (int64_t)(r0 + r) * ld + (r0 + t));
             blk[r][t][0] = v.re;
             if constexpr (C) blk[r][t][C ? 1 : 0] = v.im;
         }
@@ -373,7 +377,7 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const T* A, int64_t ld, i
         return {(double)blk[r][c][0], C ? (double)blk[r][c][C ? 1 : 0] : 0.0};
     };
     if (LOWER) {
-        for (int r = 1; r < nb; ++r) {
+        for (int r = UNIT ? 1 : 0; r < nb; ++r) {  // a non-unit row 0 still divides
             double sr = 0.0, si = 0.0;
             if (t < r) {
                 cx<double> l = ldb(r, t);
@@ -381,7 +385,14 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const T* A, int64_t ld, i
                 si = l.re * xs[t][1] + l.im * xs[t][0];
             }
             sr = wave_sum(sr); si = wave_sum(si);
-            if (t == 0) { xs[r][0] -= sr; xs[r][1] -= si; }
+            if (t == 0) {
+                if (UNIT) { xs[r][0] -= sr; xs[r][1] -= si; }
+                else {
+                    cx<double> num = {xs[r][0] - sr, xs[r][1] - si};
+                    cx<double> q = cdiv(num, ldb(r, r));
+                    xs[r][0] = q.re; xs[r][1] = q.im;
+                }
+            }
             __syncthreads();
         }
     } else {
@@ -394,9 +405,12 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const T* A, int64_t ld, i
             }
             sr = wave_sum(sr); si = wave_sum(si);
             if (t == 0) {
-                cx<double> num = {xs[r][0] - sr, xs[r][1] - si};
-                cx<double> q = cdiv(num, ldb(r, r));
-                xs[r][0] = q.re; xs[r][1] = q.im;
+                if (UNIT) { xs[r][0] -= sr; xs[r][1] -= si; }
+                else {
+                    cx<double> num = {xs[r][0] - sr, xs[r][1] - si};
+                    cx<double> q = cdiv(num, ldb(r, r));
+                    xs[r][0] = q.re; xs[r][1] = q.im;
+                }
             }
             __syncthreads();
         }
@@ -420,6 +434,29 @@ __global__ __launch_bounds__(256) void trsv_update_kernel(const T* A, int64_t ld
     }
     sr = wave_sum(sr); si = wave_sum(si);
     if (lane == 0) { x[2 * r] -= sr; x[2 * r + 1] -= si; }
+}
+
+// Transposed block update x[rows] -= A[c0:c0+nb, rows]^T x[c0:c0+nb]: one
+// thread per row, so the column-block reads A[c][r] are coalesced over r.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void trsv_update_t_kernel(const T* A, int64_t ld, int row_begin,
+                                                            int row_end, int c0, int nb, double* x) {
+    __shared__ double xs[TB][2];
+    if (threadIdx.x < nb) {
+        xs[threadIdx.x][0] = x[2 * (c0 + threadIdx.x)];
+        xs[threadIdx.x][1] = x[2 * (c0 + threadIdx.x) + 1];
+    }
+    __syncthreads();
+    const int r = row_begin + blockIdx.x * 256 + threadIdx.x;
+    if (r >= row_end) return;
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < nb; ++c) {
+        cx<T> a = Acc<T, C>::ld(A, (int64_t)(c0 + c) * ld + r);
+        sr += (double)a.re * xs[c][0] - (double)a.im * xs[c][1];
+        si += (double)a.re * xs[c][1] + (double)a.im * xs[c][0];
+    }
+    x[2 * r] -= sr;
+    x[2 * r + 1] -= si;
 }
 
 // r = b - A x in FP64 (A of T/C, x/b/r as complex128 work vectors): one wave
@@ -447,6 +484,13 @@ __global__ void permute_kernel(const double* x, const int32_t* perm, int n, doub
     if (i >= n) return;
     y[2 * i] = x[2 * perm[i]];
     y[2 * i + 1] = x[2 * perm[i] + 1];
+}
+// scatter y[perm[i]] = x[i] (the inverse permutation)
+__global__ void unpermute_kernel(const double* x, const int32_t* perm, int n, double* y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[2 * perm[i]] = x[2 * i];
+    y[2 * perm[i] + 1] = x[2 * i + 1];
 }
 
 // complex128 work vector <-> caller vector of dtype
@@ -621,6 +665,35 @@ void tri_solve_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
     }
 }
 
+// Solve U^T L^T z = w in place (the transposed factors of the same storage).
+template <typename T, bool C>
+void tri_solve_trans_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
+    for (int r0 = 0; r0 < n; r0 += TB) {
+        const int nb = min(TB, n - r0);
+        hipLaunchKernelGGL((trsv_diag_kernel<T, C, true, true>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
+        if (r0 + nb < n)
+            hipLaunchKernelGGL((trsv_update_t_kernel<T, C>), dim3((n - r0 - nb + 255) / 256), dim3(256), 0, s, LU,
+                               ld, r0 + nb, n, r0, nb, w);
+    }
+    const int last = ((n - 1) / TB) * TB;
+    for (int r0 = last; r0 >= 0; r0 -= TB) {
+        const int nb = min(TB, n - r0);
+        hipLaunchKernelGGL((trsv_diag_kernel<T, C, false, true>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
+        if (r0 > 0)
+            hipLaunchKernelGGL((trsv_update_t_kernel<T, C>), dim3((r0 + 255) / 256), dim3(256), 0, s, LU, ld, 0, r0,
+                               r0, nb, w);
+    }
+}
+
+void tri_solve_trans(const void* LU, int64_t ld, int n, int dtype, double* w, hipStream_t s) {
+    switch (dtype) {
+        case BEM_F64: tri_solve_trans_t<double, false>((const double*)LU, ld, n, w, s); break;
+        case BEM_C128: tri_solve_trans_t<double, true>((const double*)LU, ld, n, w, s); break;
+        case BEM_F32: tri_solve_trans_t<float, false>((const float*)LU, ld, n, w, s); break;
+        default: tri_solve_trans_t<float, true>((const float*)LU, ld, n, w, s); break;
+    }
+}
+
 void tri_solve(const void* LU, int64_t ld, int n, int dtype, double* w, hipStream_t s) {
     switch (dtype) {
         case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, s); break;
@@ -704,6 +777,28 @@ int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const i
     tri_solve(LU, ld, n, dtype, w1, s);
     hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
     return check_hip(hipGetLastError(), "LU solve");
+}
+
+// A^T x = b (plain transpose, no conjugation): with P A = L U,
+// A^T = U^T L^T P, so z = L^-T U^-T b and x = P^T z.
+int lu_solve_transpose(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv, void* b,
+                       hipStream_t s) {
+    if (!LU || !ipiv || !b) return set_error(BEM_ERR_INVALID, "null argument");
+    if (n <= 0 || ld < n) return set_error(BEM_ERR_INVALID, "invalid matrix size");
+    if (!valid_dtype(dtype)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    int rc = ensure_work(ctx, (size_t)n * 16 * 2 + (size_t)n * 4);
+    if (rc) return rc;
+    double* w0 = (double*)ctx->work;
+    double* w1 = w0 + 2 * (size_t)n;
+    int32_t* perm = (int32_t*)(w1 + 2 * (size_t)n);
+    rc = make_perm(ipiv, n, perm, s);
+    if (rc) return rc;
+    dim3 g((n + 255) / 256);
+    hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, (const void*)b, dtype, n, w0);
+    tri_solve_trans(LU, ld, n, dtype, w0, s);
+    hipLaunchKernelGGL(unpermute_kernel, g, dim3(256), 0, s, (const double*)w0, (const int32_t*)perm, n, w1);
+    hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
+    return check_hip(hipGetLastError(), "transposed LU solve");
 }
 
 int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, const void* x,
