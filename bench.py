@@ -33,6 +33,13 @@ PEAK_FP32_TFLOPS = 157.3
 # d = x - y 3, r^2 5, r 1, kr 1, w/r 1, V accumulate 4, (x-y).n 5,
 # w (x-y).n / r^3 3, (cos + kr sin, sin - kr cos) 4, K accumulate 4.
 FLOPS_PER_EVAL_FUSED = 31
+# HBM bytes per launch of far_dirichlet_kernel<double, Helmholtz> on the L5
+# workload from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the
+# same counter on a 512 MiB copy (FETCH x 2.00, WRITE x 1.00):
+# (3.42451e6 * 2 + 6.60208e6) KiB.  Hardware FP64 VALU flops of that launch
+# (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes): 2.258e13.
+FAR_L5_TRAFFIC_BYTES = int((3.42451e6 * 2 + 6.60208e6) * 1024)
+FAR_L5_HW_FLOPS = 2.258e13
 
 
 def log(msg: str) -> None:
@@ -49,6 +56,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (the reference kernel that scales "
+                         "FETCH_SIZE/WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -61,6 +71,15 @@ def main() -> int:
     bem = importlib.import_module("taichi-bem_amd")
     L = bem._lib
     dev = torch.device("cuda", 0)
+    if args.calibrate_copy_mib > 0:
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        del src, dst
     k = args.k
     v, f = bem.meshes.perturbed_icosphere(args.level, seed=0)
     d = bem.meshes.seeded_direction(2)
@@ -128,8 +147,13 @@ def main() -> int:
     achieved = far_flops / (mean["far_ms"] * 1e-3) / 1e12
     roofline = {"bound": "valu", "kernel": "far_dirichlet_kernel<double,Helmholtz> (V + K conj(g))",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                "traffic": FAR_L5_TRAFFIC_BYTES if args.level == 5 else None,
                 "flops_per_launch": far_flops, "launch_ms": round(mean["far_ms"], 3)}
+    if args.level == 5:
+        # hardware VALU rate (counters), beside the algorithmic one above
+        roofline["hw_valu_tflops"] = round(FAR_L5_HW_FLOPS / (mean["far_ms"] * 1e-3) / 1e12, 3)
+        roofline["hw_valu_frac"] = round(roofline["hw_valu_tflops"] / PEAK_FP64_TFLOPS, 4)
 
     result = {
         "metric": "BIO entries/s assembled; time-to-solution at 1e-8 rel err, mixed vs FP64 GPU",
