@@ -210,6 +210,77 @@ int bem_lu_factor(bem_context* ctx, void* A, int64_t ld, int32_t n, int32_t dtyp
 int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
                  const int32_t* ipiv, void* b, void* stream);
 
+/* ---- batches of uneven sizes (wave-number sweeps, several meshes) ---- */
+
+/* One problem of a combined-field batch. */
+typedef struct {
+    bem_context* ctx;  /* context holding this problem's mesh (bem_set_mesh) */
+    double k;          /* wave number */
+    double eta;        /* single-layer coupling (Brakhage-Werner: eta = k) */
+    void* A;           /* device, nv x nv of the batch dtype, row stride ld */
+    int64_t ld;
+} bem_cf_problem;
+
+/* Launch shape of the persistent far-field kernel (chosen by an autotuner;
+ * zeros select the defaults). */
+typedef struct {
+    int32_t test_panels_per_block;  /* 1, 2, 4 or 8 (default 2) */
+    int32_t blocks_per_cu;          /* grid = blocks_per_cu x compute units (default 4) */
+} bem_tuning;
+
+/* A_q += 1/2 M + K - i eta_q V (P1 x P1) for every problem of the batch: the
+ * exterior Dirichlet (sound-soft) combined-field operator, built from the
+ * reference's K and V blocks (DoubleLayer3d.forward, double_layer_3d.py:
+ * 376-438; SingleLayer3d.forward, single_layer_3d.py:371-433; mass,
+ * identity_layer_3d.py:117-146); the reference has no combined-field
+ * operator of its own.  Meshes and wave numbers may differ per problem; the
+ * whole batch runs in three persistent launches (regular pairs, near-field
+ * pairs, mass) over one work list.  dtype BEM_C64 or BEM_C128; precision
+ * BEM_PREC_FP32 evaluates the regular pairs in FP32 (near field FP64).  The
+ * batch context `ctx` owns the work-list table (uploaded, with a stream
+ * synchronisation, only when it changes: repeated identical calls may be
+ * captured in a HIP graph). */
+int bem_assemble_combined_batched(bem_context* ctx, int32_t count, const bem_cf_problem* problems,
+                                  int32_t dtype, int32_t precision, int32_t flags,
+                                  const bem_tuning* tuning, void* stream);
+
+/* Factors `count` matrices in one launch sequence: A[q] (device, n[q] x
+ * n[q], row stride ld[q], all of `dtype`) in place, pivots into ipiv[q]
+ * (device) and info[q] (device int32 array of `count`: 0, or 1 + the first
+ * exactly-zero pivot column).  Problems of different n share every panel
+ * step.  The host arrays are read during the call.  The first call with a
+ * given set of arguments uploads a problem table and synchronises the
+ * stream; a repeated identical call only launches kernels and may be
+ * captured in a HIP graph.  Batched form of bem_lu_factor (the reference
+ * solves one system per call with torch.linalg.solve, BEM_manager.py:1056,
+ * inside its Python wave-number loop, demo/demo_3d_plot_Sio.py:67-76). */
+int bem_lu_factor_batched(bem_context* ctx, int32_t count, void* const* A, const int64_t* ld,
+                          const int32_t* n, int32_t dtype, int32_t* const* ipiv, int32_t* info,
+                          void* stream);
+
+/* Solves A_q x_q = b_q for the factors of bem_lu_factor_batched (LU[q],
+ * ipiv[q], dtype), then runs refine_iters steps of x += LU^-1 (b - A x)
+ * with residuals accumulated in FP64 against A[q] (same dtype as the LU).
+ * b[q], x[q]: complex128 device vectors (float64 for real dtypes).  resid
+ * (device, count doubles, may be NULL) gets ||b - A x|| / ||b||.  A/lda may
+ * be NULL when refine_iters = 0 and resid = NULL.  Capturable like
+ * bem_lu_factor_batched. */
+int bem_batch_solve(bem_context* ctx, int32_t count, const void* const* LU, const int64_t* ld,
+                    const int32_t* n, int32_t dtype, const int32_t* const* ipiv,
+                    const void* const* A, const int64_t* lda, const void* const* b, void* const* x,
+                    int32_t refine_iters, double* resid, void* stream);
+
+/* 1-norm condition estimates: out[2q] = ||A_q||_1 (exact) and out[2q+1] = a
+ * lower-bound estimate of ||A_q^-1||_1 from `iters` Hager-Higham steps on
+ * the factors (solves with A and A^H) plus the alternating-sign test
+ * vector.  out: device, 2 * count doubles.  The reference's norm analysis
+ * (BEMManager.get_mat_A1_inv_norm, BEM_manager.py:874-936) takes a dense
+ * SVD per wave number; this is the factor-based estimate for the sweep. */
+int bem_batch_condest(bem_context* ctx, int32_t count, const void* const* LU, const int64_t* ld,
+                      const int32_t* n, int32_t dtype, const int32_t* const* ipiv,
+                      const void* const* A, const int64_t* lda, int32_t iters, double* out,
+                      void* stream);
+
 /* Solves A^T x = b (transpose, not conjugate transpose) in place of b from
  * the same bem_lu_factor output: the adjoint solve of the gradient below.
  * No reference counterpart (the reference has no derivatives). */

@@ -2,7 +2,7 @@
 dense solve of GeCao/taichi-BEM, with the reference's manager/layer class
 names over hand-written HIP kernels (libbem_hip.so, C ABI in
 include/bem_hip.h)."""
-from . import _lib, adjoint, meshes
+from . import _lib, adjoint, meshes, sweep
 from ._lib import BEMError, Context, SingularMatrixError, build, convert, matrix_axpy
 from .bem_functions.utils import (AssembleType, CellFluxType, KernelType, PanelsRelation, ScopeType,
                                   VertAttachType, get_gaussion_integration_points_and_weights,
@@ -10,7 +10,7 @@ from .bem_functions.utils import (AssembleType, CellFluxType, KernelType, Panels
 from .managers import BEMManager, CoreManager, MeshManager
 
 __all__ = [
-    "_lib", "adjoint", "meshes", "BEMError", "Context", "SingularMatrixError", "build", "convert", "matrix_axpy",
+    "_lib", "adjoint", "meshes", "sweep", "BEMError", "Context", "SingularMatrixError", "build", "convert", "matrix_axpy",
     "AssembleType", "CellFluxType", "KernelType", "PanelsRelation", "ScopeType", "VertAttachType",
     "get_gaussion_integration_points_and_weights", "warp_tensor", "unwarp_tensor",
     "BEMManager", "CoreManager", "MeshManager",

@@ -29,7 +29,8 @@ EXPORTS = (
     "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_assemble_dirichlet", "bem_apply",
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
-    "bem_shape_gradient", "bem_area_gradient",
+    "bem_shape_gradient", "bem_area_gradient", "bem_assemble_combined_batched", "bem_lu_factor_batched",
+    "bem_batch_solve", "bem_batch_condest",
 )
 
 
@@ -49,6 +50,15 @@ class BemRefineStats(ctypes.Structure):
         ("residual_norm0", ctypes.c_double), ("residual_norm", ctypes.c_double),
         ("b_norm", ctypes.c_double),
     ]
+
+
+class BemCfProblem(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("k", ctypes.c_double), ("eta", ctypes.c_double),
+                ("A", ctypes.c_void_p), ("ld", ctypes.c_int64)]
+
+
+class BemTuning(ctypes.Structure):
+    _fields_ = [("test_panels_per_block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32)]
 
 
 class BEMError(RuntimeError):
@@ -95,6 +105,16 @@ def lib() -> ctypes.CDLL:
             "bem_lu_solve_transpose": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
             "bem_shape_gradient": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, vp, dbl, dbl, vp, vp, vp]),
             "bem_area_gradient": (ctypes.c_int, [vp, vp, vp, vp]),
+            "bem_assemble_combined_batched": (ctypes.c_int, [vp, i32, ctypes.POINTER(BemCfProblem), i32, i32, i32,
+                                                             ctypes.POINTER(BemTuning), vp]),
+            "bem_lu_factor_batched": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(i64),
+                                                     ctypes.POINTER(i32), i32, ctypes.POINTER(vp), vp, vp]),
+            "bem_batch_solve": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(i64), ctypes.POINTER(i32),
+                                               i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64),
+                                               ctypes.POINTER(vp), ctypes.POINTER(vp), i32, vp, vp]),
+            "bem_batch_condest": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(i64),
+                                                 ctypes.POINTER(i32), i32, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                                 ctypes.POINTER(i64), i32, vp, vp]),
             "bem_residual": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp, vp]),
             "bem_refine_solve": (ctypes.c_int, [vp, vp, i64, i32, vp, i64, i32, i32, vp, vp, vp, i32, dbl,
                                                 ctypes.POINTER(BemRefineStats), vp]),
@@ -305,6 +325,66 @@ class Context:
                                      b.data_ptr(), x.data_ptr(), int(max_iter), float(tol), ctypes.byref(st),
                                      current_stream()))
         return st
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[None if t is None else t.data_ptr() for t in ts])
+
+
+def _i64s(xs):
+    return (ctypes.c_int64 * len(xs))(*[int(x) for x in xs])
+
+
+def _i32s(xs):
+    return (ctypes.c_int32 * len(xs))(*[int(x) for x in xs])
+
+
+class Batch:
+    """Batched entry points over a list of square device matrices (uneven
+    sizes allowed).  Tables are cached in the owning context, so a repeated
+    call with the same tensors can run inside a HIP graph capture."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def assemble_combined(self, problems, dtype_code_: int, precision: int = PREC_FP32, flags: int = 0,
+                          tuning: tuple[int, int] | None = None) -> None:
+        """problems: list of (Context, k, eta, A) with A an (nv, nv) complex
+        device tensor: A += 1/2 M + K - i eta V."""
+        arr = (BemCfProblem * len(problems))()
+        for q, (c, k, eta, A) in enumerate(problems):
+            _require_cuda(A)
+            arr[q] = BemCfProblem(c.handle.value, float(k), float(eta), A.data_ptr(), int(A.stride(0)))
+        tn = BemTuning(*(tuning or (0, 0)))
+        check(lib().bem_assemble_combined_batched(self.ctx.handle, len(problems), arr, int(dtype_code_),
+                                                  int(precision), int(flags), ctypes.byref(tn), current_stream()))
+
+    def lu_factor(self, mats, ipivs, info) -> None:
+        _require_cuda(*mats, *ipivs, info)
+        self._keep = (_ptrs(mats), _i64s([m.stride(0) for m in mats]), _i32s([m.shape[0] for m in mats]),
+                      _ptrs(ipivs))
+        a, ld, n, ip = self._keep
+        check(lib().bem_lu_factor_batched(self.ctx.handle, len(mats), a, ld, n, dtype_code(mats[0]), ip,
+                                          info.data_ptr(), current_stream()))
+
+    def solve(self, lus, ipivs, mats, bs, xs, refine_iters: int = 0, resid=None) -> None:
+        """x_q = A_q^-1 b_q from the factors, then refine_iters FP64-residual
+        refinement steps against mats (same dtype as the factors; may be None
+        without refinement or residuals).  b, x complex128 (float64 for real)."""
+        _require_cuda(*lus, *ipivs, *bs, *xs)
+        A = _ptrs(mats) if mats is not None else None
+        lda = _i64s([m.stride(0) for m in mats]) if mats is not None else None
+        check(lib().bem_batch_solve(self.ctx.handle, len(lus), _ptrs(lus), _i64s([m.stride(0) for m in lus]),
+                                    _i32s([m.shape[0] for m in lus]), dtype_code(lus[0]), _ptrs(ipivs), A, lda,
+                                    _ptrs(bs), _ptrs(xs), int(refine_iters), _ptr(resid), current_stream()))
+
+    def condest(self, lus, ipivs, mats, out, iters: int = 3) -> None:
+        """out [count, 2] float64: ||A||_1 and an estimate of ||A^-1||_1."""
+        _require_cuda(*lus, *ipivs, *mats, out)
+        check(lib().bem_batch_condest(self.ctx.handle, len(lus), _ptrs(lus), _i64s([m.stride(0) for m in lus]),
+                                      _i32s([m.shape[0] for m in lus]), dtype_code(lus[0]), _ptrs(ipivs),
+                                      _ptrs(mats), _i64s([m.stride(0) for m in mats]), int(iters), out.data_ptr(),
+                                      current_stream()))
 
 
 def convert(src, dst) -> None:

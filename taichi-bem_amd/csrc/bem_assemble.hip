@@ -17,6 +17,10 @@
 // Output: out[row * ld + col] += alpha * value with row/col the test/trial
 // dof (panel for P0, vertex for P1).  P1 rows or columns collect from several
 // panels and use atomics; P0 x P0 entries have a single writer.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "bem_device.h"
 #include "bem_internal.h"
 
@@ -698,6 +702,175 @@ __global__ void kpart_reduce_kernel(const double* kpart, int nf, int njb, void* 
     add_elem(y, dtype, i, ar * a - ai * b, ar * b + ai * a, false);
 }
 
+// ---------------------------------------------------------------------------
+// Combined-field operator of the exterior Dirichlet problem,
+//   A = 1/2 M + K - i eta V      (P1 x P1; Brakhage-Werner, eta = k usually),
+// for a batch of problems (meshes of different sizes, wave numbers) in
+// persistent launches: a fixed grid walks one work list spanning every
+// problem.  Far items are (problem, TI test panels, 256 trial panels); V and
+// K share each point-pair evaluation.  Near items are the near-field pairs
+// of every problem (FP64, as in assembly), mass items the panels.  P1 rows and
+// columns collect from several panels through atomics.
+// ---------------------------------------------------------------------------
+struct CfProb {
+    OpParams p;          // the problem's mesh tables, kap = k, out = A
+    const double* mass_w;
+    double eta;
+    int64_t far_begin;   // first far item of the problem in the work list
+    int64_t near_begin;  // first near item
+    int64_t mass_begin;  // first mass item
+    int nib, njb;        // far tiling: test-panel groups x trial blocks
+};
+
+__device__ __forceinline__ int cf_find(const CfProb* tbl, int count, int64_t item, int which) {
+    // last problem whose begin <= item (begins are non-decreasing)
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t b = which == 0 ? tbl[mid].far_begin : (which == 1 ? tbl[mid].near_begin : tbl[mid].mass_begin);
+        if (b <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// V and K of a regular P1 x P1 pair from one evaluation per point pair.
+template <typename R, bool HELM>
+__device__ __forceinline__ void far_pair_vk(const OpParams& p, int i, int j, double (&Vr)[3][3], double (&Vi)[3][3],
+                                            double (&Kr)[3][3], double (&Ki)[3][3]) {
+    const int q2 = p.q2, nf = p.nf;
+    const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+    const R* __restrict__ PL = Ptr<R>::pl(p);
+    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    R dcx = 0, dcy = 0, dcz = 0;
+    if constexpr (sizeof(R) == 4) {
+        dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
+        dcy = (R)(p.cen[3 * i + 1] - p.cen[3 * j + 1]);
+        dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
+    }
+    const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
+    const R kap = (R)p.kap;
+    R avr[3][3], avi[3][3], akr[3][3], aki[3][3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int t = 0; t < 3; ++t) { avr[s][t] = avi[s][t] = akr[s][t] = aki[s][t] = 0; }
+    for (int b = 0; b < q2; ++b) {
+        const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
+        const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
+        const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
+        const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
+        R pvr[3], pvi[3], pkr[3], pki[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) { pvr[s] = pvi[s] = pkr[s] = pki[s] = 0; }
+        for (int a = 0; a < q2; ++a) {
+            const R dx = PU[4 * a + 0] + ex;
+            const R dy = PU[4 * a + 1] + ey;
+            const R dz = PU[4 * a + 2] + ez;
+            const R wa = PU[4 * a + 3];
+            const R r2 = dx * dx + dy * dy + dz * dz;
+            const R rinv = Math<R>::rsqrt(r2);
+            const R g = wa * rinv;
+            const R base = g * (dx * nnx + dy * nny + dz * nnz) * (rinv * rinv);
+            R fvr, fvi, fkr, fki;
+            if constexpr (HELM) {
+                const R kr = kap * (r2 * rinv);
+                R sn, cs;
+                Math<R>::sincos(kr, &sn, &cs);
+                fvr = g * cs; fvi = g * sn;
+                fkr = base * (cs + kr * sn); fki = base * (sn - kr * cs);
+            } else { fvr = g; fvi = 0; fkr = base; fki = 0; }
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const R ph = PHI[3 * a + s];
+                pvr[s] += fvr * ph; pvi[s] += fvi * ph;
+                pkr[s] += fkr * ph; pki[s] += fki * ph;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const R wt = wb * PHI[3 * b + t];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                avr[s][t] += pvr[s] * wt; avi[s][t] += pvi[s] * wt;
+                akr[s][t] += pkr[s] * wt; aki[s][t] += pki[s] * wt;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            Vr[s][t] = kInv4Pi * (double)avr[s][t]; Vi[s][t] = kInv4Pi * (double)avi[s][t];
+            Kr[s][t] = kInv4Pi * (double)akr[s][t]; Ki[s][t] = kInv4Pi * (double)aki[s][t];
+        }
+}
+
+// K - i eta V into A at (tri_i[s], tri_j[t])
+__device__ __forceinline__ void cf_scatter(const OpParams& p, double eta, int i, int j, const double (&Vr)[3][3],
+                                           const double (&Vi)[3][3], const double (&Kr)[3][3],
+                                           const double (&Ki)[3][3]) {
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int64_t row = p.tris[3 * i + s];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int64_t col = p.tris[3 * j + t];
+            // -i eta (Vr + i Vi) = eta Vi - i eta Vr
+            add_elem(p.out, p.dtype, row * p.ld + col, Kr[s][t] + eta * Vi[s][t], Ki[s][t] - eta * Vr[s][t], true);
+        }
+    }
+}
+
+template <typename R, int TI>
+__global__ __launch_bounds__(256) void cf_far_kernel(const CfProb* tbl, int count, int64_t total) {
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int q = cf_find(tbl, count, item, 0);
+        const CfProb& P = tbl[q];
+        const OpParams& p = P.p;
+        const int64_t local = item - P.far_begin;
+        const int ib = (int)(local / P.njb), jb = (int)(local % P.njb);
+        const int j = jb * 256 + threadIdx.x;
+        if (j >= p.nf) continue;
+        const int i_end = min(p.nf, (ib + 1) * TI);
+        for (int i = ib * TI; i < i_end; ++i) {
+            if (shares_vertex(p.tris, i, j)) continue;
+            double Vr[3][3], Vi[3][3], Kr[3][3], Ki[3][3];
+            far_pair_vk<R, true>(p, i, j, Vr, Vi, Kr, Ki);
+            cf_scatter(p, P.eta, i, j, Vr, Vi, Kr, Ki);
+        }
+    }
+}
+
+// near items: one wave per near pair (V and K P1 x P1, FP64)
+__global__ __launch_bounds__(256) void cf_near_kernel(const CfProb* tbl, int count, int64_t total) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); item < total; item += (int64_t)gridDim.x * 4) {
+        const int q = cf_find(tbl, count, item, 1);
+        const CfProb& P = tbl[q];
+        const NearPair np = P.p.near[item - P.near_begin];
+        double Vr[3][3], Vi[3][3], Kr[3][3], Ki[3][3];
+        near_pair<BEM_OP_V, true, 1, 1>(P.p, np, lane, Vr, Vi);
+        near_pair<BEM_OP_K, true, 1, 1>(P.p, np, lane, Kr, Ki);
+        if (lane == 0) cf_scatter(P.p, P.eta, np.i, np.j, Vr, Vi, Kr, Ki);
+    }
+}
+
+// 1/2 M (P1 x P1), one thread per panel item
+__global__ void cf_mass_kernel(const CfProb* tbl, int count, int64_t total) {
+    for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < total;
+         item += (int64_t)gridDim.x * blockDim.x) {
+        const int q = cf_find(tbl, count, item, 2);
+        const CfProb& P = tbl[q];
+        const int i = (int)(item - P.mass_begin);
+        double m[3][3];
+        mass_block(P.p, P.mass_w, i, 1, 1, m);
+        for (int s = 0; s < 3; ++s)
+            for (int t = 0; t < 3; ++t)
+                add_elem(P.p.out, P.p.dtype, (int64_t)P.p.tris[3 * i + s] * P.p.ld + P.p.tris[3 * i + t],
+                         0.5 * m[s][t], 0.0, true);
+    }
+}
+
 // ------------------------- dispatch -----------------------------------------
 using FarFn = void (*)(OpParams, int);
 using NearFn = void (*)(OpParams);
@@ -965,6 +1138,87 @@ int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void*
     hipLaunchKernelGGL(mass_apply_kernel, dim3((p.nf + 255) / 256), dim3(256), 0, s, p,
                        ctx->mesh.mass_w, deg_test, deg_trial);
     return check_hip(hipGetLastError(), "mass apply launch");
+}
+
+int assemble_combined_batched(Context* ctx, int count, const CfInput* probs, int dtype, int precision,
+                              int flags, const bem_tuning* tune, hipStream_t s) {
+    if (count <= 0) return BEM_OK;
+    if (!probs) return set_error(BEM_ERR_INVALID, "null problem array");
+    if (dtype != BEM_C128 && dtype != BEM_C64)
+        return set_error(BEM_ERR_INVALID, "the combined-field operator is complex: dtype must be c128 or c64");
+    if (precision != BEM_PREC_FP64 && precision != BEM_PREC_FP32) return set_error(BEM_ERR_INVALID, "unknown precision");
+    if (flags & ~(BEM_FLAG_ROTATE_NEAR | BEM_FLAG_SYM_COINCIDE)) return set_error(BEM_ERR_INVALID, "unknown flags");
+    int ti = tune && tune->test_panels_per_block > 0 ? tune->test_panels_per_block : 2;
+    int bpc = tune && tune->blocks_per_cu > 0 ? tune->blocks_per_cu : 4;
+    if (ti != 1 && ti != 2 && ti != 4 && ti != 8)
+        return set_error(BEM_ERR_INVALID, "test_panels_per_block must be 1, 2, 4 or 8");
+    if (bpc > 64) return set_error(BEM_ERR_INVALID, "blocks_per_cu must be at most 64");
+    std::vector<CfProb> tbl(count);
+    int64_t nfar = 0, nnear = 0, nmass = 0;
+    for (int q = 0; q < count; ++q) {
+        const CfInput& pr = probs[q];
+        if (!pr.ctx || !pr.A) return set_error(BEM_ERR_INVALID, "null context or matrix in the batch");
+        Context* c = pr.ctx;
+        if (!c->mesh.nf) return set_error(BEM_ERR_INVALID, "a problem's context has no mesh");
+        if (pr.ld < c->mesh.nv) return set_error(BEM_ERR_INVALID, "matrix row stride below the vertex count");
+        int rc = mesh_build_near(c, flags, s);
+        if (rc) return rc;
+        bem_op_desc d{};
+        d.op = BEM_OP_V; d.kernel = BEM_HELMHOLTZ; d.k = pr.k; d.sqrt_n = 1.0; d.flags = flags;
+        CfProb& P = tbl[q];
+        P.p = make_params(c, &d);
+        P.p.out = pr.A; P.p.ld = pr.ld; P.p.dtype = dtype;
+        P.mass_w = c->mesh.mass_w;
+        P.eta = pr.eta;
+        P.nib = (c->mesh.nf + ti - 1) / ti;
+        P.njb = (c->mesh.nf + 255) / 256;
+        P.far_begin = nfar;
+        P.near_begin = nnear;
+        P.mass_begin = nmass;
+        nfar += (int64_t)P.nib * P.njb;
+        nnear += c->mesh.n_near;
+        nmass += c->mesh.nf;
+    }
+    const size_t tb = sizeof(CfProb) * count;
+    if (ctx->cf_bytes < tb) {
+        if (ctx->cf_ws) hipFree(ctx->cf_ws);
+        ctx->cf_ws = nullptr;
+        ctx->cf_bytes = 0;
+        ctx->cf_table.clear();
+        int rc = check_hip(hipMalloc(&ctx->cf_ws, tb), "combined-field table");
+        if (rc) return rc;
+        ctx->cf_bytes = tb;
+    }
+    if (ctx->cf_table.size() != tb || memcmp(ctx->cf_table.data(), tbl.data(), tb) != 0) {
+        int rc = check_hip(hipMemcpyAsync(ctx->cf_ws, tbl.data(), tb, hipMemcpyHostToDevice, s), "combined-field table");
+        if (rc) return rc;
+        rc = check_hip(hipStreamSynchronize(s), "combined-field table sync");
+        if (rc) return rc;
+        ctx->cf_table.assign((const char*)tbl.data(), (const char*)tbl.data() + tb);
+    }
+    if (ctx->num_cu <= 0) {
+        hipDeviceProp_t prop;
+        int rc = check_hip(hipGetDeviceProperties(&prop, ctx->device), "device properties");
+        if (rc) return rc;
+        ctx->num_cu = prop.multiProcessorCount;
+    }
+    const CfProb* d_tbl = (const CfProb*)ctx->cf_ws;
+    const int64_t grid_cap = (int64_t)ctx->num_cu * bpc;
+    auto grid_of = [&](int64_t items) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(items, grid_cap)); };
+    using CfFn = void (*)(const CfProb*, int, int64_t);
+    CfFn far = nullptr;
+    const bool f32 = precision == BEM_PREC_FP32;
+    switch (ti) {
+        case 1: far = f32 ? cf_far_kernel<float, 1> : cf_far_kernel<double, 1>; break;
+        case 2: far = f32 ? cf_far_kernel<float, 2> : cf_far_kernel<double, 2>; break;
+        case 4: far = f32 ? cf_far_kernel<float, 4> : cf_far_kernel<double, 4>; break;
+        default: far = f32 ? cf_far_kernel<float, 8> : cf_far_kernel<double, 8>; break;
+    }
+    hipLaunchKernelGGL(far, dim3(grid_of(nfar)), dim3(256), 0, s, d_tbl, count, nfar);
+    if (nnear > 0)
+        hipLaunchKernelGGL(cf_near_kernel, dim3(grid_of((nnear + 3) / 4)), dim3(256), 0, s, d_tbl, count, nnear);
+    hipLaunchKernelGGL(cf_mass_kernel, dim3(grid_of((nmass + 255) / 256)), dim3(256), 0, s, d_tbl, count, nmass);
+    return check_hip(hipGetLastError(), "combined-field launch");
 }
 
 }  // namespace bem

@@ -1,5 +1,6 @@
 // bem_capi.hip -- the extern "C" entry points declared in include/bem_hip.h.
 #include <string>
+#include <vector>
 
 #include "bem_internal.h"
 
@@ -81,6 +82,9 @@ int bem_destroy(bem_context* ctx) {
         if (ctx->impl.work) hipFree(ctx->impl.work);
         if (ctx->impl.counters) hipFree(ctx->impl.counters);
         if (ctx->impl.lu_scratch) hipFree(ctx->impl.lu_scratch);
+        for (void* w : ctx->impl.batch_ws)
+            if (w) hipFree(w);
+        if (ctx->impl.cf_ws) hipFree(ctx->impl.cf_ws);
         return BEM_OK;
     });
     delete ctx;
@@ -166,6 +170,46 @@ int bem_lu_factor(bem_context* ctx, void* A, int64_t ld, int32_t n, int32_t dtyp
 int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype, const int32_t* ipiv,
                  void* b, void* stream) {
     return with_device(ctx, [&]() -> int { return bem::lu_solve(&ctx->impl, LU, ld, n, dtype, ipiv, b, as_stream(stream)); });
+}
+
+int bem_assemble_combined_batched(bem_context* ctx, int32_t count, const bem_cf_problem* problems, int32_t dtype,
+                                  int32_t precision, int32_t flags, const bem_tuning* tuning, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        if (count > 0 && !problems) return set_error(BEM_ERR_INVALID, "null problem array");
+        std::vector<bem::CfInput> in(count > 0 ? count : 0);
+        for (int q = 0; q < count; ++q) {
+            if (!problems[q].ctx) return set_error(BEM_ERR_INVALID, "null context in the batch");
+            if (problems[q].ctx->impl.device != ctx->impl.device)
+                return set_error(BEM_ERR_INVALID, "every problem of a batch must be on the batch context's device");
+            in[q] = {&problems[q].ctx->impl, problems[q].k, problems[q].eta, problems[q].A, problems[q].ld};
+        }
+        return bem::assemble_combined_batched(&ctx->impl, count, in.data(), dtype, precision, flags, tuning,
+                                              as_stream(stream));
+    });
+}
+
+int bem_lu_factor_batched(bem_context* ctx, int32_t count, void* const* A, const int64_t* ld, const int32_t* n,
+                          int32_t dtype, int32_t* const* ipiv, int32_t* info, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::lu_factor_batched(&ctx->impl, count, A, ld, n, dtype, ipiv, info, as_stream(stream));
+    });
+}
+
+int bem_batch_solve(bem_context* ctx, int32_t count, const void* const* LU, const int64_t* ld, const int32_t* n,
+                    int32_t dtype, const int32_t* const* ipiv, const void* const* A, const int64_t* lda,
+                    const void* const* b, void* const* x, int32_t refine_iters, double* resid, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::batch_solve(&ctx->impl, count, LU, ld, n, dtype, ipiv, A, lda, b, x, refine_iters, resid,
+                                as_stream(stream));
+    });
+}
+
+int bem_batch_condest(bem_context* ctx, int32_t count, const void* const* LU, const int64_t* ld, const int32_t* n,
+                      int32_t dtype, const int32_t* const* ipiv, const void* const* A, const int64_t* lda,
+                      int32_t iters, double* out, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::batch_condest(&ctx->impl, count, LU, ld, n, dtype, ipiv, A, lda, iters, out, as_stream(stream));
+    });
 }
 
 int bem_lu_solve_transpose(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,

@@ -68,8 +68,17 @@ struct Context {
     size_t work_bytes = 0;
     int* counters = nullptr;      // small per-context flags
     size_t counters_len = 0;
-    void* lu_scratch = nullptr;   // cooperative-panel candidates, barrier words, inv(L11)
+    void* lu_scratch = nullptr;   // LU counters, pivot partials, S_L (per problem of a batch)
     size_t lu_scratch_bytes = 0;
+    std::vector<char> lu_table;   // host copy of the batch table last uploaded
+    void* lu_table_dev = nullptr; // its device address
+    void* batch_ws[2] = {nullptr, nullptr};  // batched solves / condition estimates:
+    size_t batch_bytes[2] = {0, 0};           // per-problem vectors + tables
+    std::vector<char> batch_tables[2];        // host copies of the tables last uploaded
+    void* cf_ws = nullptr;        // combined-field batch table
+    size_t cf_bytes = 0;
+    std::vector<char> cf_table;
+    int num_cu = 0;
 };
 
 // error helpers (bem_capi.hip)
@@ -92,6 +101,15 @@ int mass_assemble(Context* ctx, int deg_test, int deg_trial, int dtype, void* ou
 int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, double ar, double ai,
                        const void* g, int conj_g, void* rhs, int dtype_rhs, double alpha_k, double alpha_m,
                        hipStream_t s);
+// one problem of a combined-field batch (bem_cf_problem with the context resolved)
+struct CfInput {
+    Context* ctx;
+    double k, eta;
+    void* A;
+    int64_t ld;
+};
+int assemble_combined_batched(Context* ctx, int count, const CfInput* probs, int dtype, int precision,
+                              int flags, const bem_tuning* tune, hipStream_t s);
 int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void* x, int conj_x,
                void* y, double alpha, hipStream_t s);
 
@@ -102,6 +120,14 @@ int area_gradient(Context* ctx, const double* coef, double* grad, hipStream_t s)
 
 // dense linear algebra (bem_linalg.hip)
 int lu_factor(Context* ctx, void* A, int64_t ld, int n, int dtype, int32_t* ipiv, hipStream_t s);
+int lu_factor_batched(Context* ctx, int count, void* const* A, const int64_t* ld, const int32_t* n, int dtype,
+                      int32_t* const* ipiv, int32_t* info, hipStream_t s);
+int batch_solve(Context* ctx, int count, const void* const* LU, const int64_t* ld, const int32_t* n, int dtype,
+                const int32_t* const* ipiv, const void* const* A, const int64_t* lda, const void* const* b,
+                void* const* x, int refine_iters, double* resid, hipStream_t s);
+int batch_condest(Context* ctx, int count, const void* const* LU, const int64_t* ld, const int32_t* n, int dtype,
+                  const int32_t* const* ipiv, const void* const* A, const int64_t* lda, int iters, double* out,
+                  hipStream_t s);
 int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv,
              void* b, hipStream_t s);
 int lu_solve_transpose(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv,

@@ -14,6 +14,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "bem_device.h"
@@ -68,11 +69,39 @@ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
     return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
 
+// One factorisation of a (possibly batched) LU: the matrix and its
+// per-problem scratch.  Kernels take it by value (one problem) or from a
+// device table indexed by blockIdx.z (B = true: a batch of uneven sizes
+// stepping through the same panel index k together).
+struct LuProb {
+    void* A;
+    int64_t ld;
+    int n;
+    int32_t* ipiv;
+    int* info;
+    unsigned* counter;
+    double* pval;
+    int* pidx;
+    void* SL;
+};
+
+template <bool B>
+__device__ __forceinline__ LuProb get_prob(const LuProb& inl, const LuProb* tbl) {
+    if constexpr (B) return tbl[blockIdx.z];
+    else return inl;
+}
+
 // Row interchanges of the panel [k, kend) applied to columns outside it.
-template <typename T, bool C>
-__global__ void laswp_kernel(T* A, int64_t ld, int n, int k, int kend, const int32_t* ipiv) {
+template <typename T, bool C, bool B>
+__global__ void laswp_kernel(LuProb inl, const LuProb* tbl, int k) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    const int n = P.n;
+    const int32_t* ipiv = P.ipiv;
+    const int kend = min(n, k + NB);
     int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n || (col >= k && col < kend)) return;
+    if (k >= n || col >= n || (col >= k && col < kend)) return;
     for (int c = k; c < kend; ++c) {
         int p = ipiv[c];
         if (p != c) {
@@ -103,10 +132,23 @@ __device__ __forceinline__ T ld_agent(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename T, bool C>
-__global__ __launch_bounds__(256) void col_step_kernel(T* A, int64_t ld, int n, int k, int kend, int c,
-                                                       double* pval, int* pidx, unsigned* counter,
-                                                       int32_t* ipiv, int* info) {
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb* tbl, int k, int c) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    const int n = P.n;
+    const int kend = min(n, k + NB);
+    if (c > kend) return;  // this problem's panel (or matrix) has ended
+    const int nrows = n - (c < kend ? c : kend);
+    if (nrows <= 0) return;
+    const unsigned nblk = (unsigned)((nrows + RPB - 1) / RPB);  // blocks working on this problem
+    if (blockIdx.x >= nblk) return;
+    double* pval = P.pval;
+    int* pidx = P.pidx;
+    unsigned* counter = P.counter;
+    int32_t* ipiv = P.ipiv;
+    int* info = P.info;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = k + lane;  // panel column of this lane
     const bool search = c < kend;
@@ -165,7 +207,7 @@ __global__ __launch_bounds__(256) void col_step_kernel(T* A, int64_t ld, int n, 
         pidx[blockIdx.x] = bi;
         __threadfence();
         const unsigned done = atomicAdd(counter, 1u);
-        s_last = done == gridDim.x - 1;
+        s_last = done == nblk - 1;
     }
     __syncthreads();
     if (!s_last) return;
@@ -174,7 +216,7 @@ __global__ __launch_bounds__(256) void col_step_kernel(T* A, int64_t ld, int n, 
     __shared__ int ri[256];
     double b = -1.0;
     int bi = 0x7fffffff;
-    for (int g = threadIdx.x; g < (int)gridDim.x; g += 256) {
+    for (int g = threadIdx.x; g < (int)nblk; g += 256) {
         double v = ld_agent(&pval[g]);
         int i2 = ld_agent(&pidx[g]);
         if (v > b || (v == b && i2 < bi)) { b = v; bi = i2; }
@@ -212,8 +254,14 @@ __global__ __launch_bounds__(256) void col_step_kernel(T* A, int64_t ld, int n, 
 // inverse is computed right-looking by the four threads (t, g), thread g
 // updating rows i = g (mod 4): once X[j][t] is final it is eliminated from
 // the rows below with independent updates.
-template <typename T, bool C>
-__global__ __launch_bounds__(256) void linv_kernel(const T* A, int64_t ld, int k, int w, T* SL) {
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(256) void linv_kernel(LuProb inl, const LuProb* tbl, int k) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    const T* A = (const T*)P.A;
+    const int64_t ld = P.ld;
+    const int kend = min(P.n, k + NB), w = kend - k;
+    if (kend >= P.n) return;  // no trailing columns
+    T* SL = (T*)P.SL;
     __shared__ cx<T> M[NB][NB + 1];
     __shared__ cx<T> X[NB][NB + 1];
     const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -267,9 +315,43 @@ constexpr int GT = 64;   // block tile
 constexpr int WT = 32;   // wave tile
 constexpr int KC = 16;   // K chunk
 
+// Operands of one C -= A B.
+template <typename T>
+struct GemmArgs {
+    T* Cm;
+    int64_t ldc;
+    const T* Am;
+    int64_t lda;
+    const T* Bm;
+    int64_t ldb;
+    int M, N, K;
+};
+
+// The two LU products of panel k of problem P: mode 0 is U12 = A12 - S_L A12
+// (in place), mode 1 the trailing update A22 -= L21 U12.
 template <typename T, bool C>
-__global__ __launch_bounds__(256) void gemm_sub_kernel(T* Cm, int64_t ldc, const T* Am, int64_t lda,
-                                                       const T* Bm, int64_t ldb, int M, int N, int K) {
+__device__ __forceinline__ GemmArgs<T> lu_gemm_args(const LuProb& P, int mode, int k) {
+    constexpr int E = C ? 2 : 1;
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    const int kend = min(P.n, k + NB), w = kend - k, N = P.n - kend;
+    if (mode == 0)
+        return {A + E * ((int64_t)k * ld + kend), ld, (const T*)P.SL, (int64_t)NB,
+                A + E * ((int64_t)k * ld + kend), ld, w, N, w};
+    return {A + E * ((int64_t)kend * ld + kend), ld, A + E * ((int64_t)kend * ld + k), ld,
+            A + E * ((int64_t)k * ld + kend), ld, N, N, w};
+}
+
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const LuProb* tbl, int mode, int k) {
+    GemmArgs<T> ga = inl;
+    if constexpr (B) ga = lu_gemm_args<T, C>(tbl[blockIdx.z], mode, k);
+    T* Cm = ga.Cm;
+    const T* Am = ga.Am;
+    const T* Bm = ga.Bm;
+    const int64_t ldc = ga.ldc, lda = ga.lda, ldb = ga.ldb;
+    const int M = ga.M, N = ga.N, K = ga.K;
+    if (M <= 0 || N <= 0 || (int)blockIdx.y * GT >= M || (int)blockIdx.x * GT >= N) return;
     using V4 = typename Mfma<T>::v4;
     constexpr int NBLK = WT / 16;
     __shared__ T sAr[GT][KC + 1], sAi[C ? GT : 1][KC + 1];
@@ -357,8 +439,28 @@ constexpr int TB = 64;
 // first so the sequential sweep touches LDS only.  Without TRANS the lower
 // factor is L (unit) and the upper U; with TRANS they are U^T (lower,
 // non-unit) and L^T (upper, unit), read from the same storage.
-template <typename T, bool C, bool LOWER, bool TRANS = false>
-__global__ __launch_bounds__(64) void trsv_diag_kernel(const T* A, int64_t ld, int n, int r0, double* x) {
+// One triangular solve of a (possibly batched) sweep: the factors and the
+// complex128 work vector, by value or from a table indexed by blockIdx.z.
+struct SolveProb {
+    const void* LU;
+    int64_t ld;
+    int n;
+    double* w;
+};
+template <bool B>
+__device__ __forceinline__ SolveProb get_sp(const SolveProb& inl, const SolveProb* tbl) {
+    if constexpr (B) return tbl[blockIdx.z];
+    else return inl;
+}
+
+template <typename T, bool C, bool LOWER, bool TRANS, bool B>
+__global__ __launch_bounds__(64) void trsv_diag_kernel(SolveProb inl, const SolveProb* tbl, int r0) {
+    const SolveProb P = get_sp<B>(inl, tbl);
+    const T* A = (const T*)P.LU;
+    const int64_t ld = P.ld;
+    const int n = P.n;
+    double* x = P.w;
+    if (r0 >= n) return;
     constexpr bool UNIT = LOWER != TRANS;
     __shared__ double xs[TB][2];
     __shared__ T blk[TB][TB][C ? 2 : 1];
@@ -418,10 +520,17 @@ __global__ __launch_bounds__(64) void trsv_diag_kernel(const T* A, int64_t ld, i
     if (t < nb) { x[2 * (r0 + t)] = xs[t][0]; x[2 * (r0 + t) + 1] = xs[t][1]; }
 }
 
-// x[rows] -= A[rows, c0:c0+TB] x[c0:c0+TB]: one wave per row.
-template <typename T, bool C>
-__global__ __launch_bounds__(256) void trsv_update_kernel(const T* A, int64_t ld, int row_begin,
-                                                          int row_end, int c0, int nb, double* x) {
+// x[rows] -= A[rows, c0:c0+nb] x[c0:c0+nb]: one wave per row.  Rows below
+// the block (LOWER sweep: [c0+nb, n)) or above it ([0, c0)).
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(256) void trsv_update_kernel(SolveProb inl, const SolveProb* tbl, int c0, int lower) {
+    const SolveProb P = get_sp<B>(inl, tbl);
+    const T* A = (const T*)P.LU;
+    const int64_t ld = P.ld;
+    double* x = P.w;
+    if (c0 >= P.n) return;
+    const int nb = min(TB, P.n - c0);
+    const int row_begin = lower ? c0 + nb : 0, row_end = lower ? P.n : c0;
     const int lane = threadIdx.x & 63;
     const int r = row_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= row_end) return;
@@ -438,9 +547,15 @@ __global__ __launch_bounds__(256) void trsv_update_kernel(const T* A, int64_t ld
 
 // Transposed block update x[rows] -= A[c0:c0+nb, rows]^T x[c0:c0+nb]: one
 // thread per row, so the column-block reads A[c][r] are coalesced over r.
-template <typename T, bool C>
-__global__ __launch_bounds__(256) void trsv_update_t_kernel(const T* A, int64_t ld, int row_begin,
-                                                            int row_end, int c0, int nb, double* x) {
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(256) void trsv_update_t_kernel(SolveProb inl, const SolveProb* tbl, int c0, int lower) {
+    const SolveProb P = get_sp<B>(inl, tbl);
+    const T* A = (const T*)P.LU;
+    const int64_t ld = P.ld;
+    double* x = P.w;
+    if (c0 >= P.n) return;
+    const int nb = min(TB, P.n - c0);
+    const int row_begin = lower ? c0 + nb : 0, row_end = lower ? P.n : c0;
     __shared__ double xs[TB][2];
     if (threadIdx.x < nb) {
         xs[threadIdx.x][0] = x[2 * (c0 + threadIdx.x)];
@@ -586,58 +701,123 @@ __global__ __launch_bounds__(256) void axpy_kernel(const T* src, int64_t lds, T*
 }
 
 template <typename T, bool C>
-int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_t s) {
+GemmArgs<T> lu_gemm_args_host(const LuProb& P, int mode, int k) {
     constexpr int E = C ? 2 : 1;
-    const size_t esz = sizeof(cx<T>);
-    const int gmax = (n + RPB - 1) / RPB;
-    // scratch: counter + info, per-block partials, S_L (NB x NB)
-    const size_t need = 512 + (size_t)gmax * (sizeof(double) + sizeof(int)) + NB * NB * esz + 1024;
-    int rc;
-    if (ctx->lu_scratch_bytes < need) {
-        if (ctx->lu_scratch) hipFree(ctx->lu_scratch);
-        ctx->lu_scratch = nullptr;
-        ctx->lu_scratch_bytes = 0;
-        rc = check_hip(hipMalloc(&ctx->lu_scratch, need), "LU scratch");
-        if (rc) return rc;
-        ctx->lu_scratch_bytes = need;
-    }
-    char* base = (char*)ctx->lu_scratch;
-    unsigned* counter = (unsigned*)base;
-    int* info = (int*)(base + 64);
-    double* pval = (double*)(base + 512);
-    int* pidx = (int*)(pval + gmax);
-    T* SL = (T*)(((uintptr_t)(pidx + gmax) + 255) & ~(uintptr_t)255);
-    rc = check_hip(hipMemsetAsync(base, 0, 128, s), "LU scratch reset");
-    if (rc) return rc;
-    for (int k = 0; k < n; k += NB) {
-        const int kend = min(n, k + NB), w = kend - k;
-        for (int c = k; c <= kend; ++c) {
-            const int rows = n - (c < kend ? c : kend);
-            if (rows <= 0) break;
-            const int G = (rows + RPB - 1) / RPB;
-            hipLaunchKernelGGL((col_step_kernel<T, C>), dim3(G), dim3(256), 0, s, A, ld, n, k, kend, c, pval, pidx,
-                               counter, ipiv, info);
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    const int kend = std::min(P.n, k + NB), w = kend - k, N = P.n - kend;
+    if (mode == 0)
+        return {A + E * ((int64_t)k * ld + kend), ld, (const T*)P.SL, (int64_t)NB,
+                A + E * ((int64_t)k * ld + kend), ld, w, N, w};
+    return {A + E * ((int64_t)kend * ld + kend), ld, A + E * ((int64_t)kend * ld + k), ld,
+            A + E * ((int64_t)k * ld + kend), ld, N, N, w};
+}
+
+// Launch sequence of the right-looking LU for one problem (B = false, the
+// problem by value) or a batch (B = true, table on the device, problems
+// sorted by decreasing n so that the first `active` entries are the ones
+// with columns left at panel k).
+template <typename T, bool C, bool B>
+void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, hipStream_t s) {
+    const int nmax = ns[0];
+    for (int k = 0; k < nmax; k += NB) {
+        int active = 0;
+        while (active < (int)ns.size() && ns[active] > k) ++active;
+        auto max_over = [&](auto f) {
+            int m = 0;
+            for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
+            return m;
+        };
+        const int kendmax = std::min(nmax, k + NB);
+        for (int c = k; c <= kendmax; ++c) {
+            const int G = max_over([&](int n) {
+                const int kend = std::min(n, k + NB);
+                if (c > kend) return 0;
+                const int rows = n - (c < kend ? c : kend);
+                return rows > 0 ? (rows + RPB - 1) / RPB : 0;
+            });
+            if (G <= 0) break;
+            hipLaunchKernelGGL((col_step_kernel<T, C, B>), dim3(G, 1, active), dim3(256), 0, s, one, d_tbl, k, c);
         }
-        hipLaunchKernelGGL((laswp_kernel<T, C>), dim3((n + 255) / 256), dim3(256), 0, s, A, ld, n, k, kend, ipiv);
-        if (kend < n) {
-            const int N = n - kend;
-            hipLaunchKernelGGL((linv_kernel<T, C>), dim3(1), dim3(256), 0, s, (const T*)A, ld, k, w, SL);
+        hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((nmax + 255) / 256, 1, active), dim3(256), 0, s, one,
+                           d_tbl, k);
+        const int N = max_over([&](int n) { return n - std::min(n, k + NB); });
+        if (N > 0) {
+            hipLaunchKernelGGL((linv_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
+            GemmArgs<T> g0{}, g1{};
+            if constexpr (!B) {
+                g0 = lu_gemm_args_host<T, C>(one, 0, k);
+                g1 = lu_gemm_args_host<T, C>(one, 1, k);
+            }
             // U12 = A12 - S_L A12 (in place: one block row, each block owns its columns)
-            hipLaunchKernelGGL((gemm_sub_kernel<T, C>), dim3((N + GT - 1) / GT, 1), dim3(256), 0, s,
-                               A + E * ((int64_t)k * ld + kend), ld, (const T*)SL, (int64_t)NB,
-                               (const T*)(A + E * ((int64_t)k * ld + kend)), ld, w, N, w);
+            hipLaunchKernelGGL((gemm_sub_kernel<T, C, B>), dim3((N + GT - 1) / GT, 1, active), dim3(256), 0, s,
+                               g0, d_tbl, 0, k);
             // A22 -= L21 U12
-            dim3 grid((N + GT - 1) / GT, (N + GT - 1) / GT);
-            hipLaunchKernelGGL((gemm_sub_kernel<T, C>), grid, dim3(256), 0, s,
-                               A + E * ((int64_t)kend * ld + kend), ld,
-                               (const T*)(A + E * ((int64_t)kend * ld + k)), ld,
-                               (const T*)(A + E * ((int64_t)k * ld + kend)), ld, N, N, w);
+            hipLaunchKernelGGL((gemm_sub_kernel<T, C, B>), dim3((N + GT - 1) / GT, (N + GT - 1) / GT, active),
+                               dim3(256), 0, s, g1, d_tbl, 1, k);
         }
-        rc = check_hip(hipGetLastError(), "LU step");
-        if (rc) return rc;
     }
+}
+
+// Per-problem scratch: counter and info words, the per-block pivot
+// partials and S_L (NB x NB).
+template <typename T, bool C>
+size_t lu_scratch_bytes(int n) {
+    const int gmax = (n + RPB - 1) / RPB;
+    return 512 + (((size_t)gmax * (sizeof(double) + sizeof(int)) + 255) & ~(size_t)255) +
+           NB * NB * sizeof(cx<T>) + 256;
+}
+
+template <typename T, bool C>
+LuProb lu_carve(char* base, void* A, int64_t ld, int n, int32_t* ipiv, int* info) {
+    const int gmax = (n + RPB - 1) / RPB;
+    LuProb P;
+    P.A = A; P.ld = ld; P.n = n; P.ipiv = ipiv;
+    P.counter = (unsigned*)base;
+    P.info = info ? info : (int*)(base + 64);
+    P.pval = (double*)(base + 512);
+    P.pidx = (int*)(P.pval + gmax);
+    P.SL = (void*)(((uintptr_t)(P.pidx + gmax) + 255) & ~(uintptr_t)255);
+    return P;
+}
+
+int ensure_lu_scratch(Context* ctx, size_t need) {
+    if (ctx->lu_scratch_bytes >= need) return BEM_OK;
+    if (ctx->lu_scratch) hipFree(ctx->lu_scratch);
+    ctx->lu_scratch = nullptr;
+    ctx->lu_scratch_bytes = 0;
+    ctx->lu_table.clear();
+    int rc = check_hip(hipMalloc(&ctx->lu_scratch, need), "LU scratch");
+    if (rc) return rc;
+    ctx->lu_scratch_bytes = need;
+    return BEM_OK;
+}
+
+// counter = 0 and info = 0 of every problem (a kernel, not a memset, so the
+// sequence can be captured in a graph)
+__global__ void lu_reset_kernel(const LuProb* tbl, int count) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    *tbl[q].counter = 0u;
+    *tbl[q].info = 0;
+}
+__global__ void lu_reset_one_kernel(LuProb P) {
+    *P.counter = 0u;
+    *P.info = 0;
+}
+
+template <typename T, bool C>
+int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_t s) {
+    int rc = ensure_lu_scratch(ctx, lu_scratch_bytes<T, C>(n));
+    if (rc) return rc;
+    ctx->lu_table.clear();  // this call reuses the scratch a batch table may occupy
+    const LuProb P = lu_carve<T, C>((char*)ctx->lu_scratch, A, ld, n, ipiv, nullptr);
+    hipLaunchKernelGGL(lu_reset_one_kernel, dim3(1), dim3(1), 0, s, P);
+    lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, s);
+    rc = check_hip(hipGetLastError(), "LU launch");
+    if (rc) return rc;
     int h_info = 0;
-    rc = check_hip(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, s), "LU info");
+    rc = check_hip(hipMemcpyAsync(&h_info, P.info, sizeof(int), hipMemcpyDeviceToHost, s), "LU info");
     if (rc) return rc;
     rc = check_hip(hipStreamSynchronize(s), "LU sync");
     if (rc) return rc;
@@ -645,44 +825,90 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     return BEM_OK;
 }
 
-// Solve on the complex128 work vector w (already permuted).
+// Batched factorisation: one launch sequence for all problems.  The problem
+// table is uploaded only when it differs from the previous call's, so a
+// second call with the same arguments issues kernels only (capturable in a
+// graph).  info[q] (device) gets 0 or 1 + the first zero-pivot column.
 template <typename T, bool C>
-void tri_solve_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
-    for (int r0 = 0; r0 < n; r0 += TB) {
-        const int nb = min(TB, n - r0);
-        hipLaunchKernelGGL((trsv_diag_kernel<T, C, true>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
-        if (r0 + nb < n)
-            hipLaunchKernelGGL((trsv_update_kernel<T, C>), dim3((n - r0 - nb + 3) / 4), dim3(256), 0, s, LU, ld,
-                               r0 + nb, n, r0, nb, w);
+int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* ld, const int32_t* n,
+                        int32_t* const* ipiv, int32_t* info, hipStream_t s) {
+    std::vector<int> order(count);
+    for (int q = 0; q < count; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+    size_t off = 0;
+    std::vector<size_t> offs(count);
+    for (int r = 0; r < count; ++r) {
+        offs[r] = off;
+        off += (lu_scratch_bytes<T, C>(n[order[r]]) + 255) & ~(size_t)255;
     }
-    const int last = ((n - 1) / TB) * TB;
+    const size_t tbl_bytes = (sizeof(LuProb) * count + 255) & ~(size_t)255;
+    int rc = ensure_lu_scratch(ctx, off + tbl_bytes);
+    if (rc) return rc;
+    char* base = (char*)ctx->lu_scratch;
+    std::vector<LuProb> tbl(count);
+    std::vector<int> ns(count);
+    for (int r = 0; r < count; ++r) {
+        const int q = order[r];
+        tbl[r] = lu_carve<T, C>(base + offs[r], A[q], ld[q], n[q], ipiv[q], info + q);
+        ns[r] = n[q];
+    }
+    LuProb* d_tbl = (LuProb*)(base + off);
+    const size_t tb = sizeof(LuProb) * count;
+    if (ctx->lu_table.size() != tb || memcmp(ctx->lu_table.data(), tbl.data(), tb) != 0 ||
+        ctx->lu_table_dev != (void*)d_tbl) {
+        rc = check_hip(hipMemcpyAsync(d_tbl, tbl.data(), tb, hipMemcpyHostToDevice, s), "LU table");
+        if (rc) return rc;
+        rc = check_hip(hipStreamSynchronize(s), "LU table sync");
+        if (rc) return rc;
+        ctx->lu_table.assign((const char*)tbl.data(), (const char*)tbl.data() + tb);
+        ctx->lu_table_dev = (void*)d_tbl;
+    }
+    hipLaunchKernelGGL(lu_reset_kernel, dim3((count + 63) / 64), dim3(64), 0, s, (const LuProb*)d_tbl, count);
+    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, s);
+    return check_hip(hipGetLastError(), "batched LU launch");
+}
+
+// Forward then backward sweep on the complex128 work vectors (already
+// permuted): L U w = w, or with TRANS U^T L^T w = w.  nmax = largest n of
+// the batch (B) or the one problem's n.
+template <typename T, bool C, bool TRANS, bool B>
+void tri_launch(const SolveProb& one, const SolveProb* d_tbl, int nmax, int count, hipStream_t s) {
+    for (int r0 = 0; r0 < nmax; r0 += TB) {
+        hipLaunchKernelGGL((trsv_diag_kernel<T, C, true, TRANS, B>), dim3(1, 1, count), dim3(64), 0, s, one, d_tbl,
+                           r0);
+        const int rows = nmax - r0 - TB;
+        if (rows > 0) {
+            if constexpr (TRANS)
+                hipLaunchKernelGGL((trsv_update_t_kernel<T, C, B>), dim3((rows + 255) / 256, 1, count), dim3(256),
+                                   0, s, one, d_tbl, r0, 1);
+            else
+                hipLaunchKernelGGL((trsv_update_kernel<T, C, B>), dim3((rows + 3) / 4, 1, count), dim3(256), 0, s,
+                                   one, d_tbl, r0, 1);
+        }
+    }
+    const int last = ((nmax - 1) / TB) * TB;
     for (int r0 = last; r0 >= 0; r0 -= TB) {
-        const int nb = min(TB, n - r0);
-        hipLaunchKernelGGL((trsv_diag_kernel<T, C, false>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
-        if (r0 > 0)
-            hipLaunchKernelGGL((trsv_update_kernel<T, C>), dim3((r0 + 3) / 4), dim3(256), 0, s, LU, ld, 0, r0,
-                               r0, nb, w);
+        hipLaunchKernelGGL((trsv_diag_kernel<T, C, false, TRANS, B>), dim3(1, 1, count), dim3(64), 0, s, one,
+                           d_tbl, r0);
+        if (r0 > 0) {
+            if constexpr (TRANS)
+                hipLaunchKernelGGL((trsv_update_t_kernel<T, C, B>), dim3((r0 + 255) / 256, 1, count), dim3(256), 0,
+                                   s, one, d_tbl, r0, 0);
+            else
+                hipLaunchKernelGGL((trsv_update_kernel<T, C, B>), dim3((r0 + 3) / 4, 1, count), dim3(256), 0, s,
+                                   one, d_tbl, r0, 0);
+        }
     }
 }
 
-// Solve U^T L^T z = w in place (the transposed factors of the same storage).
+template <typename T, bool C>
+void tri_solve_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
+    tri_launch<T, C, false, false>(SolveProb{LU, ld, n, w}, nullptr, n, 1, s);
+}
+
 template <typename T, bool C>
 void tri_solve_trans_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
-    for (int r0 = 0; r0 < n; r0 += TB) {
-        const int nb = min(TB, n - r0);
-        hipLaunchKernelGGL((trsv_diag_kernel<T, C, true, true>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
-        if (r0 + nb < n)
-            hipLaunchKernelGGL((trsv_update_t_kernel<T, C>), dim3((n - r0 - nb + 255) / 256), dim3(256), 0, s, LU,
-                               ld, r0 + nb, n, r0, nb, w);
-    }
-    const int last = ((n - 1) / TB) * TB;
-    for (int r0 = last; r0 >= 0; r0 -= TB) {
-        const int nb = min(TB, n - r0);
-        hipLaunchKernelGGL((trsv_diag_kernel<T, C, false, true>), dim3(1), dim3(64), 0, s, LU, ld, n, r0, w);
-        if (r0 > 0)
-            hipLaunchKernelGGL((trsv_update_t_kernel<T, C>), dim3((r0 + 255) / 256), dim3(256), 0, s, LU, ld, 0, r0,
-                               r0, nb, w);
-    }
+    tri_launch<T, C, true, false>(SolveProb{LU, ld, n, w}, nullptr, n, 1, s);
 }
 
 void tri_solve_trans(const void* LU, int64_t ld, int n, int dtype, double* w, hipStream_t s) {
@@ -756,6 +982,23 @@ int lu_factor(Context* ctx, void* A, int64_t ld, int n, int dtype, int32_t* ipiv
         case BEM_C128: return lu_factor_t<double, true>(ctx, (double*)A, ld, n, ipiv, s);
         case BEM_F32: return lu_factor_t<float, false>(ctx, (float*)A, ld, n, ipiv, s);
         default: return lu_factor_t<float, true>(ctx, (float*)A, ld, n, ipiv, s);
+    }
+}
+
+int lu_factor_batched(Context* ctx, int count, void* const* A, const int64_t* ld, const int32_t* n, int dtype,
+                      int32_t* const* ipiv, int32_t* info, hipStream_t s) {
+    if (count <= 0) return BEM_OK;
+    if (!A || !ld || !n || !ipiv || !info) return set_error(BEM_ERR_INVALID, "null argument");
+    if (!valid_dtype(dtype)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    for (int q = 0; q < count; ++q) {
+        if (!A[q] || !ipiv[q]) return set_error(BEM_ERR_INVALID, "null matrix or pivot array in the batch");
+        if (n[q] <= 0 || ld[q] < n[q]) return set_error(BEM_ERR_INVALID, "invalid matrix size in the batch");
+    }
+    switch (dtype) {
+        case BEM_F64: return lu_factor_batched_t<double, false>(ctx, count, A, ld, n, ipiv, info, s);
+        case BEM_C128: return lu_factor_batched_t<double, true>(ctx, count, A, ld, n, ipiv, info, s);
+        case BEM_F32: return lu_factor_batched_t<float, false>(ctx, count, A, ld, n, ipiv, info, s);
+        default: return lu_factor_batched_t<float, true>(ctx, count, A, ld, n, ipiv, info, s);
     }
 }
 
@@ -920,6 +1163,417 @@ int convert_matrix(const void* src, int ds, int64_t lds, void* dst, int dd, int6
     else
         hipLaunchKernelGGL((convert_kernel<float, false, float, false>), g, dim3(256), 0, s, (const float*)src, lds, (float*)dst, ldd, rows, cols);
     return check_hip(hipGetLastError(), "convert");
+}
+
+// ---------------------------------------------------------------------------
+// Batched solves, FP64 refinement and 1-norm condition estimates over the
+// factors of bem_lu_factor_batched.  Everything runs on the device (the
+// pivot permutation included), so a repeated call with the same arguments
+// issues kernels only and can be captured in a graph.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct VecProb {
+    const void* A;      // matrix of the residuals (LU dtype), may be null
+    int64_t lda;
+    int n;
+    const int32_t* ipiv;
+    int32_t* perm;
+    const double* b;    // complex128 right-hand side (caller)
+    double* x;          // complex128 solution (caller)
+    double* w;          // work vector (the triangular-solve vector)
+    double* r;          // work vector
+    double* acc;        // 8 doubles: reductions and the condition estimate state
+    double* out;        // caller outputs (2 doubles) or null
+};
+
+__device__ __forceinline__ const VecProb& vp(const VecProb* t) { return t[blockIdx.z]; }
+
+// perm from LAPACK interchanges, one problem per block (thread 0 replays
+// the swaps in order; the identity is written by the whole block first)
+__global__ void vb_perm_kernel(const VecProb* tbl) {
+    const VecProb& P = tbl[blockIdx.x];
+    for (int i = threadIdx.x; i < P.n; i += blockDim.x) P.perm[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t* pm = P.perm;
+        for (int c = 0; c < P.n; ++c) {
+            const int p = P.ipiv[c];
+            if (p != c) { const int t = pm[c]; pm[c] = pm[p]; pm[p] = t; }
+        }
+    }
+}
+
+// w[i] = src[perm[i]], src = b (which == 0), r (1) or w-copy in r (2 same as 1)
+__global__ void vb_gather_kernel(const VecProb* tbl, int which) {
+    const VecProb& P = vp(tbl);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double* src = which == 0 ? P.b : P.r;
+    const int j = P.perm[i];
+    P.w[2 * i] = src[2 * j];
+    P.w[2 * i + 1] = src[2 * j + 1];
+}
+
+// r[perm[i]] = w[i] (the inverse permutation, transposed solves)
+__global__ void vb_scatter_kernel(const VecProb* tbl) {
+    const VecProb& P = vp(tbl);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int j = P.perm[i];
+    P.r[2 * j] = P.w[2 * i];
+    P.r[2 * j + 1] = P.w[2 * i + 1];
+}
+
+// x = w (accumulate = 0) or x += w
+__global__ void vb_update_x_kernel(const VecProb* tbl, int accumulate) {
+    const VecProb& P = vp(tbl);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    if (accumulate) { P.x[2 * i] += P.w[2 * i]; P.x[2 * i + 1] += P.w[2 * i + 1]; }
+    else { P.x[2 * i] = P.w[2 * i]; P.x[2 * i + 1] = P.w[2 * i + 1]; }
+}
+
+// r = b - A x in FP64 (one wave per row)
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void vb_residual_kernel(const VecProb* tbl) {
+    const VecProb& P = vp(tbl);
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= P.n) return;
+    const T* A = (const T*)P.A;
+    double sr = 0.0, si = 0.0;
+    for (int c = lane; c < P.n; c += 64) {
+        cx<T> a = Acc<T, C>::ld(A, (int64_t)row * P.lda + c);
+        const double xr = P.x[2 * c], xi = P.x[2 * c + 1];
+        sr += (double)a.re * xr - (double)a.im * xi;
+        si += (double)a.re * xi + (double)a.im * xr;
+    }
+    sr = wave_sum(sr); si = wave_sum(si);
+    if (lane == 0) { P.r[2 * row] = P.b[2 * row] - sr; P.r[2 * row + 1] = P.b[2 * row + 1] - si; }
+}
+
+__global__ void vb_zero_acc_kernel(const VecProb* tbl, int count, int first, int num) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    for (int e = first; e < first + num; ++e) tbl[q].acc[e] = 0.0;
+}
+
+// acc[slot] += sum |v_i|^2 (kind 0) or sum |v_i| (kind 1) of v = b (which 0),
+// r (1) or w (2)
+__global__ void vb_norm_kernel(const VecProb* tbl, int which, int kind, int slot) {
+    const VecProb& P = vp(tbl);
+    const double* v = which == 0 ? P.b : (which == 1 ? P.r : P.w);
+    double sum = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x) {
+        const double a = v[2 * i], b = v[2 * i + 1];
+        sum += kind ? sqrt(a * a + b * b) : a * a + b * b;
+    }
+    __shared__ double red[4];
+    sum = wave_sum(sum);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(P.acc + slot, red[0] + red[1] + red[2] + red[3]);
+}
+
+// out[0] = sqrt(acc[1] / acc[0]): relative residual ||b - A x|| / ||b||
+__global__ void vb_resid_out_kernel(const VecProb* tbl, int count) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const VecProb& P = tbl[q];
+    P.out[0] = P.acc[0] > 0.0 ? sqrt(P.acc[1] / P.acc[0]) : sqrt(P.acc[1]);
+}
+
+// Hager-Higham steps.  acc[2] = current ||A^-1 x||_1 sum, acc[3] = estimate,
+// acc[4] = argmax bits, acc[5] = max |z| bits.
+__global__ void ce_init_kernel(const VecProb* tbl) {
+    const VecProb& P = vp(tbl);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    P.r[2 * i] = 1.0 / P.n;  // x = (1/n) 1, kept in r for the gather
+    P.r[2 * i + 1] = 0.0;
+}
+// est = max(est, scale * acc[2]); w (= y) -> conj(sign(y)) in w for the
+// transposed solve
+__global__ void ce_sign_kernel(const VecProb* tbl) {
+    const VecProb& P = vp(tbl);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double a = P.w[2 * i], b = P.w[2 * i + 1];
+    const double m = sqrt(a * a + b * b);
+    P.w[2 * i] = m > 0.0 ? a / m : 1.0;
+    P.w[2 * i + 1] = m > 0.0 ? -b / m : 0.0;
+}
+__global__ void ce_est_kernel(const VecProb* tbl, int count, double scale_num, int scale_by_n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const VecProb& P = tbl[q];
+    const double v = scale_by_n ? P.acc[2] * scale_num / P.n : P.acc[2];
+    if (v > P.acc[3]) P.acc[3] = v;
+    P.acc[2] = 0.0;
+}
+// j = argmax |r_i| (r = A^-T conj(xi); |conj| is the same), first on ties;
+// one block per problem
+__global__ void ce_argmax_kernel(const VecProb* tbl) {
+    const VecProb& P = tbl[blockIdx.x];
+    double best = -1.0;
+    int bi = 0;
+    for (int i = threadIdx.x; i < P.n; i += blockDim.x) {
+        const double a = P.r[2 * i], b = P.r[2 * i + 1];
+        const double m = a * a + b * b;
+        if (m > best) { best = m; bi = i; }
+    }
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const double ov = sv[threadIdx.x + off];
+            const int oi = si[threadIdx.x + off];
+            if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = ov; si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) P.acc[4] = (double)si[0];
+}
+// r = e_j (next x), or the alternating test vector (alt = 1):
+// x_i = (-1)^i (1 + i / (n - 1))
+__global__ void ce_next_kernel(const VecProb* tbl, int alt) {
+    const VecProb& P = vp(tbl);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    double v;
+    if (alt) v = (i & 1 ? -1.0 : 1.0) * (1.0 + (P.n > 1 ? (double)i / (P.n - 1) : 0.0));
+    else v = i == (int)P.acc[4] ? 1.0 : 0.0;
+    P.r[2 * i] = v;
+    P.r[2 * i + 1] = 0.0;
+}
+// ||A||_1 = max_j sum_i |a_ij| (columns coalesced over threads); the
+// maximum goes through the ordered bits of the non-negative doubles
+template <typename T, bool C>
+__global__ void ce_norm1_kernel(const VecProb* tbl) {
+    const VecProb& P = vp(tbl);
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= P.n) return;
+    const T* A = (const T*)P.A;
+    double sum = 0.0;
+    for (int r = 0; r < P.n; ++r) {
+        cx<T> a = Acc<T, C>::ld(A, (int64_t)r * P.lda + col);
+        sum += sqrt((double)a.re * a.re + (double)a.im * a.im);
+    }
+    atomicMax((unsigned long long*)(P.acc + 6), (unsigned long long)__double_as_longlong(sum));
+}
+__global__ void ce_out_kernel(const VecProb* tbl, int count) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const VecProb& P = tbl[q];
+    P.out[0] = P.acc[6];
+    P.out[1] = P.acc[3];
+}
+
+struct BatchPlan {
+    std::vector<int> ns;
+    int nmax = 0, count = 0;
+    VecProb* d_vec = nullptr;
+    SolveProb* d_sp = nullptr;
+};
+
+// Lays out the per-problem work vectors and both tables in the batch
+// workspace of the context (reused while it is large enough) and uploads the
+// tables when they changed.
+int batch_plan(Context* ctx, int slot, int count, const void* const* LU, const int64_t* ld, const int32_t* n,
+               const int32_t* const* ipiv, const void* const* A, const int64_t* lda, const void* const* b,
+               void* const* x, double* out, int out_stride, BatchPlan& plan, hipStream_t s) {
+    // slot: 0 = solves, 1 = condition estimates (separate workspaces, so an
+    // alternating sequence of both keeps its cached tables)
+    void*& ws = ctx->batch_ws[slot];
+    size_t& ws_bytes = ctx->batch_bytes[slot];
+    std::vector<char>& cached = ctx->batch_tables[slot];
+    size_t bytes = 0;
+    std::vector<size_t> off(count);
+    for (int q = 0; q < count; ++q) {
+        off[q] = bytes;
+        bytes += ((size_t)n[q] * 4 + 255) / 256 * 256;          // perm
+        bytes += ((size_t)n[q] * 16 + 255) / 256 * 256 * 2;     // w, r
+        bytes += 256;                                           // acc
+    }
+    const size_t vt = (sizeof(VecProb) * count + 255) / 256 * 256;
+    const size_t st = (sizeof(SolveProb) * count + 255) / 256 * 256;
+    const size_t need = bytes + vt + st;
+    if (ws_bytes < need) {
+        if (ws) hipFree(ws);
+        ws = nullptr;
+        ws_bytes = 0;
+        cached.clear();
+        int rc = check_hip(hipMalloc(&ws, need), "batch workspace");
+        if (rc) return rc;
+        ws_bytes = need;
+    }
+    char* base = (char*)ws;
+    std::vector<VecProb> vt_h(count);
+    std::vector<SolveProb> st_h(count);
+    plan.ns.assign(n, n + count);
+    plan.nmax = 0;
+    plan.count = count;
+    for (int q = 0; q < count; ++q) {
+        char* p = base + off[q];
+        VecProb v;
+        v.A = A ? A[q] : nullptr;
+        v.lda = lda ? lda[q] : 0;
+        v.n = n[q];
+        v.ipiv = ipiv[q];
+        v.perm = (int32_t*)p;
+        p += ((size_t)n[q] * 4 + 255) / 256 * 256;
+        v.b = b ? (const double*)b[q] : nullptr;
+        v.x = x ? (double*)x[q] : nullptr;
+        v.w = (double*)p;
+        p += ((size_t)n[q] * 16 + 255) / 256 * 256;
+        v.r = (double*)p;
+        p += ((size_t)n[q] * 16 + 255) / 256 * 256;
+        v.acc = (double*)p;
+        v.out = out ? out + (size_t)out_stride * q : nullptr;
+        vt_h[q] = v;
+        st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w};
+        plan.nmax = std::max(plan.nmax, n[q]);
+    }
+    plan.d_vec = (VecProb*)(base + bytes);
+    plan.d_sp = (SolveProb*)(base + bytes + vt);
+    std::vector<char> blob(vt + st, 0);
+    memcpy(blob.data(), vt_h.data(), sizeof(VecProb) * count);
+    memcpy(blob.data() + vt, st_h.data(), sizeof(SolveProb) * count);
+    if (cached != blob) {
+        int rc = check_hip(hipMemcpyAsync(plan.d_vec, blob.data(), blob.size(), hipMemcpyHostToDevice, s),
+                           "batch tables");
+        if (rc) return rc;
+        rc = check_hip(hipStreamSynchronize(s), "batch tables sync");
+        if (rc) return rc;
+        cached.swap(blob);
+    }
+    return BEM_OK;
+}
+
+template <typename T, bool C>
+void batch_tri(const BatchPlan& pl, bool trans, hipStream_t s) {
+    if (trans) tri_launch<T, C, true, true>(SolveProb{}, pl.d_sp, pl.nmax, pl.count, s);
+    else tri_launch<T, C, false, true>(SolveProb{}, pl.d_sp, pl.nmax, pl.count, s);
+}
+
+template <typename T, bool C>
+void batch_solve_t(const BatchPlan& pl, int iters, bool resid, hipStream_t s) {
+    const dim3 gv((pl.nmax + 255) / 256, 1, pl.count), b256(256);
+    const dim3 gr((pl.nmax + 3) / 4, 1, pl.count);
+    hipLaunchKernelGGL(vb_perm_kernel, dim3(pl.count), dim3(256), 0, s, (const VecProb*)pl.d_vec);
+    hipLaunchKernelGGL(vb_gather_kernel, gv, b256, 0, s, (const VecProb*)pl.d_vec, 0);
+    batch_tri<T, C>(pl, false, s);
+    hipLaunchKernelGGL(vb_update_x_kernel, gv, b256, 0, s, (const VecProb*)pl.d_vec, 0);
+    for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL((vb_residual_kernel<T, C>), gr, b256, 0, s, (const VecProb*)pl.d_vec);
+        hipLaunchKernelGGL(vb_gather_kernel, gv, b256, 0, s, (const VecProb*)pl.d_vec, 1);
+        batch_tri<T, C>(pl, false, s);
+        hipLaunchKernelGGL(vb_update_x_kernel, gv, b256, 0, s, (const VecProb*)pl.d_vec, 1);
+    }
+    if (resid) {
+        hipLaunchKernelGGL(vb_zero_acc_kernel, dim3((pl.count + 63) / 64), dim3(64), 0, s,
+                           (const VecProb*)pl.d_vec, pl.count, 0, 2);
+        hipLaunchKernelGGL((vb_residual_kernel<T, C>), gr, b256, 0, s, (const VecProb*)pl.d_vec);
+        const dim3 gn(std::min(64, (pl.nmax + 255) / 256), 1, pl.count);
+        hipLaunchKernelGGL(vb_norm_kernel, gn, b256, 0, s, (const VecProb*)pl.d_vec, 0, 0, 0);
+        hipLaunchKernelGGL(vb_norm_kernel, gn, b256, 0, s, (const VecProb*)pl.d_vec, 1, 0, 1);
+        hipLaunchKernelGGL(vb_resid_out_kernel, dim3((pl.count + 63) / 64), dim3(64), 0, s,
+                           (const VecProb*)pl.d_vec, pl.count);
+    }
+}
+
+template <typename T, bool C>
+void batch_condest_t(const BatchPlan& pl, int iters, hipStream_t s) {
+    const VecProb* tb = pl.d_vec;
+    const dim3 gv((pl.nmax + 255) / 256, 1, pl.count), b256(256);
+    const dim3 gn(std::min(64, (pl.nmax + 255) / 256), 1, pl.count);
+    const dim3 gq((pl.count + 63) / 64), b64(64);
+    hipLaunchKernelGGL(vb_perm_kernel, dim3(pl.count), dim3(256), 0, s, tb);
+    hipLaunchKernelGGL(vb_zero_acc_kernel, gq, b64, 0, s, tb, pl.count, 0, 8);
+    hipLaunchKernelGGL((ce_norm1_kernel<T, C>), gv, b256, 0, s, tb);
+    hipLaunchKernelGGL(ce_init_kernel, gv, b256, 0, s, tb);
+    for (int it = 0; it < iters; ++it) {
+        // y = A^-1 x: w = P x, then the two sweeps
+        hipLaunchKernelGGL(vb_gather_kernel, gv, b256, 0, s, tb, 1);
+        batch_tri<T, C>(pl, false, s);
+        hipLaunchKernelGGL(vb_norm_kernel, gn, b256, 0, s, tb, 2, 1, 2);
+        hipLaunchKernelGGL(ce_est_kernel, gq, b64, 0, s, tb, pl.count, 1.0, 0);
+        // z = A^-H sign(y) = conj(A^-T conj(sign(y))): the conjugate is taken
+        // in ce_sign, the outer one does not change |z|
+        hipLaunchKernelGGL(ce_sign_kernel, gv, b256, 0, s, tb);
+        batch_tri<T, C>(pl, true, s);
+        hipLaunchKernelGGL(vb_scatter_kernel, gv, b256, 0, s, tb);
+        hipLaunchKernelGGL(ce_argmax_kernel, dim3(pl.count), b256, 0, s, tb);
+        hipLaunchKernelGGL(ce_next_kernel, gv, b256, 0, s, tb, 0);
+    }
+    // the alternating-sign test vector: est >= 2 ||A^-1 x||_1 / (3 n)
+    hipLaunchKernelGGL(ce_next_kernel, gv, b256, 0, s, tb, 1);
+    hipLaunchKernelGGL(vb_gather_kernel, gv, b256, 0, s, tb, 1);
+    batch_tri<T, C>(pl, false, s);
+    hipLaunchKernelGGL(vb_norm_kernel, gn, b256, 0, s, tb, 2, 1, 2);
+    hipLaunchKernelGGL(ce_est_kernel, gq, b64, 0, s, tb, pl.count, 2.0 / 3.0, 1);
+    hipLaunchKernelGGL(ce_out_kernel, gq, b64, 0, s, tb, pl.count);
+}
+
+int batch_check(int count, const void* const* LU, const int64_t* ld, const int32_t* n, int dtype,
+                const int32_t* const* ipiv) {
+    if (!LU || !ld || !n || !ipiv) return set_error(BEM_ERR_INVALID, "null argument");
+    if (!valid_dtype(dtype)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    for (int q = 0; q < count; ++q) {
+        if (!LU[q] || !ipiv[q]) return set_error(BEM_ERR_INVALID, "null factor or pivot array in the batch");
+        if (n[q] <= 0 || ld[q] < n[q]) return set_error(BEM_ERR_INVALID, "invalid matrix size in the batch");
+    }
+    return BEM_OK;
+}
+
+}  // namespace
+
+int batch_solve(Context* ctx, int count, const void* const* LU, const int64_t* ld, const int32_t* n, int dtype,
+                const int32_t* const* ipiv, const void* const* A, const int64_t* lda, const void* const* b,
+                void* const* x, int refine_iters, double* resid, hipStream_t s) {
+    if (count <= 0) return BEM_OK;
+    int rc = batch_check(count, LU, ld, n, dtype, ipiv);
+    if (rc) return rc;
+    if (!b || !x) return set_error(BEM_ERR_INVALID, "null right-hand side or solution array");
+    if ((refine_iters > 0 || resid) && (!A || !lda))
+        return set_error(BEM_ERR_INVALID, "refinement and residuals need the matrices");
+    if (refine_iters < 0) return set_error(BEM_ERR_INVALID, "negative refinement count");
+    BatchPlan pl;
+    rc = batch_plan(ctx, 0, count, LU, ld, n, ipiv, A, lda, b, x, resid, 1, pl, s);
+    if (rc) return rc;
+    switch (dtype) {
+        case BEM_F64: batch_solve_t<double, false>(pl, refine_iters, resid != nullptr, s); break;
+        case BEM_C128: batch_solve_t<double, true>(pl, refine_iters, resid != nullptr, s); break;
+        case BEM_F32: batch_solve_t<float, false>(pl, refine_iters, resid != nullptr, s); break;
+        default: batch_solve_t<float, true>(pl, refine_iters, resid != nullptr, s); break;
+    }
+    return check_hip(hipGetLastError(), "batched solve");
+}
+
+int batch_condest(Context* ctx, int count, const void* const* LU, const int64_t* ld, const int32_t* n, int dtype,
+                  const int32_t* const* ipiv, const void* const* A, const int64_t* lda, int iters, double* out,
+                  hipStream_t s) {
+    if (count <= 0) return BEM_OK;
+    int rc = batch_check(count, LU, ld, n, dtype, ipiv);
+    if (rc) return rc;
+    if (!A || !lda || !out) return set_error(BEM_ERR_INVALID, "null matrix or output array");
+    if (iters < 1 || iters > 20) return set_error(BEM_ERR_INVALID, "iteration count must be 1..20");
+    BatchPlan pl;
+    rc = batch_plan(ctx, 1, count, LU, ld, n, ipiv, A, lda, nullptr, nullptr, out, 2, pl, s);
+    if (rc) return rc;
+    switch (dtype) {
+        case BEM_F64: batch_condest_t<double, false>(pl, iters, s); break;
+        case BEM_C128: batch_condest_t<double, true>(pl, iters, s); break;
+        case BEM_F32: batch_condest_t<float, false>(pl, iters, s); break;
+        default: batch_condest_t<float, true>(pl, iters, s); break;
+    }
+    return check_hip(hipGetLastError(), "batched condition estimate");
 }
 
 }  // namespace bem
