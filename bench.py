@@ -56,6 +56,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--workload", choices=("dirichlet", "sweep", "transmission"), default="dirichlet",
+                    help="dirichlet: the L5 Helmholtz Dirichlet solve (default); sweep: the batched "
+                         "combined-field wave-number sweep (60 k at L4 plus L3 and L5); transmission: the "
+                         "L5 2x2 Calderon transmission solve with the interior-k gradient")
+    ap.add_argument("--no-fd", action="store_true", help="transmission: skip the finite-difference check")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the reference kernel that scales "
                          "FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -71,6 +76,10 @@ def main() -> int:
     bem = importlib.import_module("taichi-bem_amd")
     L = bem._lib
     dev = torch.device("cuda", 0)
+    if args.workload == "sweep":
+        return sweep_main(args, bem)
+    if args.workload == "transmission":
+        return transmission_main(args, bem)
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -253,6 +262,276 @@ def main() -> int:
         result["cpu_baseline"]["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
         log(f"cpu baseline: {rows2} rows in {dt2:.1f}s")
 
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+# Algorithmic flops per point pair of the combined-field far kernel
+# (cf_far_kernel, FP32): d 3, r^2 5, w/r 1, (x-y).n 5, w (x-y).n / r^3 3,
+# kr 2, phase reduction 2, eta w/r 1, K - i eta V value 8, accumulation of
+# the combination into the three test bases 12.
+FLOPS_PER_EVAL_CF = 42
+
+
+def sweep_main(args, bem) -> int:
+    """Batched exterior combined-field sweep: 60 wave numbers k = 16 + i/30
+    on icosphere L4 plus L3 and L5 at k = 16.5, c64 with FP32 regular pairs,
+    FP64 near field and FP64 refinement, condition estimates per k.  One step
+    = one replay of the captured graph (assembly, LU, refined solves,
+    condition estimates of all 62 problems)."""
+    import numpy as np
+    import torch
+
+    L = bem._lib
+    S = bem.sweep
+    meshes = [bem.meshes.icosphere(4), bem.meshes.icosphere(3), bem.meshes.icosphere(5)]
+    problems = [(0, 16.0 + i / 30.0) for i in range(60)] + [(1, 16.5), (2, 16.5)]
+    log(f"sweep: {len(problems)} problems, vertices {[len(m[0]) for m in meshes]}")
+    sw = S.WavenumberSweep(meshes, problems, dtype=torch.complex64, precision=L.PREC_FP32, refine_iters=2,
+                           condest_iters=3, direction=bem.meshes.seeded_direction(2))
+    tune = S.autotune(sw)
+    log(f"tuning {sw.tuning} ({'cached' if tune.get('cached') else 'timed'}): {tune.get('candidates_ms')}")
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    # eager stage timing
+    stages = {}
+    for w in range(2):
+        e = [ev() for _ in range(6)]
+        e[0].record()
+        sw.assemble()
+        e[1].record()
+        L.convert(sw.A_flat, sw.LU_flat)
+        sw.batch.lu_factor(sw.LU, sw.ipiv, sw.info)
+        e[2].record()
+        sw.batch.solve(sw.LU, sw.ipiv, sw.A, sw.b, sw.x, sw.refine_iters, sw.resid)
+        e[3].record()
+        sw.batch.condest(sw.LU, sw.ipiv, sw.A, sw.cond, sw.condest_iters)
+        e[4].record()
+        torch.cuda.synchronize()
+        stages = {"assembly_ms": e[0].elapsed_time(e[1]), "lu_ms": e[1].elapsed_time(e[2]),
+                  "solve_refine_ms": e[2].elapsed_time(e[3]), "condest_ms": e[3].elapsed_time(e[4])}
+        log(f"eager {w}: " + ", ".join(f"{kk}={vv:.1f}" for kk, vv in stages.items()))
+    sw.capture()
+    for _ in range(args.warmup):
+        sw.replay()
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        sw.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    res = sw.results()
+    entries = sum(2 * r["n"] ** 2 for r in res)  # V and K entries of every problem
+    far_pairs = 0
+    for m, _ in problems:
+        c = sw.ctxs[m]
+        far_pairs += c.nf * c.nf - sum(c.nearfield_counts()[1:])
+    far_flops = far_pairs * 625 * FLOPS_PER_EVAL_CF
+    # accuracy: the k = 16.5 problems against all-FP64 complex128 solves
+    errs = {}
+    for q in (len(problems) - 2, len(problems) - 1, 15):
+        m, k = problems[q]
+        ref = S.WavenumberSweep([meshes[m]], [(0, k)], dtype=torch.complex128, precision=L.PREC_FP64,
+                                refine_iters=0, condest_iters=1, direction=bem.meshes.seeded_direction(2))
+        ref.run()
+        torch.cuda.synchronize()
+        errs[f"L{[4, 3, 5][m]} k={k:.4f}"] = float(torch.linalg.vector_norm(sw.x[q] - ref.x[0]) /
+                                                   torch.linalg.vector_norm(ref.x[0]))
+        del ref
+    # CPU baseline: the oracle assembling V and K (P1 x P1) of the L3 problem
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        v3, f3 = meshes[1]
+        t = time.perf_counter()
+        for op in (O.OP_V, O.OP_K):
+            O.assemble(O.make_op(op, O.HELMHOLTZ, 16.5, 1.0, 5, 1, 1, 0), v3, f3, nthreads=threads)
+        dt = time.perf_counter() - t
+        cpu = {"value": 2 * len(v3) ** 2 / dt, "unit": "entries/s", "cores": threads, "kind": "port",
+               "sample": f"V and K (P1 x P1, Helmholtz k=16.5, GaussQR 5, FP64) of icosphere L3 "
+                         f"({len(f3)} panels), {dt:.1f} s, C/OpenMP oracle"}
+    asm_s = stages["assembly_ms"] * 1e-3
+    result = {
+        "metric": "BIO entries/s assembled; time-to-solution of the batched combined-field sweep",
+        "value": entries / asm_s, "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+        "data": "synthetic (icospheres, seeded plane wave)",
+        "config": {"workload": "exterior combined-field sweep: 60 k in [16, 18) step 1/30 on icosphere L4 "
+                               "(5120 panels, 2562 vertices) + L3 and L5 at k=16.5, P1, GaussQR 5, c64 with "
+                               "FP32 regular pairs, FP64 near field, 2 FP64 refinement steps",
+                   "problems": len(problems), "tuning": list(sw.tuning)},
+        "stages_ms_eager": {kk: round(vv, 3) for kk, vv in stages.items()},
+        "graph_replay_ms": round(ms, 3),
+        "roofline": {"bound": "valu", "kernel": "cf_far_kernel<float> (V + K, P1 x P1)",
+                     "achieved": round(far_flops / asm_s / 1e12, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(far_flops / asm_s / 1e12 / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "note": "achieved uses the whole assembly time (far + near + mass launches)"},
+        "max_rel_residual": max(r["rel_residual"] for r in res),
+        "rel_err_vs_fp64": errs,
+        "cond1_est_per_k": [round(r["cond1_est"], 3) for r in res[:60]],
+        "cond1_est_L3_L5": [round(r["cond1_est"], 3) for r in res[60:]],
+        "wall_s_steps": round(wall, 3),
+    }
+    if cpu:
+        result["cpu_baseline"] = cpu
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def transmission_main(args, bem) -> int:
+    """Helmholtz transmission problem: 2 x 2 Calderon block system of V, K, K^T,
+    W at exterior k = 5 and interior k = 5 sqrt(2.25) on the seeded perturbed
+    icosphere L5 (P1 Dirichlet + P0 Neumann unknowns, n = 30722), incident
+    plane wave, GaussQR 5, through the reference-shaped CoreManager /
+    BEMManager API.  FP64 pipeline (c128) against the mixed ones (c64 LU with
+    FP64 refinement against the c128 matrix; FP32 regular-pair assembly in c64
+    with refinement against that matrix), and the adjoint gradient of
+    J = ||x - x*||^2 with respect to the interior wave number (incident data
+    held fixed), checked by central differences."""
+    import numpy as np
+    import torch
+
+    L = bem._lib
+    dev = torch.device("cuda", 0)
+    k, ni = 5.0, 2.25
+    v, f = bem.meshes.perturbed_icosphere(args.level, seed=0)
+    d = torch.tensor(bem.meshes.seeded_direction(2), device=dev)
+    nv, nf = len(v), len(f)
+
+    def fD(x, sqrt_n):
+        return torch.exp(1j * k * sqrt_n * (x @ d))
+
+    def fN(x, n, sqrt_n):
+        return 1j * k * sqrt_n * (n @ d) * torch.exp(1j * k * sqrt_n * (x @ d))
+
+    def manager(prec, n_i=ni):
+        params = {"kernel": "Helmholtz_Transmission", "k": k, "n_i": n_i, "n_o": 1.0, "boundary": "Full",
+                  "GaussQR": 5, "Q_Neumann": 0, "Q_Dirichlet": 1, "vertices": v, "panels": f, "precision": prec}
+        cm = bem.CoreManager(params)
+        cm.initialization(fD, fN)
+        return cm
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    ST = bem.ScopeType
+    AT = bem.AssembleType
+
+    def assemble(bm):
+        bm._mat_A.zero_()
+        bm.matrix_layer_forward(k=bm._k, sqrt_n=bm._sqrt_no, scope_type=int(ST.EXTERIOR))
+        bm.assemble_matA(assemble_type=int(AT.ADD_P_MINUS), multiplier=1.0)
+        bm.rhs_constructor.set_boundaries(scope_type=int(ST.INTERIOR), sqrt_ni=bm._sqrt_ni, sqrt_no=bm._sqrt_no)
+        bm.assemble_rhs()
+        bm.matrix_layer_forward(k=bm._k, sqrt_n=bm._sqrt_ni, scope_type=int(ST.INTERIOR))
+        bm.assemble_matA(assemble_type=int(AT.ADD_P_PLUS), multiplier=1.0)
+
+    n = nf + nv
+    log(f"transmission L{args.level}: n = {n}")
+    cm = manager("fp64")
+    bm = cm._BEM_manager
+    times = []
+    for rep in range(1 + max(1, args.steps)):
+        e = [ev() for _ in range(3)]
+        e[0].record()
+        assemble(bm)
+        e[1].record()
+        bm.solve()
+        e[2].record()
+        torch.cuda.synchronize()
+        times.append((e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])))
+        log(f"fp64 run {rep}: assembly {times[-1][0]:.1f} ms, LU + solve {times[-1][1]:.1f} ms")
+    asm64, sol64 = (float(np.mean([t[q] for t in times[1:]])) for q in (0, 1))
+    x64 = bm.raw_solved.clone()
+    b = bm.rhs[:n].clone()
+    xn = torch.linalg.vector_norm(x64).item()
+    # mixed A: c64 LU of the FP64 matrix, FP64 refinement against it
+    A64 = bm._mat_A
+    LU32 = torch.empty(n, n, dtype=torch.complex64, device=dev)
+    ipiv = torch.empty(n, dtype=torch.int32, device=dev)
+    x = torch.zeros(n, dtype=torch.complex128, device=dev)
+    for rep in range(2):
+        e = [ev() for _ in range(3)]
+        e[0].record()
+        L.convert(A64, LU32)
+        bm.ctx.lu_factor(LU32, ipiv)
+        e[1].record()
+        st = bm.ctx.refine_solve(A64, LU32, ipiv, b, x, max_iter=30, tol=1e-14)
+        e[2].record()
+        torch.cuda.synchronize()
+    mixA = {"lu_c64_ms": e[0].elapsed_time(e[1]), "refine_ms": e[1].elapsed_time(e[2]),
+            "refine_iters": int(st.iterations), "converged": int(st.converged),
+            "rel_err_vs_fp64": torch.linalg.vector_norm(x - x64).item() / xn}
+    mixA["ms"] = asm64 + mixA["lu_c64_ms"] + mixA["refine_ms"]
+    del LU32
+    # gradient with respect to the interior wave number
+    rng = np.random.default_rng(9)
+    xs = torch.tensor(rng.standard_normal(n) + 1j * rng.standard_normal(n), device=dev) * x64.abs().mean()
+    e = [ev() for _ in range(2)]
+    e[0].record()
+    gk = bem.adjoint.transmission_k_gradient(bm, 2 * (x64 - xs))
+    e[1].record()
+    torch.cuda.synchronize()
+    grad = {"dJ_dki": gk, "ms": e[0].elapsed_time(e[1])}
+    cm.kill()
+    del bm, cm, A64
+    torch.cuda.empty_cache()
+    # mixed B: FP32 regular pairs assembled into c64, c64 LU, refinement against it
+    cm32 = manager("fp32")
+    b32 = cm32._BEM_manager
+    for rep in range(2):
+        e = [ev() for _ in range(4)]
+        e[0].record()
+        assemble(b32)
+        e[1].record()
+        LU = b32._mat_A.clone()
+        b32.ctx.lu_factor(LU, ipiv)
+        e[2].record()
+        st = b32.ctx.refine_solve(b32._mat_A, LU, ipiv, b, x, max_iter=30, tol=1e-12)
+        e[3].record()
+        torch.cuda.synchronize()
+        del LU
+    mixB = {"assembly_ms": e[0].elapsed_time(e[1]), "lu_c64_ms": e[1].elapsed_time(e[2]),
+            "refine_ms": e[2].elapsed_time(e[3]), "refine_iters": int(st.iterations),
+            "rel_err_vs_fp64": torch.linalg.vector_norm(x - x64).item() / xn}
+    mixB["ms"] = mixB["assembly_ms"] + mixB["lu_c64_ms"] + mixB["refine_ms"]
+    cm32.kill()
+    del b32, cm32
+    torch.cuda.empty_cache()
+    if not args.no_fd:
+        h = 1e-6
+        js = []
+        for sgn in (1, -1):
+            ki = k * np.sqrt(ni) + sgn * h
+            c2 = manager("fp64", (ki / k) ** 2)
+            b2 = c2._BEM_manager
+            assemble(b2)
+            b2.rhs[:n] = b
+            xx = b2.solve()
+            js.append(float(((xx - xs).abs() ** 2).sum()))
+            c2.kill()
+            del b2, c2, xx
+            torch.cuda.empty_cache()
+        fd = (js[0] - js[1]) / (2 * h)
+        grad["fd"] = fd
+        grad["rel_err_vs_fd"] = abs(gk - fd) / abs(fd)
+    tts64 = asm64 + sol64
+    result = {
+        "metric": "time-to-solution of the transmission system, mixed vs FP64 GPU; dJ/dk_i by adjoint",
+        "value": tts64 / mixA["ms"], "unit": "x (FP64 time / mixed time, c64 LU + FP64 refinement)",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": tts64, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic (seeded perturbed icosphere, "
+        "plane wave)",
+        "config": {"workload": f"Helmholtz transmission k_o=5, k_i=7.5, perturbed icosphere L{args.level} "
+                               f"({nf} panels, {nv} vertices), n={n}, GaussQR 5, P1 Dirichlet + P0 Neumann"},
+        "fp64": {"assembly_ms": round(asm64, 3), "lu_solve_ms": round(sol64, 3), "ms": round(tts64, 3)},
+        "mixed_c64lu_fp64asm": mixA, "mixed_fp32asm": mixB, "speedup_fp32asm": tts64 / mixB["ms"],
+        "gradient": grad,
+    }
     print(json.dumps(result), flush=True)
     return 0
 

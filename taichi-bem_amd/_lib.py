@@ -22,6 +22,7 @@ F64, C128, F32, C64 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE, FLAG_DK = 1, 2, 4
 PART_FAR, PART_NEAR = 1, 2
+KernelType_LAPLACE, KernelType_HELMHOLTZ, KernelType_TRANSMISSION = 0, 1, 2
 
 # Every symbol include/bem_hip.h declares.
 EXPORTS = (

@@ -134,3 +134,53 @@ def point_source(vertices: torch.Tensor, k: torch.Tensor, x0, kernel: int) -> to
 def misfit(t: torch.Tensor, t_target: torch.Tensor, vertices: torch.Tensor, panels: torch.Tensor) -> torch.Tensor:
     """J = ||t - t_target||^2_M with the P0 mass (panel areas)."""
     return (panel_areas(vertices, panels) * (t - t_target).abs() ** 2).sum()
+
+
+# ---------------------------------------------------------------------------
+# Transmission problem: derivative with respect to the interior wave number
+# ---------------------------------------------------------------------------
+def transmission_dA_contract(bm, mu: torch.Tensor, x: torch.Tensor) -> complex:
+    """mu^T (dA/dk_i) x for the assembled 2 x 2 Calderon system of a
+    transmission BEMManager, k_i = k sqrt(n_i) the interior wave number.  The
+    interior blocks enter A as -W at (D,D), -K^T at (D,N), +K at (N,D) and -V
+    at (N,N) (BEMManager.matA_add_M with multiplier +1, BEM_manager.py:608-628),
+    so with the k-derivative products of the library (BEM_FLAG_DK)
+
+        mu^T dA x = -mu_D.dW x_D - x_N.dK mu_D + mu_N.dK x_D - mu_N.dV x_N
+
+    (plain bilinear dots; the transposed block is taken as x_N . dK mu_D, so
+    no transposed product is needed)."""
+    if not bm._is_transmission:
+        raise RuntimeError("transmission_dA_contract needs a transmission BEMManager")
+    C = bm.ctx
+    ki = bm._k * bm._sqrt_ni
+    qN, qD = bm.shape_func_degree_Neumann, bm.shape_func_degree_Dirichlet
+    D0, N0 = bm.get_Dirichlet_offset_j(), bm.get_Neumann_offset_j()
+    nD, nN = bm.M_Dirichlet, bm.N_Neumann
+    mu = mu.to(torch.complex128).contiguous()
+    x = x.to(torch.complex128).contiguous()
+    xD, xN = x[D0:D0 + nD].contiguous(), x[N0:N0 + nN].contiguous()
+    mD, mN = mu[D0:D0 + nD].contiguous(), mu[N0:N0 + nN].contiguous()
+    flags = bm._quadrature_flags | _lib.FLAG_DK
+
+    def dapply(op, dt, dr, v, n_out):
+        y = torch.zeros(n_out, dtype=torch.complex128, device=x.device)
+        d = _lib.Context.op_desc(op, _lib.KernelType_TRANSMISSION, ki, 1.0, dt, dr, _lib.PREC_FP64, _lib.C128, flags)
+        C.apply(d, v, y, alpha=1.0, conj_x=False)
+        return y
+
+    s = -(mD * dapply(_lib.OP_W, qD, qD, xD, nD)).sum()
+    s = s - (xN * dapply(_lib.OP_K, qN, qD, mD, nN)).sum()
+    s = s + (mN * dapply(_lib.OP_K, qN, qD, xD, nN)).sum()
+    s = s - (mN * dapply(_lib.OP_V, qN, qN, xN, nN)).sum()
+    return complex(s.item())
+
+
+def transmission_k_gradient(bm, grad_x: torch.Tensor) -> float:
+    """dJ/dk_i of a real objective J(x) of the transmission solution x =
+    A(k_i)^-1 b (b, the incident-field data, held fixed), given
+    grad_x = dJ/dRe x + i dJ/dIm x: one transposed solve with the factors of
+    BEMManager.solve, A^T mu = conj(grad_x), then -Re(mu^T dA/dk_i x)."""
+    mu = grad_x.conj().to(bm._LU.dtype).contiguous().clone()
+    bm.ctx.lu_solve_transpose(bm._LU, bm._ipiv, mu)
+    return -transmission_dA_contract(bm, mu, bm.raw_solved).real

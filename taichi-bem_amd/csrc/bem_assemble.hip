@@ -733,10 +733,12 @@ __device__ __forceinline__ int cf_find(const CfProb* tbl, int count, int64_t ite
     return lo;
 }
 
-// V and K of a regular P1 x P1 pair from one evaluation per point pair.
-template <typename R, bool HELM>
-__device__ __forceinline__ void far_pair_vk(const OpParams& p, int i, int j, double (&Vr)[3][3], double (&Vi)[3][3],
-                                            double (&Kr)[3][3], double (&Ki)[3][3]) {
+// K - i eta V of a regular P1 x P1 pair: one evaluation of r, 1/r and
+// exp(i kap r) per point pair feeds both kernels, and only the combination
+// is accumulated (six partial sums per trial point instead of twelve).
+template <typename R>
+__device__ __forceinline__ void far_pair_cf(const OpParams& p, double eta, int i, int j, double (&Cr)[3][3],
+                                            double (&Ci)[3][3]) {
     const int q2 = p.q2, nf = p.nf;
     const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
     const R* __restrict__ PL = Ptr<R>::pl(p);
@@ -748,20 +750,20 @@ __device__ __forceinline__ void far_pair_vk(const OpParams& p, int i, int j, dou
         dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
     }
     const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
-    const R kap = (R)p.kap;
-    R avr[3][3], avi[3][3], akr[3][3], aki[3][3];
+    const R kap = (R)p.kap, et = (R)eta;
+    R acr[3][3], aci[3][3];
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int t = 0; t < 3; ++t) { avr[s][t] = avi[s][t] = akr[s][t] = aki[s][t] = 0; }
+        for (int t = 0; t < 3; ++t) { acr[s][t] = aci[s][t] = 0; }
     for (int b = 0; b < q2; ++b) {
         const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
         const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
         const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
         const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
-        R pvr[3], pvi[3], pkr[3], pki[3];
+        R pcr[3], pci[3];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) { pvr[s] = pvi[s] = pkr[s] = pki[s] = 0; }
+        for (int s = 0; s < 3; ++s) { pcr[s] = pci[s] = 0; }
         for (int a = 0; a < q2; ++a) {
             const R dx = PU[4 * a + 0] + ex;
             const R dy = PU[4 * a + 1] + ey;
@@ -771,38 +773,30 @@ __device__ __forceinline__ void far_pair_vk(const OpParams& p, int i, int j, dou
             const R rinv = Math<R>::rsqrt(r2);
             const R g = wa * rinv;
             const R base = g * (dx * nnx + dy * nny + dz * nnz) * (rinv * rinv);
-            R fvr, fvi, fkr, fki;
-            if constexpr (HELM) {
-                const R kr = kap * (r2 * rinv);
-                R sn, cs;
-                Math<R>::sincos(kr, &sn, &cs);
-                fvr = g * cs; fvi = g * sn;
-                fkr = base * (cs + kr * sn); fki = base * (sn - kr * cs);
-            } else { fvr = g; fvi = 0; fkr = base; fki = 0; }
+            const R kr = kap * (r2 * rinv);
+            R sn, cs;
+            Math<R>::sincos(kr, &sn, &cs);
+            // K: base (cs + kr sn, sn - kr cs); -i eta V: eta g (sn, -cs)
+            const R eg = et * g;
+            const R fr = base * (cs + kr * sn) + eg * sn;
+            const R fi = base * (sn - kr * cs) - eg * cs;
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
                 const R ph = PHI[3 * a + s];
-                pvr[s] += fvr * ph; pvi[s] += fvi * ph;
-                pkr[s] += fkr * ph; pki[s] += fki * ph;
+                pcr[s] += fr * ph; pci[s] += fi * ph;
             }
         }
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             const R wt = wb * PHI[3 * b + t];
 #pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                avr[s][t] += pvr[s] * wt; avi[s][t] += pvi[s] * wt;
-                akr[s][t] += pkr[s] * wt; aki[s][t] += pki[s] * wt;
-            }
+            for (int s = 0; s < 3; ++s) { acr[s][t] += pcr[s] * wt; aci[s][t] += pci[s] * wt; }
         }
     }
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            Vr[s][t] = kInv4Pi * (double)avr[s][t]; Vi[s][t] = kInv4Pi * (double)avi[s][t];
-            Kr[s][t] = kInv4Pi * (double)akr[s][t]; Ki[s][t] = kInv4Pi * (double)aki[s][t];
-        }
+        for (int t = 0; t < 3; ++t) { Cr[s][t] = kInv4Pi * (double)acr[s][t]; Ci[s][t] = kInv4Pi * (double)aci[s][t]; }
 }
 
 // K - i eta V into A at (tri_i[s], tri_j[t])
@@ -834,9 +828,14 @@ __global__ __launch_bounds__(256) void cf_far_kernel(const CfProb* tbl, int coun
         const int i_end = min(p.nf, (ib + 1) * TI);
         for (int i = ib * TI; i < i_end; ++i) {
             if (shares_vertex(p.tris, i, j)) continue;
-            double Vr[3][3], Vi[3][3], Kr[3][3], Ki[3][3];
-            far_pair_vk<R, true>(p, i, j, Vr, Vi, Kr, Ki);
-            cf_scatter(p, P.eta, i, j, Vr, Vi, Kr, Ki);
+            double Cr[3][3], Ci[3][3];
+            far_pair_cf<R>(p, P.eta, i, j, Cr, Ci);
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+                    add_elem(p.out, p.dtype, (int64_t)p.tris[3 * i + s] * p.ld + p.tris[3 * j + t], Cr[s][t],
+                             Ci[s][t], true);
         }
     }
 }

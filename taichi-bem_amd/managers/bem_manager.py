@@ -332,6 +332,7 @@ class BEMManager:
         self.raw_solved = None
         self.raw_analytical_solved = None
         self._mat_A = None
+        self._LU = self._ipiv = None
         self.rhs = None
         for layer in ("single_layer", "double_layer", "adj_double_layer", "hypersingular_layer",
                       "identity_layer", "rhs_constructor"):
