@@ -210,6 +210,20 @@ int bem_lu_factor(bem_context* ctx, void* A, int64_t ld, int32_t n, int32_t dtyp
 int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
                  const int32_t* ipiv, void* b, void* stream);
 
+/* ---- options ---------------------------------------------------------- */
+enum {
+    BEM_OPT_C64_LU_GEMM = 1  /* arithmetic of the trailing updates of complex64
+                                LU factorisations: 0 (default) = FP32 MFMA;
+                                6 = bf16 MFMA on operands split in three bf16
+                                parts, six part products (about FP32
+                                accuracy); 3 = two parts, three products
+                                (about 16 bits, for refinement of
+                                well-conditioned systems) */
+};
+
+/* Sets a context option.  No reference counterpart. */
+int bem_set_option(bem_context* ctx, int32_t option, int32_t value);
+
 /* ---- batches of uneven sizes (wave-number sweeps, several meshes) ---- */
 
 /* One problem of a combined-field batch. */

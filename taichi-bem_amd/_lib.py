@@ -23,6 +23,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE, FLAG_DK = 1, 2, 4
 PART_FAR, PART_NEAR = 1, 2
 KernelType_LAPLACE, KernelType_HELMHOLTZ, KernelType_TRANSMISSION = 0, 1, 2
+OPT_C64_LU_GEMM = 1
 
 # Every symbol include/bem_hip.h declares.
 EXPORTS = (
@@ -31,7 +32,7 @@ EXPORTS = (
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
     "bem_shape_gradient", "bem_area_gradient", "bem_assemble_combined_batched", "bem_lu_factor_batched",
-    "bem_batch_solve", "bem_batch_condest",
+    "bem_batch_solve", "bem_batch_condest", "bem_set_option",
 )
 
 
@@ -106,6 +107,7 @@ def lib() -> ctypes.CDLL:
             "bem_lu_solve_transpose": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
             "bem_shape_gradient": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, vp, dbl, dbl, vp, vp, vp]),
             "bem_area_gradient": (ctypes.c_int, [vp, vp, vp, vp]),
+            "bem_set_option": (ctypes.c_int, [vp, i32, i32]),
             "bem_assemble_combined_batched": (ctypes.c_int, [vp, i32, ctypes.POINTER(BemCfProblem), i32, i32, i32,
                                                              ctypes.POINTER(BemTuning), vp]),
             "bem_lu_factor_batched": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(i64),
@@ -273,6 +275,10 @@ class Context:
         _require_cuda(x, y)
         check(lib().bem_mass_apply(self._h, int(deg_test), int(deg_trial), dtype_code(y), x.data_ptr(),
                                    int(bool(conj_x)), y.data_ptr(), float(alpha), current_stream()))
+
+    def set_option(self, option: int, value: int) -> None:
+        """bem_set_option (OPT_C64_LU_GEMM: 0 FP32 MFMA (default), 6 split-bf16 x6, 3 split-bf16 x3)."""
+        check(lib().bem_set_option(self._h, int(option), int(value)))
 
     def lu_factor(self, A, ipiv) -> None:
         _require_cuda(A, ipiv)

@@ -188,6 +188,19 @@ int bem_assemble_combined_batched(bem_context* ctx, int32_t count, const bem_cf_
     });
 }
 
+int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
+    if (!ctx) return set_error(BEM_ERR_INVALID, "null context");
+    switch (option) {
+        case BEM_OPT_C64_LU_GEMM:
+            if (value != 0 && value != 3 && value != 6)
+                return set_error(BEM_ERR_INVALID, "BEM_OPT_C64_LU_GEMM takes 0, 3 or 6");
+            ctx->impl.c64_gemm_parts = value == 0 ? 0 : (value == 3 ? 2 : 3);
+            return BEM_OK;
+        default:
+            return set_error(BEM_ERR_INVALID, "unknown option");
+    }
+}
+
 int bem_lu_factor_batched(bem_context* ctx, int32_t count, void* const* A, const int64_t* ld, const int32_t* n,
                           int32_t dtype, int32_t* const* ipiv, int32_t* info, void* stream) {
     return with_device(ctx, [&]() -> int {

@@ -79,6 +79,7 @@ struct Context {
     size_t cf_bytes = 0;
     std::vector<char> cf_table;
     int num_cu = 0;
+    int c64_gemm_parts = 0;       // c64 LU trailing updates: 0 FP32 MFMA, 2/3 split-bf16 parts
 };
 
 // error helpers (bem_capi.hip)

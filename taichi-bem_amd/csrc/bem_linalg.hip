@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "bem_device.h"
@@ -91,18 +92,22 @@ __device__ __forceinline__ LuProb get_prob(const LuProb& inl, const LuProb* tbl)
     else return inl;
 }
 
-// Row interchanges of the panel [k, kend) applied to columns outside it.
+// Row interchanges of pivots [p_lo, p_hi) applied to the columns in
+// [c_lo, c_hi) outside [x_lo, x_hi) (all bounds clamped to the problem's n).
 template <typename T, bool C, bool B>
-__global__ void laswp_kernel(LuProb inl, const LuProb* tbl, int k) {
+__global__ void laswp_kernel(LuProb inl, const LuProb* tbl, int p_lo, int p_hi, int c_lo, int c_hi, int x_lo,
+                             int x_hi) {
     const LuProb P = get_prob<B>(inl, tbl);
     T* A = (T*)P.A;
     const int64_t ld = P.ld;
     const int n = P.n;
     const int32_t* ipiv = P.ipiv;
-    const int kend = min(n, k + NB);
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || col >= n || (col >= k && col < kend)) return;
-    for (int c = k; c < kend; ++c) {
+    p_hi = min(n, p_hi);
+    c_hi = min(n, c_hi);
+    x_hi = min(n, x_hi);
+    const int col = c_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p_lo >= p_hi || col >= c_hi || (col >= x_lo && col < x_hi)) return;
+    for (int c = p_lo; c < p_hi; ++c) {
         int p = ipiv[c];
         if (p != c) {
             cx<T> a = Acc<T, C>::ld(A, (int64_t)c * ld + col);
@@ -255,13 +260,13 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
 // updating rows i = g (mod 4): once X[j][t] is final it is eliminated from
 // the rows below with independent updates.
 template <typename T, bool C, bool B>
-__global__ __launch_bounds__(256) void linv_kernel(LuProb inl, const LuProb* tbl, int k) {
+__global__ __launch_bounds__(256) void linv_kernel(LuProb inl, const LuProb* tbl, int k, int slot) {
     const LuProb P = get_prob<B>(inl, tbl);
     const T* A = (const T*)P.A;
     const int64_t ld = P.ld;
     const int kend = min(P.n, k + NB), w = kend - k;
-    if (kend >= P.n) return;  // no trailing columns
-    T* SL = (T*)P.SL;
+    if (kend >= P.n) return;  // no columns right of the block
+    T* SL = (T*)P.SL + (C ? 2 : 1) * (int64_t)slot * NB * NB;
     __shared__ cx<T> M[NB][NB + 1];
     __shared__ cx<T> X[NB][NB + 1];
     const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -327,25 +332,53 @@ struct GemmArgs {
     int M, N, K;
 };
 
-// The two LU products of panel k of problem P: mode 0 is U12 = A12 - S_L A12
-// (in place), mode 1 the trailing update A22 -= L21 U12.
+// The LU products of one step for problem P (bounds clamped to its n).  The
+// factorisation is blocked twice: inner panels of NB columns inside outer
+// blocks [K0, K1) of W2 columns.
+//   mode 0  inner U12:      A[k:kend, kend:K1] -= S_L A[k:kend, kend:K1]   (in place)
+//   mode 1  inner trailing: A[kend:n, kend:K1] -= A[kend:n, k:kend] A[k:kend, kend:K1]
+//   mode 2  outer U12, block b, elimination of the solved blocks above it:
+//           A[r0:r0+wb, K1:n] -= A[r0:r0+wb, K0:r0] A[K0:r0, K1:n]
+//   mode 3  outer U12, block b: A[r0:r0+wb, K1:n] -= S_L,b A[r0:r0+wb, K1:n]   (in place)
+//   mode 4  outer trailing: A[K1:n, K1:n] -= A[K1:n, K0:K1] A[K0:K1, K1:n]
+// with r0 = K0 + b NB, S_L = I - inv(L_bb) of the block's unit lower factor.
+constexpr int W2 = 256;
+
+struct LuStep {
+    int mode, K0, k, b;
+};
+
 template <typename T, bool C>
-__device__ __forceinline__ GemmArgs<T> lu_gemm_args(const LuProb& P, int mode, int k) {
+__host__ __device__ inline GemmArgs<T> lu_gemm_args(const LuProb& P, const LuStep& st) {
     constexpr int E = C ? 2 : 1;
     T* A = (T*)P.A;
     const int64_t ld = P.ld;
-    const int kend = min(P.n, k + NB), w = kend - k, N = P.n - kend;
-    if (mode == 0)
-        return {A + E * ((int64_t)k * ld + kend), ld, (const T*)P.SL, (int64_t)NB,
-                A + E * ((int64_t)k * ld + kend), ld, w, N, w};
-    return {A + E * ((int64_t)kend * ld + kend), ld, A + E * ((int64_t)kend * ld + k), ld,
-            A + E * ((int64_t)k * ld + kend), ld, N, N, w};
+    const int n = P.n;
+    const int K1 = n < st.K0 + W2 ? n : st.K0 + W2;
+    auto at = [&](int r, int c) { return A + E * ((int64_t)r * ld + c); };
+    GemmArgs<T> g{A, ld, A, ld, A, ld, 0, 0, 0};
+    if (st.K0 >= n) return g;
+    if (st.mode <= 1) {
+        const int k = st.k, kend = n < k + NB ? n : k + NB, w = kend - k;
+        if (k >= n) return g;
+        if (st.mode == 0)
+            return {at(k, kend), ld, (const T*)P.SL + E * (int64_t)st.b * NB * NB, (int64_t)NB, at(k, kend), ld, w,
+                    K1 - kend, w};
+        return {at(kend, kend), ld, at(kend, k), ld, at(k, kend), ld, n - kend, K1 - kend, w};
+    }
+    if (st.mode == 4) return {at(K1, K1), ld, at(K1, st.K0), ld, at(st.K0, K1), ld, n - K1, n - K1, K1 - st.K0};
+    const int r0 = st.K0 + st.b * NB;
+    if (r0 >= K1) return g;
+    const int wb = K1 - r0 < NB ? K1 - r0 : NB;
+    if (st.mode == 2) return {at(r0, K1), ld, at(r0, st.K0), ld, at(st.K0, K1), ld, wb, n - K1, r0 - st.K0};
+    return {at(r0, K1), ld, (const T*)P.SL + E * (int64_t)st.b * NB * NB, (int64_t)NB, at(r0, K1), ld, wb, n - K1,
+            wb};
 }
 
 template <typename T, bool C, bool B>
-__global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const LuProb* tbl, int mode, int k) {
+__global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const LuProb* tbl, LuStep st) {
     GemmArgs<T> ga = inl;
-    if constexpr (B) ga = lu_gemm_args<T, C>(tbl[blockIdx.z], mode, k);
+    if constexpr (B) ga = lu_gemm_args<T, C>(tbl[blockIdx.z], st);
     T* Cm = ga.Cm;
     const T* Am = ga.Am;
     const T* Bm = ga.Bm;
@@ -700,72 +733,221 @@ __global__ __launch_bounds__(256) void axpy_kernel(const T* src, int64_t lds, T*
     }
 }
 
-template <typename T, bool C>
-GemmArgs<T> lu_gemm_args_host(const LuProb& P, int mode, int k) {
-    constexpr int E = C ? 2 : 1;
-    T* A = (T*)P.A;
-    const int64_t ld = P.ld;
-    const int kend = std::min(P.n, k + NB), w = kend - k, N = P.n - kend;
-    if (mode == 0)
-        return {A + E * ((int64_t)k * ld + kend), ld, (const T*)P.SL, (int64_t)NB,
-                A + E * ((int64_t)k * ld + kend), ld, w, N, w};
-    return {A + E * ((int64_t)kend * ld + kend), ld, A + E * ((int64_t)kend * ld + k), ld,
-            A + E * ((int64_t)k * ld + kend), ld, N, N, w};
+// ---------------------------------------------------------------------------
+// Complex64 C -= A B on the bf16 MFMA (v_mfma_f32_16x16x32_bf16, FP32
+// accumulation) with split operands: every FP32 value is a sum of NP bf16
+// parts (x = h + m [+ l], each part the bf16 rounding of the remainder) and a
+// real product keeps the part products of order >= 2^-16 (NP = 2: hh, hm, mh)
+// or >= 2^-24 (NP = 3: also hl, mm, lh), so the update carries about 16 or 24
+// bits.  Operand layout measured on gfx950: lane l supplies
+// A[l%16][8*(l/16)+i] and B[8*(l/16)+i][l%16] (i = 0..7); result register r
+// holds row 4*(l/16)+r, column l%16.  The parts are formed once per element
+// when a K chunk is staged in LDS (A row-major, B column-major, rows padded to
+// avoid bank conflicts); the complex product uses four real products, the
+// real part accumulated as (re re) and (im im) separately.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int BKC = 32;       // K chunk (the MFMA depth)
+constexpr int BKP = BKC + 8;  // padded row length in LDS (80 bytes)
+
+template <int NP>
+__device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[NP]) {
+    const __bf16 h = (__bf16)x;
+    p[0] = h;
+    float r = x - (float)h;
+    if constexpr (NP >= 2) {
+        const __bf16 m = (__bf16)r;
+        p[1] = m;
+        r -= (float)m;
+    }
+    if constexpr (NP >= 3) p[2] = (__bf16)r;
 }
 
-// Launch sequence of the right-looking LU for one problem (B = false, the
-// problem by value) or a batch (B = true, table on the device, problems
-// sorted by decreasing n so that the first `active` entries are the ones
-// with columns left at panel k).
-template <typename T, bool C, bool B>
-void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, hipStream_t s) {
-    const int nmax = ns[0];
-    for (int k = 0; k < nmax; k += NB) {
-        int active = 0;
-        while (active < (int)ns.size() && ns[active] > k) ++active;
-        auto max_over = [&](auto f) {
-            int m = 0;
-            for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
-            return m;
-        };
-        const int kendmax = std::min(nmax, k + NB);
-        for (int c = k; c <= kendmax; ++c) {
-            const int G = max_over([&](int n) {
-                const int kend = std::min(n, k + NB);
-                if (c > kend) return 0;
-                const int rows = n - (c < kend ? c : kend);
-                return rows > 0 ? (rows + RPB - 1) / RPB : 0;
-            });
-            if (G <= 0) break;
-            hipLaunchKernelGGL((col_step_kernel<T, C, B>), dim3(G, 1, active), dim3(256), 0, s, one, d_tbl, k, c);
+template <int NP, bool B>
+__global__ __launch_bounds__(256) void gemm_sub_c64bf_kernel(GemmArgs<float> inl, const LuProb* tbl, LuStep st) {
+    GemmArgs<float> ga = inl;
+    if constexpr (B) ga = lu_gemm_args<float, true>(tbl[blockIdx.z], st);
+    const int M = ga.M, N = ga.N, K = ga.K;
+    if (M <= 0 || N <= 0 || (int)blockIdx.y * GT >= M || (int)blockIdx.x * GT >= N) return;
+    __shared__ __bf16 sA[2][NP][GT][BKP];
+    __shared__ __bf16 sB[2][NP][GT][BKP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bm = blockIdx.y * GT, bn = blockIdx.x * GT;
+    const int wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
+    f32x4_t accR[2][2], accII[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accR[a][b] = f32x4_t{0, 0, 0, 0};
+            accII[a][b] = f32x4_t{0, 0, 0, 0};
+            accI[a][b] = f32x4_t{0, 0, 0, 0};
         }
-        hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((nmax + 255) / 256, 1, active), dim3(256), 0, s, one,
-                           d_tbl, k);
-        const int N = max_over([&](int n) { return n - std::min(n, k + NB); });
-        if (N > 0) {
-            hipLaunchKernelGGL((linv_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
-            GemmArgs<T> g0{}, g1{};
-            if constexpr (!B) {
-                g0 = lu_gemm_args_host<T, C>(one, 0, k);
-                g1 = lu_gemm_args_host<T, C>(one, 1, k);
+    const float* Am = ga.Am;
+    const float* Bm = ga.Bm;
+    for (int k0 = 0; k0 < K; k0 += BKC) {
+        // stage A (GT rows x BKC) and B (BKC x GT, stored column-major)
+        for (int e = tid; e < GT * BKC; e += 256) {
+            const int r = e / BKC, c = e % BKC;
+            const int gr = bm + r, gc = k0 + c;
+            float re = 0.f, im = 0.f;
+            if (gr < M && gc < K) {
+                re = Am[2 * ((int64_t)gr * ga.lda + gc)];
+                im = Am[2 * ((int64_t)gr * ga.lda + gc) + 1];
             }
-            // U12 = A12 - S_L A12 (in place: one block row, each block owns its columns)
-            hipLaunchKernelGGL((gemm_sub_kernel<T, C, B>), dim3((N + GT - 1) / GT, 1, active), dim3(256), 0, s,
-                               g0, d_tbl, 0, k);
-            // A22 -= L21 U12
-            hipLaunchKernelGGL((gemm_sub_kernel<T, C, B>), dim3((N + GT - 1) / GT, (N + GT - 1) / GT, active),
-                               dim3(256), 0, s, g1, d_tbl, 1, k);
+            __bf16 pr[NP], pi[NP];
+            split_bf16<NP>(re, pr);
+            split_bf16<NP>(im, pi);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) { sA[0][q][r][c] = pr[q]; sA[1][q][r][c] = pi[q]; }
+        }
+        for (int e = tid; e < BKC * GT; e += 256) {
+            const int r = e / GT, c = e % GT;
+            const int gr = k0 + r, gc = bn + c;
+            float re = 0.f, im = 0.f;
+            if (gr < K && gc < N) {
+                re = Bm[2 * ((int64_t)gr * ga.ldb + gc)];
+                im = Bm[2 * ((int64_t)gr * ga.ldb + gc) + 1];
+            }
+            __bf16 pr[NP], pi[NP];
+            split_bf16<NP>(re, pr);
+            split_bf16<NP>(im, pi);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) { sB[0][q][c][r] = pr[q]; sB[1][q][c][r] = pi[q]; }
+        }
+        __syncthreads();
+        const int kof = 8 * (lane >> 4);
+        bf16x8_t aR[2][NP], aI[2][NP], bR[2][NP], bI[2][NP];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const int row = wm + 16 * a + (lane & 15);
+                aR[a][q] = *(const bf16x8_t*)&sA[0][q][row][kof];
+                aI[a][q] = *(const bf16x8_t*)&sA[1][q][row][kof];
+                const int col = wn + 16 * a + (lane & 15);
+                bR[a][q] = *(const bf16x8_t*)&sB[0][q][col][kof];
+                bI[a][q] = *(const bf16x8_t*)&sB[1][q][col][kof];
+            }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int pa = 0; pa < NP; ++pa)
+#pragma unroll
+                    for (int pb = 0; pb < NP; ++pb) {
+                        if (pa + pb >= NP) continue;  // part products below 2^-(8 NP)
+                        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aR[a][pa], bR[b][pb], accR[a][b], 0, 0, 0);
+                        accII[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aI[a][pa], bI[b][pb], accII[a][b], 0, 0, 0);
+                        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aR[a][pa], bI[b][pb], accI[a][b], 0, 0, 0);
+                        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aI[a][pa], bR[b][pb], accI[a][b], 0, 0, 0);
+                    }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = bm + wm + 16 * a + 4 * (lane >> 4) + r;
+                const int gc = bn + wn + 16 * b + (lane & 15);
+                if (gr < M && gc < N) {
+                    float* c = ga.Cm + 2 * ((int64_t)gr * ga.ldc + gc);
+                    c[0] -= accR[a][b][r] - accII[a][b][r];
+                    c[1] -= accI[a][b][r];
+                }
+            }
+}
+
+// bf16_parts: 0 = FP32 MFMA for every c64 product; 2 or 3 = the split bf16
+// kernel for the trailing updates (modes 1 and 4) of c64 factorisations.
+template <typename T, bool C, bool B>
+void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, int N, int active, int bf16_parts,
+             hipStream_t s) {
+    if (M <= 0 || N <= 0) return;
+    GemmArgs<T> g{};
+    if constexpr (!B) g = lu_gemm_args<T, C>(one, st);
+    const dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, active);
+    if constexpr (std::is_same<T, float>::value && C) {
+        if (bf16_parts && (st.mode == 1 || st.mode == 4)) {
+            if (bf16_parts == 2) hipLaunchKernelGGL((gemm_sub_c64bf_kernel<2, B>), grid, dim3(256), 0, s, g, d_tbl, st);
+            else hipLaunchKernelGGL((gemm_sub_c64bf_kernel<3, B>), grid, dim3(256), 0, s, g, d_tbl, st);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((gemm_sub_kernel<T, C, B>), grid, dim3(256), 0, s, g, d_tbl, st);
+}
+
+// Launch sequence of the two-level right-looking LU for one problem (B =
+// false, the problem by value) or a batch (B = true, table on the device,
+// problems sorted by decreasing n so that the first `active` entries are
+// the ones with columns left at a given step).
+template <typename T, bool C, bool B>
+void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int bf16_parts, hipStream_t s) {
+    const int nmax = ns[0];
+    auto count_active = [&](int from) {
+        int a = 0;
+        while (a < (int)ns.size() && ns[a] > from) ++a;
+        return a;
+    };
+    for (int K0 = 0; K0 < nmax; K0 += W2) {
+        const int K1max = std::min(nmax, K0 + W2);
+        const int act_outer = count_active(K0);
+        for (int k = K0; k < K1max; k += NB) {
+            const int active = count_active(k);
+            auto max_over = [&](auto f) {
+                int m = 0;
+                for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
+                return m;
+            };
+            const int kendmax = std::min(nmax, k + NB);
+            for (int c = k; c <= kendmax; ++c) {
+                const int G = max_over([&](int n) {
+                    const int kend = std::min(n, k + NB);
+                    if (c > kend) return 0;
+                    const int rows = n - (c < kend ? c : kend);
+                    return rows > 0 ? (rows + RPB - 1) / RPB : 0;
+                });
+                if (G <= 0) break;
+                hipLaunchKernelGGL((col_step_kernel<T, C, B>), dim3(G, 1, active), dim3(256), 0, s, one, d_tbl, k,
+                                   c);
+            }
+            // this panel's interchanges inside the outer block
+            hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((W2 + 255) / 256, 1, active), dim3(256), 0, s, one,
+                               d_tbl, k, k + NB, K0, K0 + W2, k, k + NB);
+            const int slot = (k - K0) / NB;
+            hipLaunchKernelGGL((linv_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, slot);
+            const int Nin = max_over([&](int n) { return std::min(n, K0 + W2) - std::min(n, k + NB); });
+            const int Min = max_over([&](int n) { return n - std::min(n, k + NB); });
+            lu_gemm<T, C, B>(one, d_tbl, LuStep{0, K0, k, slot}, NB, Nin, active, bf16_parts, s);
+            lu_gemm<T, C, B>(one, d_tbl, LuStep{1, K0, k, slot}, Min, Nin, active, bf16_parts, s);
+        }
+        // the outer block's interchanges on the columns left and right of it
+        hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((nmax + 255) / 256, 1, act_outer), dim3(256), 0, s, one,
+                           d_tbl, K0, K0 + W2, 0, nmax, K0, K0 + W2);
+        int Nout = 0;
+        for (int q = 0; q < act_outer; ++q) Nout = std::max(Nout, ns[q] - std::min(ns[q], K0 + W2));
+        if (Nout > 0) {
+            const int nb_blocks = (K1max - K0 + NB - 1) / NB;
+            for (int b = 0; b < nb_blocks; ++b) {
+                if (b > 0) lu_gemm<T, C, B>(one, d_tbl, LuStep{2, K0, K0, b}, NB, Nout, act_outer, bf16_parts, s);
+                lu_gemm<T, C, B>(one, d_tbl, LuStep{3, K0, K0, b}, NB, Nout, act_outer, bf16_parts, s);
+            }
+            lu_gemm<T, C, B>(one, d_tbl, LuStep{4, K0, K0, 0}, Nout, Nout, act_outer, bf16_parts, s);
         }
     }
 }
 
 // Per-problem scratch: counter and info words, the per-block pivot
-// partials and S_L (NB x NB).
+// partials and S_L of each NB block of an outer block (W2 / NB of them).
 template <typename T, bool C>
 size_t lu_scratch_bytes(int n) {
     const int gmax = (n + RPB - 1) / RPB;
     return 512 + (((size_t)gmax * (sizeof(double) + sizeof(int)) + 255) & ~(size_t)255) +
-           NB * NB * sizeof(cx<T>) + 256;
+           (W2 / NB) * NB * NB * sizeof(cx<T>) + 256;
 }
 
 template <typename T, bool C>
@@ -813,7 +995,7 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     ctx->lu_table.clear();  // this call reuses the scratch a batch table may occupy
     const LuProb P = lu_carve<T, C>((char*)ctx->lu_scratch, A, ld, n, ipiv, nullptr);
     hipLaunchKernelGGL(lu_reset_one_kernel, dim3(1), dim3(1), 0, s, P);
-    lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, s);
+    lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->c64_gemm_parts, s);
     rc = check_hip(hipGetLastError(), "LU launch");
     if (rc) return rc;
     int h_info = 0;
@@ -864,7 +1046,7 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
         ctx->lu_table_dev = (void*)d_tbl;
     }
     hipLaunchKernelGGL(lu_reset_kernel, dim3((count + 63) / 64), dim3(64), 0, s, (const LuProb*)d_tbl, count);
-    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, s);
+    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->c64_gemm_parts, s);
     return check_hip(hipGetLastError(), "batched LU launch");
 }
 

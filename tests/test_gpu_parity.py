@@ -184,15 +184,17 @@ def test_lu_solve(bem, dtype, tol, n):
     assert np.all(piv >= np.arange(n)) and np.all(piv < n)
 
 
-@pytest.mark.parametrize("n", [150, 321])
-def test_lu_factors_reconstruct(bem, n):
-    """Packed factors and LAPACK-style interchanges satisfy P A = L U
-    (tournament pivoting picks the pivot rows, so they need not be
-    LAPACK's), with bounded multipliers."""
+@pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-13), (torch.complex64, 2e-5)])
+@pytest.mark.parametrize("n", [150, 321, 600])
+def test_lu_factors_reconstruct(bem, n, dtype, tol):
+    """Packed factors and LAPACK-style interchanges satisfy P A = L U with
+    bounded multipliers (sizes crossing the 64-column panels and the
+    256-column outer blocks)."""
     rng = np.random.default_rng(11)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     ctx = bem.Context(0)
-    At = torch.tensor(A, device=DEV).contiguous()
+    At = torch.tensor(A, device=DEV).to(dtype).contiguous()
+    A = At.cpu().numpy().astype(np.complex128)
     ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
     ctx.lu_factor(At, ipiv)
     lu, piv = to_np(At), to_np(ipiv)
@@ -202,8 +204,42 @@ def test_lu_factors_reconstruct(bem, n):
         perm[[c, p]] = perm[[p, c]]
     L = np.tril(lu, -1) + np.eye(n)
     U = np.triu(lu)
-    assert rel(L @ U, A[perm]) <= 1e-13
+    assert rel(L @ U, A[perm]) <= tol
     assert np.abs(L).max() < 20.0
+
+
+@pytest.mark.parametrize("mode,tol", [(0, 2e-3), (6, 2e-3), (3, 5e-2)])
+def test_lu_c64_gemm_modes(bem, mode, tol):
+    """The c64 trailing-update arithmetic (FP32 MFMA, split-bf16 with six or
+    three part products) solves at its stated accuracy, and FP64 refinement
+    against the complex128 matrix converges to 1e-12 with each."""
+    n = 600
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + np.eye(n) * np.sqrt(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ctx = bem.Context(0)
+    ctx.set_option(bem._lib.OPT_C64_LU_GEMM, mode)
+    A64 = torch.tensor(A, device=DEV)
+    LU = A64.to(torch.complex64).contiguous()
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    print(f"[lu c64 mode {mode}]", flush=True)
+    ctx.lu_factor(LU, ipiv)
+    x32 = torch.tensor(b, device=DEV).to(torch.complex64)
+    ctx.lu_solve(LU, ipiv, x32)
+    ref = np.linalg.solve(A, b)
+    assert rel(to_np(x32).astype(np.complex128), ref) <= tol
+    x = torch.zeros(n, dtype=torch.complex128, device=DEV)
+    st = ctx.refine_solve(A64, LU, ipiv, torch.tensor(b, device=DEV), x, max_iter=30, tol=1e-14)
+    assert st.converged == 1
+    assert rel(to_np(x), ref) <= 1e-12
+
+
+def test_set_option_rejects(bem):
+    ctx = bem.Context(0)
+    with pytest.raises(bem.BEMError):
+        ctx.set_option(bem._lib.OPT_C64_LU_GEMM, 5)
+    with pytest.raises(bem.BEMError):
+        ctx.set_option(99, 0)
 
 
 def test_lu_singular_raises(bem):
