@@ -218,7 +218,21 @@ enum {
                                 parts, six part products (about FP32
                                 accuracy); 3 = two parts, three products
                                 (about 16 bits, for refinement of
-                                well-conditioned systems) */
+                                well-conditioned systems) */,
+    BEM_OPT_LU_LOOKAHEAD = 2 /* bem_lu_factor outside graph capture: 0 = one
+                                stream; N > 0 (default 32) = the panels of the
+                                next 256-column block run on N compute units
+                                while the rest of the trailing update runs on
+                                the others (two internal streams, default
+                                priority, ordered by events against the
+                                caller's stream); -1 = the same two streams
+                                without compute-unit masks */,
+    BEM_OPT_LU_PIVOTING = 3  /* 1 (default) = tournament pivoting per 64-column
+                                panel (pivot rows chosen by a reduction tree
+                                of LDS-resident eliminations, about ten
+                                launches per panel); 0 = partial pivoting
+                                with one launch per column, the pivot rows
+                                LAPACK's getrf chooses */
 };
 
 /* Sets a context option.  No reference counterpart. */

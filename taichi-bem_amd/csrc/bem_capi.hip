@@ -84,6 +84,10 @@ int bem_destroy(bem_context* ctx) {
         if (ctx->impl.lu_scratch) hipFree(ctx->impl.lu_scratch);
         for (void* w : ctx->impl.batch_ws)
             if (w) hipFree(w);
+        for (hipStream_t st : ctx->impl.lu_stream)
+            if (st) hipStreamDestroy(st);
+        for (hipEvent_t e : ctx->impl.lu_event)
+            if (e) hipEventDestroy(e);
         if (ctx->impl.cf_ws) hipFree(ctx->impl.cf_ws);
         return BEM_OK;
     });
@@ -191,6 +195,14 @@ int bem_assemble_combined_batched(bem_context* ctx, int32_t count, const bem_cf_
 int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
     if (!ctx) return set_error(BEM_ERR_INVALID, "null context");
     switch (option) {
+        case BEM_OPT_LU_PIVOTING:
+            if (value != 0 && value != 1) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_PIVOTING takes 0 or 1");
+            ctx->impl.lu_pivoting = value;
+            return BEM_OK;
+        case BEM_OPT_LU_LOOKAHEAD:
+            if (value < -1 || value > 1024) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_LOOKAHEAD takes -1..1024");
+            ctx->impl.lu_lookahead_cus = value;
+            return BEM_OK;
         case BEM_OPT_C64_LU_GEMM:
             if (value != 0 && value != 3 && value != 6)
                 return set_error(BEM_ERR_INVALID, "BEM_OPT_C64_LU_GEMM takes 0, 3 or 6");

@@ -84,6 +84,8 @@ struct LuProb {
     double* pval;
     int* pidx;
     void* SL;
+    int* tids;   // tournament candidates: two levels of ceil(n / TRW) groups x NB row ids
+    void* SU;    // -inv(U11) of the current panel (NB x NB)
 };
 
 template <bool B>
@@ -254,6 +256,198 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tournament pivoting of a panel (columns [k, kend), rows [k, n)), the
+// communication-avoiding alternative to one global pivot search per column:
+//   level 0: every block of TRW rows runs GEPP in LDS and keeps its w pivot
+//            rows as candidates;
+//   level L: a block stacks the candidates of four groups of level L-1
+//            (their original rows, gathered from the matrix) and runs GEPP
+//            again, until one group is left;
+//   the surviving rows, in their GEPP order, are the panel's pivot rows.
+// Selection runs in FP32 (|re| + |im|, first row on ties).  The panel is
+// then permuted, its top w x w block factored without pivoting and
+// L21 = A21 inv(U11) formed on MFMA.  This replaces 64 dependent launches per
+// panel by about ten.
+// ---------------------------------------------------------------------------
+constexpr int TRW = 256;     // candidate rows per tournament block
+constexpr int TLD = NB + 1;  // padded LDS row
+
+__device__ __forceinline__ int tourn_stride(int n) { return ((n + TRW - 1) / TRW) * NB; }
+
+// GEPP of the TRW rows held in sm (w columns); rows with id < 0 are empty.
+// Leaves the chosen rows at the top of sm/id, in pivot order, and returns how
+// many were chosen.  256 threads, thread t owns row t.  The row update runs
+// in aligned chunks of 8 columns (vector LDS access); columns at or left of
+// the pivot column get harmless values, they are never read again.
+__device__ int tourn_gepp(float2 (*sm)[TLD], int* id, int w, float* rv, int* ri) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int chosen = 0;
+    for (int j = 0; j < w; ++j) {
+        float v = -1.f;
+        if (t >= j && id[t] >= 0) {
+            const float2 a = sm[t][j];
+            v = fabsf(a.x) + fabsf(a.y);
+        }
+        int bi = t;
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ov = __shfl_xor(v, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ov > v || (ov == v && oi < bi)) { v = ov; bi = oi; }
+        }
+        if (lane == 0) { rv[wave] = v; ri[wave] = bi; }
+        __syncthreads();
+        float bv = rv[0];
+        int p = ri[0];
+        for (int q = 1; q < 4; ++q)
+            if (rv[q] > bv || (rv[q] == bv && ri[q] < p)) { bv = rv[q]; p = ri[q]; }
+        if (bv < 0.f) break;  // no candidate rows left (uniform)
+        if (p != j) {
+            if (t < w) { const float2 x = sm[j][t]; sm[j][t] = sm[p][t]; sm[p][t] = x; }
+            if (t == 0) { const int x = id[j]; id[j] = id[p]; id[p] = x; }
+        }
+        __syncthreads();
+        if (t > j && id[t] >= 0 && bv > 0.f) {
+            const cx<float> d = {sm[j][j].x, sm[j][j].y};
+            const cx<float> l = cdiv(cx<float>{sm[t][j].x, sm[t][j].y}, d);
+            for (int c0 = (j + 1) & ~7; c0 < w; c0 += 8) {
+                float2 u[8], x[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) { u[q] = sm[j][c0 + q]; x[q] = sm[t][c0 + q]; }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    x[q].x -= l.re * u[q].x - l.im * u[q].y;
+                    x[q].y -= l.re * u[q].y + l.im * u[q].x;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sm[t][c0 + q] = x[q];
+            }
+        }
+        __syncthreads();
+        ++chosen;
+    }
+    return chosen;
+}
+
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(256) void tourn_kernel(LuProb inl, const LuProb* tbl, int k, int level) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    const int n = P.n;
+    if (k >= n) return;
+    const int kend = min(n, k + NB), w = kend - k, m = n - k;
+    int G = (m + TRW - 1) / TRW, Gprev = 0;
+    for (int L = 0; L < level; ++L) { Gprev = G; G = (G + 3) / 4; }
+    const int g = blockIdx.x;
+    if (g >= G) return;
+    __shared__ float2 sm[TRW][TLD];
+    __shared__ int id[TRW];
+    __shared__ float rv[4];
+    __shared__ int ri[4];
+    const T* A = (const T*)P.A;
+    const int stride = tourn_stride(n);
+    const int t = threadIdx.x;
+    if (level == 0) {
+        const int r = k + g * TRW + t;
+        id[t] = r < n ? r : -1;
+    } else {
+        const int child = 4 * g + (t >> 6), slot = t & 63;
+        id[t] = child < Gprev ? P.tids[((level - 1) & 1) * stride + child * NB + slot] : -1;
+    }
+    __syncthreads();
+    for (int e = t; e < TRW * NB; e += 256) {
+        const int r = e / NB, c = e % NB;
+        float2 v = make_float2(0.f, 0.f);
+        if (c < w && id[r] >= 0) {
+            const cx<T> a = Acc<T, C>::ld(A, (int64_t)id[r] * P.ld + k + c);
+            v = make_float2((float)a.re, (float)a.im);
+        }
+        sm[r][c] = v;
+    }
+    __syncthreads();
+    const int chosen = tourn_gepp(sm, id, w, rv, ri);
+    if (t < NB) P.tids[(level & 1) * stride + g * NB + t] = t < chosen ? id[t] : -1;
+}
+
+// LAPACK-style interchanges from the chosen rows sel[0..w): the row chosen
+// for position k+c sits at its own index unless a previous interchange moved
+// it, and a moved row is found by following ipiv from its index.
+template <bool B>
+__global__ void tourn_ipiv_kernel(LuProb inl, const LuProb* tbl, int k, int level) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    const int n = P.n;
+    if (k >= n || threadIdx.x != 0) return;
+    const int kend = min(n, k + NB), w = kend - k;
+    const int* sel = P.tids + (level & 1) * tourn_stride(n);
+    for (int c = 0; c < w; ++c) {
+        int pos = sel[c];
+        if (pos < 0) pos = k + c;  // cannot happen: the panel has at least w rows
+        while (pos < k + c) pos = P.ipiv[pos];
+        P.ipiv[k + c] = pos;
+    }
+}
+
+// Unpivoted LU of the (permuted) top w x w block in LDS, then
+// S_U = -inv(U11) for L21 = A21 inv(U11) (GEMM mode 5 stores -(A21 S_U)).
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb* tbl, int k) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    const int n = P.n;
+    if (k >= n) return;
+    const int kend = min(n, k + NB), w = kend - k;
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    __shared__ cx<T> M[NB][NB + 1];
+    __shared__ cx<T> X[NB][NB + 1];
+    const int t = threadIdx.x;
+    for (int e = t; e < NB * NB; e += 256) {
+        const int r = e / NB, c = e % NB;
+        M[r][c] = (r < w && c < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + c)
+                                   : cx<T>{T(r == c ? 1 : 0), T(0)};
+    }
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        const cx<T> d = M[j][j];
+        const bool zero = d.re == T(0) && d.im == T(0);
+        if (zero && t == 0 && *P.info == 0) *P.info = k + j + 1;
+        if (!zero)
+            for (int i = j + 1 + t; i < w; i += 256) M[i][j] = cdiv(M[i][j], d);
+        __syncthreads();
+        const int rem = w - j - 1;
+        for (int e = t; e < rem * rem; e += 256) {
+            const int i = j + 1 + e / rem, c = j + 1 + e % rem;
+            const cx<T> lu = cmul(M[i][j], M[j][c]);
+            M[i][c].re -= lu.re;
+            M[i][c].im -= lu.im;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < w * w; e += 256) {
+        const int r = e / w, c = e % w;
+        Acc<T, C>::st(A, (int64_t)(k + r) * ld + k + c, M[r][c]);
+    }
+    // X = inv(U11) column by column (thread t: column t, bottom up)
+    if (t < NB) {
+        for (int i = NB - 1; i >= 0; --i) {
+            cx<T> sum = {T(i == t ? 1 : 0), T(0)};
+            if (i < t)
+                for (int j = i + 1; j <= t; ++j) {
+                    const cx<T> ux = cmul(M[i][j], X[j][t]);
+                    sum.re -= ux.re;
+                    sum.im -= ux.im;
+                }
+            const cx<T> dii = M[i][i];
+            const bool z = dii.re == T(0) && dii.im == T(0);
+            X[i][t] = i > t ? cx<T>{T(0), T(0)} : (z ? sum : cdiv(sum, dii));
+        }
+    }
+    __syncthreads();
+    T* SU = (T*)P.SU;
+    for (int e = t; e < NB * NB; e += 256) {
+        const int i = e / NB, c = e % NB;
+        Acc<T, C>::st(SU, e, cx<T>{-X[i][c].re, -X[i][c].im});
+    }
+}
+
 // S_L = I - inv(L11) for the unit lower w x w block at (k, k), so that
 // U12 = inv(L11) A12 = A12 - S_L A12 runs on the MFMA GEMM.  Column t of the
 // inverse is computed right-looking by the four threads (t, g), thread g
@@ -340,12 +534,17 @@ struct GemmArgs {
 //   mode 2  outer U12, block b, elimination of the solved blocks above it:
 //           A[r0:r0+wb, K1:n] -= A[r0:r0+wb, K0:r0] A[K0:r0, K1:n]
 //   mode 3  outer U12, block b: A[r0:r0+wb, K1:n] -= S_L,b A[r0:r0+wb, K1:n]   (in place)
-//   mode 4  outer trailing: A[K1:n, K1:n] -= A[K1:n, K0:K1] A[K0:K1, K1:n]
+//   mode 4  outer trailing: A[K1:n, c0:c1] -= A[K1:n, K0:K1] A[K0:K1, c0:c1]
+//   mode 5  tournament panel: A[kend:n, k:kend] = A[kend:n, k:kend] inv(U11)
+//           (the kernel stores -(A B) with B = -inv(U11) instead of
+//           subtracting; in place: N <= 64, so every block owns whole rows)
+//           (columns c0 < 0 -> K1, c1 <= 0 -> n: the whole trailing matrix)
 // with r0 = K0 + b NB, S_L = I - inv(L_bb) of the block's unit lower factor.
 constexpr int W2 = 256;
 
 struct LuStep {
     int mode, K0, k, b;
+    int c0 = -1, c1 = 0;
 };
 
 template <typename T, bool C>
@@ -358,6 +557,11 @@ __host__ __device__ inline GemmArgs<T> lu_gemm_args(const LuProb& P, const LuSte
     auto at = [&](int r, int c) { return A + E * ((int64_t)r * ld + c); };
     GemmArgs<T> g{A, ld, A, ld, A, ld, 0, 0, 0};
     if (st.K0 >= n) return g;
+    if (st.mode == 5) {
+        const int k = st.k, kend = n < k + NB ? n : k + NB, w = kend - k;
+        if (k >= n) return g;
+        return {at(kend, k), ld, at(kend, k), ld, (const T*)P.SU, (int64_t)NB, n - kend, w, w};
+    }
     if (st.mode <= 1) {
         const int k = st.k, kend = n < k + NB ? n : k + NB, w = kend - k;
         if (k >= n) return g;
@@ -366,7 +570,11 @@ __host__ __device__ inline GemmArgs<T> lu_gemm_args(const LuProb& P, const LuSte
                     K1 - kend, w};
         return {at(kend, kend), ld, at(kend, k), ld, at(k, kend), ld, n - kend, K1 - kend, w};
     }
-    if (st.mode == 4) return {at(K1, K1), ld, at(K1, st.K0), ld, at(st.K0, K1), ld, n - K1, n - K1, K1 - st.K0};
+    if (st.mode == 4) {
+        const int c0 = st.c0 < 0 ? K1 : (st.c0 < n ? st.c0 : n);
+        const int c1 = st.c1 <= 0 ? n : (st.c1 < n ? st.c1 : n);
+        return {at(K1, c0), ld, at(K1, st.K0), ld, at(st.K0, c0), ld, n - K1, c1 - c0, K1 - st.K0};
+    }
     const int r0 = st.K0 + st.b * NB;
     if (r0 >= K1) return g;
     const int wb = K1 - r0 < NB ? K1 - r0 : NB;
@@ -455,7 +663,8 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const Lu
                 const int gr = bm + wm + 16 * a + Mfma<T>::row(lane, r);
                 const int gc = bn + wn + 16 * b + (lane & 15);
                 if (gr < M && gc < N) {
-                    cx<T> v = Acc<T, C>::ld(Cm, (int64_t)gr * ldc + gc);
+                    cx<T> v = {T(0), T(0)};
+                    if (st.mode != 5) v = Acc<T, C>::ld(Cm, (int64_t)gr * ldc + gc);
                     v.re -= accR[a][b][r];
                     if constexpr (C) v.im -= accI[a][b][r];
                     Acc<T, C>::st(Cm, (int64_t)gr * ldc + gc, v);
@@ -881,12 +1090,54 @@ void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, in
     hipLaunchKernelGGL((gemm_sub_kernel<T, C, B>), grid, dim3(256), 0, s, g, d_tbl, st);
 }
 
+// Factorisation of the NB-column panel at k (rows k..n of every active
+// problem): partial pivoting with one fused update/search launch per column
+// (pivoting 0, LAPACK's pivot rows), or tournament pivoting (pivoting 1).
+template <typename T, bool C, bool B>
+void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int k, int active,
+                  int pivoting, int bf16_parts, hipStream_t s) {
+    auto max_over = [&](auto f) {
+        int m = 0;
+        for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
+        return m;
+    };
+    if (pivoting == 0) {
+        const int kendmax = std::min(ns[0], k + NB);
+        for (int c = k; c <= kendmax; ++c) {
+            const int G = max_over([&](int n) {
+                const int kend = std::min(n, k + NB);
+                if (c > kend) return 0;
+                const int rows = n - (c < kend ? c : kend);
+                return rows > 0 ? (rows + RPB - 1) / RPB : 0;
+            });
+            if (G <= 0) break;
+            hipLaunchKernelGGL((col_step_kernel<T, C, B>), dim3(G, 1, active), dim3(256), 0, s, one, d_tbl, k, c);
+        }
+        return;
+    }
+    const int mmax = max_over([&](int n) { return n - k; });
+    int G = (mmax + TRW - 1) / TRW, level = 0;
+    hipLaunchKernelGGL((tourn_kernel<T, C, B>), dim3(G, 1, active), dim3(256), 0, s, one, d_tbl, k, 0);
+    while (G > 1) {
+        G = (G + 3) / 4;
+        ++level;
+        hipLaunchKernelGGL((tourn_kernel<T, C, B>), dim3(G, 1, active), dim3(256), 0, s, one, d_tbl, k, level);
+    }
+    hipLaunchKernelGGL((tourn_ipiv_kernel<B>), dim3(1, 1, active), dim3(64), 0, s, one, d_tbl, k, level);
+    hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, k + NB, k,
+                       k + NB, 0, 0);
+    hipLaunchKernelGGL((topblock_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
+    const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
+    lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, bf16_parts, s);
+}
+
 // Launch sequence of the two-level right-looking LU for one problem (B =
 // false, the problem by value) or a batch (B = true, table on the device,
 // problems sorted by decreasing n so that the first `active` entries are
 // the ones with columns left at a given step).
 template <typename T, bool C, bool B>
-void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int bf16_parts, hipStream_t s) {
+void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int bf16_parts,
+               hipStream_t s) {
     const int nmax = ns[0];
     auto count_active = [&](int from) {
         int a = 0;
@@ -903,18 +1154,7 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
                 for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
                 return m;
             };
-            const int kendmax = std::min(nmax, k + NB);
-            for (int c = k; c <= kendmax; ++c) {
-                const int G = max_over([&](int n) {
-                    const int kend = std::min(n, k + NB);
-                    if (c > kend) return 0;
-                    const int rows = n - (c < kend ? c : kend);
-                    return rows > 0 ? (rows + RPB - 1) / RPB : 0;
-                });
-                if (G <= 0) break;
-                hipLaunchKernelGGL((col_step_kernel<T, C, B>), dim3(G, 1, active), dim3(256), 0, s, one, d_tbl, k,
-                                   c);
-            }
+            panel_factor<T, C, B>(one, d_tbl, ns, k, active, pivoting, bf16_parts, s);
             // this panel's interchanges inside the outer block
             hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((W2 + 255) / 256, 1, active), dim3(256), 0, s, one,
                                d_tbl, k, k + NB, K0, K0 + W2, k, k + NB);
@@ -941,13 +1181,74 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
     }
 }
 
+// One problem with look-ahead over two internal streams: the inner panels of
+// outer block j+1 (stream sp) run while the trailing update of block j
+// beyond block j+1's columns (stream sg) is still in flight.  The streams
+// may be restricted to disjoint compute units (context option), so the
+// latency-bound panel kernels do not wait behind GEMM blocks.
+//   sp: panels of block j  -> event ep
+//   sg: wait ep; interchanges outside block j; U12; update of block j+1's
+//       columns -> event eg; update of the remaining columns
+//   sp: wait eg; panels of block j+1 ...
+template <typename T, bool C>
+void lu_launch_lookahead(const LuProb& P, int pivoting, int bf16_parts, hipStream_t s, hipStream_t sp,
+                         hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0) {
+    const int n = P.n;
+    const std::vector<int> ns1{n};
+    hipEventRecord(e0, s);
+    hipStreamWaitEvent(sp, e0, 0);
+    hipStreamWaitEvent(sg, e0, 0);
+    for (int K0 = 0; K0 < n; K0 += W2) {
+        const int K1 = std::min(n, K0 + W2);
+        for (int k = K0; k < K1; k += NB) {
+            const int kend = std::min(n, k + NB);
+            panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, bf16_parts, sp);
+            hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((W2 + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
+                               k + NB, K0, K0 + W2, k, k + NB);
+            const int slot = (k - K0) / NB;
+            hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
+            lu_gemm<T, C, false>(P, nullptr, LuStep{0, K0, k, slot}, NB, K1 - kend, 1, bf16_parts, sp);
+            lu_gemm<T, C, false>(P, nullptr, LuStep{1, K0, k, slot}, n - kend, K1 - kend, 1, bf16_parts, sp);
+        }
+        hipEventRecord(ep, sp);
+        hipStreamWaitEvent(sg, ep, 0);
+        hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((n + 255) / 256), dim3(256), 0, sg, P, nullptr, K0,
+                           K0 + W2, 0, n, K0, K0 + W2);
+        const int Nout = n - K1;
+        if (Nout > 0) {
+            const int nb_blocks = (K1 - K0 + NB - 1) / NB;
+            for (int b = 0; b < nb_blocks; ++b) {
+                if (b > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, b}, NB, Nout, 1, bf16_parts, sg);
+                lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, b}, NB, Nout, 1, bf16_parts, sg);
+            }
+            const int c_next = std::min(n, K1 + W2);
+            LuStep st4{4, K0, K0, 0};
+            st4.c0 = K1;
+            st4.c1 = c_next;
+            lu_gemm<T, C, false>(P, nullptr, st4, Nout, c_next - K1, 1, bf16_parts, sg);
+            hipEventRecord(eg, sg);
+            hipStreamWaitEvent(sp, eg, 0);
+            if (c_next < n) {
+                st4.c0 = c_next;
+                st4.c1 = n;
+                lu_gemm<T, C, false>(P, nullptr, st4, Nout, n - c_next, 1, bf16_parts, sg);
+            }
+        }
+    }
+    hipEventRecord(ep, sp);
+    hipEventRecord(eg, sg);
+    hipStreamWaitEvent(s, ep, 0);
+    hipStreamWaitEvent(s, eg, 0);
+}
+
 // Per-problem scratch: counter and info words, the per-block pivot
 // partials and S_L of each NB block of an outer block (W2 / NB of them).
 template <typename T, bool C>
 size_t lu_scratch_bytes(int n) {
     const int gmax = (n + RPB - 1) / RPB;
+    const size_t tids = 2 * (size_t)((n + TRW - 1) / TRW) * NB * sizeof(int);
     return 512 + (((size_t)gmax * (sizeof(double) + sizeof(int)) + 255) & ~(size_t)255) +
-           (W2 / NB) * NB * NB * sizeof(cx<T>) + 256;
+           (W2 / NB) * NB * NB * sizeof(cx<T>) + 256 + ((tids + 255) & ~(size_t)255) + NB * NB * sizeof(cx<T>) + 256;
 }
 
 template <typename T, bool C>
@@ -960,6 +1261,8 @@ LuProb lu_carve(char* base, void* A, int64_t ld, int n, int32_t* ipiv, int* info
     P.pval = (double*)(base + 512);
     P.pidx = (int*)(P.pval + gmax);
     P.SL = (void*)(((uintptr_t)(P.pidx + gmax) + 255) & ~(uintptr_t)255);
+    P.tids = (int*)(((uintptr_t)((cx<T>*)P.SL + (W2 / NB) * NB * NB) + 255) & ~(uintptr_t)255);
+    P.SU = (void*)(((uintptr_t)(P.tids + 2 * ((n + TRW - 1) / TRW) * NB) + 255) & ~(uintptr_t)255);
     return P;
 }
 
@@ -972,6 +1275,45 @@ int ensure_lu_scratch(Context* ctx, size_t need) {
     int rc = check_hip(hipMalloc(&ctx->lu_scratch, need), "LU scratch");
     if (rc) return rc;
     ctx->lu_scratch_bytes = need;
+    return BEM_OK;
+}
+
+// The two look-ahead streams (default priority).  With a positive
+// lu_lookahead_cus the panel stream gets the first that many compute units
+// (bits of the CU mask) and the update stream the others; -1 leaves both
+// unrestricted.
+int ensure_lu_streams(Context* ctx) {
+    if (ctx->lu_stream[0] && ctx->lu_streams_cus == ctx->lu_lookahead_cus) return BEM_OK;
+    for (int q = 0; q < 2; ++q)
+        if (ctx->lu_stream[q]) { hipStreamDestroy(ctx->lu_stream[q]); ctx->lu_stream[q] = nullptr; }
+    int rc;
+    if (ctx->lu_lookahead_cus > 0) {
+        if (ctx->num_cu <= 0) {
+            hipDeviceProp_t prop;
+            rc = check_hip(hipGetDeviceProperties(&prop, ctx->device), "device properties");
+            if (rc) return rc;
+            ctx->num_cu = prop.multiProcessorCount;
+        }
+        const int ncu = ctx->num_cu, words = (ncu + 31) / 32;
+        const int np = std::min(ctx->lu_lookahead_cus, ncu - 1);
+        std::vector<uint32_t> mp(words, 0u), mg(words, 0u);
+        for (int c = 0; c < ncu; ++c) (c < np ? mp : mg)[c / 32] |= 1u << (c % 32);
+        rc = check_hip(hipExtStreamCreateWithCUMask(&ctx->lu_stream[0], words, mp.data()), "panel stream");
+        if (rc) return rc;
+        rc = check_hip(hipExtStreamCreateWithCUMask(&ctx->lu_stream[1], words, mg.data()), "update stream");
+        if (rc) return rc;
+    } else {
+        for (int q = 0; q < 2; ++q) {
+            rc = check_hip(hipStreamCreateWithFlags(&ctx->lu_stream[q], hipStreamNonBlocking), "LU stream");
+            if (rc) return rc;
+        }
+    }
+    for (int q = 0; q < 3; ++q)
+        if (!ctx->lu_event[q]) {
+            rc = check_hip(hipEventCreateWithFlags(&ctx->lu_event[q], hipEventDisableTiming), "LU event");
+            if (rc) return rc;
+        }
+    ctx->lu_streams_cus = ctx->lu_lookahead_cus;
     return BEM_OK;
 }
 
@@ -995,7 +1337,16 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     ctx->lu_table.clear();  // this call reuses the scratch a batch table may occupy
     const LuProb P = lu_carve<T, C>((char*)ctx->lu_scratch, A, ld, n, ipiv, nullptr);
     hipLaunchKernelGGL(lu_reset_one_kernel, dim3(1), dim3(1), 0, s, P);
-    lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->c64_gemm_parts, s);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
+    if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
+        rc = ensure_lu_streams(ctx);
+        if (rc) return rc;
+        lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, ctx->c64_gemm_parts, s, ctx->lu_stream[0], ctx->lu_stream[1],
+                                  ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
+    } else {
+        lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, ctx->c64_gemm_parts, s);
+    }
     rc = check_hip(hipGetLastError(), "LU launch");
     if (rc) return rc;
     int h_info = 0;
@@ -1046,7 +1397,7 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
         ctx->lu_table_dev = (void*)d_tbl;
     }
     hipLaunchKernelGGL(lu_reset_kernel, dim3((count + 63) / 64), dim3(64), 0, s, (const LuProb*)d_tbl, count);
-    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->c64_gemm_parts, s);
+    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, ctx->c64_gemm_parts, s);
     return check_hip(hipGetLastError(), "batched LU launch");
 }
 

@@ -234,12 +234,32 @@ def test_lu_c64_gemm_modes(bem, mode, tol):
     assert rel(to_np(x), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("n", [130, 300])
+def test_lu_partial_pivoting_matches_lapack(bem, n):
+    """With BEM_OPT_LU_PIVOTING = 0 the pivot rows are LAPACK's (getrf: largest
+    |re| + |im| is izamax's measure, first row on ties) and the factors agree."""
+    import scipy.linalg
+
+    rng = np.random.default_rng(n + 1)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    ctx = bem.Context(0)
+    ctx.set_option(bem._lib.OPT_LU_PIVOTING, 0)
+    At = torch.tensor(A, device=DEV).contiguous()
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    ctx.lu_factor(At, ipiv)
+    lu, piv = scipy.linalg.lu_factor(A)
+    assert np.array_equal(to_np(ipiv), piv)
+    assert rel(to_np(At), lu) <= 1e-12
+
+
 def test_set_option_rejects(bem):
     ctx = bem.Context(0)
     with pytest.raises(bem.BEMError):
         ctx.set_option(bem._lib.OPT_C64_LU_GEMM, 5)
     with pytest.raises(bem.BEMError):
         ctx.set_option(99, 0)
+    with pytest.raises(bem.BEMError):
+        ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
 
 
 def test_lu_singular_raises(bem):
