@@ -354,14 +354,20 @@ __global__ __launch_bounds__(256) void tourn_kernel(LuProb inl, const LuProb* tb
         id[t] = child < Gprev ? P.tids[((level - 1) & 1) * stride + child * NB + slot] : -1;
     }
     __syncthreads();
-    for (int e = t; e < TRW * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
-        float2 v = make_float2(0.f, 0.f);
-        if (c < w && id[r] >= 0) {
-            const cx<T> a = Acc<T, C>::ld(A, (int64_t)id[r] * P.ld + k + c);
-            v = make_float2((float)a.re, (float)a.im);
+    // thread t gathers column t % 64 of rows t / 64 + 4 q, 16 loads in flight
+    {
+        const int c = t & 63, r0 = t >> 6;
+        for (int q0 = 0; q0 < TRW / 4; q0 += 16) {
+            cx<T> a[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int r = r0 + 4 * (q0 + q);
+                const int rid = id[r];
+                a[q] = (c < w && rid >= 0) ? Acc<T, C>::ld(A, (int64_t)rid * P.ld + k + c) : cx<T>{T(0), T(0)};
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) sm[r0 + 4 * (q0 + q)][c] = make_float2((float)a[q].re, (float)a[q].im);
         }
-        sm[r][c] = v;
     }
     __syncthreads();
     const int chosen = tourn_gepp(sm, id, w, rv, ri);
@@ -399,10 +405,18 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
     __shared__ cx<T> M[NB][NB + 1];
     __shared__ cx<T> X[NB][NB + 1];
     const int t = threadIdx.x;
-    for (int e = t; e < NB * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
-        M[r][c] = (r < w && c < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + c)
-                                   : cx<T>{T(r == c ? 1 : 0), T(0)};
+    {
+        cx<T> a[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = t + 256 * q, r = e / NB, c = e % NB;
+            a[q] = (r < w && c < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + c) : cx<T>{T(r == c ? 1 : 0), T(0)};
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = t + 256 * q;
+            M[e / NB][e % NB] = a[q];
+        }
     }
     __syncthreads();
     for (int j = 0; j < w; ++j) {
@@ -425,22 +439,34 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
         const int r = e / w, c = e % w;
         Acc<T, C>::st(A, (int64_t)(k + r) * ld + k + c, M[r][c]);
     }
-    // X = inv(U11) column by column (thread t: column t, bottom up)
-    if (t < NB) {
-        for (int i = NB - 1; i >= 0; --i) {
-            cx<T> sum = {T(i == t ? 1 : 0), T(0)};
-            if (i < t)
-                for (int j = i + 1; j <= t; ++j) {
-                    const cx<T> ux = cmul(M[i][j], X[j][t]);
-                    sum.re -= ux.re;
-                    sum.im -= ux.im;
+    // X = inv(U11) by anti-diagonals d = c - i: X[i][i+d] needs only entries
+    // of smaller d.  Four threads per entry split the dot product.
+    for (int e = t; e < NB * NB; e += 256) X[e / NB][e % NB] = cx<T>{T(0), T(0)};
+    __syncthreads();
+    {
+        const int g = t >> 2, q = t & 3;
+        for (int d = 0; d < NB; ++d) {
+            cx<T> sum = {T(0), T(0)};
+            const int i = g, c = g + d;
+            if (c < NB)
+                for (int j = i + 1 + q; j <= c; j += 4) {
+                    const cx<T> ux = cmul(M[i][j], X[j][c]);
+                    sum.re += ux.re;
+                    sum.im += ux.im;
                 }
-            const cx<T> dii = M[i][i];
-            const bool z = dii.re == T(0) && dii.im == T(0);
-            X[i][t] = i > t ? cx<T>{T(0), T(0)} : (z ? sum : cdiv(sum, dii));
+            sum.re += __shfl_xor(sum.re, 1, 64);
+            sum.im += __shfl_xor(sum.im, 1, 64);
+            sum.re += __shfl_xor(sum.re, 2, 64);
+            sum.im += __shfl_xor(sum.im, 2, 64);
+            if (c < NB && q == 0) {
+                const cx<T> num = {T(d == 0 ? 1 : 0) - sum.re, -sum.im};
+                const cx<T> dii = M[i][i];
+                const bool z = dii.re == T(0) && dii.im == T(0);
+                X[i][c] = z ? num : cdiv(num, dii);
+            }
+            __syncthreads();
         }
     }
-    __syncthreads();
     T* SU = (T*)P.SU;
     for (int e = t; e < NB * NB; e += 256) {
         const int i = e / NB, c = e % NB;
@@ -449,10 +475,8 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
 }
 
 // S_L = I - inv(L11) for the unit lower w x w block at (k, k), so that
-// U12 = inv(L11) A12 = A12 - S_L A12 runs on the MFMA GEMM.  Column t of the
-// inverse is computed right-looking by the four threads (t, g), thread g
-// updating rows i = g (mod 4): once X[j][t] is final it is eliminated from
-// the rows below with independent updates.
+// U12 = inv(L11) A12 = A12 - S_L A12 runs on the MFMA GEMM (|l| <= 1 keeps
+// S_L moderate, so the subtraction does not cancel).
 template <typename T, bool C, bool B>
 __global__ __launch_bounds__(256) void linv_kernel(LuProb inl, const LuProb* tbl, int k, int slot) {
     const LuProb P = get_prob<B>(inl, tbl);
@@ -463,24 +487,46 @@ __global__ __launch_bounds__(256) void linv_kernel(LuProb inl, const LuProb* tbl
     T* SL = (T*)P.SL + (C ? 2 : 1) * (int64_t)slot * NB * NB;
     __shared__ cx<T> M[NB][NB + 1];
     __shared__ cx<T> X[NB][NB + 1];
-    const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
-    for (int r = g; r < NB; r += 4) {
-        M[r][t] = (r < w && t < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + t) : cx<T>{T(r == t ? 1 : 0), T(0)};
-        X[r][t] = {T(r == t ? 1 : 0), T(0)};
+    const int t = threadIdx.x;
+    {
+        cx<T> a[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = t + 256 * q, r = e / NB, c = e % NB;
+            a[q] = (r < w && c < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + c) : cx<T>{T(r == c ? 1 : 0), T(0)};
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = t + 256 * q;
+            M[e / NB][e % NB] = a[q];
+            X[e / NB][e % NB] = cx<T>{T(0), T(0)};
+        }
     }
     __syncthreads();
-    for (int j = 0; j < NB - 1; ++j) {
-        const cx<T> xj = X[j][t];
-        if (j >= t)
-            for (int i = j + 1 + ((g - j - 1) & 3); i < NB; i += 4) {
-                cx<T> lx = cmul(M[i][j], xj);
-                X[i][t].re -= lx.re; X[i][t].im -= lx.im;
-            }
-        __syncthreads();
+    // X = inv(L11) (unit lower) by anti-diagonals d = i - c: X[i][i-d] needs
+    // only entries of smaller d; four threads per entry split the dot product.
+    {
+        const int g = t >> 2, q = t & 3;
+        for (int d = 0; d < NB; ++d) {
+            const int i = g + d, c = g;
+            cx<T> sum = {T(0), T(0)};
+            if (i < NB)
+                for (int j = c + q; j < i; j += 4) {
+                    const cx<T> lx = cmul(M[i][j], X[j][c]);
+                    sum.re += lx.re;
+                    sum.im += lx.im;
+                }
+            sum.re += __shfl_xor(sum.re, 1, 64);
+            sum.im += __shfl_xor(sum.im, 1, 64);
+            sum.re += __shfl_xor(sum.re, 2, 64);
+            sum.im += __shfl_xor(sum.im, 2, 64);
+            if (i < NB && q == 0) X[i][c] = cx<T>{T(d == 0 ? 1 : 0) - sum.re, -sum.im};
+            __syncthreads();
+        }
     }
-    for (int i = g; i < NB; i += 4) {
-        cx<T> v = {T(i == t ? 1 : 0) - X[i][t].re, -X[i][t].im};
-        Acc<T, C>::st(SL, (int64_t)i * NB + t, v);
+    for (int e = t; e < NB * NB; e += 256) {
+        const int i = e / NB, c = e % NB;
+        Acc<T, C>::st(SL, e, cx<T>{T(i == c ? 1 : 0) - X[i][c].re, -X[i][c].im});
     }
 }
 
