@@ -227,12 +227,14 @@ enum {
                                 priority, ordered by events against the
                                 caller's stream); -1 = the same two streams
                                 without compute-unit masks */,
-    BEM_OPT_LU_PIVOTING = 3  /* 1 (default) = tournament pivoting per 64-column
+    BEM_OPT_LU_PIVOTING = 3  /* 0 (default) = partial pivoting with one launch
+                                per column, the pivot rows LAPACK's getrf
+                                chooses; 1 = tournament pivoting per 64-column
                                 panel (pivot rows chosen by a reduction tree
                                 of LDS-resident eliminations, about ten
-                                launches per panel); 0 = partial pivoting
-                                with one launch per column, the pivot rows
-                                LAPACK's getrf chooses */
+                                launches per panel): faster, but its element
+                                growth can be larger, so use it for
+                                well-conditioned systems only */
 };
 
 /* Sets a context option.  No reference counterpart. */

@@ -81,7 +81,7 @@ struct Context {
     int num_cu = 0;
     int c64_gemm_parts = 0;       // c64 LU trailing updates: 0 FP32 MFMA, 2/3 split-bf16 parts
     int lu_lookahead_cus = 32;    // LU look-ahead: 0 off, > 0 panel-stream CUs, -1 unmasked streams
-    int lu_pivoting = 1;          // LU panels: 0 partial pivoting (LAPACK rows), 1 tournament
+    int lu_pivoting = 0;          // LU panels: 0 partial pivoting (LAPACK rows), 1 tournament
     int lu_streams_cus = 0;       // the setting the streams were created for
     hipStream_t lu_stream[2] = {nullptr, nullptr};
     hipEvent_t lu_event[3] = {nullptr, nullptr, nullptr};

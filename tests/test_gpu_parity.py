@@ -407,6 +407,30 @@ def test_transmission_matrix(bem, oracle):
     assert rel(to_np(bm.get_mat_A()), A) <= TOL64
 
 
+def test_transmission_solve_accuracy(bem):
+    """The ill-conditioned 2x2 Calderon system (L3, n = 1922) is solved by the
+    default LU to the accuracy of LAPACK on the same matrix."""
+    v, f = bem.meshes.perturbed_icosphere(3, seed=0)
+    k, ni = 5.0, 2.25
+
+    def fD(x, sqrt_n):
+        return torch.exp(1j * k * sqrt_n * x[:, 0])
+
+    def fN(x, n, sqrt_n):
+        return 1j * k * sqrt_n * n[:, 0] * torch.exp(1j * k * sqrt_n * x[:, 0])
+
+    params = {"kernel": "Helmholtz_Transmission", "k": k, "n_i": ni, "n_o": 1.0, "boundary": "Full",
+              "GaussQR": 5, "Q_Neumann": 0, "Q_Dirichlet": 1, "vertices": v, "panels": f}
+    cm = bem.CoreManager(params)
+    cm.initialization(fD, fN)
+    cm.run()
+    bm = cm._BEM_manager
+    n = bm.N_Neumann + bm.M_Dirichlet
+    A = to_np(bm.get_mat_A())
+    ref = np.linalg.solve(A, to_np(bm.rhs[:n]))
+    assert rel(to_np(bm.raw_solved), ref) <= 1e-9
+
+
 @pytest.mark.parametrize("prec,tol", [(0, TOL64), (1, 5e-6)])
 @pytest.mark.parametrize("kernel,k", [(0, 0.0), (1, 3.0)])
 def test_assemble_dirichlet_fused(bem, oracle, meshes, kernel, k, prec, tol):
