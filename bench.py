@@ -28,18 +28,20 @@ sys.path.insert(0, REPO)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3
-# Algorithmic flops per quadrature-point pair of the fused far-field kernel
-# (add/mul = 1, FMA = 2; rsqrt and sin/cos are transcendentals, not counted):
-# d = x - y 3, r^2 5, r 1, kr 1, w/r 1, V accumulate 4, (x-y).n 5,
-# w (x-y).n / r^3 3, (cos + kr sin, sin - kr cos) 4, K accumulate 4.
-FLOPS_PER_EVAL_FUSED = 31
+# Algorithmic flops per quadrature-point pair of the fused far-field kernel,
+# which evaluates each unordered panel pair once (V is symmetric) and serves
+# both orderings (add/mul = 1, FMA = 2; rsqrt and sin/cos are transcendentals,
+# not counted): d = x - y 3, r^2 5, w/r 1, w/r^3 2, (x-y).n_j 6, (y-x).n_i 6,
+# kr 2, V accumulate 4, (cos + kr sin, sin - kr cos) 4, row-i K accumulate 4,
+# row-j K times the test-point data 2 + 8.
+FLOPS_PER_EVAL_FUSED = 47
 # HBM bytes per launch of far_dirichlet_kernel<double, Helmholtz> on the L5
 # workload from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the
-# same counter on a 512 MiB copy (FETCH x 2.00, WRITE x 1.00):
-# (3.42451e6 * 2 + 6.60208e6) KiB.  Hardware FP64 VALU flops of that launch
-# (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes): 2.258e13.
-FAR_L5_TRAFFIC_BYTES = int((3.42451e6 * 2 + 6.60208e6) * 1024)
-FAR_L5_HW_FLOPS = 2.258e13
+# same counter on a 512 MiB copy (FETCH x 2.00, WRITE x 1.00), and the
+# hardware FP64 VALU flops of that launch (SQ_INSTS_VALU_FLOPS_FP64 x 64
+# lanes); None until measured for the current kernel.
+FAR_L5_TRAFFIC_BYTES = None
+FAR_L5_HW_FLOPS = None
 
 
 def log(msg: str) -> None:
@@ -152,14 +154,14 @@ def main() -> int:
 
     # roofline of the dominant assembly kernel (far field, FP64 Helmholtz V)
     far_pairs = counts[0]
-    far_flops = far_pairs * 625 * FLOPS_PER_EVAL_FUSED
+    far_flops = far_pairs // 2 * 625 * FLOPS_PER_EVAL_FUSED  # unordered pairs
     achieved = far_flops / (mean["far_ms"] * 1e-3) / 1e12
     roofline = {"bound": "valu", "kernel": "far_dirichlet_kernel<double,Helmholtz> (V + K conj(g))",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                "traffic": FAR_L5_TRAFFIC_BYTES if args.level == 5 else None,
+                "traffic": FAR_L5_TRAFFIC_BYTES if args.level == 5 and FAR_L5_TRAFFIC_BYTES else None,
                 "flops_per_launch": far_flops, "launch_ms": round(mean["far_ms"], 3)}
-    if args.level == 5:
+    if args.level == 5 and FAR_L5_HW_FLOPS:
         # hardware VALU rate (counters), beside the algorithmic one above
         roofline["hw_valu_tflops"] = round(FAR_L5_HW_FLOPS / (mean["far_ms"] * 1e-3) / 1e12, 3)
         roofline["hw_valu_frac"] = round(roofline["hw_valu_tflops"] / PEAK_FP64_TFLOPS, 4)

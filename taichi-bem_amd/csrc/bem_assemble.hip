@@ -605,97 +605,188 @@ OpParams make_params(Context* ctx, const bem_op_desc* d) {
 
 // ---------------------------------------------------------------------------
 // Dirichlet system in one far-field pass (P0 test panels): V (P0 trial) into
-// the matrix and the matrix-free K conj(g) (P1 trial) for the right-hand side
-// share every point-pair evaluation of r, 1/r and exp(i kap r).  The K part
-// is reduced over the block's 256 trial panels and written as one partial per
-// (test panel, trial block); a second kernel sums the partials in order.
+// the matrix and the matrix-free K conj(g) (P1 trial) for the right-hand side.
+// The tensor rule makes V symmetric (V_ij = V_ji: the same point pairs, G
+// symmetric), so only pairs i < j are evaluated, and one evaluation of r, 1/r
+// and exp(i kap r) per point pair yields
+//   V_ij, written to A[i][j] and A[j][i],
+//   the K conj(g) term of row i (normal n_j, data on panel j) and
+//   the K conj(g) term of row j (normal n_i, data on panel i).
+// A block takes a tile of DTI test panels (wave-uniform, scalar loads) and
+// 256 trial panels j (one per lane), over the upper triangle only.  Row-i
+// terms are reduced over the block into one partial per (test panel, trial
+// block); row-j terms stay in the lane over the tile and are added once per
+// lane (atomics).  A second kernel sums the partials in a fixed order.
 // ---------------------------------------------------------------------------
-template <typename R, bool HELM>
-__global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb, double* kpart) {
-    const int i = blockIdx.x / njb;
-    const int jb = blockIdx.x % njb;
-    const int j = jb * 256 + threadIdx.x;
-    double kgr = 0.0, kgi = 0.0;
-    if (j < p.nf && !shares_vertex(p.tris, i, j)) {
-        const int q2 = p.q2, nf = p.nf;
-        const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
-        const R* __restrict__ PL = Ptr<R>::pl(p);
-        const R* __restrict__ PHI = Ptr<R>::phi(p);
-        R dcx = 0, dcy = 0, dcz = 0;
-        if constexpr (sizeof(R) == 4) {
-            dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
-            dcy = (R)(p.cen[3 * i + 1] - p.cen[3 * j + 1]);
-            dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
-        }
-        const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
-        // trial data at the three vertices of panel j
-        R gr[3], gi[3];
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            double xr, xi;
-            load_elem(p.x, p.dtype_x, p.tris[3 * j + t], p.conj_x, xr, xi);
-            gr[t] = (R)xr; gi[t] = (R)xi;
-        }
-        const R kap = (R)p.kap;
-        R vr = 0, vi = 0, kr_ = 0, ki_ = 0;
-        for (int b = 0; b < q2; ++b) {
-            const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
-            const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
-            const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
-            const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
-            R pvr = 0, pvi = 0, pkr = 0, pki = 0;
-            for (int a = 0; a < q2; ++a) {
-                const R dx = PU[4 * a + 0] + ex;
-                const R dy = PU[4 * a + 1] + ey;
-                const R dz = PU[4 * a + 2] + ez;
-                const R wa = PU[4 * a + 3];
-                const R r2 = dx * dx + dy * dy + dz * dz;
-                const R rinv = Math<R>::rsqrt(r2);
-                const R g = wa * rinv;
-                const R base = g * (dx * nnx + dy * nny + dz * nnz) * (rinv * rinv);
-                if constexpr (HELM) {
-                    const R kr = kap * (r2 * rinv);
-                    R sn, cs;
-                    Math<R>::sincos(kr, &sn, &cs);
-                    pvr += g * cs; pvi += g * sn;
-                    pkr += base * (cs + kr * sn); pki += base * (sn - kr * cs);
-                } else {
-                    pvr += g;
-                    pkr += base;
-                }
-            }
-            vr += pvr * wb; vi += pvi * wb;
-            // interpolated trial data at point b
-            const R ph0 = PHI[3 * b], ph1 = PHI[3 * b + 1], ph2 = PHI[3 * b + 2];
-            const R xr = ph0 * gr[0] + ph1 * gr[1] + ph2 * gr[2];
-            const R xi = ph0 * gi[0] + ph1 * gi[1] + ph2 * gi[2];
-            const R wr = pkr * wb, wi = pki * wb;
-            kr_ += wr * xr - wi * xi;
-            ki_ += wr * xi + wi * xr;
-        }
-        const double Vr = kInv4Pi * (double)vr, Vi = kInv4Pi * (double)vi;
-        add_elem(p.out, p.dtype, (int64_t)i * p.ld + j, p.ar * Vr - p.ai * Vi, p.ar * Vi + p.ai * Vr, false);
-        kgr = kInv4Pi * (double)kr_;
-        kgi = kInv4Pi * (double)ki_;
+constexpr int DTI = 8;             // test panels per tile
+constexpr int DTG = 256 / DTI;     // tiles per 256-row group
+
+// block b of the upper-triangle tiling -> (tile, trial block): the tiles of
+// group g (rows [256 g, 256 g + 256)) pair with trial blocks g .. njb-1.
+__device__ __forceinline__ void sym_block(int64_t b, int njb, int ngroups, int& tile, int& jb) {
+    auto prefix = [&](int g) -> int64_t { return (int64_t)DTG * ((int64_t)g * njb - (int64_t)g * (g - 1) / 2); };
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix(mid) <= b) lo = mid; else hi = mid - 1;
     }
-    __shared__ double red[4][2];
+    const int64_t local = b - prefix(lo);
+    const int cnt = njb - lo;
+    tile = lo * DTG + (int)(local / cnt);
+    jb = lo + (int)(local % cnt);
+}
+
+template <typename R, bool HELM>
+__global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb, int ngroups, double* kpart,
+                                                            double* krow) {
+    int tile, jb;
+    sym_block(blockIdx.x, njb, ngroups, tile, jb);
+    const int nf = p.nf, q2 = p.q2;
+    const int i0 = tile * DTI;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    kgr = wave_sum(kgr);
-    kgi = wave_sum(kgi);
-    if (lane == 0) { red[wid][0] = kgr; red[wid][1] = kgi; }
+    const int j = jb * 256 + threadIdx.x;
+    const bool jvalid = j < nf;
+    const int jj = jvalid ? j : 0;
+    __shared__ double gts[DTI][144][2];  // data interpolated at the test points (Q^2 <= 144) of each tile panel
+    __shared__ double red[4][2];
+    if (i0 >= nf) return;  // the last group's empty tiles (uniform)
+    for (int e = threadIdx.x; e < DTI * q2; e += 256) {
+        const int ti = e / q2, a = e % q2, i = i0 + ti;
+        double gr = 0.0, gi = 0.0;
+        if (i < nf)
+            for (int s = 0; s < 3; ++s) {
+                double xr, xi;
+                load_elem(p.x, p.dtype_x, p.tris[3 * i + s], p.conj_x, xr, xi);
+                const double ph = p.phi[3 * a + s];
+                gr += ph * xr; gi += ph * xi;
+            }
+        gts[ti][a][0] = gr;
+        gts[ti][a][1] = gi;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        kpart[((int64_t)i * njb + jb) * 2] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
-        kpart[((int64_t)i * njb + jb) * 2 + 1] = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    const R* __restrict__ PL = Ptr<R>::pl(p);
+    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    const R kap = (R)p.kap;
+    // trial-side data of this lane's panel j
+    const R nnx = (R)p.normal[3 * jj], nny = (R)p.normal[3 * jj + 1], nnz = (R)p.normal[3 * jj + 2];
+    R gjr[3], gji[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        double xr, xi;
+        load_elem(p.x, p.dtype_x, p.tris[3 * jj + t], p.conj_x, xr, xi);
+        gjr[t] = (R)xr; gji[t] = (R)xi;
+    }
+    double kjr = 0.0, kji = 0.0;  // row-j term, summed over the tile
+    double vtr[DTI], vti[DTI];    // V_ij of the tile, for A[j][i0 + ti]
+    unsigned done = 0u;           // bit ti: pair (i0 + ti, j) evaluated here
+#pragma unroll
+    for (int ti = 0; ti < DTI; ++ti) { vtr[ti] = 0.0; vti[ti] = 0.0; }
+    const int i_end = min(nf, i0 + DTI);
+#pragma unroll
+    for (int ti = 0; ti < DTI; ++ti) {
+        const int i = i0 + ti;
+        double kir = 0.0, kii = 0.0;
+        if (i < i_end && jvalid && j > i && !shares_vertex(p.tris, i, j)) {
+            const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+            R dcx = 0, dcy = 0, dcz = 0;
+            if constexpr (sizeof(R) == 4) {
+                dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
+                dcy = (R)(p.cen[3 * i + 1] - p.cen[3 * j + 1]);
+                dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
+            }
+            const R mix = (R)p.normal[3 * i], miy = (R)p.normal[3 * i + 1], miz = (R)p.normal[3 * i + 2];
+            R vr = 0, vi = 0, ar = 0, ai = 0, br = 0, bi = 0;
+            for (int b = 0; b < q2; ++b) {
+                const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
+                const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
+                const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
+                const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
+                R pvr = 0, pvi = 0, pkr = 0, pki = 0, qkr = 0, qki = 0;
+                for (int a = 0; a < q2; ++a) {
+                    const R dx = PU[4 * a + 0] + ex;
+                    const R dy = PU[4 * a + 1] + ey;
+                    const R dz = PU[4 * a + 2] + ez;
+                    const R wa = PU[4 * a + 3];
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    const R rinv = Math<R>::rsqrt(r2);
+                    const R g = wa * rinv;
+                    const R g3 = g * (rinv * rinv);
+                    const R bj = g3 * (dx * nnx + dy * nny + dz * nnz);    // (x - y).n_j: row i
+                    const R bi_ = -g3 * (dx * mix + dy * miy + dz * miz);  // (y - x).n_i: row j
+                    const R gar = (R)gts[ti][a][0], gai = (R)gts[ti][a][1];
+                    if constexpr (HELM) {
+                        const R kr = kap * (r2 * rinv);
+                        R sn, cs;
+                        Math<R>::sincos(kr, &sn, &cs);
+                        pvr += g * cs; pvi += g * sn;
+                        const R c1 = cs + kr * sn, c2 = sn - kr * cs;
+                        pkr += bj * c1; pki += bj * c2;
+                        // row j: bi_ (c1 + i c2) times the test-point data
+                        const R tr = bi_ * c1, tim = bi_ * c2;
+                        qkr += tr * gar - tim * gai;
+                        qki += tr * gai + tim * gar;
+                    } else {
+                        pvr += g;
+                        pkr += bj;
+                        qkr += bi_ * gar;
+                        qki += bi_ * gai;
+                    }
+                }
+                vr += pvr * wb; vi += pvi * wb;
+                const R ph0 = PHI[3 * b], ph1 = PHI[3 * b + 1], ph2 = PHI[3 * b + 2];
+                const R xr = ph0 * gjr[0] + ph1 * gjr[1] + ph2 * gjr[2];
+                const R xi = ph0 * gji[0] + ph1 * gji[1] + ph2 * gji[2];
+                const R wr = pkr * wb, wi = pki * wb;
+                ar += wr * xr - wi * xi;
+                ai += wr * xi + wi * xr;
+                br += qkr * wb; bi += qki * wb;
+            }
+            const double Vr = kInv4Pi * (double)vr, Vi = kInv4Pi * (double)vi;
+            add_elem(p.out, p.dtype, (int64_t)i * p.ld + j, p.ar * Vr - p.ai * Vi, p.ar * Vi + p.ai * Vr, false);
+            vtr[ti] = Vr; vti[ti] = Vi;
+            done |= 1u << ti;
+            kir = kInv4Pi * (double)ar; kii = kInv4Pi * (double)ai;
+            kjr += kInv4Pi * (double)br; kji += kInv4Pi * (double)bi;
+        }
+        // row i: reduce over the block into the (i, jb) partial
+        kir = wave_sum(kir);
+        kii = wave_sum(kii);
+        if (lane == 0) { red[wid][0] = kir; red[wid][1] = kii; }
+        __syncthreads();
+        if (threadIdx.x == 0 && i < nf) {
+            kpart[((int64_t)i * njb + jb) * 2] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+            kpart[((int64_t)i * njb + jb) * 2 + 1] = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+        }
+        __syncthreads();
+    }
+    if (!jvalid) return;
+    // the mirrored entries A[j][i0 .. i0 + DTI) of this lane (contiguous)
+#pragma unroll
+    for (int ti = 0; ti < DTI; ++ti) {
+        const int i = i0 + ti;
+        if (done & (1u << ti))
+            add_elem(p.out, p.dtype, (int64_t)j * p.ld + i, p.ar * vtr[ti] - p.ai * vti[ti],
+                     p.ar * vti[ti] + p.ai * vtr[ti], false);
+    }
+    if (kjr != 0.0 || kji != 0.0) {
+        atomicAdd(krow + 2 * j, kjr);
+        atomicAdd(krow + 2 * j + 1, kji);
     }
 }
 
-// rhs[i] += alpha * sum_jb kpart[i][jb] (fixed order).
-__global__ void kpart_reduce_kernel(const double* kpart, int nf, int njb, void* y, int dtype, double ar, double ai) {
+__global__ void zero_kernel(double* x, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) x[e] = 0.0;
+}
+
+// rhs[i] += alpha (krow[i] + sum_{jb >= i / 256} kpart[i][jb]) (fixed order
+// over the partials; krow holds the row's terms from the mirrored pairs).
+__global__ void kpart_reduce_kernel(const double* kpart, const double* krow, int nf, int njb, void* y, int dtype,
+                                    double ar, double ai) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
-    double a = 0.0, b = 0.0;
-    for (int q = 0; q < njb; ++q) {
+    double a = krow[2 * i], b = krow[2 * i + 1];
+    for (int q = i / 256; q < njb; ++q) {
         a += kpart[((int64_t)i * njb + q) * 2];
         b += kpart[((int64_t)i * njb + q) * 2 + 1];
     }
@@ -1072,7 +1163,7 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
     if (rc) return rc;
     Mesh& m = ctx->mesh;
     const int nf = m.nf, njb = (nf + 255) / 256;
-    const size_t kbytes = (size_t)nf * njb * 2 * sizeof(double);
+    const size_t kbytes = ((size_t)nf * njb + nf) * 2 * sizeof(double);
     if (m.kpart_bytes < kbytes) {
         if (m.kpart) hipFree(m.kpart);
         m.kpart = nullptr;
@@ -1089,8 +1180,14 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
     if (parts & BEM_PART_FAR) {
         auto fk = d->precision == BEM_PREC_FP32 ? (helm ? far_dirichlet_kernel<float, true> : far_dirichlet_kernel<float, false>)
                                                 : (helm ? far_dirichlet_kernel<double, true> : far_dirichlet_kernel<double, false>);
-        hipLaunchKernelGGL(fk, dim3((unsigned)((int64_t)nf * njb)), dim3(256), 0, s, p, njb, m.kpart);
-        hipLaunchKernelGGL(kpart_reduce_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, (const double*)m.kpart, nf,
+        double* krow = m.kpart + (size_t)nf * njb * 2;
+        hipLaunchKernelGGL(zero_kernel, dim3((2 * nf + 255) / 256), dim3(256), 0, s, krow, (int64_t)2 * nf);
+        const int ngroups = (nf + 255) / 256;
+        // upper-triangle blocks: DTG tiles of group g pair with njb - g trial blocks
+        const int64_t nblocks = (int64_t)DTG * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
+        hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, njb, ngroups, m.kpart, krow);
+        hipLaunchKernelGGL(kpart_reduce_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, (const double*)m.kpart,
+                           (const double*)krow, nf,
                            njb, rhs, dtype_rhs, alpha_k, 0.0);
         rc = check_hip(hipGetLastError(), "far-field Dirichlet system launch");
         if (rc) return rc;
