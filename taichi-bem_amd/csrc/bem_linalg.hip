@@ -654,24 +654,33 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const Lu
             accR[a][b] = V4{0, 0, 0, 0};
             if constexpr (C) accI[a][b] = V4{0, 0, 0, 0};
         }
-    for (int k0 = 0; k0 < K; k0 += KC) {
-        for (int e = tid; e < GT * KC; e += 256) {
-            int r = e / KC, c = e % KC;
-            int gr = bm + r, gc = k0 + c;
-            cx<T> v = {T(0), T(0)};
-            if (gr < M && gc < K) v = Acc<T, C>::ld(Am, (int64_t)gr * lda + gc);
-            sAr[r][c] = v.re;
-            if constexpr (C) sAi[r][c] = v.im;
+    // register prefetch: the next K chunk is loaded while the current one is
+    // multiplied out of LDS
+    constexpr int PE = GT * KC / 256;
+    cx<T> pa[PE], pb[PE];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PE; ++q) {
+            const int e = tid + 256 * q;
+            const int ra = e / KC, ca = e % KC, rb = e / GT, cb = e % GT;
+            const int gra = bm + ra, gca = k0 + ca, grb = k0 + rb, gcb = bn + cb;
+            pa[q] = (gra < M && gca < K) ? Acc<T, C>::ld(Am, (int64_t)gra * lda + gca) : cx<T>{T(0), T(0)};
+            pb[q] = (grb < K && gcb < N) ? Acc<T, C>::ld(Bm, (int64_t)grb * ldb + gcb) : cx<T>{T(0), T(0)};
         }
-        for (int e = tid; e < KC * GT; e += 256) {
-            int r = e / GT, c = e % GT;
-            int gr = k0 + r, gc = bn + c;
-            cx<T> v = {T(0), T(0)};
-            if (gr < K && gc < N) v = Acc<T, C>::ld(Bm, (int64_t)gr * ldb + gc);
-            sBr[r][c] = v.re;
-            if constexpr (C) sBi[r][c] = v.im;
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < K; k0 += KC) {
+#pragma unroll
+        for (int q = 0; q < PE; ++q) {
+            const int e = tid + 256 * q;
+            const int ra = e / KC, ca = e % KC, rb = e / GT, cb = e % GT;
+            sAr[ra][ca] = pa[q].re;
+            if constexpr (C) sAi[ra][ca] = pa[q].im;
+            sBr[rb][cb] = pb[q].re;
+            if constexpr (C) sBi[rb][cb] = pb[q].im;
         }
         __syncthreads();
+        if (k0 + KC < K) fetch(k0 + KC);
 #pragma unroll
         for (int kk = 0; kk < KC; kk += 4) {
             const int ka = kk + (lane >> 4);
