@@ -63,6 +63,8 @@ def main() -> int:
                          "combined-field wave-number sweep (60 k at L4 plus L3 and L5); transmission: the "
                          "L5 2x2 Calderon transmission solve with the interior-k gradient")
     ap.add_argument("--no-fd", action="store_true", help="transmission: skip the finite-difference check")
+    ap.add_argument("--lu-lookahead", type=int, default=None,
+                    help="compute units of the LU's panel stream (0: no look-ahead; default: the library's)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the reference kernel that scales "
                          "FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -97,6 +99,8 @@ def main() -> int:
     nv, nf = len(v), len(f)
     log(f"mesh level {args.level}: nv={nv} nf={nf}")
     ctx = bem.Context(0)
+    if args.lu_lookahead is not None:
+        ctx.set_option(L.OPT_LU_LOOKAHEAD, args.lu_lookahead)
     ctx.set_mesh(torch.tensor(v, device=dev), torch.tensor(f, device=dev), 5)
     counts = ctx.nearfield_counts()
     g = torch.exp(1j * k * (torch.tensor(v, device=dev) @ torch.tensor(d, device=dev))).to(torch.complex128)

@@ -234,7 +234,10 @@ enum {
                                 of LDS-resident eliminations, about ten
                                 launches per panel): faster, but its element
                                 growth can be larger, so use it for
-                                well-conditioned systems only */
+                                well-conditioned systems only */,
+    BEM_OPT_LU_GEMM_TILE = 4 /* 1 (default) = trailing updates of at least
+                                256 rows use 128-row MFMA tiles; 0 = 64 x 64
+                                tiles for every update */
 };
 
 /* Sets a context option.  No reference counterpart. */

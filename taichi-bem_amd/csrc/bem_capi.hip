@@ -82,6 +82,7 @@ int bem_destroy(bem_context* ctx) {
         if (ctx->impl.work) hipFree(ctx->impl.work);
         if (ctx->impl.counters) hipFree(ctx->impl.counters);
         if (ctx->impl.lu_scratch) hipFree(ctx->impl.lu_scratch);
+        if (ctx->impl.tri_sync) hipFree(ctx->impl.tri_sync);
         for (void* w : ctx->impl.batch_ws)
             if (w) hipFree(w);
         for (hipStream_t st : ctx->impl.lu_stream)
@@ -202,6 +203,10 @@ int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
         case BEM_OPT_LU_LOOKAHEAD:
             if (value < -1 || value > 1024) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_LOOKAHEAD takes -1..1024");
             ctx->impl.lu_lookahead_cus = value;
+            return BEM_OK;
+        case BEM_OPT_LU_GEMM_TILE:
+            if (value != 0 && value != 1) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_GEMM_TILE takes 0 or 1");
+            ctx->impl.lu_gemm_tile = value;
             return BEM_OK;
         case BEM_OPT_C64_LU_GEMM:
             if (value != 0 && value != 3 && value != 6)

@@ -68,6 +68,8 @@ struct Context {
     size_t work_bytes = 0;
     int* counters = nullptr;      // small per-context flags
     size_t counters_len = 0;
+    void* tri_sync = nullptr;     // ticket and ready words of single-problem triangular sweeps
+    size_t tri_sync_words = 0;
     void* lu_scratch = nullptr;   // LU counters, pivot partials, S_L (per problem of a batch)
     size_t lu_scratch_bytes = 0;
     std::vector<char> lu_table;   // host copy of the batch table last uploaded
@@ -82,6 +84,7 @@ struct Context {
     int c64_gemm_parts = 0;       // c64 LU trailing updates: 0 FP32 MFMA, 2/3 split-bf16 parts
     int lu_lookahead_cus = 32;    // LU look-ahead: 0 off, > 0 panel-stream CUs, -1 unmasked streams
     int lu_pivoting = 0;          // LU panels: 0 partial pivoting (LAPACK rows), 1 tournament
+    int lu_gemm_tile = 1;         // LU trailing updates: 1 128-row tiles where they fit, 0 64 x 64 only
     int lu_streams_cus = 0;       // the setting the streams were created for
     hipStream_t lu_stream[2] = {nullptr, nullptr};
     hipEvent_t lu_event[3] = {nullptr, nullptr, nullptr};

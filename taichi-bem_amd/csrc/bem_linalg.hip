@@ -728,21 +728,174 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const Lu
 }
 
 // ---------------------------------------------------------------------------
+// Trailing updates of large factorisations (modes 1 and 4): C -= A B with a
+// 128 x BN block tile (BN = 128 or 64), eight waves of 64 x BN/4 (4 x BN/64
+// MFMA 16x16 blocks each), so every A and B element staged in LDS feeds
+// twice as many MFMAs as in the 64 x 64 kernel.  K runs in chunks of one
+// 128-byte row segment (KB = 8 complex doubles, 16 complex floats or
+// doubles, 32 floats) through two LDS buffers: the next chunk is loaded into
+// registers before the MFMAs of the current one and stored to the other
+// buffer after them, one barrier per chunk.  A is staged k-major (transposed)
+// with one element of padding so both the transposing stores and the MFMA
+// operand reads are conflict-free.  The complex products are issued as four
+// sweeps over the accumulator grid (re*re, re*im, -im*im, im*re), so
+// dependent MFMAs on one accumulator are a whole sweep apart.
+// ---------------------------------------------------------------------------
+constexpr int BT = 128;
+
+template <typename T, bool C>
+struct Vec2 {
+    // one element of a row-major matrix (C: interleaved complex) as one load
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ cx<T> ld(const T* a, int64_t idx) {
+        if constexpr (C) {
+            const v2 v = *(const v2*)(a + 2 * idx);
+            return {v.x, v.y};
+        } else {
+            return {a[idx], T(0)};
+        }
+    }
+};
+
+template <typename T, bool C, int BN, bool B>
+__global__ __launch_bounds__(512) void gemm_big_kernel(GemmArgs<T> inl, const LuProb* tbl, LuStep st) {
+    GemmArgs<T> ga = inl;
+    if constexpr (B) ga = lu_gemm_args<T, C>(tbl[blockIdx.z], st);
+    T* Cm = ga.Cm;
+    const T* Am = ga.Am;
+    const T* Bm = ga.Bm;
+    const int64_t ldc = ga.ldc, lda = ga.lda, ldb = ga.ldb;
+    const int M = ga.M, N = ga.N, K = ga.K;
+    if (M <= 0 || N <= 0 || K <= 0 || (int)blockIdx.y * BT >= M || (int)blockIdx.x * BN >= N) return;
+    using V4 = typename Mfma<T>::v4;
+    constexpr int E = C ? 2 : 1;
+    constexpr int KB = 128 / (int)(sizeof(T) * E);
+    constexpr int NT = 512;                      // eight waves, 2 x 4
+    constexpr int MA = 4, NBB = BN / 64;         // MFMA blocks per wave (rows, columns)
+    constexpr int PA = BT * KB / NT, PB = KB * BN / NT;
+    __shared__ T sA[2][E][KB][BT + 1];
+    __shared__ T sB[2][E][KB][BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bm = blockIdx.y * BT, bn = blockIdx.x * BN;
+    const int wm = (wave >> 2) * 64, wn = (wave & 3) * (BN / 4);
+    V4 accR[MA][NBB], accI[C ? MA : 1][C ? NBB : 1];
+#pragma unroll
+    for (int a = 0; a < MA; ++a)
+#pragma unroll
+        for (int b = 0; b < NBB; ++b) {
+            accR[a][b] = V4{0, 0, 0, 0};
+            if constexpr (C) accI[a][b] = V4{0, 0, 0, 0};
+        }
+    cx<T> pa[PA], pb[PB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = tid + NT * q, r = e / KB, c = e % KB;
+            const int gr = bm + r, gc = k0 + c;
+            pa[q] = (gr < M && gc < K) ? Vec2<T, C>::ld(Am, (int64_t)gr * lda + gc) : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int e = tid + NT * q, r = e / BN, c = e % BN;
+            const int gr = k0 + r, gc = bn + c;
+            pb[q] = (gr < K && gc < N) ? Vec2<T, C>::ld(Bm, (int64_t)gr * ldb + gc) : cx<T>{T(0), T(0)};
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = tid + NT * q, r = e / KB, c = e % KB;
+            sA[buf][0][c][r] = pa[q].re;
+            if constexpr (C) sA[buf][1][c][r] = pa[q].im;
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int e = tid + NT * q, r = e / BN, c = e % BN;
+            sB[buf][0][r][c] = pb[q].re;
+            if constexpr (C) sB[buf][1][r][c] = pb[q].im;
+        }
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    const int nk = (K + KB - 1) / KB;
+    for (int i = 0; i < nk; ++i) {
+        const int cur = i & 1;
+        if (i + 1 < nk) fetch((i + 1) * KB);
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 4) {
+            const int ka = kk + (lane >> 4);
+            T ar[MA], ai[MA], br[NBB], bi[NBB];
+#pragma unroll
+            for (int a = 0; a < MA; ++a) {
+                ar[a] = sA[cur][0][ka][wm + 16 * a + (lane & 15)];
+                if constexpr (C) ai[a] = sA[cur][1][ka][wm + 16 * a + (lane & 15)];
+            }
+#pragma unroll
+            for (int b = 0; b < NBB; ++b) {
+                br[b] = sB[cur][0][ka][wn + 16 * b + (lane & 15)];
+                if constexpr (C) bi[b] = sB[cur][1][ka][wn + 16 * b + (lane & 15)];
+            }
+#pragma unroll
+            for (int a = 0; a < MA; ++a)
+#pragma unroll
+                for (int b = 0; b < NBB; ++b) accR[a][b] = Mfma<T>::op(ar[a], br[b], accR[a][b]);
+            if constexpr (C) {
+#pragma unroll
+                for (int a = 0; a < MA; ++a)
+#pragma unroll
+                    for (int b = 0; b < NBB; ++b) accI[a][b] = Mfma<T>::op(ar[a], bi[b], accI[a][b]);
+#pragma unroll
+                for (int a = 0; a < MA; ++a)
+#pragma unroll
+                    for (int b = 0; b < NBB; ++b) accR[a][b] = Mfma<T>::op(-ai[a], bi[b], accR[a][b]);
+#pragma unroll
+                for (int a = 0; a < MA; ++a)
+#pragma unroll
+                    for (int b = 0; b < NBB; ++b) accI[a][b] = Mfma<T>::op(ai[a], br[b], accI[a][b]);
+            }
+        }
+        if (i + 1 < nk) stash(cur ^ 1);
+        __syncthreads();
+    }
+    // read-modify-write of C, one row of MFMA blocks at a time (the compiler
+    // memory barrier keeps it from hoisting every C load at once, which
+    // would need as many registers as the accumulators)
+#pragma unroll
+    for (int a = 0; a < MA; ++a) {
+#pragma unroll
+        for (int b = 0; b < NBB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = bm + wm + 16 * a + Mfma<T>::row(lane, r);
+                const int gc = bn + wn + 16 * b + (lane & 15);
+                if (gr < M && gc < N) {
+                    cx<T> v = Vec2<T, C>::ld(Cm, (int64_t)gr * ldc + gc);
+                    v.re -= accR[a][b][r];
+                    if constexpr (C) v.im -= accI[a][b][r];
+                    Acc<T, C>::st(Cm, (int64_t)gr * ldc + gc, v);
+                }
+            }
+        asm volatile("" ::: "memory");
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Triangular solves with one right-hand side, blocked by TB rows.
 // ---------------------------------------------------------------------------
 constexpr int TB = 64;
 
-// x[blk] solve with the TB x TB diagonal block: the block is staged in LDS
-// first so the sequential sweep touches LDS only.  Without TRANS the lower
-// factor is L (unit) and the upper U; with TRANS they are U^T (lower,
-// non-unit) and L^T (upper, unit), read from the same storage.
-// One triangular solve of a (possibly batched) sweep: the factors and the
-// complex128 work vector, by value or from a table indexed by blockIdx.z.
+// One triangular solve of a (possibly batched) sweep: the factors, the
+// complex128 work vector and the sweep's synchronisation words, by value or
+// from a table indexed by blockIdx.z.  sync holds, for the lower and then
+// the upper sweep, a ticket counter and one ready flag per TB-row block
+// (2 (nblk + 1) words, zeroed before the forward sweep).
 struct SolveProb {
     const void* LU;
     int64_t ld;
     int n;
     double* w;
+    unsigned* sync;
 };
 template <bool B>
 __device__ __forceinline__ SolveProb get_sp(const SolveProb& inl, const SolveProb* tbl) {
@@ -750,125 +903,137 @@ __device__ __forceinline__ SolveProb get_sp(const SolveProb& inl, const SolvePro
     else return inl;
 }
 
+__host__ __device__ inline int tri_blocks(int n) { return (n + TB - 1) / TB; }
+
+__global__ void tri_sync_reset_kernel(SolveProb inl, const SolveProb* tbl, int batched) {
+    const SolveProb P = batched ? tbl[blockIdx.z] : inl;
+    const int words = 2 * (tri_blocks(P.n) + 1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) P.sync[i] = 0u;
+}
+
+// Chained blocked triangular sweep, one workgroup per TB-row block and one
+// launch per sweep.  Workgroups take tickets in dispatch order and solve the
+// row blocks in dependency order (forward: 0, 1, ...; backward: the last
+// first), so a workgroup only ever waits for workgroups that are already
+// running.  Each one
+//   - stages its diagonal block in LDS and reads its right-hand side,
+//   - subtracts M(b, j) x_j for every solved block j as soon as its ready
+//     flag is up (x_j read with coherent loads),
+//   - solves with the diagonal block (one wave, column by column),
+//   - stores x_b and raises its flag (release).
+// M is the lower (LOWER) or upper factor: without TRANS L (unit) and U, read
+// by rows; with TRANS U^T (lower, non-unit) and L^T (upper, unit), read from
+// the same storage by columns.  Arithmetic in FP64 whatever T is.  A wait
+// that does not end within about a second (never expected: every
+// predecessor is resident) leaves NaN in the block instead of hanging.
 template <typename T, bool C, bool LOWER, bool TRANS, bool B>
-__global__ __launch_bounds__(64) void trsv_diag_kernel(SolveProb inl, const SolveProb* tbl, int r0) {
+__global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const SolveProb* tbl) {
     const SolveProb P = get_sp<B>(inl, tbl);
     const T* A = (const T*)P.LU;
     const int64_t ld = P.ld;
     const int n = P.n;
+    const int nblk = tri_blocks(n);
     double* x = P.w;
-    if (r0 >= n) return;
+    unsigned* ticket = P.sync + (LOWER ? 0 : nblk + 1);
+    unsigned* ready = ticket + 1;
     constexpr bool UNIT = LOWER != TRANS;
-    __shared__ double xs[TB][2];
-    __shared__ T blk[TB][TB][C ? 2 : 1];
-    const int nb = min(TB, n - r0);
-    const int t = threadIdx.x;
-    for (int r = 0; r < nb; ++r)
-        if (t < nb) {
-            cx<T> v = TRANS ? Acc<T, C>::ld(A, (int64_t)(r0 + t) * ld + (r0 + r))
-                            : Acc<T, C>::ld(A, (int64_t)(r0 + r) * ld + (r0 + t));
-            blk[r][t][0] = v.re;
-            if constexpr (C) blk[r][t][C ? 1 : 0] = v.im;
-        }
-    if (t < nb) { xs[t][0] = x[2 * (r0 + t)]; xs[t][1] = x[2 * (r0 + t) + 1]; }
+    __shared__ double dr[TB][TB + 1], di[C ? TB : 1][TB + 1];
+    __shared__ double xr[TB], xi[TB];
+    __shared__ double pr[4][TB], pi[4][TB];
+    __shared__ unsigned s_tk;
+    __shared__ int s_fail;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if ((int)blockIdx.x >= nblk) return;  // grid sized for the batch's largest n
+    if (t == 0) {
+        s_tk = atomicAdd(ticket, 1u);
+        s_fail = 0;
+    }
     __syncthreads();
-    auto ldb = [&](int r, int c) -> cx<double> {
-        return {(double)blk[r][c][0], C ? (double)blk[r][c][C ? 1 : 0] : 0.0};
+    if (s_tk >= (unsigned)nblk) return;  // sync words not reset: nothing to own
+    const int b = LOWER ? (int)s_tk : nblk - 1 - (int)s_tk;
+    const int r0 = b * TB, nb = min(TB, n - r0);
+    // element (i, j) of the operator M, rows/columns relative to r0/c0
+    auto mel = [&](int i, int c0, int j) -> cx<T> {
+        return TRANS ? Acc<T, C>::ld(A, (int64_t)(c0 + j) * ld + r0 + i) : Acc<T, C>::ld(A, (int64_t)(r0 + i) * ld + c0 + j);
     };
-    if (LOWER) {
-        for (int r = UNIT ? 1 : 0; r < nb; ++r) {  // a non-unit row 0 still divides
-            double sr = 0.0, si = 0.0;
-            if (t < r) {
-                cx<double> l = ldb(r, t);
-                sr = l.re * xs[t][0] - l.im * xs[t][1];
-                si = l.re * xs[t][1] + l.im * xs[t][0];
+    for (int e = t; e < TB * TB; e += 256) {
+        const int i = e / TB, j = e % TB;
+        cx<T> v = {T(0), T(0)};
+        if (i < nb && j < nb) v = mel(i, r0, j);
+        dr[i][j] = (double)v.re;
+        if constexpr (C) di[i][j] = (double)v.im;
+    }
+    __syncthreads();  // the diagonal block is complete before any wave reads it
+    // right-hand side: only this workgroup touches rows [r0, r0 + nb)
+    double acc_r = 0.0, acc_i = 0.0;
+    if (t < nb) { acc_r = x[2 * (r0 + t)]; acc_i = x[2 * (r0 + t) + 1]; }
+    const int jfirst = LOWER ? 0 : nblk - 1, jstep = LOWER ? 1 : -1;
+    for (int j = jfirst; j != b; j += jstep) {
+        if (t == 0) {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&ready[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 21)) { s_fail = 1; break; }
             }
-            sr = wave_sum(sr); si = wave_sum(si);
-            if (t == 0) {
-                if (UNIT) { xs[r][0] -= sr; xs[r][1] -= si; }
-                else {
-                    cx<double> num = {xs[r][0] - sr, xs[r][1] - si};
-                    cx<double> q = cdiv(num, ldb(r, r));
-                    xs[r][0] = q.re; xs[r][1] = q.im;
-                }
-            }
-            __syncthreads();
         }
-    } else {
-        for (int r = nb - 1; r >= 0; --r) {
-            double sr = 0.0, si = 0.0;
-            if (t > r && t < nb) {
-                cx<double> u = ldb(r, t);
-                sr = u.re * xs[t][0] - u.im * xs[t][1];
-                si = u.re * xs[t][1] + u.im * xs[t][0];
+        __syncthreads();
+        if (s_fail) break;
+        const int c0 = j * TB, nc = min(TB, n - c0);
+        if (t < nc) { xr[t] = ld_agent(&x[2 * (c0 + t)]); xi[t] = ld_agent(&x[2 * (c0 + t) + 1]); }
+        __syncthreads();
+        // partial products: without TRANS thread (row i = t / 4, quarter q)
+        // sums columns q, q + 4, ...; with TRANS wave q sums columns q, q + 4,
+        // ... for row i = lane (rows of the factor read contiguously)
+        double sr = 0.0, si = 0.0;
+        const int i = TRANS ? lane : t >> 2, q = TRANS ? wave : t & 3;
+        if (i < nb)
+            for (int c = q; c < nc; c += 4) {
+                const cx<T> m = mel(i, c0, c);
+                sr += (double)m.re * xr[c] - (double)m.im * xi[c];
+                si += (double)m.re * xi[c] + (double)m.im * xr[c];
             }
-            sr = wave_sum(sr); si = wave_sum(si);
-            if (t == 0) {
-                if (UNIT) { xs[r][0] -= sr; xs[r][1] -= si; }
-                else {
-                    cx<double> num = {xs[r][0] - sr, xs[r][1] - si};
-                    cx<double> q = cdiv(num, ldb(r, r));
-                    xs[r][0] = q.re; xs[r][1] = q.im;
-                }
-            }
-            __syncthreads();
+        if (!TRANS) {
+            sr += __shfl_xor(sr, 1, 64); si += __shfl_xor(si, 1, 64);
+            sr += __shfl_xor(sr, 2, 64); si += __shfl_xor(si, 2, 64);
+            if (q == 0) { pr[0][i] = sr; pi[0][i] = si; }
+        } else {
+            pr[q][i] = sr; pi[q][i] = si;
         }
+        __syncthreads();
+        if (t < nb) {
+            if (TRANS) {
+                acc_r -= pr[0][t] + pr[1][t] + pr[2][t] + pr[3][t];
+                acc_i -= pi[0][t] + pi[1][t] + pi[2][t] + pi[3][t];
+            } else {
+                acc_r -= pr[0][t];
+                acc_i -= pi[0][t];
+            }
+        }
+        __syncthreads();  // xr/pr are rewritten for the next block
     }
-    if (t < nb) { x[2 * (r0 + t)] = xs[t][0]; x[2 * (r0 + t) + 1] = xs[t][1]; }
-}
-
-// x[rows] -= A[rows, c0:c0+nb] x[c0:c0+nb]: one wave per row.  Rows below
-// the block (LOWER sweep: [c0+nb, n)) or above it ([0, c0)).
-template <typename T, bool C, bool B>
-__global__ __launch_bounds__(256) void trsv_update_kernel(SolveProb inl, const SolveProb* tbl, int c0, int lower) {
-    const SolveProb P = get_sp<B>(inl, tbl);
-    const T* A = (const T*)P.LU;
-    const int64_t ld = P.ld;
-    double* x = P.w;
-    if (c0 >= P.n) return;
-    const int nb = min(TB, P.n - c0);
-    const int row_begin = lower ? c0 + nb : 0, row_end = lower ? P.n : c0;
-    const int lane = threadIdx.x & 63;
-    const int r = row_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= row_end) return;
-    double sr = 0.0, si = 0.0;
-    if (lane < nb) {
-        cx<T> a = Acc<T, C>::ld(A, (int64_t)r * ld + c0 + lane);
-        double xr = x[2 * (c0 + lane)], xi = x[2 * (c0 + lane) + 1];
-        sr = (double)a.re * xr - (double)a.im * xi;
-        si = (double)a.re * xi + (double)a.im * xr;
+    if (s_fail) {
+        if (t < nb) { x[2 * (r0 + t)] = __builtin_nan(""); x[2 * (r0 + t) + 1] = __builtin_nan(""); }
+        return;  // the flag stays down: workgroups after this one time out too
     }
-    sr = wave_sum(sr); si = wave_sum(si);
-    if (lane == 0) { x[2 * r] -= sr; x[2 * r + 1] -= si; }
-}
-
-// Transposed block update x[rows] -= A[c0:c0+nb, rows]^T x[c0:c0+nb]: one
-// thread per row, so the column-block reads A[c][r] are coalesced over r.
-template <typename T, bool C, bool B>
-__global__ __launch_bounds__(256) void trsv_update_t_kernel(SolveProb inl, const SolveProb* tbl, int c0, int lower) {
-    const SolveProb P = get_sp<B>(inl, tbl);
-    const T* A = (const T*)P.LU;
-    const int64_t ld = P.ld;
-    double* x = P.w;
-    if (c0 >= P.n) return;
-    const int nb = min(TB, P.n - c0);
-    const int row_begin = lower ? c0 + nb : 0, row_end = lower ? P.n : c0;
-    __shared__ double xs[TB][2];
-    if (threadIdx.x < nb) {
-        xs[threadIdx.x][0] = x[2 * (c0 + threadIdx.x)];
-        xs[threadIdx.x][1] = x[2 * (c0 + threadIdx.x) + 1];
+    if (wave == 0) {
+        // column-oriented solve with the diagonal block: lane i owns row i
+        for (int s = 0; s < nb; ++s) {
+            const int c = LOWER ? s : nb - 1 - s;
+            if (!UNIT && lane == c) {
+                const cx<double> q = cdiv(cx<double>{acc_r, acc_i}, cx<double>{dr[c][c], C ? di[c][c] : 0.0});
+                acc_r = q.re; acc_i = q.im;
+            }
+            const double vr = __shfl(acc_r, c, 64), vi = __shfl(acc_i, c, 64);
+            if (lane < nb && (LOWER ? lane > c : lane < c)) {
+                const double mr = dr[lane][c], mi = C ? di[lane][c] : 0.0;
+                acc_r -= mr * vr - mi * vi;
+                acc_i -= mr * vi + mi * vr;
+            }
+        }
+        if (lane < nb) { x[2 * (r0 + lane)] = acc_r; x[2 * (r0 + lane) + 1] = acc_i; }
+        __threadfence();
+        if (lane == 0) __hip_atomic_store(&ready[b], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
-    const int r = row_begin + blockIdx.x * 256 + threadIdx.x;
-    if (r >= row_end) return;
-    double sr = 0.0, si = 0.0;
-    for (int c = 0; c < nb; ++c) {
-        cx<T> a = Acc<T, C>::ld(A, (int64_t)(c0 + c) * ld + r);
-        sr += (double)a.re * xs[c][0] - (double)a.im * xs[c][1];
-        si += (double)a.re * xs[c][1] + (double)a.im * xs[c][0];
-    }
-    x[2 * r] -= sr;
-    x[2 * r + 1] -= si;
 }
 
 // r = b - A x in FP64 (A of T/C, x/b/r as complex128 work vectors): one wave
@@ -1126,21 +1291,42 @@ __global__ __launch_bounds__(256) void gemm_sub_c64bf_kernel(GemmArgs<float> inl
             }
 }
 
-// bf16_parts: 0 = FP32 MFMA for every c64 product; 2 or 3 = the split bf16
-// kernel for the trailing updates (modes 1 and 4) of c64 factorisations.
+// gopts, the GEMM options of a factorisation: bits 0-3 = split-bf16 parts of
+// c64 trailing updates (0 = FP32 MFMA for every c64 product; 2 or 3 = the
+// split bf16 kernel for modes 1 and 4); bit 4 (GEMM_SMALL_TILES) = 64 x 64
+// tiles for every update.
+constexpr int GEMM_SMALL_TILES = 16;
+
+int gemm_opts(const Context* ctx) { return ctx->c64_gemm_parts | (ctx->lu_gemm_tile ? 0 : GEMM_SMALL_TILES); }
+
 template <typename T, bool C, bool B>
-void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, int N, int active, int bf16_parts,
+void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, int N, int active, int gopts,
              hipStream_t s) {
     if (M <= 0 || N <= 0) return;
     GemmArgs<T> g{};
     if constexpr (!B) g = lu_gemm_args<T, C>(one, st);
     const dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, active);
+    const int bf16_parts = gopts & 15;
     if constexpr (std::is_same<T, float>::value && C) {
         if (bf16_parts && (st.mode == 1 || st.mode == 4)) {
             if (bf16_parts == 2) hipLaunchKernelGGL((gemm_sub_c64bf_kernel<2, B>), grid, dim3(256), 0, s, g, d_tbl, st);
             else hipLaunchKernelGGL((gemm_sub_c64bf_kernel<3, B>), grid, dim3(256), 0, s, g, d_tbl, st);
             return;
         }
+    }
+    // trailing updates with room for 128-row tiles take the large-tile kernel
+    // (64-wide column tiles when the update is narrower than 128 or would
+    // leave a half-empty last 128-column tile)
+    if ((st.mode == 1 || st.mode == 4) && M >= 2 * BT && !(gopts & GEMM_SMALL_TILES)) {
+        const int rows = (M + BT - 1) / BT;
+        if ((N >= BT && N % BT == 0) || N > 2 * BT) {
+            hipLaunchKernelGGL((gemm_big_kernel<T, C, BT, B>), dim3((N + BT - 1) / BT, rows, active), dim3(512), 0,
+                               s, g, d_tbl, st);
+        } else {
+            hipLaunchKernelGGL((gemm_big_kernel<T, C, 64, B>), dim3((N + 63) / 64, rows, active), dim3(512), 0, s,
+                               g, d_tbl, st);
+        }
+        return;
     }
     hipLaunchKernelGGL((gemm_sub_kernel<T, C, B>), grid, dim3(256), 0, s, g, d_tbl, st);
 }
@@ -1150,7 +1336,7 @@ void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, in
 // (pivoting 0, LAPACK's pivot rows), or tournament pivoting (pivoting 1).
 template <typename T, bool C, bool B>
 void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int k, int active,
-                  int pivoting, int bf16_parts, hipStream_t s) {
+                  int pivoting, int gopts, hipStream_t s) {
     auto max_over = [&](auto f) {
         int m = 0;
         for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
@@ -1183,7 +1369,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
                        k + NB, 0, 0);
     hipLaunchKernelGGL((topblock_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
     const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
-    lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, bf16_parts, s);
+    lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
 }
 
 // Launch sequence of the two-level right-looking LU for one problem (B =
@@ -1191,7 +1377,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
 // problems sorted by decreasing n so that the first `active` entries are
 // the ones with columns left at a given step).
 template <typename T, bool C, bool B>
-void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int bf16_parts,
+void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int gopts,
                hipStream_t s) {
     const int nmax = ns[0];
     auto count_active = [&](int from) {
@@ -1209,7 +1395,7 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
                 for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
                 return m;
             };
-            panel_factor<T, C, B>(one, d_tbl, ns, k, active, pivoting, bf16_parts, s);
+            panel_factor<T, C, B>(one, d_tbl, ns, k, active, pivoting, gopts, s);
             // this panel's interchanges inside the outer block
             hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((W2 + 255) / 256, 1, active), dim3(256), 0, s, one,
                                d_tbl, k, k + NB, K0, K0 + W2, k, k + NB);
@@ -1217,8 +1403,8 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
             hipLaunchKernelGGL((linv_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, slot);
             const int Nin = max_over([&](int n) { return std::min(n, K0 + W2) - std::min(n, k + NB); });
             const int Min = max_over([&](int n) { return n - std::min(n, k + NB); });
-            lu_gemm<T, C, B>(one, d_tbl, LuStep{0, K0, k, slot}, NB, Nin, active, bf16_parts, s);
-            lu_gemm<T, C, B>(one, d_tbl, LuStep{1, K0, k, slot}, Min, Nin, active, bf16_parts, s);
+            lu_gemm<T, C, B>(one, d_tbl, LuStep{0, K0, k, slot}, NB, Nin, active, gopts, s);
+            lu_gemm<T, C, B>(one, d_tbl, LuStep{1, K0, k, slot}, Min, Nin, active, gopts, s);
         }
         // the outer block's interchanges on the columns left and right of it
         hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((nmax + 255) / 256, 1, act_outer), dim3(256), 0, s, one,
@@ -1228,10 +1414,10 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
         if (Nout > 0) {
             const int nb_blocks = (K1max - K0 + NB - 1) / NB;
             for (int b = 0; b < nb_blocks; ++b) {
-                if (b > 0) lu_gemm<T, C, B>(one, d_tbl, LuStep{2, K0, K0, b}, NB, Nout, act_outer, bf16_parts, s);
-                lu_gemm<T, C, B>(one, d_tbl, LuStep{3, K0, K0, b}, NB, Nout, act_outer, bf16_parts, s);
+                if (b > 0) lu_gemm<T, C, B>(one, d_tbl, LuStep{2, K0, K0, b}, NB, Nout, act_outer, gopts, s);
+                lu_gemm<T, C, B>(one, d_tbl, LuStep{3, K0, K0, b}, NB, Nout, act_outer, gopts, s);
             }
-            lu_gemm<T, C, B>(one, d_tbl, LuStep{4, K0, K0, 0}, Nout, Nout, act_outer, bf16_parts, s);
+            lu_gemm<T, C, B>(one, d_tbl, LuStep{4, K0, K0, 0}, Nout, Nout, act_outer, gopts, s);
         }
     }
 }
@@ -1246,7 +1432,7 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
 //       columns -> event eg; update of the remaining columns
 //   sp: wait eg; panels of block j+1 ...
 template <typename T, bool C>
-void lu_launch_lookahead(const LuProb& P, int pivoting, int bf16_parts, hipStream_t s, hipStream_t sp,
+void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, hipStream_t s, hipStream_t sp,
                          hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0) {
     const int n = P.n;
     const std::vector<int> ns1{n};
@@ -1257,13 +1443,13 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int bf16_parts, hipStrea
         const int K1 = std::min(n, K0 + W2);
         for (int k = K0; k < K1; k += NB) {
             const int kend = std::min(n, k + NB);
-            panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, bf16_parts, sp);
+            panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, gopts, sp);
             hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((W2 + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
                                k + NB, K0, K0 + W2, k, k + NB);
             const int slot = (k - K0) / NB;
             hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
-            lu_gemm<T, C, false>(P, nullptr, LuStep{0, K0, k, slot}, NB, K1 - kend, 1, bf16_parts, sp);
-            lu_gemm<T, C, false>(P, nullptr, LuStep{1, K0, k, slot}, n - kend, K1 - kend, 1, bf16_parts, sp);
+            lu_gemm<T, C, false>(P, nullptr, LuStep{0, K0, k, slot}, NB, K1 - kend, 1, gopts, sp);
+            lu_gemm<T, C, false>(P, nullptr, LuStep{1, K0, k, slot}, n - kend, K1 - kend, 1, gopts, sp);
         }
         hipEventRecord(ep, sp);
         hipStreamWaitEvent(sg, ep, 0);
@@ -1273,20 +1459,20 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int bf16_parts, hipStrea
         if (Nout > 0) {
             const int nb_blocks = (K1 - K0 + NB - 1) / NB;
             for (int b = 0; b < nb_blocks; ++b) {
-                if (b > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, b}, NB, Nout, 1, bf16_parts, sg);
-                lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, b}, NB, Nout, 1, bf16_parts, sg);
+                if (b > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, b}, NB, Nout, 1, gopts, sg);
+                lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, b}, NB, Nout, 1, gopts, sg);
             }
             const int c_next = std::min(n, K1 + W2);
             LuStep st4{4, K0, K0, 0};
             st4.c0 = K1;
             st4.c1 = c_next;
-            lu_gemm<T, C, false>(P, nullptr, st4, Nout, c_next - K1, 1, bf16_parts, sg);
+            lu_gemm<T, C, false>(P, nullptr, st4, Nout, c_next - K1, 1, gopts, sg);
             hipEventRecord(eg, sg);
             hipStreamWaitEvent(sp, eg, 0);
             if (c_next < n) {
                 st4.c0 = c_next;
                 st4.c1 = n;
-                lu_gemm<T, C, false>(P, nullptr, st4, Nout, n - c_next, 1, bf16_parts, sg);
+                lu_gemm<T, C, false>(P, nullptr, st4, Nout, n - c_next, 1, gopts, sg);
             }
         }
     }
@@ -1397,10 +1583,10 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
         rc = ensure_lu_streams(ctx);
         if (rc) return rc;
-        lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, ctx->c64_gemm_parts, s, ctx->lu_stream[0], ctx->lu_stream[1],
+        lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), s, ctx->lu_stream[0], ctx->lu_stream[1],
                                   ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
     } else {
-        lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, ctx->c64_gemm_parts, s);
+        lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, gemm_opts(ctx), s);
     }
     rc = check_hip(hipGetLastError(), "LU launch");
     if (rc) return rc;
@@ -1452,69 +1638,54 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
         ctx->lu_table_dev = (void*)d_tbl;
     }
     hipLaunchKernelGGL(lu_reset_kernel, dim3((count + 63) / 64), dim3(64), 0, s, (const LuProb*)d_tbl, count);
-    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, ctx->c64_gemm_parts, s);
+    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, gemm_opts(ctx), s);
     return check_hip(hipGetLastError(), "batched LU launch");
 }
 
 // Forward then backward sweep on the complex128 work vectors (already
 // permuted): L U w = w, or with TRANS U^T L^T w = w.  nmax = largest n of
-// the batch (B) or the one problem's n.
+// the batch (B) or the one problem's n.  One launch zeroes both sweeps'
+// synchronisation words, then one chained launch per sweep.
 template <typename T, bool C, bool TRANS, bool B>
 void tri_launch(const SolveProb& one, const SolveProb* d_tbl, int nmax, int count, hipStream_t s) {
-    for (int r0 = 0; r0 < nmax; r0 += TB) {
-        hipLaunchKernelGGL((trsv_diag_kernel<T, C, true, TRANS, B>), dim3(1, 1, count), dim3(64), 0, s, one, d_tbl,
-                           r0);
-        const int rows = nmax - r0 - TB;
-        if (rows > 0) {
-            if constexpr (TRANS)
-                hipLaunchKernelGGL((trsv_update_t_kernel<T, C, B>), dim3((rows + 255) / 256, 1, count), dim3(256),
-                                   0, s, one, d_tbl, r0, 1);
-            else
-                hipLaunchKernelGGL((trsv_update_kernel<T, C, B>), dim3((rows + 3) / 4, 1, count), dim3(256), 0, s,
-                                   one, d_tbl, r0, 1);
-        }
-    }
-    const int last = ((nmax - 1) / TB) * TB;
-    for (int r0 = last; r0 >= 0; r0 -= TB) {
-        hipLaunchKernelGGL((trsv_diag_kernel<T, C, false, TRANS, B>), dim3(1, 1, count), dim3(64), 0, s, one,
-                           d_tbl, r0);
-        if (r0 > 0) {
-            if constexpr (TRANS)
-                hipLaunchKernelGGL((trsv_update_t_kernel<T, C, B>), dim3((r0 + 255) / 256, 1, count), dim3(256), 0,
-                                   s, one, d_tbl, r0, 0);
-            else
-                hipLaunchKernelGGL((trsv_update_kernel<T, C, B>), dim3((r0 + 3) / 4, 1, count), dim3(256), 0, s,
-                                   one, d_tbl, r0, 0);
-        }
-    }
+    const int nblk = tri_blocks(nmax);
+    hipLaunchKernelGGL(tri_sync_reset_kernel, dim3((2 * (nblk + 1) + 255) / 256, 1, count), dim3(256), 0, s, one,
+                       d_tbl, (int)B);
+    hipLaunchKernelGGL((trsv_chain_kernel<T, C, true, TRANS, B>), dim3(nblk, 1, count), dim3(256), 0, s, one, d_tbl);
+    hipLaunchKernelGGL((trsv_chain_kernel<T, C, false, TRANS, B>), dim3(nblk, 1, count), dim3(256), 0, s, one, d_tbl);
 }
 
 template <typename T, bool C>
-void tri_solve_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
-    tri_launch<T, C, false, false>(SolveProb{LU, ld, n, w}, nullptr, n, 1, s);
+void tri_solve_t(const T* LU, int64_t ld, int n, double* w, unsigned* sync, bool trans, hipStream_t s) {
+    const SolveProb P{LU, ld, n, w, sync};
+    if (trans) tri_launch<T, C, true, false>(P, nullptr, n, 1, s);
+    else tri_launch<T, C, false, false>(P, nullptr, n, 1, s);
 }
 
-template <typename T, bool C>
-void tri_solve_trans_t(const T* LU, int64_t ld, int n, double* w, hipStream_t s) {
-    tri_launch<T, C, true, false>(SolveProb{LU, ld, n, w}, nullptr, n, 1, s);
+// The synchronisation words of single-problem triangular sweeps.
+int ensure_tri_sync(Context* ctx, int n) {
+    const size_t words = 2 * (size_t)(tri_blocks(n) + 1);
+    if (ctx->tri_sync_words >= words) return BEM_OK;
+    if (ctx->tri_sync) hipFree(ctx->tri_sync);
+    ctx->tri_sync = nullptr;
+    ctx->tri_sync_words = 0;
+    int rc = check_hip(hipMalloc(&ctx->tri_sync, words * sizeof(unsigned)), "triangular solve sync words");
+    if (rc) return rc;
+    ctx->tri_sync_words = words;
+    return BEM_OK;
 }
 
-void tri_solve_trans(const void* LU, int64_t ld, int n, int dtype, double* w, hipStream_t s) {
+int tri_solve_any(Context* ctx, const void* LU, int64_t ld, int n, int dtype, double* w, bool trans, hipStream_t s) {
+    int rc = ensure_tri_sync(ctx, n);
+    if (rc) return rc;
+    unsigned* sync = (unsigned*)ctx->tri_sync;
     switch (dtype) {
-        case BEM_F64: tri_solve_trans_t<double, false>((const double*)LU, ld, n, w, s); break;
-        case BEM_C128: tri_solve_trans_t<double, true>((const double*)LU, ld, n, w, s); break;
-        case BEM_F32: tri_solve_trans_t<float, false>((const float*)LU, ld, n, w, s); break;
-        default: tri_solve_trans_t<float, true>((const float*)LU, ld, n, w, s); break;
+        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, sync, trans, s); break;
+        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w, sync, trans, s); break;
+        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w, sync, trans, s); break;
+        default: tri_solve_t<float, true>((const float*)LU, ld, n, w, sync, trans, s); break;
     }
-}
-
-void tri_solve(const void* LU, int64_t ld, int n, int dtype, double* w, hipStream_t s) {
-    switch (dtype) {
-        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, s); break;
-        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w, s); break;
-        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w, s); break;
-        default: tri_solve_t<float, true>((const float*)LU, ld, n, w, s); break;
-    }
+    return BEM_OK;
 }
 
 int ensure_work(Context* ctx, size_t bytes) {
@@ -1605,7 +1776,7 @@ int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const i
     dim3 g((n + 255) / 256);
     hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, (const void*)b, dtype, n, w0);
     hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)w0, (const int32_t*)perm, n, w1);
-    tri_solve(LU, ld, n, dtype, w1, s);
+    if ((rc = tri_solve_any(ctx, LU, ld, n, dtype, w1, false, s))) return rc;
     hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
     return check_hip(hipGetLastError(), "LU solve");
 }
@@ -1626,7 +1797,7 @@ int lu_solve_transpose(Context* ctx, const void* LU, int64_t ld, int n, int dtyp
     if (rc) return rc;
     dim3 g((n + 255) / 256);
     hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, (const void*)b, dtype, n, w0);
-    tri_solve_trans(LU, ld, n, dtype, w0, s);
+    if ((rc = tri_solve_any(ctx, LU, ld, n, dtype, w0, true, s))) return rc;
     hipLaunchKernelGGL(unpermute_kernel, g, dim3(256), 0, s, (const double*)w0, (const int32_t*)perm, n, w1);
     hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
     return check_hip(hipGetLastError(), "transposed LU solve");
@@ -1685,7 +1856,7 @@ int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const vo
     if ((rc = norm(wb, &bn))) return rc;
     // x0 = LU^{-1} b
     hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)wb, (const int32_t*)perm, n, wx);
-    tri_solve(LU, ldlu, n, dtype_lu, wx, s);
+    if ((rc = tri_solve_any(ctx, LU, ldlu, n, dtype_lu, wx, false, s))) return rc;
     double r0n = -1.0, rn = 0.0;
     int it = 0, conv = 0;
     for (;;) {
@@ -1695,7 +1866,7 @@ int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const vo
         if (rn <= tol * bn) { conv = 1; break; }
         if (it >= max_iter) break;
         hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)wr, (const int32_t*)perm, n, wd);
-        tri_solve(LU, ldlu, n, dtype_lu, wd, s);
+        if ((rc = tri_solve_any(ctx, LU, ldlu, n, dtype_lu, wd, false, s))) return rc;
         hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)wd, n, (void*)wx, BEM_C128, 1);
         ++it;
     }
@@ -1988,6 +2159,7 @@ int batch_plan(Context* ctx, int slot, int count, const void* const* LU, const i
         bytes += ((size_t)n[q] * 4 + 255) / 256 * 256;          // perm
         bytes += ((size_t)n[q] * 16 + 255) / 256 * 256 * 2;     // w, r
         bytes += 256;                                           // acc
+        bytes += (2 * ((size_t)tri_blocks(n[q]) + 1) * 4 + 255) / 256 * 256;  // sweep sync words
     }
     const size_t vt = (sizeof(VecProb) * count + 255) / 256 * 256;
     const size_t st = (sizeof(SolveProb) * count + 255) / 256 * 256;
@@ -2023,9 +2195,10 @@ int batch_plan(Context* ctx, int slot, int count, const void* const* LU, const i
         v.r = (double*)p;
         p += ((size_t)n[q] * 16 + 255) / 256 * 256;
         v.acc = (double*)p;
+        p += 256;
         v.out = out ? out + (size_t)out_stride * q : nullptr;
         vt_h[q] = v;
-        st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w};
+        st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w, (unsigned*)p};
         plan.nmax = std::max(plan.nmax, n[q]);
     }
     plan.d_vec = (VecProb*)(base + bytes);

@@ -252,6 +252,53 @@ def test_lu_partial_pivoting_matches_lapack(bem, n):
     assert rel(to_np(At), lu) <= 1e-12
 
 
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64, torch.float64, torch.float32])
+def test_lu_gemm_tiles_identical(bem, dtype):
+    """The 128-row-tile trailing updates accumulate every entry in the same
+    order as the 64 x 64 kernel, so both give bit-identical factors."""
+    n = 777
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if dtype.is_complex else 0)
+    At = torch.tensor(A, device=DEV).to(dtype).contiguous()
+    out = []
+    for tile in (0, 1):
+        ctx = bem.Context(0)
+        ctx.set_option(bem._lib.OPT_LU_GEMM_TILE, tile)
+        LU = At.clone()
+        ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+        print(f"[lu tiles] dtype={dtype} tile={tile}", flush=True)
+        ctx.lu_factor(LU, ipiv)
+        out.append((LU, ipiv))
+    assert torch.equal(out[0][1], out[1][1])
+    assert torch.equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-11), (torch.float64, 1e-11),
+                                       (torch.complex64, 2e-3)])
+@pytest.mark.parametrize("n", [2049, 4100])
+def test_lu_solve_many_blocks(bem, dtype, tol, n):
+    """Chained triangular sweeps over many 64-row blocks, plain and
+    transposed, against numpy."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if dtype.is_complex else 0)
+    A = A + np.eye(n) * np.sqrt(n)
+    b = rng.standard_normal(n) + (1j * rng.standard_normal(n) if dtype.is_complex else 0)
+    ctx = bem.Context(0)
+    LU = torch.tensor(A, device=DEV).to(dtype).contiguous()
+    A = LU.cpu().numpy().astype(np.complex128)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    print(f"[lu many blocks] n={n} dtype={dtype}", flush=True)
+    ctx.lu_factor(LU, ipiv)
+    for trans in (False, True):
+        x = torch.tensor(b, device=DEV).to(dtype)
+        if trans:
+            ctx.lu_solve_transpose(LU, ipiv, x)
+        else:
+            ctx.lu_solve(LU, ipiv, x)
+        ref = np.linalg.solve(A.T if trans else A, b)
+        assert rel(to_np(x).astype(np.complex128), ref) <= tol
+
+
 def test_set_option_rejects(bem):
     ctx = bem.Context(0)
     with pytest.raises(bem.BEMError):
@@ -260,6 +307,8 @@ def test_set_option_rejects(bem):
         ctx.set_option(99, 0)
     with pytest.raises(bem.BEMError):
         ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
+    with pytest.raises(bem.BEMError):
+        ctx.set_option(bem._lib.OPT_LU_GEMM_TILE, 2)
 
 
 def test_lu_singular_raises(bem):
