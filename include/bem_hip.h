@@ -237,7 +237,14 @@ enum {
                                 well-conditioned systems only */,
     BEM_OPT_LU_GEMM_TILE = 4 /* 1 (default) = trailing updates of at least
                                 256 rows use 128-row MFMA tiles; 0 = 64 x 64
-                                tiles for every update */
+                                tiles for every update */,
+    BEM_OPT_LU_PANEL = 5     /* partial-pivoting panels of bem_lu_factor:
+                                1 (default) = one persistent launch per
+                                64-column panel (one workgroup per compute
+                                unit of the panel's stream, grid barriers,
+                                8-column sub-panels in LDS) where the rows
+                                fit; N > 1 = the same with at most N
+                                workgroups; 0 = one launch per column */
 };
 
 /* Sets a context option.  No reference counterpart. */

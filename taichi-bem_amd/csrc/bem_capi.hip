@@ -204,6 +204,10 @@ int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
             if (value < -1 || value > 1024) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_LOOKAHEAD takes -1..1024");
             ctx->impl.lu_lookahead_cus = value;
             return BEM_OK;
+        case BEM_OPT_LU_PANEL:
+            if (value < 0 || value > 1024) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_PANEL takes 0..1024");
+            ctx->impl.lu_panel = value;
+            return BEM_OK;
         case BEM_OPT_LU_GEMM_TILE:
             if (value != 0 && value != 1) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_GEMM_TILE takes 0 or 1");
             ctx->impl.lu_gemm_tile = value;

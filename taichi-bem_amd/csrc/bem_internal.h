@@ -84,6 +84,7 @@ struct Context {
     int c64_gemm_parts = 0;       // c64 LU trailing updates: 0 FP32 MFMA, 2/3 split-bf16 parts
     int lu_lookahead_cus = 32;    // LU look-ahead: 0 off, > 0 panel-stream CUs, -1 unmasked streams
     int lu_pivoting = 0;          // LU panels: 0 partial pivoting (LAPACK rows), 1 tournament
+    int lu_panel = 1;             // LU partial-pivoting panels: 1 persistent launch where it fits, 0 per column
     int lu_gemm_tile = 1;         // LU trailing updates: 1 128-row tiles where they fit, 0 64 x 64 only
     int lu_streams_cus = 0;       // the setting the streams were created for
     hipStream_t lu_stream[2] = {nullptr, nullptr};

@@ -86,6 +86,7 @@ struct LuProb {
     void* SL;
     int* tids;   // tournament candidates: two levels of ceil(n / TRW) groups x NB row ids
     void* SU;    // -inv(U11) of the current panel (NB x NB)
+    void* pan;   // exchange area of the persistent panel kernel (PanelX<T>)
 };
 
 template <bool B>
@@ -253,6 +254,269 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
         cx<T> bb = Acc<T, C>::ld(A, (int64_t)p * ld + cc);
         Acc<T, C>::st(A, (int64_t)c * ld + cc, bb);
         Acc<T, C>::st(A, (int64_t)p * ld + cc, a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent panel factorisation with partial pivoting: one launch per
+// NB-column panel of a single problem.  G workgroups, each owning the R
+// consecutive rows [k + gR, k + (g+1)R) of the panel, stay resident and
+// synchronise through a grid barrier.  The panel is processed in sub-panels
+// of SB columns whose entries (for the workgroup's rows) stay in LDS:
+//   per column c: every workgroup publishes its best pivot candidate (row,
+//     |re|+|im|, its SB sub-panel values) and, if it owns row c, row c's
+//     values; barrier; every workgroup picks the same pivot row p (largest
+//     |re|+|im|, first row on ties: LAPACK's rule); workgroup 0 records
+//     ipiv[c] and swaps rows c and p in the panel columns outside the
+//     sub-panel; the owners of rows c and p swap them in LDS; every
+//     workgroup eliminates column c from its rows below c (the same
+//     arithmetic as col_step_kernel).
+//   per sub-panel: the factored sub-panel is written back; barrier; every
+//     workgroup forms U_s = inv(L_ss) A[sub-panel rows, right of it]
+//     (SB x < NB) and updates its rows below the sub-panel right of it;
+//     workgroup 0 stores U_s after the next barrier, when nobody reads
+//     those entries any more.
+// A panel costs NB + NB/SB - 1 barriers instead of NB + 1 dependent
+// launches, and its right part is read and written once per sub-panel
+// instead of once per column.  Every wait is bounded (about two seconds of
+// wall clock): a workgroup that gives up raises the shared fail word, every
+// other one then leaves at its next wait, and info becomes -2.
+// ---------------------------------------------------------------------------
+constexpr int SB = 8;        // sub-panel width
+constexpr int PGMAX = 256;   // largest workgroup count of a persistent panel
+template <typename T>
+constexpr int panel_rmax() { return (128 * 1024) / (SB * (int)sizeof(cx<T>)); }
+
+template <typename T>
+struct PanelX {
+    unsigned count;       // grid barrier arrivals since the factorisation began
+    unsigned pad0;
+    int fail, pad1;
+    double cval[2][PGMAX];        // candidates per workgroup, double-buffered by column parity
+    int crow[2][PGMAX];
+    cx<T> cvals[2][PGMAX][SB];
+    cx<T> rowc[2][SB];            // row c's sub-panel values, from its owner
+};
+
+constexpr long long BARRIER_TICKS = 200000000;  // wall-clock ticks (100 MHz): 2 s
+
+// Barrier over the workgroups of a persistent launch on a counter that only
+// grows during a factorisation: each arrival adds one (release: this
+// workgroup's stores before it), then waits with relaxed loads until the
+// count reaches target and takes one acquire fence.  (Acquire loads in the
+// wait, or a release per poll, would write back or invalidate the L2 of
+// the workgroup's XCD on every iteration, beside the trailing-update
+// kernels running there.)  Returns false for every workgroup once any wait
+// has timed out.
+__device__ bool grid_barrier(unsigned* count, int* fail, unsigned target) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const long long t0 = wall_clock64();
+        while ((int)(__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+            __builtin_amdgcn_s_sleep(1);
+            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > BARRIER_TICKS) {
+                ok = 0;
+                break;
+            }
+        }
+        if (!ok) __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// Barriers one persistent panel launch of w columns takes (one per column,
+// one after every sub-panel but the last).
+__host__ __device__ inline int panel_barriers(int w) { return w + (w + SB - 1) / SB - 1; }
+
+template <typename T, bool C>
+__device__ __forceinline__ cx<T> ld_el_agent(const T* A, int64_t idx) {
+    if constexpr (C) return {ld_agent(A + 2 * idx), ld_agent(A + 2 * idx + 1)};
+    else return {ld_agent(A + idx), T(0)};
+}
+
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int G, int R, unsigned base) {
+    constexpr int RMAX = panel_rmax<T>();
+    __shared__ cx<T> sub[RMAX][SB];   // this workgroup's rows, sub-panel columns
+    __shared__ cx<T> us[SB][NB];      // U_s of the sub-panel
+    __shared__ cx<T> lss[SB][SB];     // L_ss (unit lower) of the sub-panel
+    __shared__ cx<T> urow[SB], orow[SB];
+    __shared__ double wv[4];
+    __shared__ int wi[4], wg[4];
+    __shared__ int s_p, s_gw;
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    const int n = P.n, kend = min(n, k + NB);
+    PanelX<T>* X = (PanelX<T>*)P.pan;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = blockIdx.x;
+    const int rb = k + g * R, re = min(n, rb + R), nr = max(0, re - rb);
+    int par = 0;
+    unsigned target = base;  // barrier targets: base + G, base + 2G, ...
+    bool pending_u = false;
+    int pu_cs = 0, pu_sw = 0, pu_ce = 0;
+    auto fail_out = [&]() {
+        if (t == 0) __hip_atomic_store(P.info, -2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    for (int cs = k; cs < kend; cs += SB) {
+        const int ce = min(kend, cs + SB), sw = ce - cs;
+        const int i0 = max(0, cs - rb);
+        for (int e = i0 * SB + t; e < nr * SB; e += 256) {
+            const int i = e / SB, j = e % SB;
+            if (j < sw) sub[i][j] = ld_el_agent<T, C>(A, (int64_t)(rb + i) * ld + cs + j);
+        }
+        __syncthreads();
+        for (int c = cs; c < ce; ++c) {
+            const int jc = c - cs;
+            // this workgroup's candidate: rows ascending per thread, ties to the first row
+            double b = -1.0;
+            int bi = 0x7fffffff;
+            for (int i = max(0, c - rb) + t; i < nr; i += 256) {
+                const cx<T> v = sub[i][jc];
+                const double m = fabs((double)v.re) + fabs((double)v.im);
+                if (m > b) { b = m; bi = rb + i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(b, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > b || (ob == b && oi < bi)) { b = ob; bi = oi; }
+            }
+            if (lane == 0) { wv[wave] = b; wi[wave] = bi; }
+            __syncthreads();
+            b = wv[0];
+            bi = wi[0];
+            for (int q = 1; q < 4; ++q)
+                if (wv[q] > b || (wv[q] == b && wi[q] < bi)) { b = wv[q]; bi = wi[q]; }
+            if (t == 0) { X->cval[par][g] = b; X->crow[par][g] = bi; }
+            if (t < sw) {
+                if (bi != 0x7fffffff) X->cvals[par][g][t] = sub[bi - rb][t];
+                if (c >= rb && c < re) X->rowc[par][t] = sub[c - rb][t];
+            }
+            target += (unsigned)G;
+            if (!grid_barrier(&X->count, &X->fail, target)) { fail_out(); return; }
+            if (c == cs && pending_u) {
+                // every workgroup has read the previous sub-panel's rows: store U_s
+                if (g == 0) {
+                    const int W = kend - pu_ce;
+                    for (int e = t; e < pu_sw * W; e += 256)
+                        Acc<T, C>::st(A, (int64_t)(pu_cs + e / W) * ld + pu_ce + e % W, us[e / W][e % W]);
+                }
+                pending_u = false;
+            }
+            // the pivot: the same reduction over the G candidates in every workgroup
+            b = -1.0;
+            bi = 0x7fffffff;
+            int bg = 0;
+            for (int q = t; q < G; q += 256) {
+                const double v = ld_agent(&X->cval[par][q]);
+                const int r = ld_agent(&X->crow[par][q]);
+                if (v > b || (v == b && r < bi)) { b = v; bi = r; bg = q; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(b, off, 64);
+                const int oi = __shfl_xor(bi, off, 64), og = __shfl_xor(bg, off, 64);
+                if (ob > b || (ob == b && oi < bi)) { b = ob; bi = oi; bg = og; }
+            }
+            __syncthreads();  // wv/wi of the local search are read by every thread above
+            if (lane == 0) { wv[wave] = b; wi[wave] = bi; wg[wave] = bg; }
+            __syncthreads();
+            if (t == 0) {
+                double bb = wv[0];
+                int ii = wi[0], gg = wg[0];
+                for (int q = 1; q < 4; ++q)
+                    if (wv[q] > bb || (wv[q] == bb && wi[q] < ii)) { bb = wv[q]; ii = wi[q]; gg = wg[q]; }
+                s_p = ii;
+                s_gw = gg;
+            }
+            __syncthreads();
+            const int p = s_p, gw = s_gw;
+            if (t < sw) {
+                urow[t] = {ld_agent((const T*)&X->cvals[par][gw][t].re), ld_agent((const T*)&X->cvals[par][gw][t].im)};
+                orow[t] = {ld_agent((const T*)&X->rowc[par][t].re), ld_agent((const T*)&X->rowc[par][t].im)};
+            }
+            __syncthreads();
+            const cx<T> piv = urow[jc];
+            const bool dz = piv.re == T(0) && piv.im == T(0);
+            if (g == 0) {
+                if (t == 0) {
+                    P.ipiv[c] = p;
+                    if (dz && *P.info == 0) *P.info = c + 1;
+                }
+                const int col = k + t;
+                if (p != c && t < kend - k && (col < cs || col >= ce)) {
+                    const cx<T> a = ld_el_agent<T, C>(A, (int64_t)c * ld + col);
+                    const cx<T> a2 = ld_el_agent<T, C>(A, (int64_t)p * ld + col);
+                    Acc<T, C>::st(A, (int64_t)c * ld + col, a2);
+                    Acc<T, C>::st(A, (int64_t)p * ld + col, a);
+                }
+            }
+            if (p != c && t < sw) {
+                if (p >= rb && p < re) sub[p - rb][t] = orow[t];
+                if (c >= rb && c < re) sub[c - rb][t] = urow[t];
+            }
+            __syncthreads();
+            for (int i = max(0, c + 1 - rb) + t; i < nr; i += 256) {
+                cx<T> l = sub[i][jc];
+                if (!dz) l = cdiv(l, piv);
+                sub[i][jc] = l;
+                for (int j = jc + 1; j < sw; ++j) {
+                    const cx<T> lu = cmul(l, urow[j]);
+                    sub[i][j].re -= lu.re;
+                    sub[i][j].im -= lu.im;
+                }
+            }
+            __syncthreads();
+            par ^= 1;
+        }
+        for (int e = i0 * SB + t; e < nr * SB; e += 256) {
+            const int i = e / SB, j = e % SB;
+            if (j < sw) Acc<T, C>::st(A, (int64_t)(rb + i) * ld + cs + j, sub[i][j]);
+        }
+        if (ce >= kend) break;
+        target += (unsigned)G;
+        if (!grid_barrier(&X->count, &X->fail, target)) { fail_out(); return; }
+        // U_s = inv(L_ss) A[cs:ce, ce:kend], formed redundantly by every workgroup
+        const int W = kend - ce;
+        if (t < SB * SB) {
+            const int m = t / SB, q = t % SB;
+            lss[m][q] = (m < sw && q < m) ? ld_el_agent<T, C>(A, (int64_t)(cs + m) * ld + cs + q) : cx<T>{T(0), T(0)};
+        }
+        __syncthreads();
+        if (t < W) {
+            for (int m = 0; m < sw; ++m) {
+                cx<T> v = ld_el_agent<T, C>(A, (int64_t)(cs + m) * ld + ce + t);
+                for (int q = 0; q < m; ++q) {
+                    const cx<T> lu = cmul(lss[m][q], us[q][t]);
+                    v.re -= lu.re;
+                    v.im -= lu.im;
+                }
+                us[m][t] = v;
+            }
+        }
+        __syncthreads();
+        // rows below the sub-panel: A[r, ce:kend] -= L[r, cs:ce] U_s
+        const int i1 = max(0, ce - rb);
+        for (int e = i1 * W + t; e < nr * W; e += 256) {
+            const int i = e / W, j = e % W;
+            cx<T> v = ld_el_agent<T, C>(A, (int64_t)(rb + i) * ld + ce + j);
+            for (int m = 0; m < sw; ++m) {
+                const cx<T> lu = cmul(sub[i][m], us[m][j]);
+                v.re -= lu.re;
+                v.im -= lu.im;
+            }
+            Acc<T, C>::st(A, (int64_t)(rb + i) * ld + ce + j, v);
+        }
+        __syncthreads();  // the next sub-panel's load reads these rows through other threads
+        pending_u = true;
+        pu_cs = cs;
+        pu_sw = sw;
+        pu_ce = ce;
     }
 }
 
@@ -924,7 +1188,7 @@ __global__ void tri_sync_reset_kernel(SolveProb inl, const SolveProb* tbl, int b
 // M is the lower (LOWER) or upper factor: without TRANS L (unit) and U, read
 // by rows; with TRANS U^T (lower, non-unit) and L^T (upper, unit), read from
 // the same storage by columns.  Arithmetic in FP64 whatever T is.  A wait
-// that does not end within about a second (never expected: every
+// that does not end within about two seconds (never expected: every
 // predecessor is resident) leaves NaN in the block instead of hanging.
 template <typename T, bool C, bool LOWER, bool TRANS, bool B>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const SolveProb* tbl) {
@@ -970,11 +1234,12 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const So
     const int jfirst = LOWER ? 0 : nblk - 1, jstep = LOWER ? 1 : -1;
     for (int j = jfirst; j != b; j += jstep) {
         if (t == 0) {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&ready[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+            const long long t0 = wall_clock64();
+            while (__hip_atomic_load(&ready[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
                 __builtin_amdgcn_s_sleep(2);
-                if (++spins > (1u << 21)) { s_fail = 1; break; }
+                if (wall_clock64() - t0 > BARRIER_TICKS) { s_fail = 1; break; }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (s_fail) break;
@@ -1334,14 +1599,27 @@ void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, in
 // Factorisation of the NB-column panel at k (rows k..n of every active
 // problem): partial pivoting with one fused update/search launch per column
 // (pivoting 0, LAPACK's pivot rows), or tournament pivoting (pivoting 1).
+// pgrid > 0 (single problems): partial pivoting runs as one persistent
+// launch of at most pgrid workgroups (one per compute unit of the stream)
+// when the panel's rows fit their LDS.
 template <typename T, bool C, bool B>
 void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int k, int active,
-                  int pivoting, int gopts, hipStream_t s) {
+                  int pivoting, int gopts, int pgrid, unsigned* pbase, hipStream_t s) {
     auto max_over = [&](auto f) {
         int m = 0;
         for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
         return m;
     };
+    if (pivoting == 0 && !B && pgrid > 0) {
+        const int nrem = ns[0] - k;
+        const int G = std::max(1, std::min({pgrid, PGMAX, (nrem + 63) / 64}));
+        const int R = (nrem + G - 1) / G;
+        if (R <= panel_rmax<T>()) {
+            hipLaunchKernelGGL((panel_persist_kernel<T, C>), dim3(G), dim3(256), 0, s, one, k, G, R, *pbase);
+            *pbase += (unsigned)(panel_barriers(std::min(NB, nrem)) * G);
+            return;
+        }
+    }
     if (pivoting == 0) {
         const int kendmax = std::min(ns[0], k + NB);
         for (int c = k; c <= kendmax; ++c) {
@@ -1378,8 +1656,9 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
 // the ones with columns left at a given step).
 template <typename T, bool C, bool B>
 void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int gopts,
-               hipStream_t s) {
+               int pgrid, hipStream_t s) {
     const int nmax = ns[0];
+    unsigned pbase = 0;
     auto count_active = [&](int from) {
         int a = 0;
         while (a < (int)ns.size() && ns[a] > from) ++a;
@@ -1395,7 +1674,7 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
                 for (int q = 0; q < active; ++q) m = std::max(m, f(ns[q]));
                 return m;
             };
-            panel_factor<T, C, B>(one, d_tbl, ns, k, active, pivoting, gopts, s);
+            panel_factor<T, C, B>(one, d_tbl, ns, k, active, pivoting, gopts, pgrid, &pbase, s);
             // this panel's interchanges inside the outer block
             hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((W2 + 255) / 256, 1, active), dim3(256), 0, s, one,
                                d_tbl, k, k + NB, K0, K0 + W2, k, k + NB);
@@ -1432,10 +1711,11 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
 //       columns -> event eg; update of the remaining columns
 //   sp: wait eg; panels of block j+1 ...
 template <typename T, bool C>
-void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, hipStream_t s, hipStream_t sp,
+void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hipStream_t s, hipStream_t sp,
                          hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0) {
     const int n = P.n;
     const std::vector<int> ns1{n};
+    unsigned pbase = 0;
     hipEventRecord(e0, s);
     hipStreamWaitEvent(sp, e0, 0);
     hipStreamWaitEvent(sg, e0, 0);
@@ -1443,7 +1723,7 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, hipStream_t s
         const int K1 = std::min(n, K0 + W2);
         for (int k = K0; k < K1; k += NB) {
             const int kend = std::min(n, k + NB);
-            panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, gopts, sp);
+            panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, gopts, pgrid, &pbase, sp);
             hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((W2 + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
                                k + NB, K0, K0 + W2, k, k + NB);
             const int slot = (k - K0) / NB;
@@ -1489,7 +1769,8 @@ size_t lu_scratch_bytes(int n) {
     const int gmax = (n + RPB - 1) / RPB;
     const size_t tids = 2 * (size_t)((n + TRW - 1) / TRW) * NB * sizeof(int);
     return 512 + (((size_t)gmax * (sizeof(double) + sizeof(int)) + 255) & ~(size_t)255) +
-           (W2 / NB) * NB * NB * sizeof(cx<T>) + 256 + ((tids + 255) & ~(size_t)255) + NB * NB * sizeof(cx<T>) + 256;
+           (W2 / NB) * NB * NB * sizeof(cx<T>) + 256 + ((tids + 255) & ~(size_t)255) + NB * NB * sizeof(cx<T>) + 256 +
+           sizeof(PanelX<T>) + 256;
 }
 
 template <typename T, bool C>
@@ -1504,6 +1785,7 @@ LuProb lu_carve(char* base, void* A, int64_t ld, int n, int32_t* ipiv, int* info
     P.SL = (void*)(((uintptr_t)(P.pidx + gmax) + 255) & ~(uintptr_t)255);
     P.tids = (int*)(((uintptr_t)((cx<T>*)P.SL + (W2 / NB) * NB * NB) + 255) & ~(uintptr_t)255);
     P.SU = (void*)(((uintptr_t)(P.tids + 2 * ((n + TRW - 1) / TRW) * NB) + 255) & ~(uintptr_t)255);
+    P.pan = (void*)(((uintptr_t)((cx<T>*)P.SU + NB * NB) + 255) & ~(uintptr_t)255);
     return P;
 }
 
@@ -1560,15 +1842,35 @@ int ensure_lu_streams(Context* ctx) {
 
 // counter = 0 and info = 0 of every problem (a kernel, not a memset, so the
 // sequence can be captured in a graph)
+// The counters of a factorisation (the persistent panel's barrier words sit
+// at the start of its exchange area, the same layout for every T).
+__device__ void lu_reset_words(const LuProb& P) {
+    *P.counter = 0u;
+    *P.info = 0;
+    unsigned* w = (unsigned*)P.pan;
+    w[0] = 0u;  // barrier arrivals
+    w[2] = 0u;  // fail
+}
 __global__ void lu_reset_kernel(const LuProb* tbl, int count) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
-    *tbl[q].counter = 0u;
-    *tbl[q].info = 0;
+    lu_reset_words(tbl[q]);
 }
-__global__ void lu_reset_one_kernel(LuProb P) {
-    *P.counter = 0u;
-    *P.info = 0;
+__global__ void lu_reset_one_kernel(LuProb P) { lu_reset_words(P); }
+
+// Workgroups of a persistent panel on the CU-masked panel stream.  The mask
+// takes the first np compute units, which fill whole XCDs (num_cu / 8 each)
+// in order, and workgroups are dealt round-robin over the XCDs the mask
+// touches, so every XCD must hold its share: with a partial last XCD of r
+// units that is (XCDs touched) x r workgroups.  (Measured: a grid of 48 on
+// a 48-unit mask, 24 per XCD on 32 + 16 units, never became resident; the
+// panel's bounded waits turned that into an error.)
+int panel_grid_masked(const Context* ctx) {
+    const int ncu = ctx->num_cu;
+    const int np = std::min(ctx->lu_lookahead_cus, ncu - 1);
+    const int per = std::max(1, ncu / 8), full = np / per, rem = np % per;
+    if (rem == 0) return np;
+    return (full + 1) * rem;
 }
 
 template <typename T, bool C>
@@ -1580,13 +1882,24 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     hipLaunchKernelGGL(lu_reset_one_kernel, dim3(1), dim3(1), 0, s, P);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
+    if (ctx->num_cu <= 0) {
+        hipDeviceProp_t prop;
+        rc = check_hip(hipGetDeviceProperties(&prop, ctx->device), "device properties");
+        if (rc) return rc;
+        ctx->num_cu = prop.multiProcessorCount;
+    }
+    // workgroups of the persistent panel: one per compute unit its stream may use
+    int pgrid = ctx->lu_panel ? ctx->num_cu : 0;
     if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
         rc = ensure_lu_streams(ctx);
         if (rc) return rc;
-        lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), s, ctx->lu_stream[0], ctx->lu_stream[1],
-                                  ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
+        if (pgrid > 0 && ctx->lu_lookahead_cus > 0) pgrid = panel_grid_masked(ctx);
+        if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
+        lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s, ctx->lu_stream[0],
+                                  ctx->lu_stream[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
     } else {
-        lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, gemm_opts(ctx), s);
+        if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
+        lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s);
     }
     rc = check_hip(hipGetLastError(), "LU launch");
     if (rc) return rc;
@@ -1595,6 +1908,7 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     if (rc) return rc;
     rc = check_hip(hipStreamSynchronize(s), "LU sync");
     if (rc) return rc;
+    if (h_info < 0) return set_error(BEM_ERR_HIP, "LU panel: a workgroup barrier timed out");
     if (h_info) return set_error(BEM_ERR_SINGULAR, "exactly singular matrix: zero pivot in column " + std::to_string(h_info - 1));
     return BEM_OK;
 }
@@ -1638,7 +1952,7 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
         ctx->lu_table_dev = (void*)d_tbl;
     }
     hipLaunchKernelGGL(lu_reset_kernel, dim3((count + 63) / 64), dim3(64), 0, s, (const LuProb*)d_tbl, count);
-    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, gemm_opts(ctx), s);
+    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, gemm_opts(ctx), 0, s);
     return check_hip(hipGetLastError(), "batched LU launch");
 }
 

@@ -232,14 +232,21 @@ def test_config4_shape_gradient_l4(bem, kernel, k):
     assert gx.shape == (2562, 3)
     rng = np.random.default_rng(3)
     h = 1e-6
-    worst = 0.0
+    gnorm = float(np.linalg.norm(gx))
+    worst = worst_raw = 0.0
     for n in range(32):
         d = rng.standard_normal(v.shape)
         d /= np.linalg.norm(d)
         fd = (J(v + h * d, k) - J(v - h * d, k)) / (2 * h)
         got = float((gx * d).sum())
-        worst = max(worst, abs(got - fd) / abs(fd))
-    print(f"[config4] kernel={kernel} J={j0:.6e} worst rel err over 32 directions {worst:.3e}", flush=True)
+        # error relative to the directional derivative, floored at 1e-4 |g| |d|
+        # (a random unit direction in R^7686 sees about 1e-2 |g|; one seeded
+        # direction here is nearly orthogonal to g, |g.d| = 2e-5 |g|, where
+        # the FP64 noise of J and of g, about 1e-10 |g|, is 1e-5 of g.d)
+        worst = max(worst, abs(got - fd) / max(abs(fd), 1e-4 * gnorm))
+        worst_raw = max(worst_raw, abs(got - fd) / abs(fd))
+    print(f"[config4] kernel={kernel} J={j0:.6e} worst rel err over 32 directions {worst:.3e} "
+          f"(unfloored {worst_raw:.3e})", flush=True)
     assert worst <= 1e-5
     if kernel:
         fdk = (J(v, k + 1e-6) - J(v, k - 1e-6)) / 2e-6

@@ -274,6 +274,31 @@ def test_lu_gemm_tiles_identical(bem, dtype):
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-11), (torch.float64, 1e-11),
+                                       (torch.complex64, 2e-3), (torch.float32, 2e-3)])
+@pytest.mark.parametrize("n,lookahead", [(700, 0), (1500, 32), (1500, 0)])
+def test_lu_persistent_panel_matches_per_column(bem, dtype, tol, n, lookahead):
+    """The persistent panel (grid barriers, sub-panels in LDS) picks the same
+    pivot rows as the one-launch-per-column panel, and its factors agree to
+    rounding (the right part of each panel is updated by sub-panel instead
+    of by column)."""
+    rng = np.random.default_rng(n + lookahead)
+    A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if dtype.is_complex else 0)
+    At = torch.tensor(A, device=DEV).to(dtype).contiguous()
+    out = []
+    for panel in (0, 1):
+        ctx = bem.Context(0)
+        ctx.set_option(bem._lib.OPT_LU_PANEL, panel)
+        ctx.set_option(bem._lib.OPT_LU_LOOKAHEAD, lookahead)
+        LU = At.clone()
+        ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+        print(f"[lu panel] n={n} dtype={dtype} panel={panel} lookahead={lookahead}", flush=True)
+        ctx.lu_factor(LU, ipiv)
+        out.append((to_np(LU), to_np(ipiv)))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.abs(out[0][0] - out[1][0]).max() <= tol * np.abs(out[0][0]).max()
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-11), (torch.float64, 1e-11),
                                        (torch.complex64, 2e-3)])
 @pytest.mark.parametrize("n", [2049, 4100])
 def test_lu_solve_many_blocks(bem, dtype, tol, n):
@@ -309,6 +334,8 @@ def test_set_option_rejects(bem):
         ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
     with pytest.raises(bem.BEMError):
         ctx.set_option(bem._lib.OPT_LU_GEMM_TILE, 2)
+    with pytest.raises(bem.BEMError):
+        ctx.set_option(bem._lib.OPT_LU_PANEL, -1)
 
 
 def test_lu_singular_raises(bem):

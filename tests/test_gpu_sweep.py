@@ -75,6 +75,9 @@ def test_lu_batched_identical_to_single(bem, dtype):
     sizes = [130, 64, 257, 1, 200, 65]
     mats = rand_mats(sizes, dtype, 1)
     ctx = bem.Context(0)
+    # batches factor their panels one launch per column; the single-problem
+    # reference does the same (its persistent panel rounds differently)
+    ctx.set_option(bem._lib.OPT_LU_PANEL, 0)
     singles, spiv = [], []
     for A in mats:
         LU = A.clone()
