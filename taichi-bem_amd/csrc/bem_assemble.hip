@@ -649,7 +649,9 @@ __global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb,
     const int jj = jvalid ? j : 0;
     __shared__ double gts[DTI][144][2];  // data interpolated at the test points (Q^2 <= 144) of each tile panel
     __shared__ double red[4][2];
+    __shared__ double2 sct[64];          // sin/cos(j pi/32) for the FP64 phase
     if (i0 >= nf) return;  // the last group's empty tiles (uniform)
+    if constexpr (HELM && sizeof(R) == 8) sincos_table_fill(sct);
     for (int e = threadIdx.x; e < DTI * q2; e += 256) {
         const int ti = e / q2, a = e % q2, i = i0 + ti;
         double gr = 0.0, gi = 0.0;
@@ -717,7 +719,8 @@ __global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb,
                     if constexpr (HELM) {
                         const R kr = kap * (r2 * rinv);
                         R sn, cs;
-                        Math<R>::sincos(kr, &sn, &cs);
+                        if constexpr (sizeof(R) == 8) sincos_f64_tab(kr, sct, &sn, &cs);
+                        else Math<R>::sincos(kr, &sn, &cs);
                         pvr += g * cs; pvi += g * sn;
                         const R c1 = cs + kr * sn, c2 = sn - kr * cs;
                         pkr += bj * c1; pki += bj * c2;
