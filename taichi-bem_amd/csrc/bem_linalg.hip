@@ -278,12 +278,14 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
 //     those entries any more.
 // A panel costs NB + NB/SB - 1 barriers instead of NB + 1 dependent
 // launches, and its right part is read and written once per sub-panel
-// instead of once per column.  Every wait is bounded (about two seconds of
+// instead of once per column.  Every barrier ends in an agent-scope acquire,
+// so the data other workgroups stored before it are read with plain loads.  Every wait is bounded (about two seconds of
 // wall clock): a workgroup that gives up raises the shared fail word, every
 // other one then leaves at its next wait, and info becomes -2.
 // ---------------------------------------------------------------------------
 constexpr int SB = 8;        // sub-panel width
-constexpr int PGMAX = 256;   // largest workgroup count of a persistent panel
+constexpr int PGMAX = 64;    // largest workgroup count of a persistent panel (one wave reduces them)
+constexpr int PNT = 1024;    // threads per workgroup: sixteen waves keep many row loads in flight
 template <typename T>
 constexpr int panel_rmax() { return (128 * 1024) / (SB * (int)sizeof(cx<T>)); }
 
@@ -336,20 +338,17 @@ __device__ bool grid_barrier(unsigned* count, int* fail, unsigned target) {
 __host__ __device__ inline int panel_barriers(int w) { return w + (w + SB - 1) / SB - 1; }
 
 template <typename T, bool C>
-__device__ __forceinline__ cx<T> ld_el_agent(const T* A, int64_t idx) {
-    if constexpr (C) return {ld_agent(A + 2 * idx), ld_agent(A + 2 * idx + 1)};
-    else return {ld_agent(A + idx), T(0)};
-}
-
-template <typename T, bool C>
-__global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int G, int R, unsigned base) {
+__global__ __launch_bounds__(PNT) void panel_persist_kernel(LuProb P, int k, int G, int R, unsigned base) {
     constexpr int RMAX = panel_rmax<T>();
     __shared__ cx<T> sub[RMAX][SB];   // this workgroup's rows, sub-panel columns
     __shared__ cx<T> us[SB][NB];      // U_s of the sub-panel
     __shared__ cx<T> lss[SB][SB];     // L_ss (unit lower) of the sub-panel
     __shared__ cx<T> urow[SB], orow[SB];
-    __shared__ double wv[4];
-    __shared__ int wi[4], wg[4];
+    __shared__ cx<T> allc[PGMAX][SB];   // every workgroup's candidate row values
+    __shared__ double cv_s[PGMAX];
+    __shared__ int cr_s[PGMAX];
+    __shared__ double wv[PNT / 64];
+    __shared__ int wi[PNT / 64];
     __shared__ int s_p, s_gw;
     T* A = (T*)P.A;
     const int64_t ld = P.ld;
@@ -367,21 +366,24 @@ __global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int
     for (int cs = k; cs < kend; cs += SB) {
         const int ce = min(kend, cs + SB), sw = ce - cs;
         const int i0 = max(0, cs - rb);
-        for (int e = i0 * SB + t; e < nr * SB; e += 256) {
+        // (plain loads: every wait above ended in an agent-scope acquire)
+        for (int e = i0 * SB + t; e < nr * SB; e += PNT) {
             const int i = e / SB, j = e % SB;
-            if (j < sw) sub[i][j] = ld_el_agent<T, C>(A, (int64_t)(rb + i) * ld + cs + j);
+            if (j < sw) sub[i][j] = Acc<T, C>::ld(A, (int64_t)(rb + i) * ld + cs + j);
         }
         __syncthreads();
+        // this thread's best row of the column being searched (rows ascending
+        // per thread, ties to the first row); later columns of the sub-panel
+        // are searched during the previous column's elimination
+        double b = -1.0;
+        int bi = 0x7fffffff;
+        for (int i = i0 + t; i < nr; i += PNT) {
+            const cx<T> v = sub[i][0];
+            const double m = fabs((double)v.re) + fabs((double)v.im);
+            if (m > b) { b = m; bi = rb + i; }
+        }
         for (int c = cs; c < ce; ++c) {
             const int jc = c - cs;
-            // this workgroup's candidate: rows ascending per thread, ties to the first row
-            double b = -1.0;
-            int bi = 0x7fffffff;
-            for (int i = max(0, c - rb) + t; i < nr; i += 256) {
-                const cx<T> v = sub[i][jc];
-                const double m = fabs((double)v.re) + fabs((double)v.im);
-                if (m > b) { b = m; bi = rb + i; }
-            }
             for (int off = 32; off > 0; off >>= 1) {
                 const double ob = __shfl_xor(b, off, 64);
                 const int oi = __shfl_xor(bi, off, 64);
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int
             __syncthreads();
             b = wv[0];
             bi = wi[0];
-            for (int q = 1; q < 4; ++q)
+            for (int q = 1; q < PNT / 64; ++q)
                 if (wv[q] > b || (wv[q] == b && wi[q] < bi)) { b = wv[q]; bi = wi[q]; }
             if (t == 0) { X->cval[par][g] = b; X->crow[par][g] = bi; }
             if (t < sw) {
@@ -404,64 +406,65 @@ __global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int
                 // every workgroup has read the previous sub-panel's rows: store U_s
                 if (g == 0) {
                     const int W = kend - pu_ce;
-                    for (int e = t; e < pu_sw * W; e += 256)
+                    for (int e = t; e < pu_sw * W; e += PNT)
                         Acc<T, C>::st(A, (int64_t)(pu_cs + e / W) * ld + pu_ce + e % W, us[e / W][e % W]);
                 }
                 pending_u = false;
             }
-            // the pivot: the same reduction over the G candidates in every workgroup
-            b = -1.0;
-            bi = 0x7fffffff;
-            int bg = 0;
-            for (int q = t; q < G; q += 256) {
-                const double v = ld_agent(&X->cval[par][q]);
-                const int r = ld_agent(&X->crow[par][q]);
-                if (v > b || (v == b && r < bi)) { b = v; bi = r; bg = q; }
+            // every candidate (value, row, sub-panel values) and row c in one
+            // round of loads, then the same reduction in every workgroup
+            for (int e = t; e < G * (SB + 1) + SB; e += PNT) {
+                if (e >= G * (SB + 1)) {
+                    const int jj = e - G * (SB + 1);
+                    if (jj < sw) orow[jj] = X->rowc[par][jj];
+                    continue;
+                }
+                const int q = e / (SB + 1), jj = e % (SB + 1);
+                if (jj == SB) {
+                    cv_s[q] = X->cval[par][q];
+                    cr_s[q] = X->crow[par][q];
+                } else if (jj < sw) {
+                    allc[q][jj] = X->cvals[par][q][jj];
+                }
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(b, off, 64);
-                const int oi = __shfl_xor(bi, off, 64), og = __shfl_xor(bg, off, 64);
-                if (ob > b || (ob == b && oi < bi)) { b = ob; bi = oi; bg = og; }
-            }
-            __syncthreads();  // wv/wi of the local search are read by every thread above
-            if (lane == 0) { wv[wave] = b; wi[wave] = bi; wg[wave] = bg; }
             __syncthreads();
-            if (t == 0) {
-                double bb = wv[0];
-                int ii = wi[0], gg = wg[0];
-                for (int q = 1; q < 4; ++q)
-                    if (wv[q] > bb || (wv[q] == bb && wi[q] < ii)) { bb = wv[q]; ii = wi[q]; gg = wg[q]; }
-                s_p = ii;
-                s_gw = gg;
+            if (wave == 0) {
+                b = lane < G ? cv_s[lane] : -1.0;
+                bi = lane < G ? cr_s[lane] : 0x7fffffff;
+                int bg = lane;
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(b, off, 64);
+                    const int oi = __shfl_xor(bi, off, 64), og = __shfl_xor(bg, off, 64);
+                    if (ob > b || (ob == b && oi < bi)) { b = ob; bi = oi; bg = og; }
+                }
+                if (lane == 0) { s_p = bi; s_gw = bg; }
             }
             __syncthreads();
             const int p = s_p, gw = s_gw;
-            if (t < sw) {
-                urow[t] = {ld_agent((const T*)&X->cvals[par][gw][t].re), ld_agent((const T*)&X->cvals[par][gw][t].im)};
-                orow[t] = {ld_agent((const T*)&X->rowc[par][t].re), ld_agent((const T*)&X->rowc[par][t].im)};
-            }
-            __syncthreads();
-            const cx<T> piv = urow[jc];
+            const cx<T> piv = allc[gw][jc];
             const bool dz = piv.re == T(0) && piv.im == T(0);
-            if (g == 0) {
-                if (t == 0) {
-                    P.ipiv[c] = p;
-                    if (dz && *P.info == 0) *P.info = c + 1;
-                }
-                const int col = k + t;
-                if (p != c && t < kend - k && (col < cs || col >= ce)) {
-                    const cx<T> a = ld_el_agent<T, C>(A, (int64_t)c * ld + col);
-                    const cx<T> a2 = ld_el_agent<T, C>(A, (int64_t)p * ld + col);
-                    Acc<T, C>::st(A, (int64_t)c * ld + col, a2);
-                    Acc<T, C>::st(A, (int64_t)p * ld + col, a);
-                }
+            // workgroup 0 interchanges rows c and p in the panel columns outside
+            // the sub-panel: loads now, stores after the elimination below
+            const int scol = k + t;
+            const bool swp = g == 0 && p != c && t < kend - k && (scol < cs || scol >= ce);
+            cx<T> sa = {T(0), T(0)}, sb2 = {T(0), T(0)};
+            if (swp) {
+                sa = Acc<T, C>::ld(A, (int64_t)c * ld + scol);
+                sb2 = Acc<T, C>::ld(A, (int64_t)p * ld + scol);
             }
+            if (g == 0 && t == 0) {
+                P.ipiv[c] = p;
+                if (dz && *P.info == 0) *P.info = c + 1;
+            }
+            if (t < sw) urow[t] = allc[gw][t];
             if (p != c && t < sw) {
                 if (p >= rb && p < re) sub[p - rb][t] = orow[t];
-                if (c >= rb && c < re) sub[c - rb][t] = urow[t];
+                if (c >= rb && c < re) sub[c - rb][t] = allc[gw][t];
             }
             __syncthreads();
-            for (int i = max(0, c + 1 - rb) + t; i < nr; i += 256) {
+            b = -1.0;
+            bi = 0x7fffffff;
+            for (int i = max(0, c + 1 - rb) + t; i < nr; i += PNT) {
                 cx<T> l = sub[i][jc];
                 if (!dz) l = cdiv(l, piv);
                 sub[i][jc] = l;
@@ -470,11 +473,19 @@ __global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int
                     sub[i][j].re -= lu.re;
                     sub[i][j].im -= lu.im;
                 }
+                if (jc + 1 < sw) {
+                    const double m = fabs((double)sub[i][jc + 1].re) + fabs((double)sub[i][jc + 1].im);
+                    if (m > b) { b = m; bi = rb + i; }
+                }
+            }
+            if (swp) {
+                Acc<T, C>::st(A, (int64_t)c * ld + scol, sb2);
+                Acc<T, C>::st(A, (int64_t)p * ld + scol, sa);
             }
             __syncthreads();
             par ^= 1;
         }
-        for (int e = i0 * SB + t; e < nr * SB; e += 256) {
+        for (int e = i0 * SB + t; e < nr * SB; e += PNT) {
             const int i = e / SB, j = e % SB;
             if (j < sw) Acc<T, C>::st(A, (int64_t)(rb + i) * ld + cs + j, sub[i][j]);
         }
@@ -485,12 +496,12 @@ __global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int
         const int W = kend - ce;
         if (t < SB * SB) {
             const int m = t / SB, q = t % SB;
-            lss[m][q] = (m < sw && q < m) ? ld_el_agent<T, C>(A, (int64_t)(cs + m) * ld + cs + q) : cx<T>{T(0), T(0)};
+            lss[m][q] = (m < sw && q < m) ? Acc<T, C>::ld(A, (int64_t)(cs + m) * ld + cs + q) : cx<T>{T(0), T(0)};
         }
         __syncthreads();
         if (t < W) {
             for (int m = 0; m < sw; ++m) {
-                cx<T> v = ld_el_agent<T, C>(A, (int64_t)(cs + m) * ld + ce + t);
+                cx<T> v = Acc<T, C>::ld(A, (int64_t)(cs + m) * ld + ce + t);
                 for (int q = 0; q < m; ++q) {
                     const cx<T> lu = cmul(lss[m][q], us[q][t]);
                     v.re -= lu.re;
@@ -501,16 +512,29 @@ __global__ __launch_bounds__(256) void panel_persist_kernel(LuProb P, int k, int
         }
         __syncthreads();
         // rows below the sub-panel: A[r, ce:kend] -= L[r, cs:ce] U_s
-        const int i1 = max(0, ce - rb);
-        for (int e = i1 * W + t; e < nr * W; e += 256) {
-            const int i = e / W, j = e % W;
-            cx<T> v = ld_el_agent<T, C>(A, (int64_t)(rb + i) * ld + ce + j);
-            for (int m = 0; m < sw; ++m) {
-                const cx<T> lu = cmul(sub[i][m], us[m][j]);
-                v.re -= lu.re;
-                v.im -= lu.im;
+        // in chunks of U elements per thread whose loads are all issued first
+        const int i1 = max(0, ce - rb), tot = max(0, nr - i1) * W;
+        constexpr int U = 8;
+        for (int base = 0; base < tot; base += PNT * U) {
+            cx<T> v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = base + u * PNT + t;
+                if (e < tot) v[u] = Acc<T, C>::ld(A, (int64_t)(rb + i1 + e / W) * ld + ce + e % W);
             }
-            Acc<T, C>::st(A, (int64_t)(rb + i) * ld + ce + j, v);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = base + u * PNT + t;
+                if (e < tot) {
+                    const int i = i1 + e / W, j = e % W;
+                    for (int m = 0; m < sw; ++m) {
+                        const cx<T> lu = cmul(sub[i][m], us[m][j]);
+                        v[u].re -= lu.re;
+                        v[u].im -= lu.im;
+                    }
+                    Acc<T, C>::st(A, (int64_t)(rb + i) * ld + ce + j, v[u]);
+                }
+            }
         }
         __syncthreads();  // the next sub-panel's load reads these rows through other threads
         pending_u = true;
@@ -1615,7 +1639,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
         const int G = std::max(1, std::min({pgrid, PGMAX, (nrem + 63) / 64}));
         const int R = (nrem + G - 1) / G;
         if (R <= panel_rmax<T>()) {
-            hipLaunchKernelGGL((panel_persist_kernel<T, C>), dim3(G), dim3(256), 0, s, one, k, G, R, *pbase);
+            hipLaunchKernelGGL((panel_persist_kernel<T, C>), dim3(G), dim3(PNT), 0, s, one, k, G, R, *pbase);
             *pbase += (unsigned)(panel_barriers(std::min(NB, nrem)) * G);
             return;
         }
