@@ -236,8 +236,12 @@ enum {
                                 growth can be larger, so use it for
                                 well-conditioned systems only */,
     BEM_OPT_LU_GEMM_TILE = 4 /* 1 (default) = trailing updates of at least
-                                256 rows use 128-row MFMA tiles; 0 = 64 x 64
-                                tiles for every update */,
+                                256 rows use 128-row MFMA tiles: 128 x 64 for
+                                complex128, 128 x 64 with three real products
+                                per complex product (Gauss) for complex64,
+                                128 x 128 for real types; 2 = 128 x 128 with
+                                four real products for the complex types too;
+                                0 = 64 x 64 tiles for every update */,
     BEM_OPT_LU_PANEL = 5     /* partial-pivoting panels of bem_lu_factor:
                                 1 (default) = one persistent launch per
                                 64-column panel (one workgroup per compute

@@ -209,7 +209,7 @@ int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
             ctx->impl.lu_panel = value;
             return BEM_OK;
         case BEM_OPT_LU_GEMM_TILE:
-            if (value != 0 && value != 1) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_GEMM_TILE takes 0 or 1");
+            if (value < 0 || value > 2) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_GEMM_TILE takes 0, 1 or 2");
             ctx->impl.lu_gemm_tile = value;
             return BEM_OK;
         case BEM_OPT_C64_LU_GEMM:

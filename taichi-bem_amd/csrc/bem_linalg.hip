@@ -1168,6 +1168,132 @@ __global__ __launch_bounds__(512) void gemm_big_kernel(GemmArgs<T> inl, const Lu
     }
 }
 
+// The same trailing update for complex operands with three real products
+// per complex multiply-add (Gauss): P1 = Re a Re b, P2 = Im a Im b,
+// P3 = (Re a + Im a)(Re b + Im b), Re = P1 - P2, Im = P3 - P1 - P2.  The
+// operand sums are formed once when a chunk is staged (a third LDS plane per
+// operand), so the MFMA count per chunk drops from 4 to 3 per complex
+// product; the imaginary part carries the usual 3M rounding (normwise, a
+// few ulps of |a||b|).
+template <typename T, int BN, bool B>
+__global__ __launch_bounds__(512) void gemm_big3m_kernel(GemmArgs<T> inl, const LuProb* tbl, LuStep st) {
+    GemmArgs<T> ga = inl;
+    if constexpr (B) ga = lu_gemm_args<T, true>(tbl[blockIdx.z], st);
+    T* Cm = ga.Cm;
+    const T* Am = ga.Am;
+    const T* Bm = ga.Bm;
+    const int64_t ldc = ga.ldc, lda = ga.lda, ldb = ga.ldb;
+    const int M = ga.M, N = ga.N, K = ga.K;
+    if (M <= 0 || N <= 0 || K <= 0 || (int)blockIdx.y * BT >= M || (int)blockIdx.x * BN >= N) return;
+    using V4 = typename Mfma<T>::v4;
+    constexpr int KB = 128 / (int)(sizeof(T) * 2);
+    constexpr int NT = 512;
+    constexpr int MA = 4, NBB = BN / 64;
+    constexpr int PA = BT * KB / NT, PB = KB * BN / NT;
+    __shared__ T sA[2][3][KB][BT + 1];
+    __shared__ T sB[2][3][KB][BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bm = blockIdx.y * BT, bn = blockIdx.x * BN;
+    const int wm = (wave >> 2) * 64, wn = (wave & 3) * (BN / 4);
+    V4 acc1[MA][NBB], acc2[MA][NBB], acc3[MA][NBB];
+#pragma unroll
+    for (int a = 0; a < MA; ++a)
+#pragma unroll
+        for (int b = 0; b < NBB; ++b) {
+            acc1[a][b] = V4{0, 0, 0, 0};
+            acc2[a][b] = V4{0, 0, 0, 0};
+            acc3[a][b] = V4{0, 0, 0, 0};
+        }
+    cx<T> pa[PA], pb[PB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = tid + NT * q, r = e / KB, c = e % KB;
+            const int gr = bm + r, gc = k0 + c;
+            pa[q] = (gr < M && gc < K) ? Vec2<T, true>::ld(Am, (int64_t)gr * lda + gc) : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int e = tid + NT * q, r = e / BN, c = e % BN;
+            const int gr = k0 + r, gc = bn + c;
+            pb[q] = (gr < K && gc < N) ? Vec2<T, true>::ld(Bm, (int64_t)gr * ldb + gc) : cx<T>{T(0), T(0)};
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int e = tid + NT * q, r = e / KB, c = e % KB;
+            sA[buf][0][c][r] = pa[q].re;
+            sA[buf][1][c][r] = pa[q].im;
+            sA[buf][2][c][r] = pa[q].re + pa[q].im;
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int e = tid + NT * q, r = e / BN, c = e % BN;
+            sB[buf][0][r][c] = pb[q].re;
+            sB[buf][1][r][c] = pb[q].im;
+            sB[buf][2][r][c] = pb[q].re + pb[q].im;
+        }
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    const int nk = (K + KB - 1) / KB;
+    for (int i = 0; i < nk; ++i) {
+        const int cur = i & 1;
+        if (i + 1 < nk) fetch((i + 1) * KB);
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 4) {
+            const int ka = kk + (lane >> 4);
+            T a0[MA], a1[MA], a2[MA], b0[NBB], b1[NBB], b2[NBB];
+#pragma unroll
+            for (int a = 0; a < MA; ++a) {
+                a0[a] = sA[cur][0][ka][wm + 16 * a + (lane & 15)];
+                a1[a] = sA[cur][1][ka][wm + 16 * a + (lane & 15)];
+                a2[a] = sA[cur][2][ka][wm + 16 * a + (lane & 15)];
+            }
+#pragma unroll
+            for (int b = 0; b < NBB; ++b) {
+                b0[b] = sB[cur][0][ka][wn + 16 * b + (lane & 15)];
+                b1[b] = sB[cur][1][ka][wn + 16 * b + (lane & 15)];
+                b2[b] = sB[cur][2][ka][wn + 16 * b + (lane & 15)];
+            }
+#pragma unroll
+            for (int a = 0; a < MA; ++a)
+#pragma unroll
+                for (int b = 0; b < NBB; ++b) acc1[a][b] = Mfma<T>::op(a0[a], b0[b], acc1[a][b]);
+#pragma unroll
+            for (int a = 0; a < MA; ++a)
+#pragma unroll
+                for (int b = 0; b < NBB; ++b) acc2[a][b] = Mfma<T>::op(a1[a], b1[b], acc2[a][b]);
+#pragma unroll
+            for (int a = 0; a < MA; ++a)
+#pragma unroll
+                for (int b = 0; b < NBB; ++b) acc3[a][b] = Mfma<T>::op(a2[a], b2[b], acc3[a][b]);
+        }
+        if (i + 1 < nk) stash(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < MA; ++a) {
+#pragma unroll
+        for (int b = 0; b < NBB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = bm + wm + 16 * a + Mfma<T>::row(lane, r);
+                const int gc = bn + wn + 16 * b + (lane & 15);
+                if (gr < M && gc < N) {
+                    cx<T> v = Vec2<T, true>::ld(Cm, (int64_t)gr * ldc + gc);
+                    const T p1 = acc1[a][b][r], p2 = acc2[a][b][r], p3 = acc3[a][b][r];
+                    v.re -= p1 - p2;
+                    v.im -= p3 - p1 - p2;
+                    Acc<T, true>::st(Cm, (int64_t)gr * ldc + gc, v);
+                }
+            }
+        asm volatile("" ::: "memory");
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Triangular solves with one right-hand side, blocked by TB rows.
 // ---------------------------------------------------------------------------
@@ -1583,10 +1709,15 @@ __global__ __launch_bounds__(256) void gemm_sub_c64bf_kernel(GemmArgs<float> inl
 // gopts, the GEMM options of a factorisation: bits 0-3 = split-bf16 parts of
 // c64 trailing updates (0 = FP32 MFMA for every c64 product; 2 or 3 = the
 // split bf16 kernel for modes 1 and 4); bit 4 (GEMM_SMALL_TILES) = 64 x 64
-// tiles for every update.
+// tiles for every update; bit 5 (GEMM_WIDE4M) = complex trailing updates on
+// 128 x 128 tiles with four real products per complex product.
 constexpr int GEMM_SMALL_TILES = 16;
+constexpr int GEMM_WIDE4M = 32;
 
-int gemm_opts(const Context* ctx) { return ctx->c64_gemm_parts | (ctx->lu_gemm_tile ? 0 : GEMM_SMALL_TILES); }
+int gemm_opts(const Context* ctx) {
+    return ctx->c64_gemm_parts | (ctx->lu_gemm_tile ? 0 : GEMM_SMALL_TILES) |
+           (ctx->lu_gemm_tile == 2 ? GEMM_WIDE4M : 0);
+}
 
 template <typename T, bool C, bool B>
 void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, int N, int active, int gopts,
@@ -1607,13 +1738,24 @@ void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, in
     // (64-wide column tiles when the update is narrower than 128 or would
     // leave a half-empty last 128-column tile)
     if ((st.mode == 1 || st.mode == 4) && M >= 2 * BT && !(gopts & GEMM_SMALL_TILES)) {
+        // tile widths measured on a 20,224 x 20,224 x 256 update: complex128
+        // 128 x 64 (two workgroups per CU; 128 x 128 holds one), complex64
+        // 128 x 64 with three products per complex product, real 128 x 128
         const int rows = (M + BT - 1) / BT;
-        if ((N >= BT && N % BT == 0) || N > 2 * BT) {
-            hipLaunchKernelGGL((gemm_big_kernel<T, C, BT, B>), dim3((N + BT - 1) / BT, rows, active), dim3(512), 0,
-                               s, g, d_tbl, st);
+        const bool wide = (N >= BT && N % BT == 0) || N > 2 * BT;
+        const dim3 g64((N + 63) / 64, rows, active), g128((N + BT - 1) / BT, rows, active);
+        if constexpr (C) {
+            if (gopts & GEMM_WIDE4M) {
+                if (wide) hipLaunchKernelGGL((gemm_big_kernel<T, C, BT, B>), g128, dim3(512), 0, s, g, d_tbl, st);
+                else hipLaunchKernelGGL((gemm_big_kernel<T, C, 64, B>), g64, dim3(512), 0, s, g, d_tbl, st);
+            } else if constexpr (std::is_same<T, float>::value) {
+                hipLaunchKernelGGL((gemm_big3m_kernel<T, 64, B>), g64, dim3(512), 0, s, g, d_tbl, st);
+            } else {
+                hipLaunchKernelGGL((gemm_big_kernel<T, C, 64, B>), g64, dim3(512), 0, s, g, d_tbl, st);
+            }
         } else {
-            hipLaunchKernelGGL((gemm_big_kernel<T, C, 64, B>), dim3((N + 63) / 64, rows, active), dim3(512), 0, s,
-                               g, d_tbl, st);
+            if (wide) hipLaunchKernelGGL((gemm_big_kernel<T, C, BT, B>), g128, dim3(512), 0, s, g, d_tbl, st);
+            else hipLaunchKernelGGL((gemm_big_kernel<T, C, 64, B>), g64, dim3(512), 0, s, g, d_tbl, st);
         }
         return;
     }

@@ -253,15 +253,18 @@ def test_lu_partial_pivoting_matches_lapack(bem, n):
 
 
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64, torch.float64, torch.float32])
-def test_lu_gemm_tiles_identical(bem, dtype):
-    """The 128-row-tile trailing updates accumulate every entry in the same
-    order as the 64 x 64 kernel, so both give bit-identical factors."""
+@pytest.mark.parametrize("tile", [1, 2])
+def test_lu_gemm_tiles_identical(bem, dtype, tile):
+    """The 128-row-tile trailing updates with four real products per complex
+    product accumulate every entry in the same order as the 64 x 64 kernel,
+    so both give bit-identical factors (complex64 with the default tiles uses
+    three products: same pivots, factors equal to rounding)."""
     n = 777
     rng = np.random.default_rng(21)
     A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if dtype.is_complex else 0)
     At = torch.tensor(A, device=DEV).to(dtype).contiguous()
     out = []
-    for tile in (0, 1):
+    for tile in (0, tile):
         ctx = bem.Context(0)
         ctx.set_option(bem._lib.OPT_LU_GEMM_TILE, tile)
         LU = At.clone()
@@ -270,7 +273,11 @@ def test_lu_gemm_tiles_identical(bem, dtype):
         ctx.lu_factor(LU, ipiv)
         out.append((LU, ipiv))
     assert torch.equal(out[0][1], out[1][1])
-    assert torch.equal(out[0][0], out[1][0])
+    if dtype == torch.complex64 and tile == 1:
+        a, b = to_np(out[0][0]), to_np(out[1][0])
+        assert np.abs(a - b).max() <= 2e-3 * np.abs(a).max()
+    else:
+        assert torch.equal(out[0][0], out[1][0])
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-11), (torch.float64, 1e-11),
@@ -294,8 +301,20 @@ def test_lu_persistent_panel_matches_per_column(bem, dtype, tol, n, lookahead):
         print(f"[lu panel] n={n} dtype={dtype} panel={panel} lookahead={lookahead}", flush=True)
         ctx.lu_factor(LU, ipiv)
         out.append((to_np(LU), to_np(ipiv)))
-    assert np.array_equal(out[0][1], out[1][1])
-    assert np.abs(out[0][0] - out[1][0]).max() <= tol * np.abs(out[0][0]).max()
+    if dtype in (torch.complex128, torch.float64):
+        assert np.array_equal(out[0][1], out[1][1])
+        assert np.abs(out[0][0] - out[1][0]).max() <= tol * np.abs(out[0][0]).max()
+    else:
+        # FP32 rounding can flip a near-tie between pivot candidates: check that
+        # each factorisation reproduces the matrix instead (P A = L U)
+        a = At.cpu().numpy().astype(np.complex128)
+        for lu, piv in out:
+            perm = np.arange(n)
+            for c, p in enumerate(piv):
+                perm[[c, p]] = perm[[p, c]]
+            lu = lu.astype(np.complex128)
+            L = np.tril(lu, -1) + np.eye(n)
+            assert rel(L @ np.triu(lu), a[perm]) <= 1e-4  # about n eps for n = 1500
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-11), (torch.float64, 1e-11),
@@ -333,7 +352,7 @@ def test_set_option_rejects(bem):
     with pytest.raises(bem.BEMError):
         ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
     with pytest.raises(bem.BEMError):
-        ctx.set_option(bem._lib.OPT_LU_GEMM_TILE, 2)
+        ctx.set_option(bem._lib.OPT_LU_GEMM_TILE, 3)
     with pytest.raises(bem.BEMError):
         ctx.set_option(bem._lib.OPT_LU_PANEL, -1)
 
