@@ -95,7 +95,7 @@ __device__ __forceinline__ double cre(double ar, double ai, double br, double bi
 // with e = exp(i kap r).
 template <bool HELM>
 __device__ __forceinline__ void trig(double kap, double r, double& cs, double& sn) {
-    if constexpr (HELM) sincos_f64(kap * r, &sn, &cs);
+    if constexpr (HELM) Math<double>::sincos(kap * r, &sn, &cs);
     else { cs = 1.0; sn = 0.0; }
 }
 

@@ -697,12 +697,16 @@ __global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb,
                 dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
             }
             const R mix = (R)p.normal[3 * i], miy = (R)p.normal[3 * i + 1], miz = (R)p.normal[3 * i + 2];
+            // (x - y).n_i = x_a.n_i + e_b.n_i, and x_a.n_i is the same for every
+            // test point of the flat panel i: one dot per trial point, not per pair
+            const R pun = PU[0] * mix + PU[1] * miy + PU[2] * miz;
             R vr = 0, vi = 0, ar = 0, ai = 0, br = 0, bi = 0;
             for (int b = 0; b < q2; ++b) {
                 const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
                 const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
                 const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
                 const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
+                const R dni = pun + ex * mix + ey * miy + ez * miz;
                 R pvr = 0, pvi = 0, pkr = 0, pki = 0, qkr = 0, qki = 0;
                 for (int a = 0; a < q2; ++a) {
                     const R dx = PU[4 * a + 0] + ex;
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb,
                     const R g = wa * rinv;
                     const R g3 = g * (rinv * rinv);
                     const R bj = g3 * (dx * nnx + dy * nny + dz * nnz);    // (x - y).n_j: row i
-                    const R bi_ = -g3 * (dx * mix + dy * miy + dz * miz);  // (y - x).n_i: row j
+                    const R bi_ = -g3 * dni;  // (y - x).n_i: row j
                     const R gar = (R)gts[ti][a][0], gai = (R)gts[ti][a][1];
                     if constexpr (HELM) {
                         const R kr = kap * (r2 * rinv);

@@ -82,15 +82,47 @@ __device__ __forceinline__ void sincos_f64_tab(double x, const double2* tab, dou
     *c_out = __builtin_fma(e.y, ct, -(e.x * st));
 }
 
-// Fills tab[j] = {sin, cos}(j pi/32), j < 64, from the threads of a block
-// (the caller synchronises before use).
+// {sin, cos}(j pi/32), j < 64, correctly rounded (evaluated to 45 digits).
+static __constant__ const double2 kSinCosPi32[64] = {
+    {0.0, 1.0}, {0.0980171403295606, 0.9951847266721969},
+    {0.19509032201612828, 0.9807852804032304}, {0.2902846772544624, 0.9569403357322088},
+    {0.3826834323650898, 0.9238795325112867}, {0.47139673682599764, 0.881921264348355},
+    {0.5555702330196022, 0.8314696123025452}, {0.6343932841636455, 0.773010453362737},
+    {0.7071067811865476, 0.7071067811865476}, {0.773010453362737, 0.6343932841636455},
+    {0.8314696123025452, 0.5555702330196022}, {0.881921264348355, 0.47139673682599764},
+    {0.9238795325112867, 0.3826834323650898}, {0.9569403357322088, 0.2902846772544624},
+    {0.9807852804032304, 0.19509032201612828}, {0.9951847266721969, 0.0980171403295606},
+    {1.0, -1.6185598133277091e-46}, {0.9951847266721969, -0.0980171403295606},
+    {0.9807852804032304, -0.19509032201612828}, {0.9569403357322088, -0.2902846772544624},
+    {0.9238795325112867, -0.3826834323650898}, {0.881921264348355, -0.47139673682599764},
+    {0.8314696123025452, -0.5555702330196022}, {0.773010453362737, -0.6343932841636455},
+    {0.7071067811865476, -0.7071067811865476}, {0.6343932841636455, -0.773010453362737},
+    {0.5555702330196022, -0.8314696123025452}, {0.47139673682599764, -0.881921264348355},
+    {0.3826834323650898, -0.9238795325112867}, {0.2902846772544624, -0.9569403357322088},
+    {0.19509032201612828, -0.9807852804032304}, {0.0980171403295606, -0.9951847266721969},
+    {1.6936229514088527e-46, -1.0}, {-0.0980171403295606, -0.9951847266721969},
+    {-0.19509032201612828, -0.9807852804032304}, {-0.2902846772544624, -0.9569403357322088},
+    {-0.3826834323650898, -0.9238795325112867}, {-0.47139673682599764, -0.881921264348355},
+    {-0.5555702330196022, -0.8314696123025452}, {-0.6343932841636455, -0.773010453362737},
+    {-0.7071067811865476, -0.7071067811865476}, {-0.773010453362737, -0.6343932841636455},
+    {-0.8314696123025452, -0.5555702330196022}, {-0.881921264348355, -0.47139673682599764},
+    {-0.9238795325112867, -0.3826834323650898}, {-0.9569403357322088, -0.2902846772544624},
+    {-0.9807852804032304, -0.19509032201612828}, {-0.9951847266721969, -0.0980171403295606},
+    {-1.0, -9.059354245791371e-47}, {-0.9951847266721969, 0.0980171403295606},
+    {-0.9807852804032304, 0.19509032201612828}, {-0.9569403357322088, 0.2902846772544624},
+    {-0.9238795325112867, 0.3826834323650898}, {-0.881921264348355, 0.47139673682599764},
+    {-0.8314696123025452, 0.5555702330196022}, {-0.773010453362737, 0.6343932841636455},
+    {-0.7071067811865476, 0.7071067811865476}, {-0.6343932841636455, 0.773010453362737},
+    {-0.5555702330196022, 0.8314696123025452}, {-0.47139673682599764, 0.881921264348355},
+    {-0.3826834323650898, 0.9238795325112867}, {-0.2902846772544624, 0.9569403357322088},
+    {-0.19509032201612828, 0.9807852804032304}, {-0.0980171403295606, 0.9951847266721969},
+};
+
+// Copies the table into LDS from the threads of a block (the caller
+// synchronises before use).
 __device__ __forceinline__ void sincos_table_fill(double2* tab) {
     const int j = threadIdx.x;
-    if (j < 64) {
-        double sn, cs;
-        sincos_f64(j * 0.098174770424681038702, &sn, &cs);
-        tab[j] = double2{sn, cs};
-    }
+    if (j < 64) tab[j] = kSinCosPi32[j];
 }
 
 // 1/sqrt(x) in FP64: hardware estimate refined by one third-order step
@@ -119,7 +151,9 @@ __device__ __forceinline__ void sincos_f32(float x, float* s, float* c) {
 template <typename R> struct Math;
 template <> struct Math<double> {
     static __device__ __forceinline__ double rsqrt(double x) { return rsqrt_f64(x); }
-    static __device__ __forceinline__ void sincos(double x, double* s, double* c) { sincos_f64(x, s, c); }
+    static __device__ __forceinline__ void sincos(double x, double* s, double* c) {
+        sincos_f64_tab(x, kSinCosPi32, s, c);
+    }
 };
 template <> struct Math<float> {
     static __device__ __forceinline__ float rsqrt(float x) { return rsqrt_f32(x); }

@@ -134,6 +134,14 @@ __global__ void laswp_kernel(LuProb inl, const LuProb* tbl, int p_lo, int p_hi, 
 // ---------------------------------------------------------------------------
 constexpr int RPB = 64;  // rows per block (16 per wave)
 
+// Lane l's value of v (l wave-uniform) through scalar registers.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Coherent (agent-scope) load of a value another workgroup published.
 template <typename T>
 __device__ __forceinline__ T ld_agent(const T* p) {
@@ -1431,14 +1439,18 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const So
         return;  // the flag stays down: workgroups after this one time out too
     }
     if (wave == 0) {
-        // column-oriented solve with the diagonal block: lane i owns row i
+        // column-oriented solve with the diagonal block: lane i owns row i.
+        // Lane i's reciprocal diagonal entry is formed before the chain, and
+        // each solved value reaches the other lanes through scalar registers.
+        cx<double> rd = {1.0, 0.0};
+        if (!UNIT && lane < nb) rd = cdiv(cx<double>{1.0, 0.0}, cx<double>{dr[lane][lane], C ? di[lane][lane] : 0.0});
         for (int s = 0; s < nb; ++s) {
             const int c = LOWER ? s : nb - 1 - s;
             if (!UNIT && lane == c) {
-                const cx<double> q = cdiv(cx<double>{acc_r, acc_i}, cx<double>{dr[c][c], C ? di[c][c] : 0.0});
+                const cx<double> q = cmul(cx<double>{acc_r, acc_i}, rd);
                 acc_r = q.re; acc_i = q.im;
             }
-            const double vr = __shfl(acc_r, c, 64), vi = __shfl(acc_i, c, 64);
+            const double vr = readlane_f64(acc_r, c), vi = readlane_f64(acc_i, c);
             if (lane < nb && (LOWER ? lane > c : lane < c)) {
                 const double mr = dr[lane][c], mi = C ? di[lane][c] : 0.0;
                 acc_r -= mr * vr - mi * vi;
