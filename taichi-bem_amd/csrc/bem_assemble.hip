@@ -831,12 +831,16 @@ __device__ __forceinline__ int cf_find(const CfProb* tbl, int count, int64_t ite
     return lo;
 }
 
-// K - i eta V of a regular P1 x P1 pair: one evaluation of r, 1/r and
-// exp(i kap r) per point pair feeds both kernels, and only the combination
-// is accumulated (six partial sums per trial point instead of twelve).
+// K - i eta V of a regular P1 x P1 pair in both orders.  The point-pair
+// integrand of V is symmetric, and K with test panel j and trial panel i
+// differs from K with test panel i only by the normal it projects on
+// ((y - x).n_i instead of (x - y).n_j), so one evaluation of r, 1/r and
+// exp(i kap r) per point pair feeds all four kernels; only the two
+// combinations are accumulated.  C_ij[s][t] pairs test basis s on panel i
+// with trial basis t on panel j; C_ji[t][s] is the transposed pair.
 template <typename R>
-__device__ __forceinline__ void far_pair_cf(const OpParams& p, double eta, int i, int j, double (&Cr)[3][3],
-                                            double (&Ci)[3][3]) {
+__device__ __forceinline__ void far_pair_cf_sym(const OpParams& p, double eta, int i, int j, double (&Cr)[3][3],
+                                                double (&Ci)[3][3], double (&Dr)[3][3], double (&Di)[3][3]) {
     const int q2 = p.q2, nf = p.nf;
     const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
     const R* __restrict__ PL = Ptr<R>::pl(p);
@@ -848,20 +852,24 @@ __device__ __forceinline__ void far_pair_cf(const OpParams& p, double eta, int i
         dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
     }
     const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
+    const R mix = (R)p.normal[3 * i], miy = (R)p.normal[3 * i + 1], miz = (R)p.normal[3 * i + 2];
+    // (x - y).n_i = x_a.n_i + e_b.n_i, with x_a.n_i the same for every test point of the flat panel i
+    const R pun = PU[0] * mix + PU[1] * miy + PU[2] * miz;
     const R kap = (R)p.kap, et = (R)eta;
-    R acr[3][3], aci[3][3];
+    R acr[3][3], aci[3][3], bcr[3][3], bci[3][3];
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int t = 0; t < 3; ++t) { acr[s][t] = aci[s][t] = 0; }
+        for (int t = 0; t < 3; ++t) { acr[s][t] = aci[s][t] = bcr[s][t] = bci[s][t] = 0; }
     for (int b = 0; b < q2; ++b) {
         const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
         const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
         const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
         const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
-        R pcr[3], pci[3];
+        const R dni = pun + ex * mix + ey * miy + ez * miz;
+        R pcr[3], pci[3], qcr[3], qci[3];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) { pcr[s] = pci[s] = 0; }
+        for (int s = 0; s < 3; ++s) { pcr[s] = pci[s] = qcr[s] = qci[s] = 0; }
         for (int a = 0; a < q2; ++a) {
             const R dx = PU[4 * a + 0] + ex;
             const R dy = PU[4 * a + 1] + ey;
@@ -870,31 +878,42 @@ __device__ __forceinline__ void far_pair_cf(const OpParams& p, double eta, int i
             const R r2 = dx * dx + dy * dy + dz * dz;
             const R rinv = Math<R>::rsqrt(r2);
             const R g = wa * rinv;
-            const R base = g * (dx * nnx + dy * nny + dz * nnz) * (rinv * rinv);
+            const R g3 = g * (rinv * rinv);
             const R kr = kap * (r2 * rinv);
             R sn, cs;
             Math<R>::sincos(kr, &sn, &cs);
             // K: base (cs + kr sn, sn - kr cs); -i eta V: eta g (sn, -cs)
-            const R eg = et * g;
-            const R fr = base * (cs + kr * sn) + eg * sn;
-            const R fi = base * (sn - kr * cs) - eg * cs;
+            const R c1 = cs + kr * sn, c2 = sn - kr * cs;
+            const R egs = et * g * sn, egc = et * g * cs;
+            const R bj = g3 * (dx * nnx + dy * nny + dz * nnz), bi_ = -g3 * dni;
+            const R fr = bj * c1 + egs, fi = bj * c2 - egc;
+            const R hr = bi_ * c1 + egs, hi = bi_ * c2 - egc;
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
                 const R ph = PHI[3 * a + s];
                 pcr[s] += fr * ph; pci[s] += fi * ph;
+                qcr[s] += hr * ph; qci[s] += hi * ph;
             }
         }
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             const R wt = wb * PHI[3 * b + t];
 #pragma unroll
-            for (int s = 0; s < 3; ++s) { acr[s][t] += pcr[s] * wt; aci[s][t] += pci[s] * wt; }
+            for (int s = 0; s < 3; ++s) {
+                acr[s][t] += pcr[s] * wt; aci[s][t] += pci[s] * wt;
+                bcr[s][t] += qcr[s] * wt; bci[s][t] += qci[s] * wt;
+            }
         }
     }
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int t = 0; t < 3; ++t) { Cr[s][t] = kInv4Pi * (double)acr[s][t]; Ci[s][t] = kInv4Pi * (double)aci[s][t]; }
+        for (int t = 0; t < 3; ++t) {
+            Cr[s][t] = kInv4Pi * (double)acr[s][t];
+            Ci[s][t] = kInv4Pi * (double)aci[s][t];
+            Dr[t][s] = kInv4Pi * (double)bcr[s][t];
+            Di[t][s] = kInv4Pi * (double)bci[s][t];
+        }
 }
 
 // K - i eta V into A at (tri_i[s], tri_j[t])
@@ -913,27 +932,47 @@ __device__ __forceinline__ void cf_scatter(const OpParams& p, double eta, int i,
     }
 }
 
+// Far item -> (test group ib, trial block jb) over the upper triangle: the
+// test groups of 256-row group gr (256 / TI of them) pair with trial blocks
+// gr .. njb-1, and a lane only evaluates pairs with j > i.
+template <int TI>
+__device__ __forceinline__ void cf_sym_item(int64_t local, int njb, int nib, int& ib, int& jb) {
+    constexpr int GP = 256 / TI;
+    auto prefix = [&](int g) -> int64_t { return (int64_t)GP * ((int64_t)g * njb - (int64_t)g * (g - 1) / 2); };
+    int lo = 0, hi = (nib + GP - 1) / GP - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix(mid) <= local) lo = mid; else hi = mid - 1;
+    }
+    const int64_t off = local - prefix(lo);
+    const int cnt = njb - lo;
+    ib = lo * GP + (int)(off / cnt);
+    jb = lo + (int)(off % cnt);
+}
+
 template <typename R, int TI>
 __global__ __launch_bounds__(256) void cf_far_kernel(const CfProb* tbl, int count, int64_t total) {
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
         const int q = cf_find(tbl, count, item, 0);
         const CfProb& P = tbl[q];
         const OpParams& p = P.p;
-        const int64_t local = item - P.far_begin;
-        const int ib = (int)(local / P.njb), jb = (int)(local % P.njb);
+        int ib, jb;
+        cf_sym_item<TI>(item - P.far_begin, P.njb, P.nib, ib, jb);
         const int j = jb * 256 + threadIdx.x;
         if (j >= p.nf) continue;
         const int i_end = min(p.nf, (ib + 1) * TI);
         for (int i = ib * TI; i < i_end; ++i) {
-            if (shares_vertex(p.tris, i, j)) continue;
-            double Cr[3][3], Ci[3][3];
-            far_pair_cf<R>(p, P.eta, i, j, Cr, Ci);
+            if (j <= i || shares_vertex(p.tris, i, j)) continue;
+            double Cr[3][3], Ci[3][3], Dr[3][3], Di[3][3];
+            far_pair_cf_sym<R>(p, P.eta, i, j, Cr, Ci, Dr, Di);
 #pragma unroll
             for (int s = 0; s < 3; ++s)
 #pragma unroll
-                for (int t = 0; t < 3; ++t)
-                    add_elem(p.out, p.dtype, (int64_t)p.tris[3 * i + s] * p.ld + p.tris[3 * j + t], Cr[s][t],
-                             Ci[s][t], true);
+                for (int t = 0; t < 3; ++t) {
+                    const int64_t vi = p.tris[3 * i + s], vj = p.tris[3 * j + t];
+                    add_elem(p.out, p.dtype, vi * p.ld + vj, Cr[s][t], Ci[s][t], true);
+                    add_elem(p.out, p.dtype, vj * p.ld + vi, Dr[t][s], Di[t][s], true);
+                }
         }
     }
 }
@@ -1278,7 +1317,14 @@ int assemble_combined_batched(Context* ctx, int count, const CfInput* probs, int
         P.far_begin = nfar;
         P.near_begin = nnear;
         P.mass_begin = nmass;
-        nfar += (int64_t)P.nib * P.njb;
+        {
+            // upper-triangle items: test groups of 256-row group g pair with trial blocks g .. njb-1
+            const int gp = 256 / ti;
+            for (int ib0 = 0; ib0 < P.nib; ib0 += gp) {
+                const int g = ib0 / gp;
+                nfar += (int64_t)std::min(gp, P.nib - ib0) * (P.njb - g);
+            }
+        }
         nnear += c->mesh.n_near;
         nmass += c->mesh.nf;
     }
