@@ -268,7 +268,7 @@ typedef struct {
 /* Launch shape of the persistent far-field kernel (chosen by an autotuner;
  * zeros select the defaults). */
 typedef struct {
-    int32_t test_panels_per_block;  /* 1, 2, 4 or 8 (default 2) */
+    int32_t test_panels_per_block;  /* 1, 2, 4, 8 or 16 (default 2) */
     int32_t blocks_per_cu;          /* grid = blocks_per_cu x compute units (default 4) */
 } bem_tuning;
 

@@ -950,6 +950,14 @@ __device__ __forceinline__ void cf_sym_item(int64_t local, int njb, int nib, int
     jb = lo + (int)(off % cnt);
 }
 
+// FP32 items hold the sums of up to CF_SLOTS test vertices per lane and add
+// them to the matrix once a slot is needed for another vertex (and at the
+// end of the item): the TI consecutive test panels of an item share most
+// vertices (8 panels of a subdivided icosphere touch about 10, not 24), so
+// the lane's atomics drop by that factor.  Which slot holds which vertex is
+// the same for every lane (the test panels are wave-uniform).
+constexpr int CF_SLOTS = 6;
+
 template <typename R, int TI>
 __global__ __launch_bounds__(256) void cf_far_kernel(const CfProb* tbl, int count, int64_t total) {
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -959,20 +967,100 @@ __global__ __launch_bounds__(256) void cf_far_kernel(const CfProb* tbl, int coun
         int ib, jb;
         cf_sym_item<TI>(item - P.far_begin, P.njb, P.nib, ib, jb);
         const int j = jb * 256 + threadIdx.x;
-        if (j >= p.nf) continue;
         const int i_end = min(p.nf, (ib + 1) * TI);
-        for (int i = ib * TI; i < i_end; ++i) {
-            if (j <= i || shares_vertex(p.tris, i, j)) continue;
-            double Cr[3][3], Ci[3][3], Dr[3][3], Di[3][3];
-            far_pair_cf_sym<R>(p, P.eta, i, j, Cr, Ci, Dr, Di);
+        if constexpr (sizeof(R) == 8) {
+            if (j >= p.nf) continue;
+            for (int i = ib * TI; i < i_end; ++i) {
+                if (j <= i || shares_vertex(p.tris, i, j)) continue;
+                double Cr[3][3], Ci[3][3], Dr[3][3], Di[3][3];
+                far_pair_cf_sym<R>(p, P.eta, i, j, Cr, Ci, Dr, Di);
 #pragma unroll
-            for (int s = 0; s < 3; ++s)
+                for (int s = 0; s < 3; ++s)
 #pragma unroll
-                for (int t = 0; t < 3; ++t) {
-                    const int64_t vi = p.tris[3 * i + s], vj = p.tris[3 * j + t];
-                    add_elem(p.out, p.dtype, vi * p.ld + vj, Cr[s][t], Ci[s][t], true);
-                    add_elem(p.out, p.dtype, vj * p.ld + vi, Dr[t][s], Di[t][s], true);
+                    for (int t = 0; t < 3; ++t) {
+                        const int64_t vi = p.tris[3 * i + s], vj = p.tris[3 * j + t];
+                        add_elem(p.out, p.dtype, vi * p.ld + vj, Cr[s][t], Ci[s][t], true);
+                        add_elem(p.out, p.dtype, vj * p.ld + vi, Dr[t][s], Di[t][s], true);
+                    }
+            }
+        } else {
+            const bool jv = j < p.nf;
+            const int jj = jv ? j : 0;
+            int64_t vt[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) vt[t] = p.tris[3 * jj + t];
+            int sv[CF_SLOTS];  // test vertex of each slot (-1: free), the same in every lane
+            float ar[CF_SLOTS][3], ai[CF_SLOTS][3], br[CF_SLOTS][3], bi[CF_SLOTS][3];
+#pragma unroll
+            for (int k = 0; k < CF_SLOTS; ++k) {
+                sv[k] = -1;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) ar[k][t] = ai[k][t] = br[k][t] = bi[k][t] = 0.f;
+            }
+            // adds slot k's sums to A[u][v_t] and A[v_t][u] and frees it
+            auto flush = [&](int k) {
+#pragma unroll
+                for (int kk = 0; kk < CF_SLOTS; ++kk) {
+                    if (kk != k) continue;
+                    const int64_t u = sv[kk];
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        if (ar[kk][t] != 0.f || ai[kk][t] != 0.f)
+                            add_elem(p.out, p.dtype, u * p.ld + vt[t], ar[kk][t], ai[kk][t], true);
+                        if (br[kk][t] != 0.f || bi[kk][t] != 0.f)
+                            add_elem(p.out, p.dtype, vt[t] * p.ld + u, br[kk][t], bi[kk][t], true);
+                        ar[kk][t] = ai[kk][t] = br[kk][t] = bi[kk][t] = 0.f;
+                    }
+                    sv[kk] = -1;
                 }
+            };
+            for (int i = ib * TI; i < i_end; ++i) {
+                const int a[3] = {p.tris[3 * i], p.tris[3 * i + 1], p.tris[3 * i + 2]};
+                int slot[3];
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    slot[s] = -1;
+#pragma unroll
+                    for (int k = 0; k < CF_SLOTS; ++k)
+                        if (sv[k] == a[s]) slot[s] = k;
+                }
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    if (slot[s] >= 0) continue;
+                    int k = -1;
+#pragma unroll
+                    for (int kk = CF_SLOTS - 1; kk >= 0; --kk)
+                        if (sv[kk] < 0) k = kk;
+                    if (k < 0) {
+                        // evict the first slot this panel does not use
+#pragma unroll
+                        for (int kk = CF_SLOTS - 1; kk >= 0; --kk)
+                            if (kk != slot[0] && kk != slot[1] && kk != slot[2]) k = kk;
+                        flush(k);
+                    }
+                    sv[k] = a[s];
+                    slot[s] = k;
+                }
+                if (!jv || j <= i || shares_vertex(p.tris, i, j)) continue;
+                double Cr[3][3], Ci[3][3], Dr[3][3], Di[3][3];
+                far_pair_cf_sym<R>(p, P.eta, i, j, Cr, Ci, Dr, Di);
+#pragma unroll
+                for (int s = 0; s < 3; ++s)
+#pragma unroll
+                    for (int k = 0; k < CF_SLOTS; ++k) {
+                        if (k != slot[s]) continue;
+#pragma unroll
+                        for (int t = 0; t < 3; ++t) {
+                            ar[k][t] += (float)Cr[s][t];
+                            ai[k][t] += (float)Ci[s][t];
+                            br[k][t] += (float)Dr[t][s];
+                            bi[k][t] += (float)Di[t][s];
+                        }
+                    }
+            }
+#pragma unroll
+            for (int k = 0; k < CF_SLOTS; ++k)
+                if (sv[k] >= 0) flush(k);
         }
     }
 }
@@ -1292,8 +1380,8 @@ int assemble_combined_batched(Context* ctx, int count, const CfInput* probs, int
     if (flags & ~(BEM_FLAG_ROTATE_NEAR | BEM_FLAG_SYM_COINCIDE)) return set_error(BEM_ERR_INVALID, "unknown flags");
     int ti = tune && tune->test_panels_per_block > 0 ? tune->test_panels_per_block : 2;
     int bpc = tune && tune->blocks_per_cu > 0 ? tune->blocks_per_cu : 4;
-    if (ti != 1 && ti != 2 && ti != 4 && ti != 8)
-        return set_error(BEM_ERR_INVALID, "test_panels_per_block must be 1, 2, 4 or 8");
+    if (ti != 1 && ti != 2 && ti != 4 && ti != 8 && ti != 16)
+        return set_error(BEM_ERR_INVALID, "test_panels_per_block must be 1, 2, 4, 8 or 16");
     if (bpc > 64) return set_error(BEM_ERR_INVALID, "blocks_per_cu must be at most 64");
     std::vector<CfProb> tbl(count);
     int64_t nfar = 0, nnear = 0, nmass = 0;
@@ -1361,7 +1449,8 @@ int assemble_combined_batched(Context* ctx, int count, const CfInput* probs, int
         case 1: far = f32 ? cf_far_kernel<float, 1> : cf_far_kernel<double, 1>; break;
         case 2: far = f32 ? cf_far_kernel<float, 2> : cf_far_kernel<double, 2>; break;
         case 4: far = f32 ? cf_far_kernel<float, 4> : cf_far_kernel<double, 4>; break;
-        default: far = f32 ? cf_far_kernel<float, 8> : cf_far_kernel<double, 8>; break;
+        case 8: far = f32 ? cf_far_kernel<float, 8> : cf_far_kernel<double, 8>; break;
+        default: far = f32 ? cf_far_kernel<float, 16> : cf_far_kernel<double, 16>; break;
     }
     hipLaunchKernelGGL(far, dim3(grid_of(nfar)), dim3(256), 0, s, d_tbl, count, nfar);
     if (nnear > 0)

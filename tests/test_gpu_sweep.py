@@ -48,7 +48,7 @@ def test_combined_batch_matches_oracle(bem, oracle, flags):
         assert rel(A.cpu().numpy(), oracle_cf(oracle, v, f, k, k, flags)) <= 1e-11
 
 
-@pytest.mark.parametrize("tuning", [(1, 1), (2, 4), (8, 2)])
+@pytest.mark.parametrize("tuning", [(1, 1), (2, 4), (8, 2), (16, 4)])
 def test_combined_fp32_and_tunings(bem, oracle, tuning):
     v, f = bem.meshes.perturbed_icosphere(2, seed=0)
     c = make_ctx(bem, v, f)
