@@ -649,7 +649,7 @@ __global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb,
     const int jj = jvalid ? j : 0;
     __shared__ double gts[DTI][144][2];  // data interpolated at the test points (Q^2 <= 144) of each tile panel
     __shared__ double red[4][2];
-    __shared__ double2 sct[64];          // sin/cos(j pi/32) for the FP64 phase
+    __shared__ double2 sct[256];         // sin/cos(j pi/128) for the FP64 phase
     if (i0 >= nf) return;  // the last group's empty tiles (uniform)
     if constexpr (HELM && sizeof(R) == 8) sincos_table_fill(sct);
     for (int e = threadIdx.x; e < DTI * q2; e += 256) {
