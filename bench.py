@@ -39,9 +39,9 @@ FLOPS_PER_EVAL_FUSED = 47
 # workload from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the
 # same counter on a 512 MiB copy (FETCH x 2.00, WRITE x 1.00), and the
 # hardware FP64 VALU flops of that launch (SQ_INSTS_VALU_FLOPS_FP64 x 64
-# lanes).  Measured: 96.3 GB fetched + 8.6 GB written; 1.0455e13 flops.
-FAR_L5_TRAFFIC_BYTES = 104_915_845_120
-FAR_L5_HW_FLOPS = 10_455_360_000_000
+# lanes).  Measured: 95.7 GB fetched + 8.6 GB written; 9.930e12 flops.
+FAR_L5_TRAFFIC_BYTES = 104_298_600_000
+FAR_L5_HW_FLOPS = 9_929_664_000_000
 
 
 def log(msg: str) -> None:
