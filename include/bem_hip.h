@@ -174,6 +174,22 @@ int bem_assemble_dirichlet(bem_context* ctx, const bem_op_desc* op, void* A, int
                            void* rhs, int32_t dtype_rhs, double alpha_k, double alpha_m,
                            void* stream);
 
+/* The three blocks of the transmission (Calderon) system of one wave number
+ * in one pass: V <- V (P0 x P0, nf x nf), K <- K (P0 test x P1 trial,
+ * nf x nv), W <- W (P1 x P1, nv x nv); the call zeroes the three outputs
+ * first.  Replaces the forward() calls of BEMManager.matrix_layer_forward
+ * (BEM_manager.py:829-844) when every panel carries both traces:
+ * SingleLayer3d.forward (single_layer_3d.py:371-433),
+ * DoubleLayer3d.forward (double_layer_3d.py:376-438) and
+ * HypersingularLayer3d.forward (hypersingular_layer_3d.py:437-500), which
+ * evaluate the kernel once per operator and ordered pair; here every
+ * unordered regular panel pair is evaluated once for all three blocks and
+ * both orders.  `op` is the P0 x P0 single-layer descriptor (kernel, k,
+ * sqrt_n, precision, dtype of the outputs, flags); ldv >= nf, ldk >= nv and
+ * ldw >= nv are the row strides in elements. */
+int bem_assemble_vkw(bem_context* ctx, const bem_op_desc* op, void* V, int64_t ldv,
+                     void* K, int64_t ldk, void* W, int64_t ldw, void* stream);
+
 /* Matrix-free y[test dof] += alpha * Op . xhat, xhat = conj(x) if conj_x.
  * Replaces apply_V_dot_N_boundary (single_layer_3d.py:435-498),
  * apply_K_dot_D_boundary (double_layer_3d.py:440-501),

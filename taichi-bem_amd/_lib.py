@@ -28,7 +28,8 @@ OPT_C64_LU_GEMM, OPT_LU_LOOKAHEAD, OPT_LU_PIVOTING, OPT_LU_GEMM_TILE, OPT_LU_PAN
 # Every symbol include/bem_hip.h declares.
 EXPORTS = (
     "bem_create", "bem_destroy", "bem_last_error", "bem_version", "bem_set_mesh",
-    "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_assemble_dirichlet", "bem_apply",
+    "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_assemble_dirichlet", "bem_assemble_vkw",
+    "bem_apply",
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
     "bem_shape_gradient", "bem_area_gradient", "bem_assemble_combined_batched", "bem_lu_factor_batched",
@@ -100,6 +101,7 @@ def lib() -> ctypes.CDLL:
             "bem_apply": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i32, vp, dbl, dbl, vp]),
             "bem_assemble_dirichlet": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i64, dbl, dbl, vp, i32, vp,
                                                       i32, dbl, dbl, vp]),
+            "bem_assemble_vkw": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i64, vp, i64, vp, i64, vp]),
             "bem_mass_assemble": (ctypes.c_int, [vp, i32, i32, i32, vp, i64, dbl, i32, vp]),
             "bem_mass_apply": (ctypes.c_int, [vp, i32, i32, i32, vp, i32, vp, dbl, vp]),
             "bem_lu_factor": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp]),
@@ -254,6 +256,21 @@ class Context:
         check(lib().bem_assemble_dirichlet(self._h, ctypes.byref(desc), A.data_ptr(), int(A.stride(0)), a.real,
                                            a.imag, g.data_ptr(), int(bool(conj_g)), rhs.data_ptr(),
                                            dtype_code(rhs), float(alpha_k), float(alpha_m), current_stream()))
+
+    def assemble_vkw(self, desc: BemOpDesc, V, K, W) -> None:
+        """V <- V (P0 x P0), K <- K (P0 x P1), W <- W (P1 x P1) of one wave number
+        in one far-field pass; `desc` is the P0 single-layer descriptor."""
+        _require_cuda(V, K, W)
+        if V.dtype != K.dtype or V.dtype != W.dtype:
+            raise BEMError("V, K and W must share a dtype")
+        for M, shape in ((V, (self.nf, self.nf)), (K, (self.nf, self.nv)), (W, (self.nv, self.nv))):
+            if M.dim() != 2 or M.stride(-1) != 1:
+                raise BEMError("outputs must be matrices with unit column stride")
+            if tuple(M.shape) != shape:
+                raise BEMError(f"output of shape {tuple(M.shape)}, expected {shape}")
+        desc.dtype = dtype_code(V)
+        check(lib().bem_assemble_vkw(self._h, ctypes.byref(desc), V.data_ptr(), int(V.stride(0)), K.data_ptr(),
+                                     int(K.stride(0)), W.data_ptr(), int(W.stride(0)), current_stream()))
 
     def apply(self, desc: BemOpDesc, x, y, alpha: complex = 1.0, conj_x: bool = True) -> None:
         _require_cuda(x, y)

@@ -801,6 +801,197 @@ __global__ void kpart_reduce_kernel(const double* kpart, const double* krow, int
 }
 
 // ---------------------------------------------------------------------------
+// The blocks of the Calderon (transmission) system of one wave number in one
+// far-field pass: V (P0 x P0), K (P0 test x P1 trial) and W (P1 x P1).  The
+// tensor rule evaluates the same point pairs for (i, j) and (j, i), so only
+// pairs i < j are visited, and one evaluation of r, 1/r and exp(i kap r) per
+// point pair yields
+//   V_ij, stored to V[i][j] and V[j][i];
+//   K[i][v] for the vertices v of panel j (normal n_j, trial bases on j) and
+//   K[j][u] for the vertices u of panel i (normal n_i, trial bases on i);
+//   the W entries of the pair, W_ij[s][t] = W_ji[t][s] (G and the curl and
+//   normal products are symmetric): the curl term needs only the V integral,
+//   and the k^2 term the P1 x P1 integrals of G.
+// W entries are added once, at (min(u, v), max(u, v)), and a second kernel
+// copies the upper triangle to the lower one.  Regular pairs share no
+// vertex, so they never reach the diagonal; the near field, added after
+// the copy, brings both orders of its pairs itself.  Blocks tile the upper
+// triangle like the Dirichlet pass (DTI test panels, wave-uniform, times 256
+// trial panels, one per lane).
+// ---------------------------------------------------------------------------
+struct VkwOut {
+    void* V; int64_t ldv;
+    void* K; int64_t ldk;
+    void* W; int64_t ldw;
+};
+
+template <typename R, bool HELM>
+__global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int njb, int ngroups) {
+    int tile, jb;
+    sym_block(blockIdx.x, njb, ngroups, tile, jb);
+    const int nf = p.nf, q2 = p.q2;
+    const int i0 = tile * DTI;
+    const int j = jb * 256 + threadIdx.x;
+    __shared__ double2 sct[256];  // sin/cos(j pi/128) for the FP64 phase
+    if (i0 >= nf) return;  // the last group's empty tiles (uniform)
+    if constexpr (HELM && sizeof(R) == 8) {
+        sincos_table_fill(sct);
+        __syncthreads();
+    }
+    if (j >= nf) return;  // no barrier below
+    const R* __restrict__ PL = Ptr<R>::pl(p);
+    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    const R kap = (R)p.kap;
+    const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
+    const int64_t vj[3] = {p.tris[3 * j], p.tris[3 * j + 1], p.tris[3 * j + 2]};
+    const int i_end = min(nf, i0 + DTI);
+    for (int i = i0; i < i_end; ++i) {
+        if (j <= i || shares_vertex(p.tris, i, j)) continue;
+        const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+        R dcx = 0, dcy = 0, dcz = 0;
+        if constexpr (sizeof(R) == 4) {
+            dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
+            dcy = (R)(p.cen[3 * i + 1] - p.cen[3 * j + 1]);
+            dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
+        }
+        const R mix = (R)p.normal[3 * i], miy = (R)p.normal[3 * i + 1], miz = (R)p.normal[3 * i + 2];
+        // (x - y).n_i = x_a.n_i + e_b.n_i with x_a.n_i the same for every test point of the flat panel i
+        const R pun = PU[0] * mix + PU[1] * miy + PU[2] * miz;
+        R kar[3], kai[3], kbr[3], kbi[3], wr[3][3], wi[3][3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            kar[s] = kai[s] = kbr[s] = kbi[s] = 0;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) wr[s][t] = wi[s][t] = 0;
+        }
+        for (int b = 0; b < q2; ++b) {
+            const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
+            const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
+            const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
+            const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
+            const R dni = pun + ex * mix + ey * miy + ez * miz;
+            R pwr[3], pwi[3], qkr[3], qki[3], pkr = 0, pki = 0;
+#pragma unroll
+            for (int s = 0; s < 3; ++s) pwr[s] = pwi[s] = qkr[s] = qki[s] = 0;
+            for (int a = 0; a < q2; ++a) {
+                const R dx = PU[4 * a + 0] + ex;
+                const R dy = PU[4 * a + 1] + ey;
+                const R dz = PU[4 * a + 2] + ez;
+                const R wa = PU[4 * a + 3];
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                const R rinv = Math<R>::rsqrt(r2);
+                const R g = wa * rinv;
+                const R g3 = g * (rinv * rinv);
+                const R bj = g3 * (dx * nnx + dy * nny + dz * nnz);  // (x - y).n_j: K row i
+                const R bi_ = -g3 * dni;                               // (y - x).n_i: K row j
+                if constexpr (HELM) {
+                    const R kr = kap * (r2 * rinv);
+                    R sn, cs;
+                    if constexpr (sizeof(R) == 8) sincos_f64_tab(kr, sct, &sn, &cs);
+                    else Math<R>::sincos(kr, &sn, &cs);
+                    const R gr = g * cs, gi = g * sn;
+                    const R c1 = cs + kr * sn, c2 = sn - kr * cs;
+                    pkr += bj * c1; pki += bj * c2;
+                    const R tr = bi_ * c1, tim = bi_ * c2;
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        const R ph = PHI[3 * a + s];
+                        pwr[s] += gr * ph; pwi[s] += gi * ph;
+                        qkr[s] += tr * ph; qki[s] += tim * ph;
+                    }
+                } else {
+                    pkr += bj;
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        const R ph = PHI[3 * a + s];
+                        pwr[s] += g * ph;
+                        qkr[s] += bi_ * ph;
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const R wt = wb * PHI[3 * b + t];
+                kar[t] += pkr * wt; kai[t] += pki * wt;
+#pragma unroll
+                for (int s = 0; s < 3; ++s) { wr[s][t] += pwr[s] * wt; wi[s][t] += pwi[s] * wt; }
+            }
+#pragma unroll
+            for (int s = 0; s < 3; ++s) { kbr[s] += qkr[s] * wb; kbi[s] += qki[s] * wb; }
+        }
+        // the V integral: the P1 bases sum to one on either panel
+        double s0r = 0.0, s0i = 0.0;
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+#pragma unroll
+            for (int t = 0; t < 3; ++t) { s0r += (double)wr[s][t]; s0i += (double)wi[s][t]; }
+        const double Vr = kInv4Pi * s0r, Vi = kInv4Pi * s0i;
+        add_elem(o.V, p.dtype, (int64_t)i * o.ldv + j, Vr, Vi, false);
+        add_elem(o.V, p.dtype, (int64_t)j * o.ldv + i, Vr, Vi, false);
+        const int64_t vi[3] = {p.tris[3 * i], p.tris[3 * i + 1], p.tris[3 * i + 2]};
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+            add_elem(o.K, p.dtype, (int64_t)i * o.ldk + vj[t], kInv4Pi * (double)kar[t], kInv4Pi * (double)kai[t],
+                     true);
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            add_elem(o.K, p.dtype, (int64_t)j * o.ldk + vi[s], kInv4Pi * (double)kbr[s], kInv4Pi * (double)kbi[s],
+                     true);
+        const double nxy = p.normal[3 * i] * p.normal[3 * j] + p.normal[3 * i + 1] * p.normal[3 * j + 1] +
+                           p.normal[3 * i + 2] * p.normal[3 * j + 2];
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const double cc = p.curl[9 * i + 3 * s] * p.curl[9 * j + 3 * t] +
+                                  p.curl[9 * i + 3 * s + 1] * p.curl[9 * j + 3 * t + 1] +
+                                  p.curl[9 * i + 3 * s + 2] * p.curl[9 * j + 3 * t + 2];
+                const double er = kInv4Pi * (cc * s0r - p.k2n * nxy * (double)wr[s][t]);
+                const double ei = kInv4Pi * (cc * s0i - p.k2n * nxy * (double)wi[s][t]);
+                const int64_t lo = min(vi[s], vj[t]), hi = max(vi[s], vj[t]);
+                add_elem(o.W, p.dtype, lo * o.ldw + hi, er, ei, true);
+            }
+    }
+}
+
+// W[hi][lo] = W[lo][hi] for lo < hi, in 32 x 32 tiles through LDS: block
+// (tr, tc) with tr >= tc writes the strict lower part of tile row tr,
+// column tc from the transposed upper tile.  E is the element type.
+template <typename E>
+__global__ __launch_bounds__(256) void mirror_upper_kernel(E* W, int64_t ld, int n, int ntiles) {
+    // block -> (tr, tc), tc <= tr, row-major over the lower triangle of tiles
+    const int64_t b = blockIdx.x;
+    int tr = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((int64_t)tr * (tr + 1) / 2 > b) --tr;
+    while ((int64_t)(tr + 1) * (tr + 2) / 2 <= b) ++tr;
+    const int tc = (int)(b - (int64_t)tr * (tr + 1) / 2);
+    if (tr >= ntiles) return;
+    __shared__ E t[32][33];
+    const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+    // load the upper tile: rows tc*32 + y, cols tr*32 + x
+    for (int y = y0; y < 32; y += 8) {
+        const int r = tc * 32 + y, c = tr * 32 + x;
+        if (r < n && c < n && r < c) t[y][x] = W[(int64_t)r * ld + c];
+    }
+    __syncthreads();
+    for (int y = y0; y < 32; y += 8) {
+        const int r = tr * 32 + y, c = tc * 32 + x;
+        if (r < n && c < n && r > c) W[(int64_t)r * ld + c] = t[x][y];
+    }
+}
+
+// Zeroes row blockIdx.x of a matrix: row_bytes bytes at out + row * ld_bytes
+// (16-byte stores when the rows are 16-byte aligned, else 4-byte ones).
+__global__ __launch_bounds__(256) void zero_rows_kernel(char* out, int64_t row_bytes, int64_t ld_bytes, int wide) {
+    char* row = out + (int64_t)blockIdx.x * ld_bytes;
+    if (wide) {
+        for (int64_t w = threadIdx.x; w < row_bytes / 16; w += blockDim.x) ((uint4*)row)[w] = make_uint4(0, 0, 0, 0);
+    } else {
+        for (int64_t w = threadIdx.x; w < row_bytes / 4; w += blockDim.x) ((uint32_t*)row)[w] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Combined-field operator of the exterior Dirichlet problem,
 //   A = 1/2 M + K - i eta V      (P1 x P1; Brakhage-Werner, eta = k usually),
 // for a batch of problems (meshes of different sizes, wave numbers) in
@@ -1340,6 +1531,77 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
         rc = apply_op(ctx, &dk, g, conj_g, rhs, alpha_k, 0.0, s);
         if (rc) return rc;
         if (alpha_m != 0.0) rc = mass_apply(ctx, 0, 1, dtype_rhs, g, conj_g, rhs, alpha_m, s);
+    }
+    return rc;
+}
+
+namespace {
+int zero_matrix(void* out, int64_t rows, int64_t cols, int64_t ld, int dtype, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return BEM_OK;
+    const int64_t es = (int64_t)dtype_size(dtype);
+    const int64_t row_bytes = cols * es, ld_bytes = ld * es;
+    const int wide = ((uintptr_t)out % 16 == 0 && row_bytes % 16 == 0 && ld_bytes % 16 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(zero_rows_kernel, dim3((unsigned)rows), dim3(256), 0, s, (char*)out, row_bytes, ld_bytes, wide);
+    return check_hip(hipGetLastError(), "matrix zero launch");
+}
+
+template <typename E>
+int mirror_upper(void* W, int64_t ld, int n, hipStream_t s) {
+    const int ntiles = (n + 31) / 32;
+    const int64_t nblocks = (int64_t)ntiles * (ntiles + 1) / 2;
+    hipLaunchKernelGGL(mirror_upper_kernel<E>, dim3((unsigned)nblocks), dim3(256), 0, s, (E*)W, ld, n, ntiles);
+    return check_hip(hipGetLastError(), "W mirror launch");
+}
+}  // namespace
+
+int assemble_vkw(Context* ctx, const bem_op_desc* d, void* V, int64_t ldv, void* K, int64_t ldk, void* W,
+                 int64_t ldw, hipStream_t s) {
+    int rc = validate(ctx, d);
+    if (rc) return rc;
+    if (!V || !K || !W) return set_error(BEM_ERR_INVALID, "null output");
+    if (d->op != BEM_OP_V || d->deg_test != 0 || d->deg_trial != 0)
+        return set_error(BEM_ERR_INVALID, "the V/K/W pass takes the P0 single-layer descriptor");
+    if (d->flags & BEM_FLAG_DK)
+        return set_error(BEM_ERR_UNSUPPORTED, "k-derivative operators are matrix-free only (bem_apply)");
+    Mesh& m = ctx->mesh;
+    if (ldv < m.nf || ldk < m.nv || ldw < m.nv)
+        return set_error(BEM_ERR_INVALID, "leading dimension smaller than the block width");
+    rc = mesh_build_near(ctx, d->flags, s);
+    if (rc) return rc;
+    const int parts = d->parts ? d->parts : (BEM_PART_FAR | BEM_PART_NEAR);
+    rc = zero_matrix(V, m.nf, m.nf, ldv, d->dtype, s);
+    if (!rc) rc = zero_matrix(K, m.nf, m.nv, ldk, d->dtype, s);
+    if (!rc) rc = zero_matrix(W, m.nv, m.nv, ldw, d->dtype, s);
+    if (rc) return rc;
+    OpParams p = make_params(ctx, d);
+    p.dtype = d->dtype; p.ar = 1.0; p.ai = 0.0;
+    const bool helm = d->kernel != BEM_LAPLACE;
+    const int nf = m.nf, njb = (nf + 255) / 256, ngroups = (nf + 255) / 256;
+    if (parts & BEM_PART_FAR) {
+        auto fk = d->precision == BEM_PREC_FP32 ? (helm ? far_vkw_kernel<float, true> : far_vkw_kernel<float, false>)
+                                                : (helm ? far_vkw_kernel<double, true> : far_vkw_kernel<double, false>);
+        const int64_t nblocks = (int64_t)DTG * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
+        VkwOut o{V, ldv, K, ldk, W, ldw};
+        hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, o, njb, ngroups);
+        rc = check_hip(hipGetLastError(), "far-field V/K/W launch");
+        if (rc) return rc;
+        switch (d->dtype) {
+            case BEM_C128: rc = mirror_upper<double2>(W, ldw, m.nv, s); break;
+            case BEM_C64:
+            case BEM_F64: rc = mirror_upper<double>(W, ldw, m.nv, s); break;  // 8-byte elements
+            default: rc = mirror_upper<float>(W, ldw, m.nv, s); break;
+        }
+        if (rc) return rc;
+    }
+    if (p.n_near > 0 && (parts & BEM_PART_NEAR)) {
+        const unsigned nb = (unsigned)((p.n_near + 3) / 4);
+        p.out = V; p.ld = ldv;
+        hipLaunchKernelGGL(near_any(BEM_OP_V, helm, 0, 0, false), dim3(nb), dim3(256), 0, s, p);
+        p.out = K; p.ld = ldk;
+        hipLaunchKernelGGL(near_any(BEM_OP_K, helm, 0, 1, false), dim3(nb), dim3(256), 0, s, p);
+        p.out = W; p.ld = ldw;
+        hipLaunchKernelGGL(near_any(BEM_OP_W, helm, 1, 1, false), dim3(nb), dim3(256), 0, s, p);
+        rc = check_hip(hipGetLastError(), "near-field V/K/W launch");
     }
     return rc;
 }

@@ -152,6 +152,13 @@ int bem_assemble_dirichlet(bem_context* ctx, const bem_op_desc* op, void* A, int
     });
 }
 
+int bem_assemble_vkw(bem_context* ctx, const bem_op_desc* op, void* V, int64_t ldv, void* K, int64_t ldk, void* W,
+                     int64_t ldw, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::assemble_vkw(&ctx->impl, op, V, ldv, K, ldk, W, ldw, as_stream(stream));
+    });
+}
+
 int bem_mass_assemble(bem_context* ctx, int32_t deg_test, int32_t deg_trial, int32_t dtype, void* out,
                       int64_t ld, double alpha, int32_t transpose, void* stream) {
     return with_device(ctx, [&]() -> int {

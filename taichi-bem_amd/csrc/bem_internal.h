@@ -111,6 +111,8 @@ int mass_assemble(Context* ctx, int deg_test, int deg_trial, int dtype, void* ou
 int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, double ar, double ai,
                        const void* g, int conj_g, void* rhs, int dtype_rhs, double alpha_k, double alpha_m,
                        hipStream_t s);
+int assemble_vkw(Context* ctx, const bem_op_desc* d, void* V, int64_t ldv, void* K, int64_t ldk, void* W,
+                 int64_t ldw, hipStream_t s);
 // one problem of a combined-field batch (bem_cf_problem with the context resolved)
 struct CfInput {
     Context* ctx;

@@ -69,6 +69,8 @@ class BEMManager:
         self._L1_inv_H = None
         self._L2_inv_H = None
         self.initialized = False
+        # one fused V/K/W pass per wave number where the layout allows it
+        self.fuse_layers = True
 
     @property
     def ctx(self) -> _lib.Context:
@@ -178,10 +180,27 @@ class BEMManager:
             raise RuntimeError("Did not indicate a correct scope type!")
         self.set_scope_type(scope_type=scope_type)
         self.matrix_layer_init()
+        if self._can_fuse_layers():
+            # V, K and W of every panel pair from one kernel evaluation
+            V, K, W = self.single_layer._Vmat, self.double_layer._Kmat, self.hypersingular_layer._Wmat
+            desc = self.single_layer._desc(0, 0, k, sqrt_n)
+            self.ctx.assemble_vkw(desc, V, K, W)
+            self.adj_double_layer.forward(scope_type, k, sqrt_n)
+            return
         self.single_layer.forward(scope_type, k, sqrt_n)
         self.double_layer.forward(scope_type, k, sqrt_n)
         self.adj_double_layer.forward(scope_type, k, sqrt_n)
         self.hypersingular_layer.forward(scope_type, k, sqrt_n)
+
+    def _can_fuse_layers(self) -> bool:
+        """The fused V/K/W pass covers P0 Neumann unknowns on every panel and P1
+        Dirichlet unknowns on every vertex (the transmission layout);
+        `fuse_layers = False` selects the per-operator passes."""
+        mm = self._core_manager._mesh_manager
+        return (self.fuse_layers and self.shape_func_degree_Neumann == 0 and self.shape_func_degree_Dirichlet == 1
+                and self.N_Neumann == mm.get_num_of_panels() and self.M_Dirichlet == mm.get_num_of_vertices()
+                and self.single_layer._Vmat is not None and self.double_layer._Kmat is not None
+                and self.hypersingular_layer._Wmat is not None)
 
     def rhs_layer_forward(self, scope_type, k: float, sqrt_n: float = 1.0):
         self.rhs_constructor.forward(scope_type=scope_type, k=k, sqrt_n=sqrt_n)

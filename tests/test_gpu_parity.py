@@ -548,3 +548,81 @@ def test_assemble_dirichlet_fused(bem, oracle, meshes, kernel, k, prec, tol):
         V, r = V.real, r.real
     assert rel(to_np(A).astype(V.dtype), V) <= tol
     assert rel(to_np(rhs), r) <= tol
+
+
+VKW_KERNELS = [(0, 0.0, 1.0), (1, 3.0, 1.0), (2, 5.0, 1.5)]
+
+
+@pytest.mark.parametrize("flags", [0, 3])
+@pytest.mark.parametrize("kernel,k,sqrt_n", VKW_KERNELS)
+def test_assemble_vkw_matches_oracle(bem, oracle, meshes, kernel, k, sqrt_n, flags):
+    """The fused V/K/W pass (one evaluation per unordered point pair, W added
+    once at (min, max) and mirrored) equals the oracle's separate operators."""
+    v, f = bem.meshes.perturbed_icosphere(2, seed=1)
+    ctx = make_ctx(bem, v, f)
+    nv, nf = len(v), len(f)
+    dt = torch.float64 if kernel == 0 else torch.complex128
+    V = torch.full((nf, nf), 7.0, dtype=dt, device=DEV)  # the call zeroes its outputs
+    K = torch.full((nf, nv), 7.0, dtype=dt, device=DEV)
+    W = torch.full((nv, nv), 7.0, dtype=dt, device=DEV)
+    print(f"[vkw] kernel={kernel} k={k} sqrt_n={sqrt_n} flags={flags}", flush=True)
+    ctx.assemble_vkw(bem.Context.op_desc(0, kernel, k, sqrt_n, 0, 0, bem._lib.PREC_FP64, 0, flags), V, K, W)
+    for got, op, a, b in ((V, 0, 0, 0), (K, 1, 0, 1), (W, 3, 1, 1)):
+        ref = oracle.assemble(oracle.make_op(op, kernel, k, sqrt_n, 5, a, b, flags), v, f)
+        if kernel == 0:
+            ref = ref.real
+        err = rel(to_np(got), ref)
+        print(f"[vkw] op={op} rel Frobenius error {err:.3e}", flush=True)
+        assert err <= TOL64
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 2e-6)])
+def test_assemble_vkw_matches_separate_passes(bem, meshes, prec, tol):
+    """L3 (1280 panels: five trial blocks, several 256-row groups): fused
+    blocks against bem_assemble of V, K and W, written into offset blocks of
+    larger matrices whose other entries must stay untouched."""
+    v, f = bem.meshes.perturbed_icosphere(3, seed=2)
+    ctx = make_ctx(bem, v, f)
+    nv, nf = len(v), len(f)
+    dt = torch.complex128 if prec == 0 else torch.complex64
+    big = [torch.full((r + 3, c + 5), 2.0, dtype=dt, device=DEV) for r, c in ((nf, nf), (nf, nv), (nv, nv))]
+    outs = [b[3:, 5:] for b in big]
+    d = bem.Context.op_desc(0, 2, 4.0, 1.5, 0, 0, prec, 0, 0)
+    print(f"[vkw] separate-pass comparison prec={prec}", flush=True)
+    ctx.assemble_vkw(d, *outs)
+    for got, b, (op, a, c) in zip(outs, big, ((0, 0, 0), (1, 0, 1), (3, 1, 1))):
+        ref = torch.zeros(got.shape, dtype=dt, device=DEV)
+        ctx.assemble(bem.Context.op_desc(op, 2, 4.0, 1.5, a, c, prec, 0, 0), ref)
+        err = rel(to_np(got).astype(np.complex128), to_np(ref).astype(np.complex128))
+        print(f"[vkw] op={op} fused vs separate {err:.3e}", flush=True)
+        assert err <= tol
+        assert bool((b[:3] == 2.0).all()) and bool((b[:, :5] == 2.0).all())
+
+
+def test_assemble_vkw_real_fp32_odd_width(bem):
+    """Laplace into float32 outputs (4-byte elements: the narrow zeroing and
+    mirror paths) against the FP64 blocks."""
+    v, f = bem.meshes.perturbed_icosphere(1, seed=3)
+    ctx = make_ctx(bem, v, f)
+    nv, nf = len(v), len(f)
+    outs32 = [torch.full((r, c), 3.0, dtype=torch.float32, device=DEV) for r, c in ((nf, nf), (nf, nv), (nv, nv))]
+    outs64 = [torch.zeros(o.shape, dtype=torch.float64, device=DEV) for o in outs32]
+    ctx.assemble_vkw(bem.Context.op_desc(0, 0, 0.0, 1.0, 0, 0, bem._lib.PREC_FP32, 0, 0), *outs32)
+    ctx.assemble_vkw(bem.Context.op_desc(0, 0, 0.0, 1.0, 0, 0, bem._lib.PREC_FP64, 0, 0), *outs64)
+    for a, b in zip(outs32, outs64):
+        assert rel(to_np(a).astype(np.float64), to_np(b)) <= 2e-6
+
+
+def test_assemble_vkw_rejects(bem, meshes):
+    v, f = meshes[1]
+    ctx = make_ctx(bem, v, f)
+    nv, nf = len(v), len(f)
+    V = torch.zeros(nf, nf, dtype=torch.complex128, device=DEV)
+    K = torch.zeros(nf, nv, dtype=torch.complex128, device=DEV)
+    W = torch.zeros(nv, nv, dtype=torch.complex128, device=DEV)
+    with pytest.raises(bem.BEMError):
+        ctx.assemble_vkw(bem.Context.op_desc(1, 1, 2.0, 1.0, 0, 1, 0, 0, 0), V, K, W)  # not the V descriptor
+    with pytest.raises(NotImplementedError):
+        ctx.assemble_vkw(bem.Context.op_desc(0, 1, 2.0, 1.0, 0, 0, 0, 0, bem._lib.FLAG_DK), V, K, W)
+    with pytest.raises(bem.BEMError):
+        ctx.assemble_vkw(bem.Context.op_desc(0, 1, 2.0, 1.0, 0, 0, 0, 0, 0), V, K, W[:, : nv - 1])
