@@ -190,6 +190,20 @@ int bem_assemble_dirichlet(bem_context* ctx, const bem_op_desc* op, void* A, int
 int bem_assemble_vkw(bem_context* ctx, const bem_op_desc* op, void* V, int64_t ldv,
                      void* K, int64_t ldk, void* W, int64_t ldw, void* stream);
 
+/* The k-derivative bilinear forms of the three Calderon blocks over every
+ * panel pair, without forming a matrix: out (4 complex128 = 8 doubles, device
+ * memory) = (aN^T dV/dk bN, aN^T dK/dk bD, bN^T dK/dk aD, aD^T dW/dk bD),
+ * plain products (no conjugation).  aN, bN: complex128 on the panels (nf);
+ * aD, bD: complex128 on the vertices (nv).  `op` is the P0 single-layer
+ * descriptor of the wave number (Helmholtz kernels, FP64, complex128).  No
+ * reference counterpart (the reference has no derivatives); it serves the
+ * adjoint wave-number gradient of the transmission system, which needs
+ * mu^T dA/dk x of the blocks that BEMManager.matA_add_M places
+ * (BEM_manager.py:608-628).  Same result as four bem_apply calls with
+ * BEM_FLAG_DK and dots, from one evaluation per unordered point pair. */
+int bem_vkw_dk_forms(bem_context* ctx, const bem_op_desc* op, const void* aN, const void* bN,
+                     const void* aD, const void* bD, double* out, void* stream);
+
 /* Matrix-free y[test dof] += alpha * Op . xhat, xhat = conj(x) if conj_x.
  * Replaces apply_V_dot_N_boundary (single_layer_3d.py:435-498),
  * apply_K_dot_D_boundary (double_layer_3d.py:440-501),

@@ -29,7 +29,7 @@ OPT_C64_LU_GEMM, OPT_LU_LOOKAHEAD, OPT_LU_PIVOTING, OPT_LU_GEMM_TILE, OPT_LU_PAN
 EXPORTS = (
     "bem_create", "bem_destroy", "bem_last_error", "bem_version", "bem_set_mesh",
     "bem_get_geometry", "bem_nearfield_counts", "bem_assemble", "bem_assemble_dirichlet", "bem_assemble_vkw",
-    "bem_apply",
+    "bem_vkw_dk_forms", "bem_apply",
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
     "bem_shape_gradient", "bem_area_gradient", "bem_assemble_combined_batched", "bem_lu_factor_batched",
@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
             "bem_assemble_dirichlet": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i64, dbl, dbl, vp, i32, vp,
                                                       i32, dbl, dbl, vp]),
             "bem_assemble_vkw": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, i64, vp, i64, vp, i64, vp]),
+            "bem_vkw_dk_forms": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, vp, vp, vp, vp, vp]),
             "bem_mass_assemble": (ctypes.c_int, [vp, i32, i32, i32, vp, i64, dbl, i32, vp]),
             "bem_mass_apply": (ctypes.c_int, [vp, i32, i32, i32, vp, i32, vp, dbl, vp]),
             "bem_lu_factor": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp]),
@@ -271,6 +272,23 @@ class Context:
         desc.dtype = dtype_code(V)
         check(lib().bem_assemble_vkw(self._h, ctypes.byref(desc), V.data_ptr(), int(V.stride(0)), K.data_ptr(),
                                      int(K.stride(0)), W.data_ptr(), int(W.stride(0)), current_stream()))
+
+    def vkw_dk_forms(self, desc: BemOpDesc, aN, bN, aD, bD):
+        """The four k-derivative bilinear forms of the Calderon blocks,
+        (aN^T dV bN, aN^T dK bD, bN^T dK aD, aD^T dW bD), plain products over
+        every panel pair; complex128 vectors, `desc` the P0 single-layer
+        descriptor of the wave number.  Returns a complex128 tensor of 4."""
+        import torch
+
+        _require_cuda(aN, bN, aD, bD)
+        for x, n in ((aN, self.nf), (bN, self.nf), (aD, self.nv), (bD, self.nv)):
+            if x.dtype != torch.complex128 or x.dim() != 1 or x.shape[0] != n or not x.is_contiguous():
+                raise BEMError(f"vectors must be contiguous complex128 of length nf={self.nf} or nv={self.nv}")
+        out = torch.empty(4, dtype=torch.complex128, device=aN.device)
+        desc.dtype = C128
+        check(lib().bem_vkw_dk_forms(self._h, ctypes.byref(desc), aN.data_ptr(), bN.data_ptr(), aD.data_ptr(),
+                                     bD.data_ptr(), out.data_ptr(), current_stream()))
+        return out
 
     def apply(self, desc: BemOpDesc, x, y, alpha: complex = 1.0, conj_x: bool = True) -> None:
         _require_cuda(x, y)

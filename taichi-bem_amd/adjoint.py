@@ -149,7 +149,9 @@ def transmission_dA_contract(bm, mu: torch.Tensor, x: torch.Tensor) -> complex:
         mu^T dA x = -mu_D.dW x_D - x_N.dK mu_D + mu_N.dK x_D - mu_N.dV x_N
 
     (plain bilinear dots; the transposed block is taken as x_N . dK mu_D, so
-    no transposed product is needed)."""
+    no transposed product is needed).  With P0 Neumann unknowns on every panel
+    and P1 Dirichlet unknowns on every vertex the four forms come from one
+    fused pass (bem_vkw_dk_forms); otherwise from four matrix-free products."""
     if not bm._is_transmission:
         raise RuntimeError("transmission_dA_contract needs a transmission BEMManager")
     C = bm.ctx
@@ -161,6 +163,13 @@ def transmission_dA_contract(bm, mu: torch.Tensor, x: torch.Tensor) -> complex:
     x = x.to(torch.complex128).contiguous()
     xD, xN = x[D0:D0 + nD].contiguous(), x[N0:N0 + nN].contiguous()
     mD, mN = mu[D0:D0 + nD].contiguous(), mu[N0:N0 + nN].contiguous()
+    mm = bm._core_manager._mesh_manager
+    if bm.fuse_layers and qN == 0 and qD == 1 and nN == mm.get_num_of_panels() and nD == mm.get_num_of_vertices():
+        # the four forms from one evaluation per unordered point pair
+        d = _lib.Context.op_desc(_lib.OP_V, _lib.KernelType_TRANSMISSION, ki, 1.0, 0, 0, _lib.PREC_FP64, _lib.C128,
+                                 bm._quadrature_flags)
+        c = C.vkw_dk_forms(d, mN, xN, mD, xD)
+        return complex((-c[3] - c[2] + c[1] - c[0]).item())
     flags = bm._quadrature_flags | _lib.FLAG_DK
 
     def dapply(op, dt, dr, v, n_out):

@@ -980,6 +980,212 @@ __global__ __launch_bounds__(256) void mirror_upper_kernel(E* W, int64_t ld, int
     }
 }
 
+// ---------------------------------------------------------------------------
+// The k-derivative bilinear forms of the Calderon blocks over the regular
+// pairs, without forming a matrix or a vector:
+//   c0 = aN^T dV bN,  c1 = aN^T dK bD,  c2 = bN^T dK aD,  c3 = aD^T dW bD
+// (aN, bN on the P0 panels, aD, bD on the P1 vertices, plain products, no
+// conjugation).  Tiling and the single evaluation per unordered point pair
+// are those of far_vkw_kernel, with the k-derivative integrands
+//   dG/dk       = dkap i exp(i kap r) / (4 pi),
+//   d(dG/dn)/dk = dkap kap ((x - y).n) exp(i kap r) / (4 pi r),
+//   dW/dk       = dG/dk curl.curl - (dk2n G + k2n dG/dk) phi phi n_x.n_y.
+// Each pair is contracted with the vectors in both orders, each lane keeps
+// the four sums, and the block writes its reduction to part[block][8].
+// ---------------------------------------------------------------------------
+struct DkVecs {
+    const double* aN;  // complex, [nf]
+    const double* bN;
+    const double* aD;  // complex, [nv]
+    const double* bD;
+};
+
+__device__ __forceinline__ void cmac(double& zr, double& zi, double xr, double xi, double yr, double yi) {
+    zr += xr * yr - xi * yi;
+    zi += xr * yi + xi * yr;
+}
+
+__global__ __launch_bounds__(256) void far_vkw_dk_kernel(OpParams p, DkVecs v, int njb, int ngroups, double* part) {
+    int tile, jb;
+    sym_block(blockIdx.x, njb, ngroups, tile, jb);
+    const int nf = p.nf, q2 = p.q2;
+    const int i0 = tile * DTI;
+    const int j = jb * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ double2 sct[256];
+    __shared__ double red[4][8];
+    if (i0 >= nf) {  // the last group's empty tiles (uniform)
+        if (threadIdx.x < 8) part[(int64_t)blockIdx.x * 8 + threadIdx.x] = 0.0;
+        return;
+    }
+    sincos_table_fill(sct);
+    __syncthreads();
+    double c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = 0.0;
+    if (j < nf) {
+        const double* __restrict__ PL = p.pl;
+        const double* __restrict__ PHI = p.phi;
+        const double kap = p.kap, dkap = p.dkap, k2n = p.k2n, dk2n = p.dk2n, kd = p.kap * p.dkap;
+        const double nnx = p.normal[3 * j], nny = p.normal[3 * j + 1], nnz = p.normal[3 * j + 2];
+        const int vj[3] = {p.tris[3 * j], p.tris[3 * j + 1], p.tris[3 * j + 2]};
+        const double aNj[2] = {v.aN[2 * j], v.aN[2 * j + 1]}, bNj[2] = {v.bN[2 * j], v.bN[2 * j + 1]};
+        double aDj[3][2], bDj[3][2];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            aDj[t][0] = v.aD[2 * vj[t]]; aDj[t][1] = v.aD[2 * vj[t] + 1];
+            bDj[t][0] = v.bD[2 * vj[t]]; bDj[t][1] = v.bD[2 * vj[t] + 1];
+        }
+        const int i_end = min(nf, i0 + DTI);
+        for (int i = i0; i < i_end; ++i) {
+            if (j <= i || shares_vertex(p.tris, i, j)) continue;
+            const double* __restrict__ PU = p.pu + (int64_t)i * q2 * 4;
+            const double mix = p.normal[3 * i], miy = p.normal[3 * i + 1], miz = p.normal[3 * i + 2];
+            const double pun = PU[0] * mix + PU[1] * miy + PU[2] * miz;
+            double vr = 0, vi = 0, kar[3], kai[3], kbr[3], kbi[3], wr[3][3], wi[3][3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                kar[s] = kai[s] = kbr[s] = kbi[s] = 0;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) wr[s][t] = wi[s][t] = 0;
+            }
+            for (int b = 0; b < q2; ++b) {
+                const double ex = -PL[((int64_t)b * 4 + 0) * nf + j];
+                const double ey = -PL[((int64_t)b * 4 + 1) * nf + j];
+                const double ez = -PL[((int64_t)b * 4 + 2) * nf + j];
+                const double wb = PL[((int64_t)b * 4 + 3) * nf + j];
+                const double dni = pun + ex * mix + ey * miy + ez * miz;
+                double pvr = 0, pvi = 0, pkr = 0, pki = 0, pwr[3], pwi[3], qkr[3], qki[3];
+#pragma unroll
+                for (int s = 0; s < 3; ++s) pwr[s] = pwi[s] = qkr[s] = qki[s] = 0;
+                for (int a = 0; a < q2; ++a) {
+                    const double dx = PU[4 * a + 0] + ex;
+                    const double dy = PU[4 * a + 1] + ey;
+                    const double dz = PU[4 * a + 2] + ez;
+                    const double wa = PU[4 * a + 3];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double rinv = rsqrt_f64(r2);
+                    const double kr = kap * (r2 * rinv);
+                    double sn, cs;
+                    sincos_f64_tab(kr, sct, &sn, &cs);
+                    const double wd = wa * dkap;
+                    const double f1r = -wd * sn, f1i = wd * cs;  // dG/dk (times 4 pi / w_b)
+                    pvr += f1r; pvi += f1i;
+                    const double g = wa * rinv;
+                    const double hr = dk2n * g * cs + k2n * f1r, hi = dk2n * g * sn + k2n * f1i;
+                    const double gk = g * kd;
+                    const double bj = gk * (dx * nnx + dy * nny + dz * nnz);  // (x - y).n_j
+                    const double bi_ = -gk * dni;                              // (y - x).n_i
+                    pkr += bj * cs; pki += bj * sn;
+                    const double tr = bi_ * cs, tim = bi_ * sn;
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        const double ph = PHI[3 * a + s];
+                        pwr[s] += hr * ph; pwi[s] += hi * ph;
+                        qkr[s] += tr * ph; qki[s] += tim * ph;
+                    }
+                }
+                vr += pvr * wb; vi += pvi * wb;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const double wt = wb * PHI[3 * b + t];
+                    kar[t] += pkr * wt; kai[t] += pki * wt;
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) { wr[s][t] += pwr[s] * wt; wi[s][t] += pwi[s] * wt; }
+                }
+#pragma unroll
+                for (int s = 0; s < 3; ++s) { kbr[s] += qkr[s] * wb; kbi[s] += qki[s] * wb; }
+            }
+            // contraction of the pair in both orders
+            const int vi_[3] = {p.tris[3 * i], p.tris[3 * i + 1], p.tris[3 * i + 2]};
+            const double aNi[2] = {v.aN[2 * i], v.aN[2 * i + 1]}, bNi[2] = {v.bN[2 * i], v.bN[2 * i + 1]};
+            double aDi[3][2], bDi[3][2];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                aDi[s][0] = v.aD[2 * vi_[s]]; aDi[s][1] = v.aD[2 * vi_[s] + 1];
+                bDi[s][0] = v.bD[2 * vi_[s]]; bDi[s][1] = v.bD[2 * vi_[s] + 1];
+            }
+            const double dvr = kInv4Pi * vr, dvi = kInv4Pi * vi;
+            // c0: aN_i dV bN_j + aN_j dV bN_i
+            double xr = 0, xi = 0;
+            cmac(xr, xi, aNi[0], aNi[1], bNj[0], bNj[1]);
+            cmac(xr, xi, aNj[0], aNj[1], bNi[0], bNi[1]);
+            cmac(c[0], c[1], dvr, dvi, xr, xi);
+            // c1, c2: row i against panel j's vertices, row j against panel i's
+            double s1r = 0, s1i = 0, s2r = 0, s2i = 0, u1r = 0, u1i = 0, u2r = 0, u2i = 0;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                cmac(s1r, s1i, kar[t], kai[t], bDj[t][0], bDj[t][1]);
+                cmac(s2r, s2i, kar[t], kai[t], aDj[t][0], aDj[t][1]);
+                cmac(u1r, u1i, kbr[t], kbi[t], bDi[t][0], bDi[t][1]);
+                cmac(u2r, u2i, kbr[t], kbi[t], aDi[t][0], aDi[t][1]);
+            }
+            double yr = 0, yi = 0, zr = 0, zi = 0;
+            cmac(yr, yi, aNi[0], aNi[1], s1r, s1i);
+            cmac(yr, yi, aNj[0], aNj[1], u1r, u1i);
+            cmac(zr, zi, bNi[0], bNi[1], s2r, s2i);
+            cmac(zr, zi, bNj[0], bNj[1], u2r, u2i);
+            c[2] += kInv4Pi * yr; c[3] += kInv4Pi * yi;
+            c[4] += kInv4Pi * zr; c[5] += kInv4Pi * zi;
+            // c3: sum_st dW[s][t] (aD_is bD_jt + aD_jt bD_is)
+            const double nxy = mix * nnx + miy * nny + miz * nnz;
+            double wsr = 0, wsi = 0;
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const double cc = p.curl[9 * i + 3 * s] * p.curl[9 * j + 3 * t] +
+                                      p.curl[9 * i + 3 * s + 1] * p.curl[9 * j + 3 * t + 1] +
+                                      p.curl[9 * i + 3 * s + 2] * p.curl[9 * j + 3 * t + 2];
+                    const double er = cc * vr - nxy * wr[s][t], ei = cc * vi - nxy * wi[s][t];
+                    double mr = 0, mi = 0;
+                    cmac(mr, mi, aDi[s][0], aDi[s][1], bDj[t][0], bDj[t][1]);
+                    cmac(mr, mi, aDj[t][0], aDj[t][1], bDi[s][0], bDi[s][1]);
+                    cmac(wsr, wsi, er, ei, mr, mi);
+                }
+            c[6] += kInv4Pi * wsr; c[7] += kInv4Pi * wsi;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double t = wave_sum(c[q]);
+        if (lane == 0) red[wid][q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8)
+        part[(int64_t)blockIdx.x * 8 + threadIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// out[0..8) = the sums of part[nparts][8] (fixed order) plus the near-field
+// dots x_q . y_q (complex, plain) of the four forms; one block.
+struct DkNear {
+    const double* x[4];  // complex vectors
+    const double* y[4];
+    int64_t n[4];
+};
+__global__ __launch_bounds__(256) void dk_finish_kernel(const double* part, int64_t nparts, DkNear nr, double* out) {
+    __shared__ double red[4][8];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = 0.0;
+    for (int64_t b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] += part[b * 8 + q];
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+        for (int64_t e = threadIdx.x; e < nr.n[f]; e += blockDim.x)
+            cmac(c[2 * f], c[2 * f + 1], nr.x[f][2 * e], nr.x[f][2 * e + 1], nr.y[f][2 * e], nr.y[f][2 * e + 1]);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double t = wave_sum(c[q]);
+        if (lane == 0) red[wid][q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) out[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
 // Zeroes row blockIdx.x of a matrix: row_bytes bytes at out + row * ld_bytes
 // (16-byte stores when the rows are 16-byte aligned, else 4-byte ones).
 __global__ __launch_bounds__(256) void zero_rows_kernel(char* out, int64_t row_bytes, int64_t ld_bytes, int wide) {
@@ -1604,6 +1810,57 @@ int assemble_vkw(Context* ctx, const bem_op_desc* d, void* V, int64_t ldv, void*
         rc = check_hip(hipGetLastError(), "near-field V/K/W launch");
     }
     return rc;
+}
+
+int vkw_dk_forms(Context* ctx, const bem_op_desc* d, const void* aN, const void* bN, const void* aD,
+                 const void* bD, double* out, hipStream_t s) {
+    int rc = validate(ctx, d);
+    if (rc) return rc;
+    if (!aN || !bN || !aD || !bD || !out) return set_error(BEM_ERR_INVALID, "null argument");
+    if (d->op != BEM_OP_V || d->deg_test != 0 || d->deg_trial != 0)
+        return set_error(BEM_ERR_INVALID, "the k-derivative forms take the P0 single-layer descriptor");
+    if (d->kernel == BEM_LAPLACE) return set_error(BEM_ERR_INVALID, "the Laplace kernel does not depend on k");
+    if (d->precision != BEM_PREC_FP64 || d->dtype != BEM_C128)
+        return set_error(BEM_ERR_UNSUPPORTED, "k-derivative forms run in FP64 on complex128 vectors");
+    rc = mesh_build_near(ctx, d->flags & ~BEM_FLAG_DK, s);
+    if (rc) return rc;
+    Mesh& m = ctx->mesh;
+    const int nf = m.nf, nv = m.nv, njb = (nf + 255) / 256, ngroups = (nf + 255) / 256;
+    const int64_t nblocks = (int64_t)DTG * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
+    const size_t ylen = 2 * ((size_t)3 * nf + nv);  // doubles of the near-field vectors
+    const size_t bytes = ((size_t)nblocks * 8 + ylen) * sizeof(double);
+    if (m.kpart_bytes < bytes) {
+        if (m.kpart) hipFree(m.kpart);
+        m.kpart = nullptr;
+        m.kpart_bytes = 0;
+        rc = check_hip(hipMalloc((void**)&m.kpart, bytes), "k-derivative partials");
+        if (rc) return rc;
+        m.kpart_bytes = bytes;
+    }
+    double* part = m.kpart;
+    double* y = part + (size_t)nblocks * 8;
+    double* yv[4] = {y, y + 2 * (size_t)nf, y + 4 * (size_t)nf, y + 6 * (size_t)nf};
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ylen + 255) / 256)), dim3(256), 0, s, y, (int64_t)ylen);
+    OpParams p = make_params(ctx, d);
+    DkVecs vv{(const double*)aN, (const double*)bN, (const double*)aD, (const double*)bD};
+    hipLaunchKernelGGL(far_vkw_dk_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, p, vv, njb, ngroups, part);
+    rc = check_hip(hipGetLastError(), "far-field k-derivative forms launch");
+    if (rc) return rc;
+    // near field: the matrix-free k-derivative products, then plain dots
+    const int ops[4][3] = {{BEM_OP_V, 0, 0}, {BEM_OP_K, 0, 1}, {BEM_OP_K, 0, 1}, {BEM_OP_W, 1, 1}};
+    const void* xin[4] = {bN, bD, aD, bD};
+    for (int f = 0; f < 4; ++f) {
+        bem_op_desc dn = *d;
+        dn.op = ops[f][0]; dn.deg_test = ops[f][1]; dn.deg_trial = ops[f][2];
+        dn.flags = d->flags | BEM_FLAG_DK; dn.parts = BEM_PART_NEAR;
+        rc = apply_op(ctx, &dn, xin[f], 0, yv[f], 1.0, 0.0, s);
+        if (rc) return rc;
+    }
+    DkNear nr{{(const double*)aN, (const double*)aN, (const double*)bN, (const double*)aD},
+              {yv[0], yv[1], yv[2], yv[3]},
+              {nf, nf, nf, nv}};
+    hipLaunchKernelGGL(dk_finish_kernel, dim3(1), dim3(256), 0, s, (const double*)part, nblocks, nr, out);
+    return check_hip(hipGetLastError(), "k-derivative forms reduction launch");
 }
 
 int mass_assemble(Context* ctx, int deg_test, int deg_trial, int dtype, void* out, int64_t ld,

@@ -159,6 +159,13 @@ int bem_assemble_vkw(bem_context* ctx, const bem_op_desc* op, void* V, int64_t l
     });
 }
 
+int bem_vkw_dk_forms(bem_context* ctx, const bem_op_desc* op, const void* aN, const void* bN, const void* aD,
+                     const void* bD, double* out, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::vkw_dk_forms(&ctx->impl, op, aN, bN, aD, bD, out, as_stream(stream));
+    });
+}
+
 int bem_mass_assemble(bem_context* ctx, int32_t deg_test, int32_t deg_trial, int32_t dtype, void* out,
                       int64_t ld, double alpha, int32_t transpose, void* stream) {
     return with_device(ctx, [&]() -> int {

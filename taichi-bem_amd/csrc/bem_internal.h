@@ -113,6 +113,8 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
                        hipStream_t s);
 int assemble_vkw(Context* ctx, const bem_op_desc* d, void* V, int64_t ldv, void* K, int64_t ldk, void* W,
                  int64_t ldw, hipStream_t s);
+int vkw_dk_forms(Context* ctx, const bem_op_desc* d, const void* aN, const void* bN, const void* aD,
+                 const void* bD, double* out, hipStream_t s);
 // one problem of a combined-field batch (bem_cf_problem with the context resolved)
 struct CfInput {
     Context* ctx;

@@ -301,3 +301,56 @@ def test_transmission_k_gradient_matches_fd(bem, quadrature):
     fd = (J(ki + h) - J(ki - h)) / (2 * h)
     print(f"[transmission dk] adjoint {got:.12e} fd {fd:.12e}", flush=True)
     assert abs(got - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("flags", [0, 3])
+@pytest.mark.parametrize("kernel,k,sqrt_n", [(1, 3.0, 1.0), (2, 5.0, 1.5)])
+def test_vkw_dk_forms_match_products(bem, kernel, k, sqrt_n, flags):
+    """The fused k-derivative forms (one evaluation per unordered point pair,
+    contracted in both orders) equal dots with the matrix-free BEM_FLAG_DK
+    products of V, K and W, which are themselves checked against finite
+    differences of assembled matrices."""
+    v, f = bem.meshes.perturbed_icosphere(2, seed=4)
+    ctx = bem.Context(0)
+    ctx.set_mesh(torch.tensor(v, device=DEV), torch.tensor(f, device=DEV), 5)
+    nv, nf = len(v), len(f)
+    rng = np.random.default_rng(11)
+
+    def cvec(n):
+        return torch.tensor(rng.standard_normal(n) + 1j * rng.standard_normal(n), device=DEV)
+
+    aN, bN, aD, bD = cvec(nf), cvec(nf), cvec(nv), cvec(nv)
+    L = bem._lib
+    got = ctx.vkw_dk_forms(bem.Context.op_desc(0, kernel, k, sqrt_n, 0, 0, L.PREC_FP64, L.C128, flags),
+                           aN, bN, aD, bD).cpu().numpy()
+
+    def dprod(op, dt, dr, x, n):
+        y = torch.zeros(n, dtype=torch.complex128, device=DEV)
+        ctx.apply(bem.Context.op_desc(op, kernel, k, sqrt_n, dt, dr, L.PREC_FP64, L.C128, flags | L.FLAG_DK), x, y,
+                  alpha=1.0, conj_x=False)
+        return y
+
+    ref = np.array([
+        (aN * dprod(L.OP_V, 0, 0, bN, nf)).sum().item(),
+        (aN * dprod(L.OP_K, 0, 1, bD, nf)).sum().item(),
+        (bN * dprod(L.OP_K, 0, 1, aD, nf)).sum().item(),
+        (aD * dprod(L.OP_W, 1, 1, bD, nv)).sum().item(),
+    ])
+    err = np.abs(got - ref) / np.abs(ref)
+    print(f"[dk forms] kernel={kernel} flags={flags} rel errors {err}", flush=True)
+    assert err.max() <= 1e-12
+
+
+def test_transmission_dA_contract_fused_equals_products(bem):
+    v, f = bem.meshes.perturbed_icosphere(2, seed=0)
+    cm = transmission_manager(bem, v, f, 5.0, 2.25)
+    cm.run()
+    bm = cm._BEM_manager
+    n = bm.N_Neumann + bm.M_Dirichlet
+    rng = np.random.default_rng(5)
+    mu = torch.tensor(rng.standard_normal(n) + 1j * rng.standard_normal(n), device=DEV)
+    fused = bem.adjoint.transmission_dA_contract(bm, mu, bm.raw_solved)
+    bm.fuse_layers = False
+    sep = bem.adjoint.transmission_dA_contract(bm, mu, bm.raw_solved)
+    print(f"[dA contract] fused {fused} separate {sep}", flush=True)
+    assert abs(fused - sep) <= 1e-12 * abs(sep)
