@@ -292,9 +292,13 @@ __global__ __launch_bounds__(256) void far_assemble_kernel(OpParams p, int njb) 
 // evaluates its second term f(y, x) with the same bases (reference,
 // single_layer_3d.py:198-206) unless BEM_FLAG_SYM_COINCIDE.
 // ---------------------------------------------------------------------------
+// near_pair's pseudo-operator for the combined field K - i eta V (P1 x P1,
+// Helmholtz): one kernel evaluation per rule point serves both operators.
+constexpr int kOpCf = 16;
+
 template <int OP, bool HELM, int DT, int DR, bool DK = false>
 __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np, int lane,
-                                          double (&vr)[3][3], double (&vi)[3][3]) {
+                                          double (&vr)[3][3], double (&vi)[3][3], double eta = 0.0) {
     constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
     const int i = np.i, j = np.j, rel = np.rel;
     int px[3], py[3], invx[3], invy[3];
@@ -351,7 +355,7 @@ __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np,
             double fr, fi, f1r = 0.0, f1i = 0.0;
             if constexpr (DK && (OP == BEM_OP_V || OP == BEM_OP_W)) {
                 double sn, cs;
-                sincos_f64(kap * (r2 * rinv), &sn, &cs);
+                Math<double>::sincos(kap * (r2 * rinv), &sn, &cs);
                 f1r = -sn * p.dkap; f1i = cs * p.dkap;
                 if constexpr (OP == BEM_OP_W) {
                     fr = p.dk2n * rinv * cs + p.k2n * f1r;
@@ -361,13 +365,22 @@ __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np,
                 const double dn = (OP == BEM_OP_K) ? (dx * ny0 + dy * ny1 + dz * ny2)
                                                    : -(dx * nx0 + dy * nx1 + dz * nx2);
                 double sn, cs;
-                sincos_f64(kap * (r2 * rinv), &sn, &cs);
+                Math<double>::sincos(kap * (r2 * rinv), &sn, &cs);
                 const double base = dn * rinv * kap * p.dkap;
                 fr = base * cs; fi = base * sn;
+            } else if constexpr (OP == kOpCf) {
+                // K - i eta V from one evaluation: K (dn rinv^3)(cs + kr sn, sn - kr cs), V rinv (cs, sn)
+                const double dn = dx * ny0 + dy * ny1 + dz * ny2;
+                const double kr = kap * (r2 * rinv);
+                double sn, cs;
+                Math<double>::sincos(kr, &sn, &cs);
+                const double base = dn * (rinv * rinv * rinv);
+                fr = base * (cs + kr * sn) + eta * (rinv * sn);
+                fi = base * (sn - kr * cs) - eta * (rinv * cs);
             } else if constexpr (OP == BEM_OP_V || OP == BEM_OP_W) {
                 if constexpr (HELM) {
                     double sn, cs;
-                    sincos_f64(kap * (r2 * rinv), &sn, &cs);
+                    Math<double>::sincos(kap * (r2 * rinv), &sn, &cs);
                     fr = rinv * cs; fi = rinv * sn;
                 } else { fr = rinv; fi = 0.0; }
             } else {
@@ -377,7 +390,7 @@ __device__ __forceinline__ void near_pair(const OpParams& p, const NearPair& np,
                 if constexpr (HELM) {
                     const double kr = kap * (r2 * rinv);
                     double sn, cs;
-                    sincos_f64(kr, &sn, &cs);
+                    Math<double>::sincos(kr, &sn, &cs);
                     fr = base * (cs + kr * sn); fi = base * (sn - kr * cs);
                 } else { fr = base; fi = 0.0; }
             }
@@ -1313,22 +1326,6 @@ __device__ __forceinline__ void far_pair_cf_sym(const OpParams& p, double eta, i
         }
 }
 
-// K - i eta V into A at (tri_i[s], tri_j[t])
-__device__ __forceinline__ void cf_scatter(const OpParams& p, double eta, int i, int j, const double (&Vr)[3][3],
-                                           const double (&Vi)[3][3], const double (&Kr)[3][3],
-                                           const double (&Ki)[3][3]) {
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        const int64_t row = p.tris[3 * i + s];
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            const int64_t col = p.tris[3 * j + t];
-            // -i eta (Vr + i Vi) = eta Vi - i eta Vr
-            add_elem(p.out, p.dtype, row * p.ld + col, Kr[s][t] + eta * Vi[s][t], Ki[s][t] - eta * Vr[s][t], true);
-        }
-    }
-}
-
 // Far item -> (test group ib, trial block jb) over the upper triangle: the
 // test groups of 256-row group gr (256 / TI of them) pair with trial blocks
 // gr .. njb-1, and a lane only evaluates pairs with j > i.
@@ -1469,10 +1466,13 @@ __global__ __launch_bounds__(256) void cf_near_kernel(const CfProb* tbl, int cou
         const int q = cf_find(tbl, count, item, 1);
         const CfProb& P = tbl[q];
         const NearPair np = P.p.near[item - P.near_begin];
-        double Vr[3][3], Vi[3][3], Kr[3][3], Ki[3][3];
-        near_pair<BEM_OP_V, true, 1, 1>(P.p, np, lane, Vr, Vi);
-        near_pair<BEM_OP_K, true, 1, 1>(P.p, np, lane, Kr, Ki);
-        if (lane == 0) cf_scatter(P.p, P.eta, np.i, np.j, Vr, Vi, Kr, Ki);
+        double Cr[3][3], Ci[3][3];
+        near_pair<kOpCf, true, 1, 1>(P.p, np, lane, Cr, Ci, P.eta);
+        if (lane == 0)
+            for (int s = 0; s < 3; ++s)
+                for (int t = 0; t < 3; ++t)
+                    add_elem(P.p.out, P.p.dtype, (int64_t)P.p.tris[3 * np.i + s] * P.p.ld + P.p.tris[3 * np.j + t],
+                             Cr[s][t], Ci[s][t], true);
     }
 }
 
