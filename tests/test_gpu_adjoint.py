@@ -354,3 +354,27 @@ def test_transmission_dA_contract_fused_equals_products(bem):
     sep = bem.adjoint.transmission_dA_contract(bm, mu, bm.raw_solved)
     print(f"[dA contract] fused {fused} separate {sep}", flush=True)
     assert abs(fused - sep) <= 1e-12 * abs(sep)
+
+
+def test_vkw_dk_forms_octahedron(bem):
+    """Eight panels: four regular pairs, most tiles of the pass empty."""
+    v = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=np.float64)
+    f = np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]],
+                 dtype=np.int32)
+    ctx = bem.Context(0)
+    ctx.set_mesh(torch.tensor(v, device=DEV), torch.tensor(f, device=DEV), 5)
+    rng = np.random.default_rng(12)
+    vec = [torch.tensor(rng.standard_normal(n) + 1j * rng.standard_normal(n), device=DEV) for n in (8, 8, 6, 6)]
+    L = bem._lib
+    got = ctx.vkw_dk_forms(bem.Context.op_desc(0, 1, 2.5, 1.0, 0, 0, L.PREC_FP64, L.C128, 0), *vec).cpu().numpy()
+
+    def dprod(op, dt, dr, x, n):
+        y = torch.zeros(n, dtype=torch.complex128, device=DEV)
+        ctx.apply(bem.Context.op_desc(op, 1, 2.5, 1.0, dt, dr, L.PREC_FP64, L.C128, L.FLAG_DK), x, y, alpha=1.0,
+                  conj_x=False)
+        return y
+
+    aN, bN, aD, bD = vec
+    ref = np.array([(aN * dprod(L.OP_V, 0, 0, bN, 8)).sum().item(), (aN * dprod(L.OP_K, 0, 1, bD, 8)).sum().item(),
+                    (bN * dprod(L.OP_K, 0, 1, aD, 8)).sum().item(), (aD * dprod(L.OP_W, 1, 1, bD, 6)).sum().item()])
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()

@@ -626,3 +626,30 @@ def test_assemble_vkw_rejects(bem, meshes):
         ctx.assemble_vkw(bem.Context.op_desc(0, 1, 2.0, 1.0, 0, 0, 0, 0, bem._lib.FLAG_DK), V, K, W)
     with pytest.raises(bem.BEMError):
         ctx.assemble_vkw(bem.Context.op_desc(0, 1, 2.0, 1.0, 0, 0, 0, 0, 0), V, K, W[:, : nv - 1])
+
+
+def _octahedron():
+    v = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=np.float64)
+    f = np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]],
+                 dtype=np.int32)
+    return v, f
+
+
+def _two_triangles():
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0.2]], dtype=np.float64)
+    f = np.array([[0, 1, 2], [1, 3, 2]], dtype=np.int32)
+    return v, f
+
+
+@pytest.mark.parametrize("mesh", ["two_triangles", "octahedron", "icosahedron"])
+def test_assemble_vkw_tiny_meshes(bem, oracle, mesh):
+    """Meshes with no or few regular pairs (most upper-triangle tiles empty)."""
+    v, f = {"two_triangles": _two_triangles, "octahedron": _octahedron,
+            "icosahedron": lambda: bem.meshes.icosphere(0)}[mesh]()
+    ctx = make_ctx(bem, v, f)
+    nv, nf = len(v), len(f)
+    outs = [torch.zeros(r, c, dtype=torch.complex128, device=DEV) for r, c in ((nf, nf), (nf, nv), (nv, nv))]
+    ctx.assemble_vkw(bem.Context.op_desc(0, 2, 5.0, 1.5, 0, 0, 0, 0, 0), *outs)
+    for got, (op, a, b) in zip(outs, ((0, 0, 0), (1, 0, 1), (3, 1, 1))):
+        ref = oracle.assemble(oracle.make_op(op, 2, 5.0, 1.5, 5, a, b, 0), v, f)
+        assert rel(to_np(got), ref) <= TOL64
