@@ -1390,7 +1390,20 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const So
     double acc_r = 0.0, acc_i = 0.0;
     if (t < nb) { acc_r = x[2 * (r0 + t)]; acc_i = x[2 * (r0 + t) + 1]; }
     const int jfirst = LOWER ? 0 : nblk - 1, jstep = LOWER ? 1 : -1;
+    // partial products: without TRANS thread (row i = t / 4, quarter q) sums
+    // columns q, q + 4, ...; with TRANS wave q sums columns q, q + 4, ... for
+    // row i = lane (rows of the factor read contiguously)
+    const int pi_row = TRANS ? lane : t >> 2, pq = TRANS ? wave : t & 3;
     for (int j = jfirst; j != b; j += jstep) {
+        const int c0 = j * TB, nc = min(TB, n - c0);
+        // this thread's entries of M(b, j): the factor does not depend on x_j,
+        // so the loads are issued before the wait for block j
+        cx<T> mv[TB / 4];
+#pragma unroll
+        for (int u = 0; u < TB / 4; ++u) {
+            const int c = pq + 4 * u;
+            mv[u] = (pi_row < nb && c < nc) ? mel(pi_row, c0, c) : cx<T>{T(0), T(0)};
+        }
         if (t == 0) {
             const long long t0 = wall_clock64();
             while (__hip_atomic_load(&ready[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
@@ -1401,20 +1414,18 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const So
         }
         __syncthreads();
         if (s_fail) break;
-        const int c0 = j * TB, nc = min(TB, n - c0);
         if (t < nc) { xr[t] = ld_agent(&x[2 * (c0 + t)]); xi[t] = ld_agent(&x[2 * (c0 + t) + 1]); }
         __syncthreads();
-        // partial products: without TRANS thread (row i = t / 4, quarter q)
-        // sums columns q, q + 4, ...; with TRANS wave q sums columns q, q + 4,
-        // ... for row i = lane (rows of the factor read contiguously)
         double sr = 0.0, si = 0.0;
-        const int i = TRANS ? lane : t >> 2, q = TRANS ? wave : t & 3;
-        if (i < nb)
-            for (int c = q; c < nc; c += 4) {
-                const cx<T> m = mel(i, c0, c);
-                sr += (double)m.re * xr[c] - (double)m.im * xi[c];
-                si += (double)m.re * xi[c] + (double)m.im * xr[c];
+        const int i = pi_row, q = pq;
+#pragma unroll
+        for (int u = 0; u < TB / 4; ++u) {
+            const int c = q + 4 * u;
+            if (c < nc) {  // entries outside the block are zero
+                sr += (double)mv[u].re * xr[c] - (double)mv[u].im * xi[c];
+                si += (double)mv[u].re * xi[c] + (double)mv[u].im * xr[c];
             }
+        }
         if (!TRANS) {
             sr += __shfl_xor(sr, 1, 64); si += __shfl_xor(si, 1, 64);
             sr += __shfl_xor(sr, 2, 64); si += __shfl_xor(si, 2, 64);
