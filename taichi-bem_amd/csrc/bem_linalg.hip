@@ -2704,6 +2704,11 @@ int batch_plan(Context* ctx, int slot, int count, const void* const* LU, const i
         st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w, (unsigned*)p};
         plan.nmax = std::max(plan.nmax, n[q]);
     }
+    // The sweep kernels take the problems in table order (blockIdx.z), and
+    // workgroups are dispatched z by z: the largest problems go first, so
+    // their long chains start at once and the short ones fill in beside them.
+    // (Only the sweeps read this table; each entry carries its own vectors.)
+    std::stable_sort(st_h.begin(), st_h.end(), [](const SolveProb& x, const SolveProb& y) { return x.n > y.n; });
     plan.d_vec = (VecProb*)(base + bytes);
     plan.d_sp = (SolveProb*)(base + bytes + vt);
     std::vector<char> blob(vt + st, 0);
