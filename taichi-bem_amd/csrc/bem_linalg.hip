@@ -1845,7 +1845,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
 // the ones with columns left at a given step).
 template <typename T, bool C, bool B>
 void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int gopts,
-               int pgrid, hipStream_t s) {
+               int pgrid, hipStream_t s, int K_begin = 0, int K_end = 1 << 30) {
     const int nmax = ns[0];
     unsigned pbase = 0;
     auto count_active = [&](int from) {
@@ -1853,7 +1853,8 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
         while (a < (int)ns.size() && ns[a] > from) ++a;
         return a;
     };
-    for (int K0 = 0; K0 < nmax; K0 += W2) {
+    // outer blocks [K_begin, min(nmax, K_end)), K_begin a multiple of W2
+    for (int K0 = K_begin; K0 < std::min(nmax, K_end); K0 += W2) {
         const int K1max = std::min(nmax, K0 + W2);
         const int act_outer = count_active(K0);
         for (int k = K0; k < K1max; k += NB) {
@@ -2141,7 +2142,31 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
         ctx->lu_table_dev = (void*)d_tbl;
     }
     hipLaunchKernelGGL(lu_reset_kernel, dim3((count + 63) / 64), dim3(64), 0, s, (const LuProb*)d_tbl, count);
-    lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, gemm_opts(ctx), 0, s);
+    // Once only the largest problem is left (from the first outer block past
+    // the second-largest n), its remaining columns take the single-problem
+    // path: persistent partial-pivoting panels instead of the batch's panel
+    // launches, which for one problem are a dozen short launches per panel.
+    // Any pivot rule per panel gives a valid factorisation; partial pivoting
+    // is the stronger one.  (With persistent panels switched off the batch
+    // runs its own steps to the end.)
+    int K_switch = 1 << 30;
+    if (ctx->lu_panel != 0) {
+        const int second = count > 1 ? ns[1] : 0;
+        const int ks = (second + W2 - 1) / W2 * W2;
+        if (ns[0] > ks) K_switch = ks;
+    }
+    if (K_switch > 0) lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, gemm_opts(ctx), 0, s, 0, K_switch);
+    if (K_switch < ns[0]) {
+        if (ctx->num_cu <= 0) {
+            hipDeviceProp_t prop;
+            rc = check_hip(hipGetDeviceProperties(&prop, ctx->device), "device properties");
+            if (rc) return rc;
+            ctx->num_cu = prop.multiProcessorCount;
+        }
+        int pgrid = ctx->num_cu;
+        if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
+        lu_launch<T, C, false>(tbl[0], nullptr, std::vector<int>{ns[0]}, 0, gemm_opts(ctx), pgrid, s, K_switch);
+    }
     return check_hip(hipGetLastError(), "batched LU launch");
 }
 

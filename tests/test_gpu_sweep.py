@@ -97,6 +97,35 @@ def test_lu_batched_identical_to_single(bem, dtype):
         assert torch.equal(p, q)
 
 
+@pytest.mark.parametrize("pivoting", [0, 1])
+@pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-14), (torch.complex64, 2e-6), (torch.float64, 1e-14)])
+def test_lu_batched_lone_tail(bem, dtype, tol, pivoting):
+    """The largest problem outlives the others by several outer blocks, so
+    its last columns take the single-problem path (persistent partial-pivoting
+    panels) after the batch's steps: its factors must still solve the system."""
+    sizes = [1500, 300, 64, 1]
+    mats = rand_mats(sizes, dtype, 7)
+    ctx = bem.Context(0)
+    ctx.set_option(bem._lib.OPT_LU_PIVOTING, pivoting)
+    lus = [A.clone() for A in mats]
+    ipivs = [torch.empty(A.shape[0], dtype=torch.int32, device=DEV) for A in mats]
+    info = torch.full((len(mats),), -1, dtype=torch.int32, device=DEV)
+    print(f"[lu lone tail] dtype={dtype} pivoting={pivoting}", flush=True)
+    bem._lib.Batch(ctx).lu_factor(lus, ipivs, info)
+    torch.cuda.synchronize()
+    assert info.cpu().tolist() == [0] * len(mats)
+    rng = np.random.default_rng(8)
+    for A, LU, ip in zip(mats, lus, ipivs):
+        n = A.shape[0]
+        b = torch.tensor(rng.standard_normal(n), dtype=dtype, device=DEV)
+        x = b.clone()
+        ctx.lu_solve(LU, ip, x)
+        A64, x64, b64 = (t.to(torch.complex128) for t in (A, x, b))
+        r = (torch.linalg.vector_norm(A64 @ x64 - b64) /
+             (torch.linalg.matrix_norm(A64) * torch.linalg.vector_norm(x64))).item()
+        assert r <= tol, (n, r)
+
+
 def test_lu_batched_reports_singular(bem):
     mats = rand_mats([70, 40], torch.complex128, 2)
     mats[1][:, 5] = 0
