@@ -2,6 +2,8 @@
 the C ABI.  Bar: relative Frobenius error <= 1e-11 in FP64 (the tolerance the
 target states for this path); FP32 results are checked against stated,
 looser bounds."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -653,3 +655,56 @@ def test_assemble_vkw_tiny_meshes(bem, oracle, mesh):
     for got, (op, a, b) in zip(outs, ((0, 0, 0), (1, 0, 1), (3, 1, 1))):
         ref = oracle.assemble(oracle.make_op(op, 2, 5.0, 1.5, 5, a, b, 0), v, f)
         assert rel(to_np(got), ref) <= TOL64
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_goldens.npz")
+_GOPS = {"V": (0, 0, 0), "K": (1, 0, 1), "KADJ": (2, 1, 0), "W": (3, 1, 1)}
+_GKERNELS = {"lap": (0, 0.0, 1.0), "helm2": (1, 2.0, 1.0), "trans3": (2, 3.0, 1.5), "helm3": (1, 3.0, 1.0)}
+
+
+def test_library_matches_goldens(bem):
+    """Every committed fixture (two-triangle pairs of each relation and
+    vertex order, octahedron, icosahedron, L1) through bem_assemble."""
+    d = np.load(GOLDEN)
+    meshes = sorted({k[: -len("_verts")] for k in d.files if k.endswith("_verts")})
+    worst, checked = 0.0, 0
+    for m in meshes:
+        v, f = d[m + "_verts"], d[m + "_tris"]
+        ctx = make_ctx(bem, v, f)
+        for key in d.files:
+            if not key.startswith(m + "_"):
+                continue
+            parts = key[len(m) + 1:].split("_")
+            if len(parts) != 3 or parts[0] not in _GKERNELS:
+                continue
+            kernel, k, sn = _GKERNELS[parts[0]]
+            op, dt, dr = _GOPS[parts[2]]
+            ref = d[key]
+            out = torch.zeros(ref.shape, dtype=torch.float64 if kernel == 0 else torch.complex128, device=DEV)
+            ctx.assemble(bem.Context.op_desc(op, kernel, k, sn, dt, dr, bem._lib.PREC_FP64, 0, int(parts[1][1:])), out)
+            err = rel(to_np(out), ref.real if kernel == 0 else ref)
+            worst = max(worst, err)
+            assert err <= TOL64, key
+            checked += 1
+    print(f"[goldens] {checked} operators, worst rel Frobenius error {worst:.2e}", flush=True)
+    assert checked >= 250
+
+
+@pytest.mark.parametrize("mesh", ["ico0", "ico1"])
+def test_config1_matches_goldens(bem, mesh):
+    """Config 1 (Laplace interior Dirichlet) against the committed
+    right-hand side and solution: the fused Dirichlet pass, LU and solve."""
+    d = np.load(GOLDEN)
+    v, f = d[mesh + "_verts"], d[mesh + "_tris"]
+    ctx = make_ctx(bem, v, f)
+    nf = len(f)
+    g = torch.tensor(1.0 / np.linalg.norm(v - np.array([2.0, 0.5, 0.3]), axis=1), device=DEV)
+    A = torch.zeros(nf, nf, dtype=torch.float64, device=DEV)
+    b = torch.zeros(nf, dtype=torch.float64, device=DEV)
+    ctx.assemble_dirichlet(bem.Context.op_desc(0, 0, 0.0, 1.0, 0, 0, bem._lib.PREC_FP64, 0, 0), A, g, b,
+                           alpha=1.0, alpha_k=1.0, alpha_m=0.5, conj_g=True)
+    assert rel(to_np(b), d[mesh + "_config1_rhs"]) <= TOL64
+    ipiv = torch.empty(nf, dtype=torch.int32, device=DEV)
+    ctx.lu_factor(A, ipiv)
+    ctx.lu_solve(A, ipiv, b)
+    assert rel(to_np(b), d[mesh + "_config1_solution"]) <= 1e-10

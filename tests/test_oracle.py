@@ -4,6 +4,7 @@ counts, exact mass integrals, the Gauss solid-angle identity, the curl of a
 constant, sphere eigenvalues of V/K/W, and an independent closed-form
 triangle potential for the singular (Sauter-Schwab) rules."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -296,3 +297,37 @@ def test_config1_quadrature_presets(bem, oracle):
             errs[(flags, L)] = rel(t, texact)
     assert errs[(3, 2)] < errs[(3, 1)] / 2.5
     assert errs[(0, 2)] > 0.15
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_goldens.npz")
+_OPS = {"V": (0, 0, 0), "K": (1, 0, 1), "KADJ": (2, 1, 0), "W": (3, 1, 1)}
+_KERNELS = {"lap": (0, 0.0, 1.0), "helm2": (1, 2.0, 1.0), "trans3": (2, 3.0, 1.5), "helm3": (1, 3.0, 1.0)}
+
+
+def test_oracle_reproduces_goldens(oracle):
+    """The committed fixtures (tests/golden/make_golden.py) are what the
+    oracle computes today: a change of the restatement shows up here."""
+    d = np.load(GOLDEN)
+    meshes = sorted({k[: -len("_verts")] for k in d.files if k.endswith("_verts")})
+    checked = 0
+    for m in meshes:
+        v, f = d[m + "_verts"], d[m + "_tris"]
+        for key in d.files:
+            if not key.startswith(m + "_"):
+                continue
+            parts = key[len(m) + 1:].split("_")
+            if len(parts) != 3 or parts[0] not in _KERNELS:
+                continue
+            kernel, k, sn = _KERNELS[parts[0]]
+            op, dt, dr = _OPS[parts[2]]
+            got = oracle.assemble(oracle.make_op(op, kernel, k, sn, 5, dt, dr, int(parts[1][1:])), v, f)
+            ref = d[key]
+            assert np.abs(got - ref).max() <= 1e-14 * max(np.abs(ref).max(), 1e-300), key
+            checked += 1
+    assert checked >= 250
+    v, f = d["ico1_verts"], d["ico1_tris"]
+    assert np.abs(oracle.mass(5, 0, 1, v, f) - d["ico1_mass_p0p1"]).max() <= 1e-16
+    for name, rel_expected in (("edge", oracle.COMMON_EDGE), ("vertex", oracle.COMMON_VERTEX),
+                               ("separate", oracle.SEPARATE)):
+        for rot in range(3):
+            assert oracle.relation(d[f"tri_{name}_r{rot}_tris"], 0, 1) == rel_expected
