@@ -708,3 +708,41 @@ def test_config1_matches_goldens(bem, mesh):
     ctx.lu_factor(A, ipiv)
     ctx.lu_solve(A, ipiv, b)
     assert rel(to_np(b), d[mesh + "_config1_solution"]) <= 1e-10
+
+
+def test_helmholtz_neumann_pipeline(bem, oracle):
+    """Exterior Neumann problem through the reference-shaped managers (L2,
+    Helmholtz k = 2): A = W at the Dirichlet block and rhs = -1/2 M' fhat -
+    K' fhat (BEM_manager.py:608-628, rhs_constructor_3d.py:374-386), fhat =
+    conj(f), against the oracle's pieces; then the solve."""
+    k = 2.0
+    v, f = bem.meshes.perturbed_icosphere(2, seed=5)
+    x0 = torch.tensor([0.1, 0.2, -0.1], dtype=torch.float64, device=DEV)
+
+    def fD(x, sqrt_n=1.0):
+        r = torch.linalg.norm(x - x0, dim=-1)
+        return torch.exp(1j * k * r) / r
+
+    def fN(x, n, sqrt_n=1.0):
+        d = x - x0
+        r = torch.linalg.norm(d, dim=-1)
+        return (1j * k * r - 1.0) * torch.exp(1j * k * r) / r ** 3 * (d * n).sum(-1)
+
+    params = {"kernel": "Helmholtz", "k": k, "boundary": "Neumann", "scope": "Exterior", "GaussQR": 5,
+              "Q_Neumann": 0, "Q_Dirichlet": 1, "vertices": v, "panels": f}
+    cm = bem.CoreManager(params)
+    cm.initialization(fD, fN)
+    cm.run()
+    bm = cm._BEM_manager
+    assert bm.N_Neumann == 0 and bm.M_Dirichlet == len(v)
+    W = oracle.assemble(oracle.make_op(3, 1, k, 1.0, 5, 1, 1, 0), v, f)
+    Kadj = oracle.assemble(oracle.make_op(2, 1, k, 1.0, 5, 1, 0, 0), v, f)
+    Mp = oracle.mass(5, 1, 0, v, f)
+    area, normal = oracle.geometry(v, f)[:2]
+    cen = v[f].mean(axis=1)
+    fn = to_np(fN(torch.tensor(cen, device=DEV), torch.tensor(normal, device=DEV)))
+    rhs = -0.5 * (Mp @ fn.conj()) - Kadj @ fn.conj()
+    assert rel(to_np(bm.get_mat_A()), W) <= TOL64
+    assert rel(to_np(bm.rhs), rhs) <= TOL64
+    u = np.linalg.solve(W, rhs)
+    assert rel(to_np(bm.raw_solved), u) <= 1e-10
