@@ -262,8 +262,11 @@ enum {
                                 chooses; 1 = tournament pivoting per 64-column
                                 panel (pivot rows chosen by a reduction tree
                                 of LDS-resident eliminations, about ten
-                                launches per panel): faster, but its element
-                                growth can be larger, so use it for
+                                launches per panel): fewer synchronisations
+                                per panel, but slower than the persistent
+                                partial-pivoting panels of single
+                                factorisations, and its element growth can be
+                                larger, so use it for batches of
                                 well-conditioned systems only */,
     BEM_OPT_LU_GEMM_TILE = 4 /* 1 (default) = trailing updates of at least
                                 256 rows use 128-row MFMA tiles: 128 x 64 for
@@ -278,7 +281,15 @@ enum {
                                 unit of the panel's stream, grid barriers,
                                 8-column sub-panels in LDS) where the rows
                                 fit; N > 1 = the same with at most N
-                                workgroups; 0 = one launch per column */
+                                workgroups; 0 = one launch per column */,
+    BEM_OPT_WAIT_LIMIT_US = 6 /* bound, in microseconds, of every wait of a
+                                kernel on other workgroups (the persistent
+                                panel's grid barriers, the triangular sweeps'
+                                ready flags): 0..2,000,000, default 2,000,000.
+                                A wait that runs out makes bem_lu_factor
+                                return BEM_ERR_HIP and a solve leave NaN, and
+                                every kernel still ends.  Lowering it is a
+                                diagnostic for those paths */
 };
 
 /* Sets a context option.  No reference counterpart. */

@@ -23,7 +23,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE, FLAG_DK = 1, 2, 4
 PART_FAR, PART_NEAR = 1, 2
 KernelType_LAPLACE, KernelType_HELMHOLTZ, KernelType_TRANSMISSION = 0, 1, 2
-OPT_C64_LU_GEMM, OPT_LU_LOOKAHEAD, OPT_LU_PIVOTING, OPT_LU_GEMM_TILE, OPT_LU_PANEL = 1, 2, 3, 4, 5
+OPT_C64_LU_GEMM, OPT_LU_LOOKAHEAD, OPT_LU_PIVOTING, OPT_LU_GEMM_TILE, OPT_LU_PANEL, OPT_WAIT_LIMIT_US = 1, 2, 3, 4, 5, 6
 
 # Every symbol include/bem_hip.h declares.
 EXPORTS = (

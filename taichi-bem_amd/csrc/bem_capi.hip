@@ -231,6 +231,11 @@ int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
                 return set_error(BEM_ERR_INVALID, "BEM_OPT_C64_LU_GEMM takes 0, 3 or 6");
             ctx->impl.c64_gemm_parts = value == 0 ? 0 : (value == 3 ? 2 : 3);
             return BEM_OK;
+        case BEM_OPT_WAIT_LIMIT_US:
+            if (value < 0 || value > 2000000)
+                return set_error(BEM_ERR_INVALID, "BEM_OPT_WAIT_LIMIT_US takes 0..2000000");
+            ctx->impl.wait_ticks = (long long)value * 100;  // wall clock of 100 MHz
+            return BEM_OK;
         default:
             return set_error(BEM_ERR_INVALID, "unknown option");
     }

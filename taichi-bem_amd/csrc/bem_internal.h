@@ -85,6 +85,9 @@ struct Context {
     int lu_lookahead_cus = 32;    // LU look-ahead: 0 off, > 0 panel-stream CUs, -1 unmasked streams
     int lu_pivoting = 0;          // LU panels: 0 partial pivoting (LAPACK rows), 1 tournament
     int lu_panel = 1;             // LU partial-pivoting panels: 1 persistent launch where it fits, 0 per column
+    // bound of the kernels' waits on other workgroups (grid barriers, ready flags),
+    // wall-clock ticks of 100 MHz: 2 s by default
+    long long wait_ticks = 200000000;
     int lu_gemm_tile = 1;         // LU trailing updates: 1 128-row tiles where they fit, 0 64 x 64 only
     int lu_streams_cus = 0;       // the setting the streams were created for
     hipStream_t lu_stream[2] = {nullptr, nullptr};

@@ -87,6 +87,7 @@ struct LuProb {
     int* tids;   // tournament candidates: two levels of ceil(n / TRW) groups x NB row ids
     void* SU;    // -inv(U11) of the current panel (NB x NB)
     void* pan;   // exchange area of the persistent panel kernel (PanelX<T>)
+    long long wait;  // bound of every wait on other workgroups (wall-clock ticks, 100 MHz)
 };
 
 template <bool B>
@@ -111,8 +112,8 @@ __global__ void laswp_kernel(LuProb inl, const LuProb* tbl, int p_lo, int p_hi, 
     const int col = c_lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (p_lo >= p_hi || col >= c_hi || (col >= x_lo && col < x_hi)) return;
     for (int c = p_lo; c < p_hi; ++c) {
-        int p = ipiv[c];
-        if (p != c) {
+        const int p = ipiv[c];
+        if (p != c && p >= 0 && p < n) {  // rows outside the matrix (a failed panel) are skipped
             cx<T> a = Acc<T, C>::ld(A, (int64_t)c * ld + col);
             cx<T> b = Acc<T, C>::ld(A, (int64_t)p * ld + col);
             Acc<T, C>::st(A, (int64_t)c * ld + col, b);
@@ -287,8 +288,9 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
 // A panel costs NB + NB/SB - 1 barriers instead of NB + 1 dependent
 // launches, and its right part is read and written once per sub-panel
 // instead of once per column.  Every barrier ends in an agent-scope acquire,
-// so the data other workgroups stored before it are read with plain loads.  Every wait is bounded (about two seconds of
-// wall clock): a workgroup that gives up raises the shared fail word, every
+// so the data other workgroups stored before it are read with plain loads.
+// Every wait is bounded (two seconds of wall clock by default, the context's
+// wait limit): a workgroup that gives up raises the shared fail word, every
 // other one then leaves at its next wait, and info becomes -2.
 // ---------------------------------------------------------------------------
 constexpr int SB = 8;        // sub-panel width
@@ -308,7 +310,6 @@ struct PanelX {
     cx<T> rowc[2][SB];            // row c's sub-panel values, from its owner
 };
 
-constexpr long long BARRIER_TICKS = 200000000;  // wall-clock ticks (100 MHz): 2 s
 
 // Barrier over the workgroups of a persistent launch on a counter that only
 // grows during a factorisation: each arrival adds one (release: this
@@ -318,7 +319,7 @@ constexpr long long BARRIER_TICKS = 200000000;  // wall-clock ticks (100 MHz): 2
 // the workgroup's XCD on every iteration, beside the trailing-update
 // kernels running there.)  Returns false for every workgroup once any wait
 // has timed out.
-__device__ bool grid_barrier(unsigned* count, int* fail, unsigned target) {
+__device__ bool grid_barrier(unsigned* count, int* fail, unsigned target, long long ticks) {
     __shared__ int s_ok;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -327,8 +328,7 @@ __device__ bool grid_barrier(unsigned* count, int* fail, unsigned target) {
         const long long t0 = wall_clock64();
         while ((int)(__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
             __builtin_amdgcn_s_sleep(1);
-            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                wall_clock64() - t0 > BARRIER_TICKS) {
+            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || wall_clock64() - t0 > ticks) {
                 ok = 0;
                 break;
             }
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(PNT) void panel_persist_kernel(LuProb P, int k, int
                 if (c >= rb && c < re) X->rowc[par][t] = sub[c - rb][t];
             }
             target += (unsigned)G;
-            if (!grid_barrier(&X->count, &X->fail, target)) { fail_out(); return; }
+            if (!grid_barrier(&X->count, &X->fail, target, P.wait)) { fail_out(); return; }
             if (c == cs && pending_u) {
                 // every workgroup has read the previous sub-panel's rows: store U_s
                 if (g == 0) {
@@ -499,7 +499,7 @@ __global__ __launch_bounds__(PNT) void panel_persist_kernel(LuProb P, int k, int
         }
         if (ce >= kend) break;
         target += (unsigned)G;
-        if (!grid_barrier(&X->count, &X->fail, target)) { fail_out(); return; }
+        if (!grid_barrier(&X->count, &X->fail, target, P.wait)) { fail_out(); return; }
         // U_s = inv(L_ss) A[cs:ce, ce:kend], formed redundantly by every workgroup
         const int W = kend - ce;
         if (t < SB * SB) {
@@ -1318,6 +1318,7 @@ struct SolveProb {
     int n;
     double* w;
     unsigned* sync;
+    long long wait;  // bound of every wait on another block's flag (wall-clock ticks, 100 MHz)
 };
 template <bool B>
 __device__ __forceinline__ SolveProb get_sp(const SolveProb& inl, const SolveProb* tbl) {
@@ -1346,8 +1347,9 @@ __global__ void tri_sync_reset_kernel(SolveProb inl, const SolveProb* tbl, int b
 // M is the lower (LOWER) or upper factor: without TRANS L (unit) and U, read
 // by rows; with TRANS U^T (lower, non-unit) and L^T (upper, unit), read from
 // the same storage by columns.  Arithmetic in FP64 whatever T is.  A wait
-// that does not end within about two seconds (never expected: every
-// predecessor is resident) leaves NaN in the block instead of hanging.
+// that does not end within the wait limit (two seconds by default; never
+// expected: every predecessor is resident) leaves NaN in the block instead of
+// hanging.
 template <typename T, bool C, bool LOWER, bool TRANS, bool B>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const SolveProb* tbl) {
     const SolveProb P = get_sp<B>(inl, tbl);
@@ -1408,7 +1410,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const So
             const long long t0 = wall_clock64();
             while (__hip_atomic_load(&ready[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
                 __builtin_amdgcn_s_sleep(2);
-                if (wall_clock64() - t0 > BARRIER_TICKS) { s_fail = 1; break; }
+                if (wall_clock64() - t0 > P.wait) { s_fail = 1; break; }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
@@ -1964,10 +1966,10 @@ size_t lu_scratch_bytes(int n) {
 }
 
 template <typename T, bool C>
-LuProb lu_carve(char* base, void* A, int64_t ld, int n, int32_t* ipiv, int* info) {
+LuProb lu_carve(char* base, void* A, int64_t ld, int n, int32_t* ipiv, int* info, long long wait) {
     const int gmax = (n + RPB - 1) / RPB;
     LuProb P;
-    P.A = A; P.ld = ld; P.n = n; P.ipiv = ipiv;
+    P.A = A; P.ld = ld; P.n = n; P.ipiv = ipiv; P.wait = wait;
     P.counter = (unsigned*)base;
     P.info = info ? info : (int*)(base + 64);
     P.pval = (double*)(base + 512);
@@ -2047,6 +2049,14 @@ __global__ void lu_reset_kernel(const LuProb* tbl, int count) {
     lu_reset_words(tbl[q]);
 }
 __global__ void lu_reset_one_kernel(LuProb P) { lu_reset_words(P); }
+// ipiv = identity before a factorisation, so the interchanges of a panel that
+// gave up (a wait ran out) are no-ops instead of whatever the array held
+template <bool B>
+__global__ void ipiv_identity_kernel(LuProb inl, const LuProb* tbl) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) P.ipiv[i] = i;
+}
 
 // Workgroups of a persistent panel on the CU-masked panel stream.  The mask
 // takes the first np compute units, which fill whole XCDs (num_cu / 8 each)
@@ -2068,8 +2078,9 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     int rc = ensure_lu_scratch(ctx, lu_scratch_bytes<T, C>(n));
     if (rc) return rc;
     ctx->lu_table.clear();  // this call reuses the scratch a batch table may occupy
-    const LuProb P = lu_carve<T, C>((char*)ctx->lu_scratch, A, ld, n, ipiv, nullptr);
+    const LuProb P = lu_carve<T, C>((char*)ctx->lu_scratch, A, ld, n, ipiv, nullptr, ctx->wait_ticks);
     hipLaunchKernelGGL(lu_reset_one_kernel, dim3(1), dim3(1), 0, s, P);
+    hipLaunchKernelGGL(ipiv_identity_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, s, P, (const LuProb*)nullptr);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
     if (ctx->num_cu <= 0) {
@@ -2127,7 +2138,7 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
     std::vector<int> ns(count);
     for (int r = 0; r < count; ++r) {
         const int q = order[r];
-        tbl[r] = lu_carve<T, C>(base + offs[r], A[q], ld[q], n[q], ipiv[q], info + q);
+        tbl[r] = lu_carve<T, C>(base + offs[r], A[q], ld[q], n[q], ipiv[q], info + q, ctx->wait_ticks);
         ns[r] = n[q];
     }
     LuProb* d_tbl = (LuProb*)(base + off);
@@ -2142,6 +2153,8 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
         ctx->lu_table_dev = (void*)d_tbl;
     }
     hipLaunchKernelGGL(lu_reset_kernel, dim3((count + 63) / 64), dim3(64), 0, s, (const LuProb*)d_tbl, count);
+    hipLaunchKernelGGL(ipiv_identity_kernel<true>, dim3((ns[0] + 255) / 256, 1, count), dim3(256), 0, s, LuProb{},
+                       (const LuProb*)d_tbl);
     // Once only the largest problem is left (from the first outer block past
     // the second-largest n), its remaining columns take the single-problem
     // path: persistent partial-pivoting panels instead of the batch's panel
@@ -2184,8 +2197,9 @@ void tri_launch(const SolveProb& one, const SolveProb* d_tbl, int nmax, int coun
 }
 
 template <typename T, bool C>
-void tri_solve_t(const T* LU, int64_t ld, int n, double* w, unsigned* sync, bool trans, hipStream_t s) {
-    const SolveProb P{LU, ld, n, w, sync};
+void tri_solve_t(const T* LU, int64_t ld, int n, double* w, unsigned* sync, bool trans, long long wait,
+                 hipStream_t s) {
+    const SolveProb P{LU, ld, n, w, sync, wait};
     if (trans) tri_launch<T, C, true, false>(P, nullptr, n, 1, s);
     else tri_launch<T, C, false, false>(P, nullptr, n, 1, s);
 }
@@ -2208,10 +2222,10 @@ int tri_solve_any(Context* ctx, const void* LU, int64_t ld, int n, int dtype, do
     if (rc) return rc;
     unsigned* sync = (unsigned*)ctx->tri_sync;
     switch (dtype) {
-        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, sync, trans, s); break;
-        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w, sync, trans, s); break;
-        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w, sync, trans, s); break;
-        default: tri_solve_t<float, true>((const float*)LU, ld, n, w, sync, trans, s); break;
+        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
+        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
+        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
+        default: tri_solve_t<float, true>((const float*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
     }
     return BEM_OK;
 }
@@ -2486,7 +2500,7 @@ __global__ void vb_perm_kernel(const VecProb* tbl) {
         int32_t* pm = P.perm;
         for (int c = 0; c < P.n; ++c) {
             const int p = P.ipiv[c];
-            if (p != c) { const int t = pm[c]; pm[c] = pm[p]; pm[p] = t; }
+            if (p != c && p >= 0 && p < P.n) { const int t = pm[c]; pm[c] = pm[p]; pm[p] = t; }
         }
     }
 }
@@ -2726,7 +2740,7 @@ int batch_plan(Context* ctx, int slot, int count, const void* const* LU, const i
         p += 256;
         v.out = out ? out + (size_t)out_stride * q : nullptr;
         vt_h[q] = v;
-        st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w, (unsigned*)p};
+        st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w, (unsigned*)p, ctx->wait_ticks};
         plan.nmax = std::max(plan.nmax, n[q]);
     }
     // The sweep kernels take the problems in table order (blockIdx.z), and

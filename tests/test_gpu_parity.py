@@ -746,3 +746,38 @@ def test_helmholtz_neumann_pipeline(bem, oracle):
     assert rel(to_np(bm.rhs), rhs) <= TOL64
     u = np.linalg.solve(W, rhs)
     assert rel(to_np(bm.raw_solved), u) <= 1e-10
+
+
+def test_wait_limit_turns_stuck_waits_into_errors(bem):
+    """With the wait limit at 0, every wait on another workgroup that is not
+    satisfied at once gives up, and every kernel still ends: the factorisation
+    reports the barrier time-out, a solve leaves NaN, and at the default limit
+    both work again on the same context."""
+    L = bem._lib
+    n = 2000
+    rng = np.random.default_rng(21)
+    A = torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n),
+                     dtype=torch.complex128, device=DEV)
+    b = torch.tensor(rng.standard_normal(n) + 0j, dtype=torch.complex128, device=DEV)
+    ctx = bem.Context(0)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    print("[wait limit] factor with limit 0", flush=True)
+    ctx.set_option(L.OPT_WAIT_LIMIT_US, 0)
+    with pytest.raises(bem.BEMError, match="timed out"):
+        ctx.lu_factor(A.clone(), ipiv)
+    ctx.set_option(L.OPT_WAIT_LIMIT_US, 2000000)
+    LU = A.clone()
+    ctx.lu_factor(LU, ipiv)
+    print("[wait limit] solve with limit 0", flush=True)
+    ctx.set_option(L.OPT_WAIT_LIMIT_US, 0)
+    x = b.clone()
+    ctx.lu_solve(LU, ipiv, x)
+    assert bool(torch.isnan(x).any())
+    ctx.set_option(L.OPT_WAIT_LIMIT_US, 2000000)
+    x = b.clone()
+    ctx.lu_solve(LU, ipiv, x)
+    r = (torch.linalg.vector_norm(A @ x - b) / torch.linalg.vector_norm(b)).item()
+    assert r <= 1e-12
+    for bad in (-1, 2000001):
+        with pytest.raises(bem.BEMError):
+            ctx.set_option(L.OPT_WAIT_LIMIT_US, bad)
