@@ -58,10 +58,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--workload", choices=("dirichlet", "sweep", "transmission"), default="dirichlet",
+    ap.add_argument("--workload", choices=("dirichlet", "sweep", "transmission", "gradient"), default="dirichlet",
                     help="dirichlet: the L5 Helmholtz Dirichlet solve (default); sweep: the batched "
                          "combined-field wave-number sweep (60 k at L4 plus L3 and L5); transmission: the "
-                         "L5 2x2 Calderon transmission solve with the interior-k gradient")
+                         "L5 2x2 Calderon transmission solve with the interior-k gradient; gradient: the adjoint "
+                         "shape and wave-number gradient of the Dirichlet misfit (L4, and L5 beside it)")
     ap.add_argument("--no-fd", action="store_true", help="transmission: skip the finite-difference check")
     ap.add_argument("--lu-lookahead", type=int, default=None,
                     help="compute units of the LU's panel stream (0: no look-ahead; default: the library's)")
@@ -84,6 +85,8 @@ def main() -> int:
         return sweep_main(args, bem)
     if args.workload == "transmission":
         return transmission_main(args, bem)
+    if args.workload == "gradient":
+        return gradient_main(args, bem)
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -389,6 +392,108 @@ def sweep_main(args, bem) -> int:
     if cpu:
         result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
+    return 0
+
+
+def gradient_main(args, bem) -> int:
+    """Adjoint gradient of the Dirichlet misfit J = ||t - t*||^2_M with
+    respect to the vertex positions and k (Helmholtz k = 2, point-source data
+    g = exp(i k r)/r from (2, 0.5, 0.3), target trace t* of a seeded
+    perturbed sphere), through the torch.autograd.Function NeumannTrace, on
+    icosphere L4 (2562 vertices, the config) and L5.  One step = the forward
+    solve (fused V + K g assembly, LU, solve) plus the backward (transposed
+    solve, the adjoint contractions of dV/dX, dK/dX and dOp/dk, the area
+    term).  The two contraction launches are also timed alone.  A central
+    difference along one seeded direction checks the L4 gradient."""
+    import numpy as np
+    import torch
+
+    A = bem.adjoint
+    L = bem._lib
+    dev = torch.device("cuda", 0)
+    k, x0 = 2.0, (2.0, 0.5, 0.3)
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    out = {}
+    for level in (4, 5):
+        v, f = bem.meshes.icosphere(level)
+        vt, _ = bem.meshes.perturbed_icosphere(level, seed=1, degrees=(3,))
+        prob = A.DirichletProblem(f, kernel=1, gauss_qr=5)
+        kt = torch.tensor(k, dtype=torch.float64, device=dev)
+        vt_t = torch.tensor(vt, device=dev)
+        with torch.no_grad():
+            t_target = A.neumann_trace(vt_t, kt, A.point_source(vt_t, kt, x0, 1), prob)
+
+        def J(vv, kk, grad):
+            vv = torch.as_tensor(vv, device=dev).clone().requires_grad_(grad)
+            kk = torch.tensor(float(kk), dtype=torch.float64, device=dev, requires_grad=grad)
+            e = [ev() for _ in range(3)]
+            e[0].record()
+            t = A.neumann_trace(vv, kk, A.point_source(vv, kk, x0, 1), prob)
+            j = A.misfit(t, t_target, vv, prob.panels)
+            e[1].record()
+            if grad:
+                j.backward()
+            e[2].record()
+            torch.cuda.synchronize()
+            tm = (e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]))
+            if not grad:
+                return float(j.detach()), tm
+            return float(j.detach()), tm, vv.grad, kk.grad
+
+        for _ in range(max(1, args.warmup)):
+            J(v, k, True)
+        fw, bw = [], []
+        for _ in range(args.steps):
+            j0, tm, gx, gk = J(v, k, True)
+            fw.append(tm[0])
+            bw.append(tm[1])
+        # the two adjoint contraction launches alone (V with t, K with g)
+        nf = len(f)
+        C = prob.ctx
+        prob.set_vertices(torch.tensor(v, device=dev))
+        lam = torch.randn(nf, dtype=torch.complex128, device=dev)
+        u_t = torch.randn(nf, dtype=torch.complex128, device=dev)
+        u_g = torch.randn(len(v), dtype=torch.complex128, device=dev)
+        grad = torch.zeros(len(v), 3, dtype=torch.float64, device=dev)
+        grad_k = torch.zeros(1, dtype=torch.float64, device=dev)
+        ts = []
+        for rep in range(3):
+            e = [ev() for _ in range(3)]
+            e[0].record()
+            C.shape_gradient(prob.desc(L.OP_V, k, 0), lam, u_t, grad, grad_k, alpha=-1.0)
+            e[1].record()
+            C.shape_gradient(prob.desc(L.OP_K, k, 1), lam, u_g, grad, grad_k, alpha=1.0)
+            e[2].record()
+            torch.cuda.synchronize()
+            ts.append((e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])))
+        rec = {"vertices": len(v), "panels": nf, "forward_ms": round(float(np.median(fw)), 3),
+               "backward_ms": round(float(np.median(bw)), 3),
+               "adjoint_V_ms": round(float(np.median([t[0] for t in ts[1:]])), 3),
+               "adjoint_K_ms": round(float(np.median([t[1] for t in ts[1:]])), 3),
+               "separate_ordered_panel_pairs": int(C.nearfield_counts()[0])}
+        if level == 4:
+            rng = np.random.default_rng(3)
+            d = rng.standard_normal(v.shape)
+            d /= np.linalg.norm(d)
+            h = 1e-6
+            fd = (J(v + h * d, k, False)[0] - J(v - h * d, k, False)[0]) / (2 * h)
+            got = float((gx.cpu().numpy() * d).sum())
+            rec["fd_check"] = {"adjoint": got, "central_difference": fd, "rel_err": abs(got - fd) / abs(fd)}
+            rec["dJ_dk"] = float(gk)
+        out[f"L{level}"] = rec
+        log(f"gradient L{level}: {rec}")
+    l4 = out["L4"]
+    line = {
+        "metric": "adjoint shape + wave-number gradient of the Dirichlet misfit (forward + backward)",
+        "value": round(l4["forward_ms"] + l4["backward_ms"], 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(l4["forward_ms"] + l4["backward_ms"], 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (icosphere, seeded perturbed target, point source)",
+        "config": {"workload": "Helmholtz k=2 interior Dirichlet misfit gradient, icosphere L4 (2562 vertices, "
+                               "5120 panels), GaussQR 5; L5 beside it"},
+        "levels": out,
+    }
+    print(json.dumps(line))
     return 0
 
 

@@ -98,18 +98,36 @@ __device__ __forceinline__ void trig(double kap, double r, double& cs, double& s
     if constexpr (HELM) Math<double>::sincos(kap * r, &sn, &cs);
     else { cs = 1.0; sn = 0.0; }
 }
+// The same from a block's LDS copy of the sin/cos table.
+template <bool HELM>
+__device__ __forceinline__ void trig_lds(double kap, double r, const double2* tab, double& cs, double& sn) {
+    if constexpr (HELM) sincos_f64_tab(kap * r, tab, &sn, &cs);
+    else { cs = 1.0; sn = 0.0; }
+}
 
 // ---------------------------------------------------------------------------
 // Regular pairs.  Block: 256 trial panels (one per lane) x TI test panels
 // (wave-uniform, scalar loads of the test points).  The trial-vertex
 // gradient stays in registers over the TI test panels; the test-vertex
 // gradient is reduced over the block per test panel.
+//
+// V is symmetric pair by pair (V_ji sums the same point pairs as V_ij with
+// the roles of the two rule copies exchanged, and G(-d) = G(d)), so V runs
+// over the pairs i < j only, contracted with lam_i u_j + lam_j u_i; blocks
+// with no such pair return at once.  K runs over all ordered pairs.
 // ---------------------------------------------------------------------------
 constexpr int TI = 8;
 
 template <int OP, bool HELM>
 __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) {
     const int ib = blockIdx.x / njb, jb = blockIdx.x % njb;
+    if constexpr (OP == BEM_OP_V)
+        if (ib * TI >= min(p.nf, (jb + 1) * 256) - 1) return;  // every i >= every j
+    __shared__ double2 sct[256];  // sin/cos(j pi/128)
+    if constexpr (HELM) {
+        sincos_table_fill(sct);
+        __syncthreads();
+    }
     const int j = jb * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const bool jvalid = j < p.nf;
@@ -119,11 +137,8 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
     const double* __restrict__ PHI = p.phi;
     const double kap = p.kap;
 
-    double Y[3][3];
-    load_panel(p, jj, Y);
-    double dlj[3][3];
-    dlog_area(Y, dlj);
-    const double a2j = 2.0 * p.area[jj];
+    // the trial panel's vertices are reloaded per pair in the epilogue
+    // rather than held in registers over the point loops
     const double nj[3] = {p.normal[3 * jj], p.normal[3 * jj + 1], p.normal[3 * jj + 2]};
     // trial coefficients: u_j (V) or u at the three vertices (K)
     double ur[3], ui[3];
@@ -132,6 +147,9 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
         const int idx = OP == BEM_OP_V ? jj : p.tris[3 * jj + t];
         ur[t] = p.u[2 * idx]; ui[t] = p.u[2 * idx + 1];
     }
+    // V: lam_j for the mirrored order (j, i)
+    double ljr = 0.0, lji = 0.0;
+    if constexpr (OP == BEM_OP_V) { ljr = p.lam[2 * jj]; lji = p.lam[2 * jj + 1]; }
     double gy[3][3];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
@@ -148,11 +166,14 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
 #pragma unroll
             for (int d = 0; d < 3; ++d) gx[m][d] = 0.0;
         double gkp = 0.0;
-        if (jvalid && !shares_vertex_adj(p.tris, i, j)) {
+        if (jvalid && (OP != BEM_OP_V || i < j) && !shares_vertex_adj(p.tris, i, j)) {
             const double* __restrict__ PU = p.pu + (int64_t)i * q2 * 4;
             // accumulators over the pair (complex): test-slot gradient EX,
             // trial-slot gradient EY, value S0, k-derivative SK, and for K the
-            // N_j-derivative vector T
+            // N_j-derivative vector T.  EX[0] is not accumulated: the six
+            // point-map gradients sum to zero (d = x - y, sum_m phi_m = 1),
+            // so EX[0] = -(EY[0] + EY[1] + EY[2]) - EX[1] - EX[2] after the
+            // loops, which keeps the K kernel within two waves per SIMD.
             double EXr[3][3], EXi[3][3], EYr[3][3], EYi[3][3];
 #pragma unroll
             for (int m = 0; m < 3; ++m)
@@ -179,7 +200,7 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
                     const double rinv = rsqrt_f64(r2);
                     const double r = r2 * rinv;
                     double cs, sn;
-                    trig<HELM>(kap, r, cs, sn);
+                    trig_lds<HELM>(kap, r, sct, cs, sn);
                     const double kr = kap * r;
                     const double ri3 = rinv * rinv * rinv;
                     double gdr[3], gdi[3];  // wa * grad_d of the integrand
@@ -209,8 +230,11 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
                             tr[d] += whr * dv[d]; ti[d] += whi * dv[d];
                         }
                     }
+                    // P[0] = sum_a grad (all slots), P[m] = sum_a phi_m(a) grad, m = 1, 2
 #pragma unroll
-                    for (int m = 0; m < 3; ++m) {
+                    for (int d = 0; d < 3; ++d) { Pr[0][d] += gdr[d]; Pi[0][d] += gdi[d]; }
+#pragma unroll
+                    for (int m = 1; m < 3; ++m) {
                         const double ph = PHI[3 * a + m];
 #pragma unroll
                         for (int d = 0; d < 3; ++d) { Pr[m][d] += ph * gdr[d]; Pi[m][d] += ph * gdi[d]; }
@@ -223,19 +247,13 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
                     cr = wb * (f0 * ur[0] + f1 * ur[1] + f2 * ur[2]);
                     ci = wb * (f0 * ui[0] + f1 * ui[1] + f2 * ui[2]);
                 }
-                double Qr[3], Qi[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    Qr[d] = Pr[0][d] + Pr[1][d] + Pr[2][d];
-                    Qi[d] = Pi[0][d] + Pi[1][d] + Pi[2][d];
-                }
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const double ph = PHI[3 * b + m];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        cmac(EXr[m][d], EXi[m][d], cr, ci, Pr[m][d], Pi[m][d]);
-                        cmac(EYr[m][d], EYi[m][d], -cr * ph, -ci * ph, Qr[d], Qi[d]);
+                        if (m > 0) cmac(EXr[m][d], EXi[m][d], cr, ci, Pr[m][d], Pi[m][d]);
+                        cmac(EYr[m][d], EYi[m][d], -cr * ph, -ci * ph, Pr[0][d], Pi[0][d]);
                     }
                 }
                 cmac(S0r, S0i, cr, ci, s0r, s0i);
@@ -245,18 +263,27 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
                     for (int d = 0; d < 3; ++d) cmac(Tr[d], Ti[d], cr, ci, tr[d], ti[d]);
                 }
             }
-            // contraction coefficient c = alpha lam_i (u_j for V)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                EXr[0][d] = -(EYr[0][d] + EYr[1][d] + EYr[2][d]) - EXr[1][d] - EXr[2][d];
+                EXi[0][d] = -(EYi[0][d] + EYi[1][d] + EYi[2][d]) - EXi[1][d] - EXi[2][d];
+            }
+            // contraction coefficient c = alpha lam_i (K) or
+            // alpha (lam_i u_j + lam_j u_i) (V)
             double lr = p.lam[2 * i], li = p.lam[2 * i + 1];
             if constexpr (OP == BEM_OP_V) {
-                const double t = lr * ur[0] - li * ui[0];
-                li = lr * ui[0] + li * ur[0];
+                const double uir = p.u[2 * i], uii = p.u[2 * i + 1];
+                const double t = lr * ur[0] - li * ui[0] + (ljr * uir - lji * uii);
+                li = lr * ui[0] + li * ur[0] + (ljr * uii + lji * uir);
                 lr = t;
             }
             const double cr = p.ar * lr - p.ai * li, ci = p.ar * li + p.ai * lr;
-            double X[3][3];
+            double X[3][3], Y[3][3];
             load_panel(p, i, X);
-            double dli[3][3];
+            load_panel(p, j, Y);
+            double dli[3][3], dlj[3][3];
             dlog_area(X, dli);
+            if constexpr (OP == BEM_OP_V) dlog_area(Y, dlj);
 #pragma unroll
             for (int m = 0; m < 3; ++m)
 #pragma unroll
@@ -273,7 +300,7 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
                 for (int d = 0; d < 3; ++d) { e1[d] = Y[1][d] - Y[0][d]; e2[d] = Y[2][d] - Y[0][d]; }
                 double tr_[3];
 #pragma unroll
-                for (int d = 0; d < 3; ++d) tr_[d] = cre(cr, ci, Tr[d], Ti[d]) / a2j;
+                for (int d = 0; d < 3; ++d) tr_[d] = cre(cr, ci, Tr[d], Ti[d]) / (2.0 * p.area[j]);
                 const double c1[3] = {e2[1] * tr_[2] - e2[2] * tr_[1], e2[2] * tr_[0] - e2[0] * tr_[2],
                                       e2[0] * tr_[1] - e2[1] * tr_[0]};
                 const double c2[3] = {tr_[1] * e1[2] - tr_[2] * e1[1], tr_[2] * e1[0] - tr_[0] * e1[2],
