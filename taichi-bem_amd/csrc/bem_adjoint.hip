@@ -119,7 +119,7 @@ __device__ __forceinline__ void trig_lds(double kap, double r, const double2* ta
 constexpr int TI = 8;
 
 template <int OP, bool HELM>
-__global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) {
+__global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int njb) {
     const int ib = blockIdx.x / njb, jb = blockIdx.x % njb;
     if constexpr (OP == BEM_OP_V)
         if (ib * TI >= min(p.nf, (jb + 1) * 256) - 1) return;  // every i >= every j
@@ -181,63 +181,119 @@ __global__ __launch_bounds__(256) void adjoint_far_kernel(AdjParams p, int njb) 
                 for (int d = 0; d < 3; ++d) { EXr[m][d] = EXi[m][d] = EYr[m][d] = EYi[m][d] = 0.0; }
             double S0r = 0.0, S0i = 0.0, SKr = 0.0, SKi = 0.0;
             double Tr[3] = {0.0, 0.0, 0.0}, Ti[3] = {0.0, 0.0, 0.0};
+            // test panel (wave-uniform): corner X0 and edges E1, E2 in stored slot order
+            double X0[3], E1[3], E2[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                X0[d] = p.verts[3 * p.tris[3 * i] + d];
+                E1[d] = p.verts[3 * p.tris[3 * i + 1] + d] - X0[d];
+                E2[d] = p.verts[3 * p.tris[3 * i + 2] + d] - X0[d];
+            }
             for (int b = 0; b < q2; ++b) {
                 const double yx = PL[((int64_t)b * 4 + 0) * nf + j];
                 const double yy = PL[((int64_t)b * 4 + 1) * nf + j];
                 const double yz = PL[((int64_t)b * 4 + 2) * nf + j];
                 const double wb = PL[((int64_t)b * 4 + 3) * nf + j];
+                // P[0] = sum_a grad (all slots), P[m] = sum_a phi_m(a) grad, m = 1, 2
                 double Pr[3][3], Pi[3][3];
-#pragma unroll
-                for (int m = 0; m < 3; ++m)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) { Pr[m][d] = 0.0; Pi[m][d] = 0.0; }
                 double s0r = 0.0, s0i = 0.0, skr = 0.0, ski = 0.0;
                 double tr[3] = {0.0, 0.0, 0.0}, ti[3] = {0.0, 0.0, 0.0};
-                for (int a = 0; a < q2; ++a) {
-                    const double dv[3] = {PU[4 * a + 0] - yx, PU[4 * a + 1] - yy, PU[4 * a + 2] - yz};
-                    const double wa = PU[4 * a + 3];
-                    const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
-                    const double rinv = rsqrt_f64(r2);
-                    const double r = r2 * rinv;
-                    double cs, sn;
-                    trig_lds<HELM>(kap, r, sct, cs, sn);
-                    const double kr = kap * r;
-                    const double ri3 = rinv * rinv * rinv;
-                    double gdr[3], gdi[3];  // wa * grad_d of the integrand
-                    if constexpr (OP == BEM_OP_V) {
+                if constexpr (OP == BEM_OP_V) {
+                    // V: grad_d of the integrand is wh d with wh = wa (i kap r - 1) e/r^3;
+                    // the phi-weighted sums follow from the moments as for K below
+                    double Gr = 0.0, Gi = 0.0, G1r = 0.0, G1i = 0.0, G2r = 0.0, G2i = 0.0;
+                    double G11r = 0.0, G11i = 0.0, G12r = 0.0, G12i = 0.0, G22r = 0.0, G22i = 0.0;
+                    for (int a = 0; a < q2; ++a) {
+                        const double dv[3] = {PU[4 * a + 0] - yx, PU[4 * a + 1] - yy, PU[4 * a + 2] - yz};
+                        const double wa = PU[4 * a + 3];
+                        const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+                        const double rinv = rsqrt_f64(r2);
+                        const double r = r2 * rinv;
+                        double cs, sn;
+                        trig_lds<HELM>(kap, r, sct, cs, sn);
+                        const double kr = kap * r;
+                        const double ri3 = rinv * rinv * rinv;
                         s0r += wa * cs * rinv; s0i += wa * sn * rinv;
                         if constexpr (HELM) { skr -= wa * sn; ski += wa * cs; }
-                        const double hr = wa * (-cs - kr * sn) * ri3, hi = wa * (kr * cs - sn) * ri3;
+                        const double whr = wa * (-cs - kr * sn) * ri3, whi = wa * (kr * cs - sn) * ri3;
+                        const double p1 = PHI[3 * a + 1], p2 = PHI[3 * a + 2];
+                        Gr += whr; Gi += whi;
+                        const double t1r = p1 * whr, t1i = p1 * whi;
+                        G1r += t1r; G1i += t1i;
+                        G11r += p1 * t1r; G11i += p1 * t1i;
+                        G12r += p2 * t1r; G12i += p2 * t1i;
+                        const double t2r = p2 * whr, t2i = p2 * whi;
+                        G2r += t2r; G2i += t2i;
+                        G22r += p2 * t2r; G22i += p2 * t2i;
+                    }
+                    const double Dx[3] = {X0[0] - yx, X0[1] - yy, X0[2] - yz};
 #pragma unroll
-                        for (int d = 0; d < 3; ++d) { gdr[d] = hr * dv[d]; gdi[d] = hi * dv[d]; }
-                    } else {
+                    for (int d = 0; d < 3; ++d) {
+                        Pr[0][d] = Dx[d] * Gr + E1[d] * G1r + E2[d] * G2r;
+                        Pi[0][d] = Dx[d] * Gi + E1[d] * G1i + E2[d] * G2i;
+                        Pr[1][d] = Dx[d] * G1r + E1[d] * G11r + E2[d] * G12r;
+                        Pi[1][d] = Dx[d] * G1i + E1[d] * G11i + E2[d] * G12i;
+                        Pr[2][d] = Dx[d] * G2r + E1[d] * G12r + E2[d] * G22r;
+                        Pi[2][d] = Dx[d] * G2i + E1[d] * G12i + E2[d] * G22i;
+                    }
+                } else {
+                    // K: grad_d of the integrand is wh n_j + wp d with
+                    // wh = wa h, wp = wa (d.n_j) hp.  Since d = x_a - y_b =
+                    // (X0 - y_b) + phi_1(a) E1 + phi_2(a) E2, the sums over a
+                    // of grad, phi_1 grad, phi_2 grad and T = sum wh d follow
+                    // from the complex moments sum {1, phi_1, phi_2} wh and
+                    // sum {1, phi_1, phi_2, phi_1^2, phi_1 phi_2, phi_2^2} wp,
+                    // with no 3-vector accumulated per point pair.
+                    double Hr = 0.0, Hi = 0.0, H1r = 0.0, H1i = 0.0, H2r = 0.0, H2i = 0.0;
+                    double Gr = 0.0, Gi = 0.0, G1r = 0.0, G1i = 0.0, G2r = 0.0, G2i = 0.0;
+                    double G11r = 0.0, G11i = 0.0, G12r = 0.0, G12i = 0.0, G22r = 0.0, G22i = 0.0;
+                    for (int a = 0; a < q2; ++a) {
+                        const double dv[3] = {PU[4 * a + 0] - yx, PU[4 * a + 1] - yy, PU[4 * a + 2] - yz};
+                        const double wa = PU[4 * a + 3];
+                        const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+                        const double rinv = rsqrt_f64(r2);
+                        const double r = r2 * rinv;
+                        double cs, sn;
+                        trig_lds<HELM>(kap, r, sct, cs, sn);
+                        const double kr = kap * r;
+                        const double ri3 = rinv * rinv * rinv;
                         const double dn = dv[0] * nj[0] + dv[1] * nj[1] + dv[2] * nj[2];
                         const double hr = (cs + kr * sn) * ri3, hi = (sn - kr * cs) * ri3;
                         const double ri5 = ri3 * rinv * rinv;
                         const double A = kr * kr - 3.0, B = 3.0 * kr;
                         const double hpr = (cs * A - sn * B) * ri5, hpi = (sn * A + cs * B) * ri5;
-                        s0r += wa * dn * hr; s0i += wa * dn * hi;
+                        const double wd = wa * dn;
+                        s0r += wd * hr; s0i += wd * hi;
                         if constexpr (HELM) {
-                            const double f = wa * dn * kap * rinv;
+                            const double f = wd * kap * rinv;
                             skr += f * cs; ski += f * sn;
                         }
                         const double whr = wa * hr, whi = wa * hi;
-                        const double wpr = wa * dn * hpr, wpi = wa * dn * hpi;
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            gdr[d] = whr * nj[d] + wpr * dv[d];
-                            gdi[d] = whi * nj[d] + wpi * dv[d];
-                            tr[d] += whr * dv[d]; ti[d] += whi * dv[d];
-                        }
+                        const double wpr = wd * hpr, wpi = wd * hpi;
+                        const double p1 = PHI[3 * a + 1], p2 = PHI[3 * a + 2];
+                        Hr += whr; Hi += whi;
+                        H1r += p1 * whr; H1i += p1 * whi;
+                        H2r += p2 * whr; H2i += p2 * whi;
+                        Gr += wpr; Gi += wpi;
+                        const double t1r = p1 * wpr, t1i = p1 * wpi;
+                        G1r += t1r; G1i += t1i;
+                        G11r += p1 * t1r; G11i += p1 * t1i;
+                        G12r += p2 * t1r; G12i += p2 * t1i;
+                        const double t2r = p2 * wpr, t2i = p2 * wpi;
+                        G2r += t2r; G2i += t2i;
+                        G22r += p2 * t2r; G22i += p2 * t2i;
                     }
-                    // P[0] = sum_a grad (all slots), P[m] = sum_a phi_m(a) grad, m = 1, 2
+                    const double Dx[3] = {X0[0] - yx, X0[1] - yy, X0[2] - yz};
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) { Pr[0][d] += gdr[d]; Pi[0][d] += gdi[d]; }
-#pragma unroll
-                    for (int m = 1; m < 3; ++m) {
-                        const double ph = PHI[3 * a + m];
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) { Pr[m][d] += ph * gdr[d]; Pi[m][d] += ph * gdi[d]; }
+                    for (int d = 0; d < 3; ++d) {
+                        Pr[0][d] = nj[d] * Hr + Dx[d] * Gr + E1[d] * G1r + E2[d] * G2r;
+                        Pi[0][d] = nj[d] * Hi + Dx[d] * Gi + E1[d] * G1i + E2[d] * G2i;
+                        Pr[1][d] = nj[d] * H1r + Dx[d] * G1r + E1[d] * G11r + E2[d] * G12r;
+                        Pi[1][d] = nj[d] * H1i + Dx[d] * G1i + E1[d] * G11i + E2[d] * G12i;
+                        Pr[2][d] = nj[d] * H2r + Dx[d] * G2r + E1[d] * G12r + E2[d] * G22r;
+                        Pi[2][d] = nj[d] * H2i + Dx[d] * G2i + E1[d] * G12i + E2[d] * G22i;
+                        tr[d] = Dx[d] * Hr + E1[d] * H1r + E2[d] * H2r;
+                        ti[d] = Dx[d] * Hi + E1[d] * H1i + E2[d] * H2i;
                     }
                 }
                 // trial-point factor: wb (V) or wb * u interpolated at y_b (K)
