@@ -66,11 +66,18 @@ enum {
     BEM_FLAG_SYM_COINCIDE = 2, /* coincident rule: f(y,x) with swapped bases
                                   instead of f(x,y) twice (single_layer_3d.py:
                                   198-206) */
-    BEM_FLAG_DK = 4            /* bem_apply only: the product with dOp/dk, the
+    BEM_FLAG_DK = 4,           /* bem_apply only: the product with dOp/dk, the
                                   derivative of every entry with respect to the
                                   wave number k (kap = k or k*sqrt_n); FP64.
                                   No reference counterpart (the reference has
                                   no derivatives) */
+    BEM_FLAG_TRANSPOSE = 8     /* bem_apply only: y[trial dof] += alpha Op^T
+                                  xhat (transpose, not conjugate transpose) of
+                                  the same discrete operator, pair rules
+                                  unchanged; FP64, not with BEM_FLAG_DK.  No
+                                  reference counterpart; serves the adjoint's
+                                  gradient with respect to the Dirichlet data,
+                                  (1/2 M + K)^T mu, without forming K */
 };
 
 typedef struct bem_context bem_context;
@@ -404,6 +411,17 @@ int bem_refine_solve(bem_context* ctx, const void* A, int64_t lda, int32_t dtype
 int bem_shape_gradient(bem_context* ctx, const bem_op_desc* op, const void* lam, const void* u,
                        double alpha_re, double alpha_im, double* grad_vertices, double* grad_k,
                        void* stream);
+
+/* bem_shape_gradient for K (P0 test x P1 trial) that also adds
+ *   kt[v] += (K^T lam)_v   (complex128 [nv], transpose, no conjugation)
+ * from the same pair evaluations: the regular pairs inside the contraction,
+ * the near pairs by the transposed near-field product.  kt = NULL is
+ * bem_shape_gradient.  No reference counterpart: the adjoint of the Dirichlet
+ * problem (BEM_manager.py:990-1011) needs (1/2 M + K)^T mu for the gradient
+ * with respect to the boundary data g, next to mu^T dK/dX g. */
+int bem_shape_gradient_kt(bem_context* ctx, const bem_op_desc* op, const void* lam, const void* u,
+                          double alpha_re, double alpha_im, double* grad_vertices, double* grad_k,
+                          void* kt, void* stream);
 
 /* grad_vertices[3 v + d] += sum_i coef_i dA_i/dX_{v,d} (panel areas A_i of
  * mesh_manager.py:114-124; coef f64 [nf]).  The derivative of the P0 mass

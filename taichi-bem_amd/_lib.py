@@ -20,7 +20,7 @@ BEM_OK, BEM_ERR_INVALID, BEM_ERR_UNSUPPORTED, BEM_ERR_HIP, BEM_ERR_SINGULAR = 0,
 OP_V, OP_K, OP_KADJ, OP_W = 0, 1, 2, 3
 F64, C128, F32, C64 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
-FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE, FLAG_DK = 1, 2, 4
+FLAG_ROTATE_NEAR, FLAG_SYM_COINCIDE, FLAG_DK, FLAG_TRANSPOSE = 1, 2, 4, 8
 PART_FAR, PART_NEAR = 1, 2
 KernelType_LAPLACE, KernelType_HELMHOLTZ, KernelType_TRANSMISSION = 0, 1, 2
 OPT_C64_LU_GEMM, OPT_LU_LOOKAHEAD, OPT_LU_PIVOTING, OPT_LU_GEMM_TILE, OPT_LU_PANEL, OPT_WAIT_LIMIT_US = 1, 2, 3, 4, 5, 6
@@ -32,7 +32,7 @@ EXPORTS = (
     "bem_vkw_dk_forms", "bem_apply",
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
-    "bem_shape_gradient", "bem_area_gradient", "bem_assemble_combined_batched", "bem_lu_factor_batched",
+    "bem_shape_gradient", "bem_shape_gradient_kt", "bem_area_gradient", "bem_assemble_combined_batched", "bem_lu_factor_batched",
     "bem_batch_solve", "bem_batch_condest", "bem_set_option",
 )
 
@@ -109,6 +109,8 @@ def lib() -> ctypes.CDLL:
             "bem_lu_solve": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
             "bem_lu_solve_transpose": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
             "bem_shape_gradient": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, vp, dbl, dbl, vp, vp, vp]),
+            "bem_shape_gradient_kt": (ctypes.c_int,
+                                      [vp, ctypes.POINTER(BemOpDesc), vp, vp, dbl, dbl, vp, vp, vp, vp]),
             "bem_area_gradient": (ctypes.c_int, [vp, vp, vp, vp]),
             "bem_set_option": (ctypes.c_int, [vp, i32, i32]),
             "bem_assemble_combined_batched": (ctypes.c_int, [vp, i32, ctypes.POINTER(BemCfProblem), i32, i32, i32,
@@ -291,6 +293,8 @@ class Context:
         return out
 
     def apply(self, desc: BemOpDesc, x, y, alpha: complex = 1.0, conj_x: bool = True) -> None:
+        """y[test dof] += alpha Op xhat, or y[trial dof] += alpha Op^T xhat when
+        desc.flags has FLAG_TRANSPOSE (xhat = conj(x) if conj_x)."""
         _require_cuda(x, y)
         if x.dtype != y.dtype:
             raise BEMError("x and y must share a dtype")
@@ -332,20 +336,28 @@ class Context:
         check(lib().bem_lu_solve_transpose(self._h, LU.data_ptr(), int(LU.stride(0)), int(LU.shape[0]),
                                            dtype_code(LU), ipiv.data_ptr(), b.data_ptr(), current_stream()))
 
-    def shape_gradient(self, desc: BemOpDesc, lam, u, grad, grad_k=None, alpha: complex = 1.0) -> None:
+    def shape_gradient(self, desc: BemOpDesc, lam, u, grad, grad_k=None, alpha: complex = 1.0, kt=None) -> None:
         """grad [nv,3] += Re(alpha sum lam_s u_t dOp_st/dX); grad_k [1] +=
-        Re(alpha sum lam_s u_t dOp_st/dk).  lam, u complex128 device vectors."""
+        Re(alpha sum lam_s u_t dOp_st/dk).  lam, u complex128 device vectors.
+        kt (K only, complex128 [nv]) += K^T lam from the same evaluations."""
         import torch
 
-        _require_cuda(lam, u, grad, grad_k)
+        _require_cuda(lam, u, grad, grad_k, kt)
+        if kt is not None and (kt.dtype != torch.complex128 or not kt.is_contiguous()):
+            raise BEMError("kt must be a contiguous complex128 tensor")
         if lam.dtype != torch.complex128 or u.dtype != torch.complex128:
             raise BEMError("shape gradient coefficients must be complex128")
         if grad.dtype != torch.float64 or not grad.is_contiguous():
             raise BEMError("grad must be a contiguous float64 tensor")
         desc.dtype = C128
         a = complex(alpha)
-        check(lib().bem_shape_gradient(self._h, ctypes.byref(desc), lam.data_ptr(), u.data_ptr(), a.real, a.imag,
-                                       grad.data_ptr(), _ptr(grad_k), current_stream()))
+        if kt is None:
+            check(lib().bem_shape_gradient(self._h, ctypes.byref(desc), lam.data_ptr(), u.data_ptr(), a.real,
+                                           a.imag, grad.data_ptr(), _ptr(grad_k), current_stream()))
+        else:
+            check(lib().bem_shape_gradient_kt(self._h, ctypes.byref(desc), lam.data_ptr(), u.data_ptr(), a.real,
+                                              a.imag, grad.data_ptr(), _ptr(grad_k), kt.data_ptr(),
+                                              current_stream()))
 
     def area_gradient(self, coef, grad) -> None:
         """grad [nv,3] += sum_i coef_i dA_i/dX (coef float64 [nf])."""

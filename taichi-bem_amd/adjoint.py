@@ -100,17 +100,17 @@ class NeumannTrace(torch.autograd.Function):
         grad_x = torch.zeros(nv, 3, dtype=torch.float64, device=prob.device)
         grad_k = torch.zeros(1, dtype=torch.float64, device=prob.device)
         C.shape_gradient(prob.desc(_lib.OP_V, ctx.k, 0), mu, t, grad_x, grad_k, alpha=-1.0)
-        C.shape_gradient(prob.desc(_lib.OP_K, ctx.k, 1), mu, gc, grad_x, grad_k, alpha=1.0)
+        # the K contraction also returns K^T mu when g needs a gradient
+        bt = torch.zeros(nv, dtype=torch.complex128, device=prob.device) if ctx.needs_input_grad[2] else None
+        C.shape_gradient(prob.desc(_lib.OP_K, ctx.k, 1), mu, gc, grad_x, grad_k, alpha=1.0, kt=bt)
         # 1/2 dM g: M_iv = A_i / 3 for the three vertices v of panel i
         gbar = gc[prob.panels.long()].mean(dim=1)
         C.area_gradient((0.5 * (mu * gbar).real).contiguous(), grad_x)
         grad_g = None
-        if ctx.needs_input_grad[2]:
-            nf = int(prob.panels.shape[0])
-            B = torch.zeros(nf, nv, dtype=torch.complex128, device=prob.device)
-            C.assemble(prob.desc(_lib.OP_K, ctx.k, 1), B)
-            C.mass_assemble(0, 1, B, alpha=0.5)
-            grad_g = (B.transpose(0, 1) @ mu).conj()
+        if bt is not None:
+            # (1/2 M + K)^T mu: K^T mu from the contraction, P1 x P0 mass
+            C.mass_apply(1, 0, mu, bt, alpha=0.5, conj_x=False)
+            grad_g = bt.conj()
             if ctx.g_real:
                 grad_g = grad_g.real.contiguous()
         gk = grad_k.reshape(()) if ctx.needs_input_grad[1] else None

@@ -50,6 +50,7 @@ struct AdjParams {
     double ar, ai;                   // alpha / (4 pi) folded in on the host
     double* grad;                    // [nv*3]
     double* gk;                      // [1] or null
+    double* kt;                      // K only: complex128 [nv] += (K^T lam) of the regular pairs, or null
 };
 
 // d(2A)/dX_m / (2A) for the stored slots of panel with vertices P[3][3]:
@@ -156,6 +157,8 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
 #pragma unroll
         for (int d = 0; d < 3; ++d) gy[m][d] = 0.0;
     double gk_lane = 0.0;
+    // K: (K^T lam) at the trial panel's three vertices, over the block's test panels
+    double ktr[3] = {0.0, 0.0, 0.0}, kti[3] = {0.0, 0.0, 0.0};
 
     __shared__ double red[4][10];
     const int i_end = min(nf, (ib + 1) * TI);
@@ -181,6 +184,9 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
                 for (int d = 0; d < 3; ++d) { EXr[m][d] = EXi[m][d] = EYr[m][d] = EYi[m][d] = 0.0; }
             double S0r = 0.0, S0i = 0.0, SKr = 0.0, SKi = 0.0;
             double Tr[3] = {0.0, 0.0, 0.0}, Ti[3] = {0.0, 0.0, 0.0};
+            // K: the pair's entries K_iv (times 4 pi) for the trial vertices t;
+            // the value sum is then S0 = sum_t u_t KB_t
+            double KBr[3] = {0.0, 0.0, 0.0}, KBi[3] = {0.0, 0.0, 0.0};
             // test panel (wave-uniform): corner X0 and edges E1, E2 in stored slot order
             double X0[3], E1[3], E2[3];
 #pragma unroll
@@ -312,7 +318,15 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
                         cmac(EYr[m][d], EYi[m][d], -cr * ph, -ci * ph, Pr[0][d], Pi[0][d]);
                     }
                 }
-                cmac(S0r, S0i, cr, ci, s0r, s0i);
+                if constexpr (OP == BEM_OP_K) {
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        const double wt = wb * PHI[3 * b + t];
+                        KBr[t] += wt * s0r; KBi[t] += wt * s0i;
+                    }
+                } else {
+                    cmac(S0r, S0i, cr, ci, s0r, s0i);
+                }
                 cmac(SKr, SKi, cr, ci, skr, ski);
                 if constexpr (OP == BEM_OP_K) {
 #pragma unroll
@@ -327,6 +341,13 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
             // contraction coefficient c = alpha lam_i (K) or
             // alpha (lam_i u_j + lam_j u_i) (V)
             double lr = p.lam[2 * i], li = p.lam[2 * i + 1];
+            if constexpr (OP == BEM_OP_K) {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    cmac(S0r, S0i, ur[t], ui[t], KBr[t], KBi[t]);
+                    cmac(ktr[t], kti[t], lr, li, KBr[t], KBi[t]);
+                }
+            }
             if constexpr (OP == BEM_OP_V) {
                 const double uir = p.u[2 * i], uii = p.u[2 * i + 1];
                 const double t = lr * ur[0] - li * ui[0] + (ljr * uir - lji * uii);
@@ -393,6 +414,16 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
 #pragma unroll
             for (int d = 0; d < 3; ++d)
                 if (gy[m][d] != 0.0) atomicAdd(p.grad + 3 * p.tris[3 * j + m] + d, gy[m][d]);
+        if constexpr (OP == BEM_OP_K) {
+            if (p.kt) {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    double* o = p.kt + 2 * p.tris[3 * j + t];
+                    atomicAdd(o, kInv4Pi * ktr[t]);
+                    atomicAdd(o + 1, kInv4Pi * kti[t]);
+                }
+            }
+        }
     }
     if (p.gk) {
         const double s = wave_sum(gk_lane);
@@ -615,7 +646,7 @@ void launch_adjoint(const AdjParams& p, bool helm, int njb, hipStream_t s) {
 }  // namespace
 
 int shape_gradient(Context* ctx, const bem_op_desc* d, const void* lam, const void* u, double ar, double ai,
-                   double* grad, double* grad_k, hipStream_t s) {
+                   double* grad, double* grad_k, void* kt, hipStream_t s) {
     Mesh& m = ctx->mesh;
     if (!m.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
     if (!d || !lam || !u || !grad) return set_error(BEM_ERR_INVALID, "null argument");
@@ -625,8 +656,11 @@ int shape_gradient(Context* ctx, const bem_op_desc* d, const void* lam, const vo
     const bool k_op = d->op == BEM_OP_K && d->deg_test == 0 && d->deg_trial == 1;
     if (!v_op && !k_op)
         return set_error(BEM_ERR_UNSUPPORTED, "shape gradient: V (P0 x P0) and K (P0 x P1) only");
+    if (kt && !k_op) return set_error(BEM_ERR_INVALID, "K^T lam output: the operator must be K (P0 x P1)");
     if (d->dtype != BEM_C128) return set_error(BEM_ERR_INVALID, "shape gradient: coefficients must be complex128");
     if (d->precision != BEM_PREC_FP64) return set_error(BEM_ERR_UNSUPPORTED, "shape gradient runs in FP64 only");
+    if (d->flags & (BEM_FLAG_DK | BEM_FLAG_TRANSPOSE))
+        return set_error(BEM_ERR_INVALID, "BEM_FLAG_DK and BEM_FLAG_TRANSPOSE are bem_apply flags");
     int rc = mesh_build_near(ctx, d->flags, s);
     if (rc) return rc;
     AdjParams p;
@@ -641,10 +675,17 @@ int shape_gradient(Context* ctx, const bem_op_desc* d, const void* lam, const vo
     p.lam = (const double*)lam; p.u = (const double*)u;
     p.ar = ar * kInv4Pi; p.ai = ai * kInv4Pi;
     p.grad = grad; p.gk = helm ? grad_k : nullptr;
+    p.kt = (double*)kt;
     const int njb = (m.nf + 255) / 256;
     if (v_op) launch_adjoint<BEM_OP_V>(p, helm, njb, s);
     else launch_adjoint<BEM_OP_K>(p, helm, njb, s);
-    return check_hip(hipGetLastError(), "shape gradient launch");
+    rc = check_hip(hipGetLastError(), "shape gradient launch");
+    if (rc || !kt) return rc;
+    // the near pairs of K^T lam: the transposed near-field product
+    bem_op_desc dn = *d;
+    dn.flags |= BEM_FLAG_TRANSPOSE;
+    dn.parts = BEM_PART_NEAR;
+    return apply_op(ctx, &dn, lam, 0, kt, 1.0, 0.0, s);
 }
 
 int area_gradient(Context* ctx, const double* coef, double* grad, hipStream_t s) {

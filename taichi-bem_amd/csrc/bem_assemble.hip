@@ -524,6 +524,71 @@ __global__ __launch_bounds__(256) void near_apply_kernel(OpParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Transposed products y[trial] += alpha Op^T xhat (BEM_FLAG_TRANSPOSE), FP64,
+// from the same pair evaluations as the forward products.  Far: a block
+// holds 256 trial panels (one per lane) and a chunk of TCH test panels
+// (wave-uniform, as in far_apply_kernel); a lane sums Op[s][t] x[s] over its
+// chunk and adds its trial dofs once.  Near: one wave per pair, its trial
+// dofs added atomically.
+// ---------------------------------------------------------------------------
+constexpr int TCH = 64;
+
+template <int OP, bool HELM, int DT, int DR>
+__global__ __launch_bounds__(256) void far_apply_t_kernel(OpParams p, int njb) {
+    const int ib = blockIdx.x / njb, jb = blockIdx.x % njb;
+    const int j = jb * 256 + threadIdx.x;
+    if (j >= p.nf) return;
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    double yr[NT], yi[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) { yr[t] = 0.0; yi[t] = 0.0; }
+    const int i_end = min(p.nf, (ib + 1) * TCH);
+    for (int i = ib * TCH; i < i_end; ++i) {
+        if (shares_vertex(p.tris, i, j)) continue;
+        double vr[3][3], vi[3][3];
+        far_pair<double, OP, HELM, DT, DR>(p, i, j, vr, vi);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double xr, xi;
+            load_elem(p.x, p.dtype_x, DT ? p.tris[3 * i + s] : i, p.conj_x, xr, xi);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                yr[t] += vr[s][t] * xr - vi[s][t] * xi;
+                yi[t] += vr[s][t] * xi + vi[s][t] * xr;
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int64_t col = DR ? p.tris[3 * j + t] : j;
+        add_elem(p.y, p.dtype_x, col, p.ar * yr[t] - p.ai * yi[t], p.ar * yi[t] + p.ai * yr[t], true);
+    }
+}
+
+template <int OP, bool HELM, int DT, int DR>
+__global__ __launch_bounds__(256) void near_apply_t_kernel(OpParams p) {
+    const int lane = threadIdx.x & 63;
+    const int64_t pid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pid >= p.n_near) return;
+    const NearPair np = p.near[pid];
+    constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
+    double vr[3][3], vi[3][3];
+    near_pair<OP, HELM, DT, DR>(p, np, lane, vr, vi);
+    if (lane != 0) return;
+    for (int t = 0; t < NT; ++t) {
+        double yr = 0.0, yi = 0.0;
+        for (int s = 0; s < NS; ++s) {
+            double xr, xi;
+            load_elem(p.x, p.dtype_x, DT ? p.tris[3 * np.i + s] : np.i, p.conj_x, xr, xi);
+            yr += vr[s][t] * xr - vi[s][t] * xi;
+            yi += vr[s][t] * xi + vi[s][t] * xr;
+        }
+        const int64_t col = DR ? p.tris[3 * np.j + t] : np.j;
+        add_elem(p.y, p.dtype_x, col, p.ar * yr - p.ai * yi, p.ar * yi + p.ai * yr, true);
+    }
+}
+
 // Deterministic per-row sum of the near contributions (P0 test rows), or an
 // atomic scatter to vertex rows (P1 test).
 __global__ void near_reduce_kernel(OpParams p, int deg_test) {
@@ -1594,6 +1659,27 @@ NearFn near_apply_dk_any(int op, int dt, int dr) {
     }
 }
 
+template <int OP, bool HELM>
+void apply_t_fns(int dt, int dr, FarFn& far, NearFn& nearf) {
+    if (dt == 0 && dr == 0) { far = far_apply_t_kernel<OP, HELM, 0, 0>; nearf = near_apply_t_kernel<OP, HELM, 0, 0>; }
+    else if (dt == 0 && dr == 1) { far = far_apply_t_kernel<OP, HELM, 0, 1>; nearf = near_apply_t_kernel<OP, HELM, 0, 1>; }
+    else if (dt == 1 && dr == 0) { far = far_apply_t_kernel<OP, HELM, 1, 0>; nearf = near_apply_t_kernel<OP, HELM, 1, 0>; }
+    else { far = far_apply_t_kernel<OP, HELM, 1, 1>; nearf = near_apply_t_kernel<OP, HELM, 1, 1>; }
+}
+template <int OP>
+void apply_t_k(bool helm, int dt, int dr, FarFn& far, NearFn& nearf) {
+    if (helm) apply_t_fns<OP, true>(dt, dr, far, nearf);
+    else apply_t_fns<OP, false>(dt, dr, far, nearf);
+}
+void apply_t_any(int op, bool helm, int dt, int dr, FarFn& far, NearFn& nearf) {
+    switch (op) {
+        case BEM_OP_V: apply_t_k<BEM_OP_V>(helm, dt, dr, far, nearf); break;
+        case BEM_OP_K: apply_t_k<BEM_OP_K>(helm, dt, dr, far, nearf); break;
+        case BEM_OP_KADJ: apply_t_k<BEM_OP_KADJ>(helm, dt, dr, far, nearf); break;
+        default: apply_t_k<BEM_OP_W>(helm, dt, dr, far, nearf); break;
+    }
+}
+
 int validate(Context* ctx, const bem_op_desc* d) {
     if (!ctx->mesh.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
     if (!d) return set_error(BEM_ERR_INVALID, "null operator descriptor");
@@ -1607,7 +1693,7 @@ int validate(Context* ctx, const bem_op_desc* d) {
         return set_error(BEM_ERR_INVALID, "Helmholtz operators need a complex dtype");
     if (d->precision != BEM_PREC_FP64 && d->precision != BEM_PREC_FP32)
         return set_error(BEM_ERR_INVALID, "unknown precision");
-    if (d->flags & ~(BEM_FLAG_ROTATE_NEAR | BEM_FLAG_SYM_COINCIDE | BEM_FLAG_DK))
+    if (d->flags & ~(BEM_FLAG_ROTATE_NEAR | BEM_FLAG_SYM_COINCIDE | BEM_FLAG_DK | BEM_FLAG_TRANSPOSE))
         return set_error(BEM_ERR_INVALID, "unknown flags");
     return BEM_OK;
 }
@@ -1621,6 +1707,8 @@ int assemble_op(Context* ctx, const bem_op_desc* d, void* out, int64_t ld, doubl
     if (!out) return set_error(BEM_ERR_INVALID, "null output");
     if (d->flags & BEM_FLAG_DK)
         return set_error(BEM_ERR_UNSUPPORTED, "k-derivative operators are matrix-free only (bem_apply)");
+    if (d->flags & BEM_FLAG_TRANSPOSE)
+        return set_error(BEM_ERR_INVALID, "BEM_FLAG_TRANSPOSE is a bem_apply flag");
     rc = mesh_build_near(ctx, d->flags, s);
     if (rc) return rc;
     OpParams p = make_params(ctx, d);
@@ -1652,12 +1740,32 @@ int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void
     const bool helm = d->kernel != BEM_LAPLACE;
     if (dk && d->precision != BEM_PREC_FP64)
         return set_error(BEM_ERR_UNSUPPORTED, "k-derivative products run in FP64 only");
+    const bool tr = (d->flags & BEM_FLAG_TRANSPOSE) != 0;
+    if (tr && dk) return set_error(BEM_ERR_UNSUPPORTED, "transposed k-derivative products are not provided");
+    if (tr && d->precision != BEM_PREC_FP64)
+        return set_error(BEM_ERR_UNSUPPORTED, "transposed products run in FP64 only");
     if (dk && !helm) return BEM_OK;  // the Laplace kernel does not depend on k
     rc = mesh_build_near(ctx, d->flags, s);
     if (rc) return rc;
     OpParams p = make_params(ctx, d);
     p.x = x; p.conj_x = conj_x; p.dtype_x = d->dtype; p.y = y; p.ar = ar; p.ai = ai;
     const int parts = d->parts ? d->parts : (BEM_PART_FAR | BEM_PART_NEAR);
+    if (tr) {
+        FarFn far = nullptr;
+        NearFn nearf = nullptr;
+        apply_t_any(d->op, helm, d->deg_test, d->deg_trial, far, nearf);
+        const int njb = (p.nf + 255) / 256, nib = (p.nf + TCH - 1) / TCH;
+        if (parts & BEM_PART_FAR) {
+            hipLaunchKernelGGL(far, dim3((unsigned)((int64_t)nib * njb)), dim3(256), 0, s, p, njb);
+            rc = check_hip(hipGetLastError(), "transposed far-field apply launch");
+            if (rc) return rc;
+        }
+        if (p.n_near > 0 && (parts & BEM_PART_NEAR)) {
+            hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
+            rc = check_hip(hipGetLastError(), "transposed near-field apply launch");
+        }
+        return rc;
+    }
     if (parts & BEM_PART_FAR) {
         NearFn far = dk ? far_apply_dk_any(d->op, d->deg_test, d->deg_trial)
                         : d->precision == BEM_PREC_FP32 ? far_apply_any<float>(d->op, helm, d->deg_test, d->deg_trial)
@@ -1687,6 +1795,7 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
     if (d->op != BEM_OP_V || d->deg_test != 0 || d->deg_trial != 0)
         return set_error(BEM_ERR_INVALID, "the Dirichlet system takes the P0 single-layer descriptor");
     if (d->flags & BEM_FLAG_DK) return set_error(BEM_ERR_UNSUPPORTED, "k-derivative: use bem_apply");
+    if (d->flags & BEM_FLAG_TRANSPOSE) return set_error(BEM_ERR_INVALID, "BEM_FLAG_TRANSPOSE is a bem_apply flag");
     if (dtype_rhs < BEM_F64 || dtype_rhs > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown rhs dtype");
     if (dtype_complex(d->dtype) != dtype_complex(dtype_rhs))
         return set_error(BEM_ERR_INVALID, "matrix and right-hand side must both be real or both complex");
@@ -1769,6 +1878,7 @@ int assemble_vkw(Context* ctx, const bem_op_desc* d, void* V, int64_t ldv, void*
         return set_error(BEM_ERR_INVALID, "the V/K/W pass takes the P0 single-layer descriptor");
     if (d->flags & BEM_FLAG_DK)
         return set_error(BEM_ERR_UNSUPPORTED, "k-derivative operators are matrix-free only (bem_apply)");
+    if (d->flags & BEM_FLAG_TRANSPOSE) return set_error(BEM_ERR_INVALID, "BEM_FLAG_TRANSPOSE is a bem_apply flag");
     Mesh& m = ctx->mesh;
     if (ldv < m.nf || ldk < m.nv || ldw < m.nv)
         return set_error(BEM_ERR_INVALID, "leading dimension smaller than the block width");
@@ -1822,6 +1932,7 @@ int vkw_dk_forms(Context* ctx, const bem_op_desc* d, const void* aN, const void*
     if (d->kernel == BEM_LAPLACE) return set_error(BEM_ERR_INVALID, "the Laplace kernel does not depend on k");
     if (d->precision != BEM_PREC_FP64 || d->dtype != BEM_C128)
         return set_error(BEM_ERR_UNSUPPORTED, "k-derivative forms run in FP64 on complex128 vectors");
+    if (d->flags & BEM_FLAG_TRANSPOSE) return set_error(BEM_ERR_INVALID, "BEM_FLAG_TRANSPOSE is a bem_apply flag");
     rc = mesh_build_near(ctx, d->flags & ~BEM_FLAG_DK, s);
     if (rc) return rc;
     Mesh& m = ctx->mesh;

@@ -275,7 +275,16 @@ int bem_lu_solve_transpose(bem_context* ctx, const void* LU, int64_t ld, int32_t
 int bem_shape_gradient(bem_context* ctx, const bem_op_desc* op, const void* lam, const void* u, double alpha_re,
                        double alpha_im, double* grad_vertices, double* grad_k, void* stream) {
     return with_device(ctx, [&]() -> int {
-        return bem::shape_gradient(&ctx->impl, op, lam, u, alpha_re, alpha_im, grad_vertices, grad_k,
+        return bem::shape_gradient(&ctx->impl, op, lam, u, alpha_re, alpha_im, grad_vertices, grad_k, nullptr,
+                                   as_stream(stream));
+    });
+}
+
+int bem_shape_gradient_kt(bem_context* ctx, const bem_op_desc* op, const void* lam, const void* u,
+                          double alpha_re, double alpha_im, double* grad_vertices, double* grad_k, void* kt,
+                          void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::shape_gradient(&ctx->impl, op, lam, u, alpha_re, alpha_im, grad_vertices, grad_k, kt,
                                    as_stream(stream));
     });
 }

@@ -132,7 +132,7 @@ int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void*
 
 // adjoint contractions (bem_adjoint.hip)
 int shape_gradient(Context* ctx, const bem_op_desc* d, const void* lam, const void* u, double ar, double ai,
-                   double* grad, double* grad_k, hipStream_t s);
+                   double* grad, double* grad_k, void* kt, hipStream_t s);
 int area_gradient(Context* ctx, const double* coef, double* grad, hipStream_t s);
 
 // dense linear algebra (bem_linalg.hip)

@@ -112,6 +112,37 @@ def test_shape_gradient_translation_invariance(bem):
     assert s <= 1e-11 * grad.abs().max().item()
 
 
+@pytest.mark.parametrize("kernel,k,flags", [(1, 2.0, 0), (1, 3.0, 3), (0, 0.0, 0)])
+def test_shape_gradient_kt(bem, kernel, k, flags):
+    """bem_shape_gradient_kt: kt = K^T lam equals the assembled K, transposed,
+    times lam; its vertex and k gradients equal bem_shape_gradient's.  L3
+    (1280 panels): five trial blocks of the far kernel, near pairs of every
+    relation."""
+    v, f = bem.meshes.perturbed_icosphere(3, seed=4)
+    ctx = make_ctx(bem, v, f)
+    rng = np.random.default_rng(9)
+    lam, u = crandn(rng, len(f)), crandn(rng, len(v))
+    desc = bem.Context.op_desc(1, kernel, k, 1.0, 0, 1, 0, 1, flags)
+    g1 = torch.zeros(len(v), 3, dtype=torch.float64, device=DEV)
+    k1 = torch.zeros(1, dtype=torch.float64, device=DEV)
+    ctx.shape_gradient(desc, lam, u, g1, k1, alpha=0.5 - 1.5j)
+    g2 = torch.zeros_like(g1)
+    k2 = torch.zeros_like(k1)
+    kt = torch.zeros(len(v), dtype=torch.complex128, device=DEV)
+    ctx.shape_gradient(desc, lam, u, g2, k2, alpha=0.5 - 1.5j, kt=kt)
+    assert (g2 - g1).abs().max().item() <= 1e-12 * g1.abs().max().item()
+    if kernel:
+        assert abs(float(k2) - float(k1)) <= 1e-12 * abs(float(k1))
+    K = torch.zeros(len(f), len(v), dtype=torch.complex128, device=DEV)
+    ctx.assemble(desc, K)
+    ref = K.transpose(0, 1) @ lam
+    err = (kt - ref).norm().item() / ref.norm().item()
+    assert err <= 1e-13, err
+    with pytest.raises(bem.BEMError):  # kt is for K only
+        ctx.shape_gradient(bem.Context.op_desc(0, kernel, k, 1.0, 0, 0, 0, 1, flags), lam, crandn(rng, len(f)),
+                           g1, k1, kt=kt)
+
+
 def test_area_gradient(bem):
     v, f = bem.meshes.perturbed_icosphere(2, seed=0)
     rng = np.random.default_rng(5)

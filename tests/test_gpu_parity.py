@@ -145,6 +145,63 @@ def test_apply_fp64(bem, oracle, meshes, op, dt, dr, kernel, k, sqrt_n):
     assert rel(to_np(yt), ref) <= TOL64
 
 
+@pytest.mark.parametrize("flags", [0, 3])
+@pytest.mark.parametrize("kernel,k,sqrt_n", KERNELS)
+@pytest.mark.parametrize("op,dt,dr", OPS)
+def test_apply_transpose_fp64(bem, oracle, meshes, op, dt, dr, kernel, k, sqrt_n, flags):
+    """BEM_FLAG_TRANSPOSE: y[trial dof] += alpha Op^T x against the oracle's
+    assembled matrix, both quadrature presets (80 panels: two test-panel
+    chunks of the far kernel)."""
+    v, f = meshes["p1"]
+    ctx = make_ctx(bem, v, f)
+    rng = np.random.default_rng(11)
+    nx = len(v) if dt else len(f)  # x on the test dofs
+    ny = len(v) if dr else len(f)
+    x = rng.standard_normal(nx) + 1j * rng.standard_normal(nx)
+    y0 = rng.standard_normal(ny) + 1j * rng.standard_normal(ny)
+    yt = torch.tensor(y0, device=DEV)
+    desc = bem.Context.op_desc(op, kernel, k, sqrt_n, dt, dr, 0, 0, flags | bem._lib.FLAG_TRANSPOSE)
+    ctx.apply(desc, torch.tensor(x, device=DEV), yt, 0.75 - 0.25j, conj_x=False)
+    A = oracle.assemble(oracle.make_op(op, kernel, k, sqrt_n, 5, dt, dr, flags), v, f)
+    ref = y0 + (0.75 - 0.25j) * (A.T @ x)
+    assert rel(to_np(yt), ref) <= TOL64
+
+
+@pytest.mark.parametrize("op,dt,dr", [(1, 0, 1), (0, 0, 0), (3, 1, 1)])
+def test_apply_transpose_matches_assembled_l3(bem, op, dt, dr):
+    """At 1280 panels (20 test-panel chunks x 5 trial blocks) the transposed
+    product equals the library's own assembled matrix, transposed, times x;
+    conj_x conjugates x as in the forward product."""
+    v, f = bem.meshes.perturbed_icosphere(3, seed=2)
+    ctx = make_ctx(bem, v, f)
+    nx = len(v) if dt else len(f)
+    ny = len(v) if dr else len(f)
+    A = torch.zeros(nx, ny, dtype=torch.complex128, device=DEV)
+    ctx.assemble(bem.Context.op_desc(op, 1, 2.5, 1.0, dt, dr, 0, 0, 0), A)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    x = torch.randn(nx, dtype=torch.complex128, generator=g).to(DEV)
+    y = torch.zeros(ny, dtype=torch.complex128, device=DEV)
+    desc = bem.Context.op_desc(op, 1, 2.5, 1.0, dt, dr, 0, 0, bem._lib.FLAG_TRANSPOSE)
+    ctx.apply(desc, x, y, 1.0, conj_x=True)
+    ref = A.transpose(0, 1) @ x.conj()
+    assert rel(to_np(y), to_np(ref)) <= 1e-13
+
+
+def test_apply_transpose_rejects(bem, meshes):
+    v, f = meshes["p1"]
+    ctx = make_ctx(bem, v, f)
+    T = bem._lib.FLAG_TRANSPOSE
+    y = torch.zeros(len(v), dtype=torch.complex128, device=DEV)
+    x = torch.ones(len(f), dtype=torch.complex128, device=DEV)
+    with pytest.raises(NotImplementedError):  # FP32
+        ctx.apply(bem.Context.op_desc(1, 1, 2.0, 1.0, 0, 1, 1, 0, T), x, y)
+    with pytest.raises(NotImplementedError):  # with the k-derivative
+        ctx.apply(bem.Context.op_desc(1, 1, 2.0, 1.0, 0, 1, 0, 0, T | bem._lib.FLAG_DK), x, y)
+    A = torch.zeros(len(f), len(v), dtype=torch.complex128, device=DEV)
+    with pytest.raises(bem.BEMError):  # assembly takes no transpose flag
+        ctx.assemble(bem.Context.op_desc(1, 1, 2.0, 1.0, 0, 1, 0, 0, T), A)
+
+
 @pytest.mark.parametrize("dt,dr", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_mass(bem, oracle, meshes, dt, dr):
     v, f = meshes["p1"]
