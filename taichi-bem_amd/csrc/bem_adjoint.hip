@@ -108,9 +108,11 @@ __device__ __forceinline__ void trig_lds(double kap, double r, const double2* ta
 
 // ---------------------------------------------------------------------------
 // Regular pairs.  Block: 256 trial panels (one per lane) x TI test panels
-// (wave-uniform, scalar loads of the test points).  The trial-vertex
+// (wave-uniform: every lane reads the same test points).  The trial-vertex
 // gradient stays in registers over the TI test panels; the test-vertex
-// gradient is reduced over the block per test panel.
+// gradient is reduced over the block per test panel.  Launch bounds ask for
+// two waves per SIMD; the K Helmholtz variant then spills a few registers,
+// whose loads and stores run once per panel pair, outside the point loops.
 //
 // V is symmetric pair by pair (V_ji sums the same point pairs as V_ij with
 // the roles of the two rule copies exchanged, and G(-d) = G(d)), so V runs
