@@ -89,3 +89,19 @@ def test_status_mapping(bem, monkeypatch):
         bem._lib.check(bem._lib.BEM_ERR_SINGULAR)
     with pytest.raises(RuntimeError):
         bem._lib.check(bem._lib.BEM_ERR_INVALID)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N > 1 exits with status 2 and a JSON line saying the
+    build runs on one GPU (before anything touches a device)."""
+    import json
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
