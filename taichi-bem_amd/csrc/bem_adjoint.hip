@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
     const int jj = jvalid ? j : 0;
     const int q2 = p.q2, nf = p.nf;
     const double* __restrict__ PL = p.pl;
-    const double* __restrict__ PHI = p.phi;
+    const ConstPtr<double> PHI = as_const(p.phi);  // wave-uniform: scalar loads
     const double kap = p.kap;
 
     // the trial panel's vertices are reloaded per pair in the epilogue
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
             for (int d = 0; d < 3; ++d) gx[m][d] = 0.0;
         double gkp = 0.0;
         if (jvalid && (OP != BEM_OP_V || i < j) && !shares_vertex_adj(p.tris, i, j)) {
-            const double* __restrict__ PU = p.pu + (int64_t)i * q2 * 4;
+            const ConstPtr<double> PU = as_const(p.pu + (int64_t)i * q2 * 4);  // wave-uniform test points
             // accumulators over the pair (complex): test-slot gradient EX,
             // trial-slot gradient EY, value S0, k-derivative SK, and for K the
             // N_j-derivative vector T.  EX[0] is not accumulated: the six

@@ -136,9 +136,11 @@ __device__ __forceinline__ void far_pair(const OpParams& p, int i, int j, double
     constexpr int NS = DT ? 3 : 1, NT = DR ? 3 : 1;
     const int q2 = p.q2;
     const int nf = p.nf;
-    const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+    // test panel i is wave-uniform in every caller: its points and the basis
+    // table are read with scalar loads
+    const ConstPtr<R> PU = as_const(Ptr<R>::pu(p) + (int64_t)i * q2 * 4);
     const R* __restrict__ PL = Ptr<R>::pl(p);
-    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    const ConstPtr<R> PHI = as_const(Ptr<R>::phi(p));
     // FP32: d = (c_i - c_j) + (o_a - o_b); FP64: absolute coordinates.
     R dcx = 0, dcy = 0, dcz = 0;
     if constexpr (sizeof(R) == 4) {
@@ -1317,9 +1319,11 @@ template <typename R>
 __device__ __forceinline__ void far_pair_cf_sym(const OpParams& p, double eta, int i, int j, double (&Cr)[3][3],
                                                 double (&Ci)[3][3], double (&Dr)[3][3], double (&Di)[3][3]) {
     const int q2 = p.q2, nf = p.nf;
-    const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+    // wave-uniform test panel (the problem table is read per item, so the
+    // compiler is told): scalar loads of its points and of the basis table
+    const ConstPtr<R> PU = as_const_uniform(Ptr<R>::pu(p) + (int64_t)i * q2 * 4);
     const R* __restrict__ PL = Ptr<R>::pl(p);
-    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    const ConstPtr<R> PHI = as_const_uniform(Ptr<R>::phi(p));
     R dcx = 0, dcy = 0, dcz = 0;
     if constexpr (sizeof(R) == 4) {
         dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
