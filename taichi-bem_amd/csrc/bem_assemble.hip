@@ -747,7 +747,7 @@ __global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb,
     }
     __syncthreads();
     const R* __restrict__ PL = Ptr<R>::pl(p);
-    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    const ConstPtr<R> PHI = as_const(Ptr<R>::phi(p));  // scalar loads
     const R kap = (R)p.kap;
     // trial-side data of this lane's panel j
     const R nnx = (R)p.normal[3 * jj], nny = (R)p.normal[3 * jj + 1], nnz = (R)p.normal[3 * jj + 2];
@@ -769,7 +769,8 @@ __global__ __launch_bounds__(256) void far_dirichlet_kernel(OpParams p, int njb,
         const int i = i0 + ti;
         double kir = 0.0, kii = 0.0;
         if (i < i_end && jvalid && j > i && !shares_vertex(p.tris, i, j)) {
-            const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+            // wave-uniform test points, read with scalar loads
+            const ConstPtr<R> PU = as_const(Ptr<R>::pu(p) + (int64_t)i * q2 * 4);
             R dcx = 0, dcy = 0, dcz = 0;
             if constexpr (sizeof(R) == 4) {
                 dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
@@ -920,14 +921,15 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
     }
     if (j >= nf) return;  // no barrier below
     const R* __restrict__ PL = Ptr<R>::pl(p);
-    const R* __restrict__ PHI = Ptr<R>::phi(p);
+    const ConstPtr<R> PHI = as_const(Ptr<R>::phi(p));  // scalar loads
     const R kap = (R)p.kap;
     const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
     const int64_t vj[3] = {p.tris[3 * j], p.tris[3 * j + 1], p.tris[3 * j + 2]};
     const int i_end = min(nf, i0 + DTI);
     for (int i = i0; i < i_end; ++i) {
         if (j <= i || shares_vertex(p.tris, i, j)) continue;
-        const R* __restrict__ PU = Ptr<R>::pu(p) + (int64_t)i * q2 * 4;
+        // wave-uniform test points, read with scalar loads
+        const ConstPtr<R> PU = as_const(Ptr<R>::pu(p) + (int64_t)i * q2 * 4);
         R dcx = 0, dcy = 0, dcz = 0;
         if constexpr (sizeof(R) == 4) {
             dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
@@ -1105,7 +1107,7 @@ __global__ __launch_bounds__(256) void far_vkw_dk_kernel(OpParams p, DkVecs v, i
     for (int q = 0; q < 8; ++q) c[q] = 0.0;
     if (j < nf) {
         const double* __restrict__ PL = p.pl;
-        const double* __restrict__ PHI = p.phi;
+        const ConstPtr<double> PHI = as_const(p.phi);  // scalar loads
         const double kap = p.kap, dkap = p.dkap, k2n = p.k2n, dk2n = p.dk2n, kd = p.kap * p.dkap;
         const double nnx = p.normal[3 * j], nny = p.normal[3 * j + 1], nnz = p.normal[3 * j + 2];
         const int vj[3] = {p.tris[3 * j], p.tris[3 * j + 1], p.tris[3 * j + 2]};
@@ -1119,7 +1121,7 @@ __global__ __launch_bounds__(256) void far_vkw_dk_kernel(OpParams p, DkVecs v, i
         const int i_end = min(nf, i0 + DTI);
         for (int i = i0; i < i_end; ++i) {
             if (j <= i || shares_vertex(p.tris, i, j)) continue;
-            const double* __restrict__ PU = p.pu + (int64_t)i * q2 * 4;
+            const ConstPtr<double> PU = as_const(p.pu + (int64_t)i * q2 * 4);  // wave-uniform test points
             const double mix = p.normal[3 * i], miy = p.normal[3 * i + 1], miz = p.normal[3 * i + 2];
             const double pun = PU[0] * mix + PU[1] * miy + PU[2] * miz;
             double vr = 0, vi = 0, kar[3], kai[3], kbr[3], kbi[3], wr[3][3], wi[3][3];
