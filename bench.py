@@ -39,8 +39,8 @@ FLOPS_PER_EVAL_FUSED = 47
 # workload from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the
 # same counter on a 512 MiB copy (FETCH x 2.00, WRITE x 1.00), and the
 # hardware FP64 VALU flops of that launch (SQ_INSTS_VALU_FLOPS_FP64 x 64
-# lanes).  Measured: 95.7 GB fetched + 8.6 GB written; 9.930e12 flops.
-FAR_L5_TRAFFIC_BYTES = 104_298_600_000
+# lanes).  Measured: 101.8 GB fetched + 12.3 GB written; 9.930e12 flops.
+FAR_L5_TRAFFIC_BYTES = 114_107_892_320
 FAR_L5_HW_FLOPS = 9_929_664_000_000
 
 
