@@ -32,7 +32,8 @@ EXPORTS = (
     "bem_vkw_dk_forms", "bem_apply",
     "bem_mass_assemble", "bem_mass_apply", "bem_lu_factor", "bem_lu_solve",
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
-    "bem_shape_gradient", "bem_shape_gradient_kt", "bem_area_gradient", "bem_assemble_combined_batched", "bem_lu_factor_batched",
+    "bem_shape_gradient", "bem_shape_gradient_kt", "bem_area_gradient", "bem_assemble_combined_batched",
+    "bem_lu_factor_batched",
     "bem_batch_solve", "bem_batch_condest", "bem_set_option",
 )
 
