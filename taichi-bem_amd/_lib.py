@@ -8,9 +8,12 @@ NotImplementedError for an unsupported GaussQR, utils/functions.py:80-83).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import subprocess
+import sys
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbem_hip.so")
@@ -169,9 +172,30 @@ def dtype_code(t) -> int:
 
 
 def _require_cuda(*ts) -> None:
+    """Every tensor crossing the C ABI is a device tensor whose storage holds
+    its values: a lazy conjugate or negative view (torch's conj/neg bits)
+    would hand the library the unconjugated, unnegated memory."""
     for t in ts:
-        if t is not None and not t.is_cuda:
+        if t is None:
+            continue
+        if t.is_conj() or t.is_neg():
+            raise BEMError("lazy conj/neg view: call resolve_conj()/resolve_neg() before passing it")
+        if not t.is_cuda:
             raise BEMError("the BEM library takes device (HIP) tensors only")
+
+
+# Contexts still open when the interpreter exits are closed by an atexit
+# hook, which runs before module teardown and before the C runtime's own exit
+# handlers (HIP's and a profiler's): the library's streams and device memory
+# are then released while the runtime is whole, never from a finaliser at
+# interpreter shutdown.
+_live_contexts: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def close_all_contexts() -> None:
+    for c in list(_live_contexts):
+        c.close()
 
 
 class Context:
@@ -184,13 +208,28 @@ class Context:
         self.device = int(device)
         self.nv = self.nf = 0
         self.gauss_qr = 0
+        _live_contexts.add(self)
 
     def close(self) -> None:
+        """Releases the context's device memory and streams (bem_destroy).
+        Idempotent; later calls on the context raise BEMError (null context)."""
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().bem_destroy(self._h)
-            self._h = None
+        self._h = None
+        _live_contexts.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def __del__(self):
+        # at interpreter shutdown the atexit hook has already closed every
+        # context; a finaliser then makes no HIP call
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

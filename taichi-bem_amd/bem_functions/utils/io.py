@@ -9,8 +9,7 @@ def warp_tensor(x: torch.Tensor) -> torch.Tensor:
     if n_ == 2:
         return torch.view_as_complex(x.contiguous())
     raise NotImplementedError(
-        "A Tensor with shape = (*, {}) can not be warped, "
-        "we only support 1 (Scalar) or 2 (Complex) for the last dim".format(n_)
+        f"last dimension {n_} has no complex reading: it must be 1 (real values) or 2 (real, imaginary pairs)"
     )
 
 

@@ -244,8 +244,22 @@ void mesh_free(Mesh* m) {
     *m = Mesh();
 }
 
-int mesh_build(Context* ctx, const double* d_verts, const int32_t* d_tris, int nv, int nf, int q,
-               hipStream_t s) {
+namespace {
+
+// Device buffers of one mesh build that do not outlive it.
+struct BuildTemps {
+    double* r12w = nullptr;
+    int32_t* vp_row = nullptr;
+    int32_t* vp_col = nullptr;
+    ~BuildTemps() {
+        if (r12w) hipFree(r12w);
+        if (vp_row) hipFree(vp_row);
+        if (vp_col) hipFree(vp_col);
+    }
+};
+
+int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, int nv, int nf, int q,
+                    hipStream_t s) {
     if (nv <= 0 || nf <= 0) return set_error(BEM_ERR_INVALID, "mesh must have vertices and panels");
     double gp[12], gw[12];
     if (!gauss01(q, gp, gw))
@@ -296,8 +310,9 @@ int mesh_build(Context* ctx, const double* d_verts, const int32_t* d_tris, int n
             mw[p] = gw[a] * gw[b] * r1;
         }
     for (int e = 0; e < q2 * 3; ++e) phi32[e] = (float)phi[e];
-    double* d_r12w = nullptr;
-    if (dev_alloc(&d_r12w, r12w.size())) return BEM_ERR_HIP;
+    BuildTemps tmp;
+    if (dev_alloc(&tmp.r12w, r12w.size())) return BEM_ERR_HIP;
+    double* d_r12w = tmp.r12w;
     hipMemcpyAsync(d_r12w, r12w.data(), r12w.size() * sizeof(double), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.phi, phi.data(), phi.size() * sizeof(double), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.phi32, phi32.data(), phi32.size() * sizeof(float), hipMemcpyHostToDevice, s);
@@ -311,8 +326,8 @@ int mesh_build(Context* ctx, const double* d_verts, const int32_t* d_tris, int n
         std::vector<int32_t> fill(vp_row.begin(), vp_row.end() - 1);
         for (int e = 0; e < nf * 3; ++e) vp_col[fill[m.h_tris[e]]++] = e / 3;
     }
-    int32_t *d_vp_row = nullptr, *d_vp_col = nullptr;
-    if (dev_alloc(&d_vp_row, vp_row.size()) || dev_alloc(&d_vp_col, vp_col.size())) return BEM_ERR_HIP;
+    if (dev_alloc(&tmp.vp_row, vp_row.size()) || dev_alloc(&tmp.vp_col, vp_col.size())) return BEM_ERR_HIP;
+    int32_t *d_vp_row = tmp.vp_row, *d_vp_col = tmp.vp_col;
     hipMemcpyAsync(d_vp_row, vp_row.data(), vp_row.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(d_vp_col, vp_col.data(), vp_col.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
 
@@ -331,9 +346,20 @@ int mesh_build(Context* ctx, const double* d_verts, const int32_t* d_tris, int n
     if (dev_alloc(&m.rule, rules.size())) return BEM_ERR_HIP;
     hipMemcpyAsync(m.rule, rules.data(), rules.size() * sizeof(NearPt), hipMemcpyHostToDevice, s);
     rc = check_hip(hipStreamSynchronize(s), "mesh build");
-    hipFree(d_r12w); hipFree(d_vp_row); hipFree(d_vp_col);
     if (rc) return rc;
     return mesh_build_near(ctx, 0, s);
+}
+
+}  // namespace
+
+// A build that fails leaves no mesh behind (nf = 0), so every later call
+// reports "no mesh" instead of running on partly filled tables or on panel
+// indices that were found out of range.
+int mesh_build(Context* ctx, const double* d_verts, const int32_t* d_tris, int nv, int nf, int q,
+               hipStream_t s) {
+    const int rc = mesh_build_impl(ctx, d_verts, d_tris, nv, nf, q, s);
+    if (rc != BEM_OK) mesh_free(&ctx->mesh);
+    return rc;
 }
 
 // Near-field list: all ordered pairs sharing at least one vertex, found

@@ -60,13 +60,16 @@ struct Mesh {
     size_t kpart_bytes = 0;
 };
 
+// Slots of Context::counters.
+constexpr int CTR_TRI_FAIL = 0;   // a triangular sweep gave up a wait (sticky within one solve call)
+
 struct Context {
     int device = 0;
     Mesh mesh;
     // LU / solver scratch
     void* work = nullptr;
     size_t work_bytes = 0;
-    int* counters = nullptr;      // small per-context flags
+    int* counters = nullptr;      // small per-context flags (CTR_* below)
     size_t counters_len = 0;
     void* tri_sync = nullptr;     // ticket and ready words of single-problem triangular sweeps
     size_t tri_sync_words = 0;

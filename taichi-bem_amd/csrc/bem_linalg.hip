@@ -1319,6 +1319,7 @@ struct SolveProb {
     double* w;
     unsigned* sync;
     long long wait;  // bound of every wait on another block's flag (wall-clock ticks, 100 MHz)
+    int* fail;       // raised when a wait gives up (the context's CTR_TRI_FAIL word)
 };
 template <bool B>
 __device__ __forceinline__ SolveProb get_sp(const SolveProb& inl, const SolveProb* tbl) {
@@ -1349,7 +1350,7 @@ __global__ void tri_sync_reset_kernel(SolveProb inl, const SolveProb* tbl, int b
 // the same storage by columns.  Arithmetic in FP64 whatever T is.  A wait
 // that does not end within the wait limit (two seconds by default; never
 // expected: every predecessor is resident) leaves NaN in the block instead of
-// hanging.
+// hanging, and raises the fail word the host reads after the solve.
 template <typename T, bool C, bool LOWER, bool TRANS, bool B>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const SolveProb* tbl) {
     const SolveProb P = get_sp<B>(inl, tbl);
@@ -1449,6 +1450,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const So
     }
     if (s_fail) {
         if (t < nb) { x[2 * (r0 + t)] = __builtin_nan(""); x[2 * (r0 + t) + 1] = __builtin_nan(""); }
+        if (t == 0) __hip_atomic_store(P.fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;  // the flag stays down: workgroups after this one time out too
     }
     if (wave == 0) {
@@ -2197,9 +2199,9 @@ void tri_launch(const SolveProb& one, const SolveProb* d_tbl, int nmax, int coun
 }
 
 template <typename T, bool C>
-void tri_solve_t(const T* LU, int64_t ld, int n, double* w, unsigned* sync, bool trans, long long wait,
+void tri_solve_t(const T* LU, int64_t ld, int n, double* w, unsigned* sync, bool trans, long long wait, int* fail,
                  hipStream_t s) {
-    const SolveProb P{LU, ld, n, w, sync, wait};
+    const SolveProb P{LU, ld, n, w, sync, wait, fail};
     if (trans) tri_launch<T, C, true, false>(P, nullptr, n, 1, s);
     else tri_launch<T, C, false, false>(P, nullptr, n, 1, s);
 }
@@ -2221,12 +2223,43 @@ int tri_solve_any(Context* ctx, const void* LU, int64_t ld, int n, int dtype, do
     int rc = ensure_tri_sync(ctx, n);
     if (rc) return rc;
     unsigned* sync = (unsigned*)ctx->tri_sync;
+    int* fail = ctx->counters + CTR_TRI_FAIL;
+    const long long wt = ctx->wait_ticks;
     switch (dtype) {
-        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
-        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
-        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
-        default: tri_solve_t<float, true>((const float*)LU, ld, n, w, sync, trans, ctx->wait_ticks, s); break;
+        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w, sync, trans, wt, fail, s); break;
+        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w, sync, trans, wt, fail, s); break;
+        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w, sync, trans, wt, fail, s); break;
+        default: tri_solve_t<float, true>((const float*)LU, ld, n, w, sync, trans, wt, fail, s); break;
     }
+    return BEM_OK;
+}
+
+__global__ void fail_reset_kernel(int* f) {
+    if (threadIdx.x == 0) *f = 0;
+}
+
+// The fail word of the triangular sweeps is cleared once per solve call (a
+// kernel, so the sequence can be captured) and stays raised through the
+// call's later sweeps.
+void tri_fail_reset(Context* ctx, hipStream_t s) {
+    hipLaunchKernelGGL(fail_reset_kernel, dim3(1), dim3(64), 0, s, ctx->counters + CTR_TRI_FAIL);
+}
+
+// Reports a sweep that gave up a wait as an error.  Under stream capture the
+// word cannot be read back; the NaN the sweep left in its block is then the
+// only sign.
+int tri_fail_check(Context* ctx, hipStream_t s, const char* what) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    int rc = check_hip(hipStreamIsCapturing(s, &cs), "capture status");
+    if (rc) return rc;
+    if (cs != hipStreamCaptureStatusNone) return BEM_OK;
+    int h = 0;
+    rc = check_hip(hipMemcpyAsync(&h, ctx->counters + CTR_TRI_FAIL, sizeof(int), hipMemcpyDeviceToHost, s),
+                   "triangular solve status");
+    if (rc) return rc;
+    rc = check_hip(hipStreamSynchronize(s), "triangular solve sync");
+    if (rc) return rc;
+    if (h) return set_error(BEM_ERR_HIP, std::string(what) + ": a triangular sweep's wait timed out");
     return BEM_OK;
 }
 
@@ -2316,11 +2349,13 @@ int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const i
     rc = make_perm(ipiv, n, perm, s);
     if (rc) return rc;
     dim3 g((n + 255) / 256);
+    tri_fail_reset(ctx, s);
     hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, (const void*)b, dtype, n, w0);
     hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)w0, (const int32_t*)perm, n, w1);
     if ((rc = tri_solve_any(ctx, LU, ld, n, dtype, w1, false, s))) return rc;
     hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
-    return check_hip(hipGetLastError(), "LU solve");
+    if ((rc = check_hip(hipGetLastError(), "LU solve"))) return rc;
+    return tri_fail_check(ctx, s, "LU solve");
 }
 
 // A^T x = b (plain transpose, no conjugation): with P A = L U,
@@ -2338,11 +2373,13 @@ int lu_solve_transpose(Context* ctx, const void* LU, int64_t ld, int n, int dtyp
     rc = make_perm(ipiv, n, perm, s);
     if (rc) return rc;
     dim3 g((n + 255) / 256);
+    tri_fail_reset(ctx, s);
     hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, (const void*)b, dtype, n, w0);
     if ((rc = tri_solve_any(ctx, LU, ld, n, dtype, w0, true, s))) return rc;
     hipLaunchKernelGGL(unpermute_kernel, g, dim3(256), 0, s, (const double*)w0, (const int32_t*)perm, n, w1);
     hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
-    return check_hip(hipGetLastError(), "transposed LU solve");
+    if ((rc = check_hip(hipGetLastError(), "transposed LU solve"))) return rc;
+    return tri_fail_check(ctx, s, "transposed LU solve");
 }
 
 int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, const void* x,
@@ -2393,6 +2430,7 @@ int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const vo
         *out = sqrt(*out);
         return r2;
     };
+    tri_fail_reset(ctx, s);
     hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, b, dtype64, n, wb);
     double bn = 0.0;
     if ((rc = norm(wb, &bn))) return rc;
@@ -2420,7 +2458,8 @@ int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const vo
         st->iterations = it; st->converged = conv; st->residual_norm0 = r0n;
         st->residual_norm = rn; st->b_norm = bn;
     }
-    return rc;
+    if (rc) return rc;
+    return tri_fail_check(ctx, s, "refined solve");
 }
 
 int matrix_axpy(const void* src, int64_t lds, void* dst, int64_t ldd, int dtype, int rows, int cols,
@@ -2740,7 +2779,7 @@ int batch_plan(Context* ctx, int slot, int count, const void* const* LU, const i
         p += 256;
         v.out = out ? out + (size_t)out_stride * q : nullptr;
         vt_h[q] = v;
-        st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w, (unsigned*)p, ctx->wait_ticks};
+        st_h[q] = SolveProb{LU[q], ld[q], n[q], v.w, (unsigned*)p, ctx->wait_ticks, ctx->counters + CTR_TRI_FAIL};
         plan.nmax = std::max(plan.nmax, n[q]);
     }
     // The sweep kernels take the problems in table order (blockIdx.z), and
@@ -2855,13 +2894,15 @@ int batch_solve(Context* ctx, int count, const void* const* LU, const int64_t* l
     BatchPlan pl;
     rc = batch_plan(ctx, 0, count, LU, ld, n, ipiv, A, lda, b, x, resid, 1, pl, s);
     if (rc) return rc;
+    tri_fail_reset(ctx, s);
     switch (dtype) {
         case BEM_F64: batch_solve_t<double, false>(pl, refine_iters, resid != nullptr, s); break;
         case BEM_C128: batch_solve_t<double, true>(pl, refine_iters, resid != nullptr, s); break;
         case BEM_F32: batch_solve_t<float, false>(pl, refine_iters, resid != nullptr, s); break;
         default: batch_solve_t<float, true>(pl, refine_iters, resid != nullptr, s); break;
     }
-    return check_hip(hipGetLastError(), "batched solve");
+    if ((rc = check_hip(hipGetLastError(), "batched solve"))) return rc;
+    return tri_fail_check(ctx, s, "batched solve");
 }
 
 int batch_condest(Context* ctx, int count, const void* const* LU, const int64_t* ld, const int32_t* n, int dtype,
@@ -2875,13 +2916,15 @@ int batch_condest(Context* ctx, int count, const void* const* LU, const int64_t*
     BatchPlan pl;
     rc = batch_plan(ctx, 1, count, LU, ld, n, ipiv, A, lda, nullptr, nullptr, out, 2, pl, s);
     if (rc) return rc;
+    tri_fail_reset(ctx, s);
     switch (dtype) {
         case BEM_F64: batch_condest_t<double, false>(pl, iters, s); break;
         case BEM_C128: batch_condest_t<double, true>(pl, iters, s); break;
         case BEM_F32: batch_condest_t<float, false>(pl, iters, s); break;
         default: batch_condest_t<float, true>(pl, iters, s); break;
     }
-    return check_hip(hipGetLastError(), "batched condition estimate");
+    if ((rc = check_hip(hipGetLastError(), "batched condition estimate"))) return rc;
+    return tri_fail_check(ctx, s, "batched condition estimate");
 }
 
 }  // namespace bem

@@ -12,7 +12,7 @@ import torch
 from .. import _lib
 from ..bem_functions import (AdjDoubleLayer3d, DoubleLayer3d, HypersingularLayer3d, IdentityLayer3d,
                              RHSConstructor3d, SingleLayer3d)
-from ..bem_functions.utils import (AssembleType, KernelType, ScopeType, VertAttachType,
+from ..bem_functions.utils import (AssembleType, CellFluxType, KernelType, ScopeType, VertAttachType,
                                    get_gaussion_integration_points_and_weights)
 
 _PRECISION = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32}
@@ -247,7 +247,9 @@ class BEMManager:
             A = self._mat_A
             A64 = A if A.dtype in (torch.complex128, torch.float64) else A.to(
                 torch.complex128 if A.is_complex() else torch.float64)
-            x = f.conj().to(A64.dtype).contiguous()
+            # conj() is a lazy view; the library reads storage, so the
+            # conjugate is materialised before the pointer is taken
+            x = f.to(A64.dtype).conj().resolve_conj().contiguous()
             zero = torch.zeros_like(x)
             r = torch.empty_like(x)
             self.ctx.residual(A64, x, zero, r)  # r = 0 - A x
@@ -319,28 +321,40 @@ class BEMManager:
             self.raw_analytical_solved[off: off + self.M_Dirichlet] = rc._as_field(v)
 
     def splat_u_from_raw_solved_to_vertices(self):
-        """Area-weighted panel -> vertex average for P0, identity for P1."""
-        mm = self._core_manager._mesh_manager
+        """BEM_manager.py:753-801: the Neumann unknowns are added into the
+        vertex fields unless only the Dirichlet field is shown, and the
+        Dirichlet unknowns unless only the Neumann field is shown (`_show` is
+        -1, both, except for transmission, where it is the Neumann field,
+        BEM_manager.py:28,52-53).  P0 values are spread area-weighted over a
+        panel's vertices, P1 values are added as they are; panels and vertices
+        of the other unknown type are skipped."""
         self.solved.zero_()
         self.analytical_solved.zero_()
-        tris = mm.panels.long()
-        show_dirichlet = self._is_transmission == 0 and self.N_Neumann == 0
-        if not show_dirichlet and self.N_Neumann > 0:
-            off, q = self._Neumann_offset_j, self.shape_func_degree_Neumann
-        else:
-            off, q = self._Dirichlet_offset_j, self.shape_func_degree_Dirichlet
-        n = self.N_Neumann if not show_dirichlet and self.N_Neumann > 0 else self.M_Dirichlet
+        show = int(VertAttachType.NEUMANN_TOBESOLVED) if self._is_transmission else -1
+        if show != int(VertAttachType.DIRICHLET_TOBESOLVED) and self.N_Neumann > 0:
+            self._splat(self._Neumann_offset_j, self.N_Neumann, self.shape_func_degree_Neumann,
+                        int(CellFluxType.DIRICHLET_TOBESOLVED), int(VertAttachType.DIRICHLET_TOBESOLVED))
+        if show != int(VertAttachType.NEUMANN_TOBESOLVED) and self.M_Dirichlet > 0:
+            self._splat(self._Dirichlet_offset_j, self.M_Dirichlet, self.shape_func_degree_Dirichlet,
+                        int(CellFluxType.NEUMANN_TOBESOLVED), int(VertAttachType.NEUMANN_TOBESOLVED))
+
+    def _splat(self, off: int, n: int, degree: int, skip_panel: int, skip_vertex: int):
+        mm = self._core_manager._mesh_manager
         sol = self.raw_solved[off: off + n]
         ana = self.raw_analytical_solved[off: off + n]
-        if q == 0:
-            w = (mm.get_panel_areas() / 3.0).to(self.solved.real.dtype)
+        if degree == 0:
+            # the dof maps are the identity: local panel I is panel I
+            tris = mm.panels.long()
+            keep = (mm.get_panel_types() != skip_panel).to(self.solved.real.dtype)
+            w = (mm.get_panel_areas() / 3.0).to(self.solved.real.dtype) * keep
             va = mm.get_vert_areas().to(self.solved.real.dtype)
             for s in range(3):
                 self.solved.index_add_(0, tris[:, s], sol * w / va[tris[:, s]])
                 self.analytical_solved.index_add_(0, tris[:, s], ana * w / va[tris[:, s]])
         else:
-            self.solved.copy_(sol)
-            self.analytical_solved.copy_(ana)
+            keep = (mm.get_vertice_types() != skip_vertex).to(self.solved.real.dtype)
+            self.solved += sol * keep
+            self.analytical_solved += ana * keep
 
     def kill(self):
         self.initialized = False

@@ -105,3 +105,80 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_lazy_conj_views_are_rejected_at_the_boundary(bem):
+    """torch's conj() and neg() of a complex tensor are views over the
+    unchanged storage; the library reads storage through data_ptr(), so such
+    views are refused before any pointer is taken."""
+    L = bem._lib
+    x = torch.ones(4, dtype=torch.complex128)
+    for bad in (x.conj(), x.conj().contiguous(), x.conj().imag):
+        with pytest.raises(L.BEMError, match="lazy conj"):
+            L._require_cuda(bad)
+    with pytest.raises(L.BEMError, match="device"):
+        L._require_cuda(x.conj().resolve_conj())
+
+
+def _splat_manager(bem, nf_kind, transmission, panel_type, vert_type, v, f):
+    """A BEMManager holding only what splat_u_from_raw_solved_to_vertices
+    reads, on CPU tensors (no device call is made)."""
+    from types import SimpleNamespace
+
+    M = bem.managers.bem_manager.BEMManager
+    bm = M.__new__(M)
+    nv, nf = len(v), len(f)
+    area = 0.5 * np.linalg.norm(np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 1]]), axis=1)
+    va = np.zeros(nv)
+    np.add.at(va, f.ravel(), np.repeat(area / 3, 3))
+    mm = SimpleNamespace(
+        panels=torch.tensor(f, dtype=torch.int32),
+        get_panel_areas=lambda: torch.tensor(area),
+        get_vert_areas=lambda: torch.tensor(va),
+        get_panel_types=lambda: torch.full((nf,), panel_type, dtype=torch.int32),
+        get_vertice_types=lambda: torch.full((nv,), vert_type, dtype=torch.int32))
+    bm._core_manager = SimpleNamespace(_mesh_manager=mm)
+    bm._is_transmission = transmission
+    bm.N_Neumann, bm.M_Dirichlet = nf_kind
+    bm.shape_func_degree_Neumann, bm.shape_func_degree_Dirichlet = 0, 1
+    bm._Dirichlet_offset_j, bm._Neumann_offset_j = 0, bm.M_Dirichlet
+    n = bm.N_Neumann + bm.M_Dirichlet
+    rng = np.random.default_rng(7)
+    bm.raw_solved = torch.tensor(rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    bm.raw_analytical_solved = torch.tensor(rng.standard_normal(n) + 0j)
+    bm.solved = torch.zeros(nv, dtype=torch.complex128)
+    bm.analytical_solved = torch.zeros(nv, dtype=torch.complex128)
+    return bm, area, va
+
+
+@pytest.mark.parametrize("case", ["dirichlet_bvp", "neumann_bvp", "transmission", "both_fields"])
+def test_splat_follows_reference_show_branches(bem, case):
+    """BEM_manager.py:753-801: the P0 Neumann values are spread with
+    area/3/vertex-area weights (per panel, per vertex, as the reference loops
+    do), the P1 Dirichlet values are added as they are; transmission shows the
+    Neumann field only, other problems add both."""
+    CF, VA = bem.CellFluxType, bem.VertAttachType
+    v, f = bem.meshes.icosphere(1)
+    nv, nf = len(v), len(f)
+    kind, tr, pt, vt = {
+        "dirichlet_bvp": ((nf, 0), 0, CF.NEUMANN_TOBESOLVED, VA.NEUMANN_TOBESOLVED),
+        "neumann_bvp": ((0, nv), 0, CF.DIRICHLET_TOBESOLVED, VA.DIRICHLET_TOBESOLVED),
+        "transmission": ((nf, nv), 1, CF.BOTH_TOBESOLVED, VA.BOTH_TOBESOLVED),
+        "both_fields": ((nf, nv), 0, CF.BOTH_TOBESOLVED, VA.BOTH_TOBESOLVED),
+    }[case]
+    bm, area, va = _splat_manager(bem, kind, tr, int(pt), int(vt), v, f)
+    bm.splat_u_from_raw_solved_to_vertices()
+    raw = bm.raw_solved.numpy()
+    ref = np.zeros(nv, dtype=complex)
+    if bm.N_Neumann:
+        for i in range(nf):
+            for ii in range(3):
+                ref[f[i, ii]] += raw[bm.M_Dirichlet + i] * area[i] / 3.0 / va[f[i, ii]]
+    if bm.M_Dirichlet and not tr:
+        ref += raw[:nv]
+    assert np.abs(bm.solved.numpy() - ref).max() < 1e-13
+    # a constant P0 field stays constant after the area-weighted spread
+    if case == "dirichlet_bvp":
+        bm.raw_analytical_solved[:] = 2.5
+        bm.splat_u_from_raw_solved_to_vertices()
+        assert np.abs(bm.analytical_solved.numpy() - 2.5).max() < 1e-13

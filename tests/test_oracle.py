@@ -165,6 +165,111 @@ def test_sphere_eigenvalues_helmholtz_single_layer(bem, oracle):
         assert abs(lam - exact) < 0.03 * abs(exact)
 
 
+def _helmholtz_sphere_eigs(l, kap):
+    """Eigenvalues of V, K (= K' on the sphere, where (x - y).n_y =
+    (y - x).n_x = x.y - 1), and W on the unit sphere for G = e^{i kap r}/(4 pi r)
+    and outward normals, from the interior and exterior single-layer
+    potentials i kap j_l(kap |x|) h_l(kap) Y_l and their jump relations:
+        V: i kap j_l h_l,   K: 1/2 + i kap^2 j_l h_l',   W: -i kap^3 j_l' h_l'
+    (h_l = h_l^(1), W = -d/dn of the double-layer potential)."""
+    jl, jlp = special.spherical_jn(l, kap), special.spherical_jn(l, kap, derivative=True)
+    hl = jl + 1j * special.spherical_yn(l, kap)
+    hlp = jlp + 1j * special.spherical_yn(l, kap, derivative=True)
+    return 1j * kap * jl * hl, 0.5 + 1j * kap ** 2 * jl * hlp, -1j * kap ** 3 * jlp * hlp
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
+def test_helmholtz_sphere_eigenvalues_reduce_to_laplace(l):
+    """The closed forms used below tend to the Laplace eigenvalues 1/(2l+1),
+    -1/(2(2l+1)) and l(l+1)/(2l+1) as kap -> 0."""
+    v, kk, w = _helmholtz_sphere_eigs(l, 1e-6)
+    assert abs(v - 1 / (2 * l + 1)) < 1e-5
+    assert abs(kk + 1 / (2 * (2 * l + 1))) < 1e-5
+    assert abs(w - l * (l + 1) / (2 * l + 1)) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def l3_helmholtz(bem, oracle):
+    """Unit icosphere L3, Helmholtz k = 2 with the adapted near-field rules:
+    V (P0 x P0), K (P0 x P1), K' (P1 x P0), W (P1 x P1) and the mass blocks."""
+    v, f = bem.meshes.icosphere(3)
+    geom = oracle.geometry(v, f)
+    k = 2.0
+    mk = lambda op, dt, dr: oracle.make_op(op, oracle.HELMHOLTZ, k, q=4, deg_test=dt, deg_trial=dr, flags=3)
+    ops = {name: oracle.assemble(mk(op, dt, dr), v, f, geom=geom)
+           for name, (op, dt, dr) in {"V": (oracle.OP_V, 0, 0), "K": (oracle.OP_K, 0, 1),
+                                      "KADJ": (oracle.OP_KADJ, 1, 0), "W": (oracle.OP_W, 1, 1)}.items()}
+    mass = {(dt, dr): oracle.mass(4, dt, dr, v, f, geom=geom) for dt in (0, 1) for dr in (0, 1)}
+    return v, f, k, ops, mass
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
+def test_sphere_eigenvalues_helmholtz_all_operators(l, l3_helmholtz):
+    """Generalised Rayleigh quotients <psi, Op phi> / <psi, phi> with psi, phi
+    the P0 (centroid) or P1 (vertex) interpolants of Y_l against the closed
+    forms above: they pin the Helmholtz dG/dn_y factor (1 - i kap r) e^{i kap r}
+    of K and K' and the k^2 (n_x . n_y) term of W.  Discretisation error is
+    O(h^2) (below 1 % at L3); the W bar is taken on the size of its two terms,
+    which nearly cancel at l = 1."""
+    v, f, k, ops, mass = l3_helmholtz
+    c = v[f].mean(1)
+    c = c / np.linalg.norm(c, axis=1, keepdims=True)
+    y1 = _ylm_vertex(v, l, 0 if l == 0 else 1)
+    y0 = _ylm_vertex(c, l, 0 if l == 0 else 1)
+    ev, ek, ew = _helmholtz_sphere_eigs(l, k)
+    q = lambda a, M, b, B: (a @ M @ b) / (a @ B @ b)
+    assert abs(q(y0, ops["V"], y0, mass[0, 0]) - ev) < 0.02 * abs(ev)
+    assert abs(q(y0, ops["K"], y1, mass[0, 1]) - ek) < 0.02 * abs(ek)
+    assert abs(q(y1, ops["KADJ"], y0, mass[1, 0]) - ek) < 0.02 * abs(ek)
+    scale_w = l * (l + 1) / (2 * l + 1) + k * k * abs(ev)
+    assert abs(q(y1, ops["W"], y1, mass[1, 1]) - ew) < 0.01 * scale_w
+
+
+@pytest.mark.parametrize("flags", [0, 3])
+def test_transmission_kernel_is_helmholtz_at_scaled_wavenumber(bem, oracle, flags):
+    """The transmission kernel at (k, sqrt_n) is the Helmholtz kernel at
+    kap = k sqrt_n in G, dG/dn_y and W's k^2 n term (BEM_manager.py:452-525,
+    hypersingular_layer_3d.py:171): all four operators agree to rounding."""
+    v, f = bem.meshes.perturbed_icosphere(1, seed=5)
+    geom = oracle.geometry(v, f)
+    k, sn = 2.0, 1.5
+    for op, dt, dr in ((oracle.OP_V, 0, 0), (oracle.OP_K, 0, 1), (oracle.OP_KADJ, 1, 0), (oracle.OP_W, 1, 1)):
+        a = oracle.assemble(oracle.make_op(op, oracle.HELMHOLTZ_TRANSMISSION, k, sn, 5, dt, dr, flags), v, f,
+                            geom=geom)
+        b = oracle.assemble(oracle.make_op(op, oracle.HELMHOLTZ, k * sn, 1.0, 5, dt, dr, flags), v, f, geom=geom)
+        c = oracle.assemble(oracle.make_op(op, oracle.HELMHOLTZ, k, 1.0, 5, dt, dr, flags), v, f, geom=geom)
+        assert rel(a, b) < 1e-13, op
+        assert rel(a, c) > 1e-2, op
+
+
+def test_helmholtz_interior_dirichlet_converges_to_analytic_trace(bem, oracle):
+    """Interior Helmholtz Dirichlet problem (k = 3) with a point source
+    outside the unit sphere, x0 = (2, 0.5, 0.3): u = e^{ik|x-x0|}/(4 pi |x-x0|)
+    solves it, so V t = (M/2 + K) g must converge to t = du/dn at the panel
+    centroids.  The data are passed conjugated, since the reference's right-
+    hand side conjugates them (rhs_constructor_3d.py:361-372).  Adapted
+    near-field rules, O(h^2) from L1 to L3."""
+    k = 3.0
+    x0 = np.array([2.0, 0.5, 0.3])
+    errs = []
+    for L in (1, 2, 3):
+        v, f = bem.meshes.icosphere(L)
+        geom = oracle.geometry(v, f)
+        V = oracle.assemble(oracle.make_op(oracle.OP_V, oracle.HELMHOLTZ, k, q=5, flags=3), v, f, geom=geom)
+        r = np.linalg.norm(v - x0, axis=1)
+        u = np.exp(1j * k * r) / (4 * math.pi * r)
+        rhs = oracle.rhs_dirichlet(oracle.HELMHOLTZ, k, 5, v, f, np.conj(u), True, 1.0, flags=3, geom=geom)
+        t = np.linalg.solve(V, rhs)
+        c = v[f].mean(1)
+        d = c - x0
+        rc = np.linalg.norm(d, axis=1)
+        texact = (1j * k * rc - 1) * np.exp(1j * k * rc) / (4 * math.pi * rc ** 3) * np.einsum("ij,ij->i", d,
+                                                                                                  geom[1])
+        errs.append(rel(t, texact))
+    assert errs[1] < errs[0] / 2.5 and errs[2] < errs[1] / 2.5, errs
+    assert errs[2] < 0.06, errs
+
+
 # --------------------------------------------- singular rules vs closed form
 def _triangle_potential(x, p):
     """int_T 1/|x - y| dy for a flat triangle p[3,3] and any point x
