@@ -116,6 +116,13 @@ class WavenumberSweep:
             raise RuntimeError("capture() first")
         self.graph.replay()
 
+    def close(self) -> None:
+        """Releases the contexts of the batch and of every mesh."""
+        self.graph = None
+        for c in self.ctxs:
+            c.close()
+        self.batch_ctx.close()
+
     def results(self) -> list[dict]:
         info = self.info.cpu().tolist()
         res = self.resid.cpu().tolist()

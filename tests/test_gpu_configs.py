@@ -1,0 +1,201 @@
+"""The benchmark configurations on the GPU at sizes the oracle assembles in
+seconds, against the FP64 oracle or an all-FP64 GPU solve:
+
+* the Laplace interior Dirichlet problem on its own mesh (icosphere L3,
+  1280 panels) through the reference-shaped managers, both quadrature presets;
+* the combined-field batch of the wave-number sweep at k = 16.0 and 17.9 (the
+  ends of its band) on an uneven batch of two meshes, FP64 and FP32, and the
+  whole mixed sweep (c64 + FP64 refinement) against the complex128 sweep;
+* the mixed-precision Dirichlet pipeline of the L5 Helmholtz case, at L3:
+  c64 LU of the BEM matrix refined in FP64 against a complex128 solve, with
+  FP64 and with FP32 regular-pair assembly.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+TOL64 = 1e-11
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def make_ctx(bem, v, f, q=5):
+    print(f"[ctx] nv={len(v)} nf={len(f)} q={q}", flush=True)
+    c = bem.Context(0)
+    c.set_mesh(torch.as_tensor(v, device=DEV), torch.as_tensor(f, device=DEV), q)
+    return c
+
+
+# ------------------------------------------------------------------ config 1
+@pytest.mark.parametrize("quadrature,flags", [("reference", 0), ("corrected", 3)])
+def test_config1_l3_matches_oracle(bem, oracle, quadrature, flags):
+    """Laplace interior Dirichlet on the unit icosphere L3 (1280 panels, 642
+    vertices), g = 1/|x - (2, 0.5, 0.3)|, P0 Neumann / P1 Dirichlet, GaussQR 5:
+    the matrix, the right-hand side and the solution of the manager pipeline
+    against the oracle; the corrected rules converge to the analytic trace."""
+    x0 = torch.tensor([2.0, 0.5, 0.3], dtype=torch.float64, device=DEV)
+
+    def fD(x):
+        return 1.0 / torch.linalg.norm(x - x0, dim=-1)
+
+    def fN(x, n):
+        d = x - x0
+        return -(d * n).sum(-1) / torch.linalg.norm(d, dim=-1) ** 3
+
+    params = {"kernel": "Laplace", "boundary": "Dirichlet", "scope": "Interior", "GaussQR": 5,
+              "Q_Neumann": 0, "Q_Dirichlet": 1, "object": "icosphere", "level": 3, "quadrature": quadrature}
+    print(f"[config1 L3] {quadrature}", flush=True)
+    cm = bem.CoreManager(params)
+    cm.initialization(fD, fN)
+    cm.run()
+    bm = cm._BEM_manager
+    v, f = bem.meshes.icosphere(3)
+    geom = oracle.geometry(v, f)
+    V = oracle.assemble(oracle.make_op(oracle.OP_V, oracle.LAPLACE, 0.0, 1.0, 5, 0, 0, flags), v, f,
+                        geom=geom).real
+    g = 1.0 / np.linalg.norm(v - np.array([2.0, 0.5, 0.3]), axis=1)
+    rhs = oracle.rhs_dirichlet(oracle.LAPLACE, 0.0, 5, v, f, g.astype(complex), True, 1.0, flags=flags,
+                               geom=geom).real
+    assert rel(to_np(bm.get_mat_A()), V) <= TOL64
+    assert rel(to_np(bm.rhs), rhs) <= TOL64
+    t = np.linalg.solve(V, rhs)
+    assert rel(to_np(bm.raw_solved), t) <= 1e-10
+    e = rel(to_np(bm.raw_solved), to_np(bm.raw_analytical_solved))
+    print(f"[config1 L3] {quadrature}: analytic rel error {e:.3e}", flush=True)
+    if flags:
+        assert e < 0.02
+    else:
+        assert 0.1 < e < 0.3
+    cm.kill()
+
+
+# ------------------------------------------------------------------ config 3
+def _oracle_cf(oracle, v, f, k, eta, flags):
+    K = oracle.assemble(oracle.make_op(oracle.OP_K, oracle.HELMHOLTZ, k, 1.0, 5, 1, 1, flags), v, f)
+    V = oracle.assemble(oracle.make_op(oracle.OP_V, oracle.HELMHOLTZ, k, 1.0, 5, 1, 1, flags), v, f)
+    M = oracle.mass(5, 1, 1, v, f)
+    return 0.5 * M + K - 1j * eta * V
+
+
+@pytest.mark.parametrize("dtype,precision,tol", [(torch.complex128, 0, TOL64), (torch.complex64, 1, 3e-5)])
+def test_config3_band_ends_uneven_batch(bem, oracle, dtype, precision, tol):
+    """Combined field 1/2 M + K - i k V at the ends of the sweep's band,
+    k = 16.0 and 17.9, on an uneven batch (perturbed L1 and L2, 42 and 162
+    vertices) in one persistent launch set, against the oracle."""
+    L = bem._lib
+    meshes = [bem.meshes.perturbed_icosphere(1, seed=0), bem.meshes.perturbed_icosphere(2, seed=1)]
+    ctxs = [make_ctx(bem, v, f) for v, f in meshes]
+    probs = [(0, 16.0), (1, 16.0), (1, 17.9), (0, 17.9)]
+    mats = [torch.zeros(len(meshes[m][0]), len(meshes[m][0]), dtype=dtype, device=DEV) for m, _ in probs]
+    batch = L.Batch(bem.Context(0))
+    print(f"[config3] dtype={dtype}", flush=True)
+    batch.assemble_combined([(ctxs[m], k, k, A) for (m, k), A in zip(probs, mats)],
+                            L.C128 if dtype == torch.complex128 else L.C64, precision, 0)
+    for (m, k), A in zip(probs, mats):
+        v, f = meshes[m]
+        e = rel(to_np(A).astype(np.complex128), _oracle_cf(oracle, v, f, k, k, 0))
+        print(f"[config3] mesh {m} k={k}: rel err {e:.3e}", flush=True)
+        assert e <= tol
+
+
+def test_config3_mixed_sweep_band_ends(bem):
+    """The whole sweep (assembly, batched LU, solves with FP64 refinement,
+    condition estimates) in c64 with FP32 regular pairs, against the same
+    sweep in complex128 with FP64 assembly, at k = 16.0 and 17.9 on the
+    uneven batch."""
+    meshes = [bem.meshes.perturbed_icosphere(1, seed=0), bem.meshes.perturbed_icosphere(2, seed=1)]
+    probs = [(0, 16.0), (1, 16.0), (1, 17.9), (0, 17.9)]
+    d = bem.meshes.seeded_direction(2)
+    print("[config3 sweep] c64 + refinement", flush=True)
+    s32 = bem.sweep.WavenumberSweep(meshes, probs, dtype=torch.complex64, precision=1, refine_iters=3,
+                                    direction=d)
+    s32.run()
+    print("[config3 sweep] c128", flush=True)
+    s64 = bem.sweep.WavenumberSweep(meshes, probs, dtype=torch.complex128, precision=0, refine_iters=0,
+                                    direction=d)
+    s64.run()
+    torch.cuda.synchronize()
+    r32 = s32.results()
+    for q, (a, b) in enumerate(zip(s32.x, s64.x)):
+        e = rel(to_np(a), to_np(b))
+        print(f"[config3 sweep] problem {q}: rel err vs c128 {e:.3e}, residual {r32[q]['rel_residual']:.2e}, "
+              f"cond1 {r32[q]['cond1_est']:.3e}", flush=True)
+        assert r32[q]["info"] == 0
+        assert r32[q]["rel_residual"] <= 1e-6
+        assert e <= 1e-4
+        assert np.isfinite(r32[q]["cond1_est"]) and r32[q]["cond1_est"] >= 1.0
+
+
+# ------------------------------------------------------------------ config 2
+@pytest.fixture(scope="module")
+def l3_dirichlet(bem):
+    """Config-2 shape at L3: Helmholtz k = 3 interior Dirichlet on the seeded
+    perturbed icosphere (1280 panels), plane-wave data, and its complex128
+    FP64 solve (fused V + K conj(g) assembly, LU, solve)."""
+    L = bem._lib
+    v, f = bem.meshes.perturbed_icosphere(3, seed=0)
+    k = 3.0
+    ctx = make_ctx(bem, v, f)
+    d = torch.tensor(bem.meshes.seeded_direction(2), device=DEV)
+    g = torch.exp(1j * k * (torch.tensor(v, device=DEV) @ d)).to(torch.complex128)
+    nf = len(f)
+    desc = lambda prec, parts=0: bem.Context.op_desc(L.OP_V, 1, k, 1.0, 0, 0, prec, 0, 0, parts)  # noqa: E731
+    A64 = torch.zeros(nf, nf, dtype=torch.complex128, device=DEV)
+    b64 = torch.zeros(nf, dtype=torch.complex128, device=DEV)
+    ctx.assemble_dirichlet(desc(L.PREC_FP64), A64, g, b64, 1.0, 1.0, 0.5)
+    LU = A64.clone()
+    ipiv = torch.empty(nf, dtype=torch.int32, device=DEV)
+    ctx.lu_factor(LU, ipiv)
+    x64 = b64.clone()
+    ctx.lu_solve(LU, ipiv, x64)
+    return ctx, desc, g, A64, b64, x64
+
+
+def test_config2_mixed_fp64_assembly(bem, l3_dirichlet):
+    """c64 LU of the FP64-assembled matrix, FP64 residuals against it: the
+    refined solution meets the 1e-8 target against the complex128 solve."""
+    L = bem._lib
+    ctx, desc, g, A64, b64, x64 = l3_dirichlet
+    LU32 = torch.empty(A64.shape, dtype=torch.complex64, device=DEV)
+    bem.convert(A64, LU32)
+    ipiv = torch.empty(A64.shape[0], dtype=torch.int32, device=DEV)
+    ctx.lu_factor(LU32, ipiv)
+    x = torch.zeros_like(x64)
+    st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=1e-14)
+    e = rel(to_np(x), to_np(x64))
+    print(f"[config2 mixed fp64 asm] {st.iterations} steps, rel err {e:.3e}", flush=True)
+    assert st.converged and e <= 1e-8
+
+
+def test_config2_mixed_fp32_assembly(bem, l3_dirichlet):
+    """FP32 regular pairs + FP64 near field into c64, c64 LU, FP64 residuals
+    against that c64 matrix: refinement converges to the solution of the
+    FP32-assembled system, whose error against the FP64 solve is the
+    assembly's (about 1e-6; bound 2e-5)."""
+    L = bem._lib
+    ctx, desc, g, A64, b64, x64 = l3_dirichlet
+    nf = A64.shape[0]
+    A32 = torch.zeros(nf, nf, dtype=torch.complex64, device=DEV)
+    b = torch.zeros(nf, dtype=torch.complex128, device=DEV)
+    ctx.assemble_dirichlet(desc(L.PREC_FP32, L.PART_FAR), A32, g, b, 1.0, 1.0, 0.5)
+    ctx.assemble_dirichlet(desc(L.PREC_FP64, L.PART_NEAR), A32, g, b, 1.0, 1.0, 0.5)
+    LU32 = A32.clone()
+    ipiv = torch.empty(nf, dtype=torch.int32, device=DEV)
+    ctx.lu_factor(LU32, ipiv)
+    x = torch.zeros_like(x64)
+    st = ctx.refine_solve(A32, LU32, ipiv, b, x, max_iter=30, tol=1e-13)
+    e = rel(to_np(x), to_np(x64))
+    eb = rel(to_np(b), to_np(b64))
+    print(f"[config2 mixed fp32 asm] {st.iterations} steps, rhs err {eb:.3e}, rel err {e:.3e}", flush=True)
+    assert st.converged
+    assert eb <= 1e-6 and e <= 2e-5
