@@ -42,6 +42,8 @@ struct OpParams {
     const float* __restrict__ pl32;
     const double* __restrict__ phi;
     const float* __restrict__ phi32;
+    const double* __restrict__ frame;  // [nf][PANEL_FRAME] local frames
+    const double* __restrict__ pq;     // [nf][q2][4] (|x - C|^2, -2 r1, -2 r2, w)
     const NearPair* __restrict__ near;
     const int32_t* __restrict__ near_row;
     const NearPt* __restrict__ rule;
@@ -667,7 +669,7 @@ OpParams make_params(Context* ctx, const bem_op_desc* d) {
     p.nf = m.nf; p.nv = m.nv; p.q2 = m.q2; p.flags = d ? d->flags : 0;
     p.tris = m.tris; p.verts = m.verts; p.area = m.area; p.normal = m.normal; p.curl = m.curl;
     p.cen = m.cen; p.pu = m.pu; p.pl = m.pl; p.pu32 = m.pu32; p.pl32 = m.pl32;
-    p.phi = m.phi; p.phi32 = m.phi32; p.near = m.near; p.near_row = m.near_row; p.rule = m.rule;
+    p.phi = m.phi; p.phi32 = m.phi32; p.frame = m.frame; p.pq = m.pq; p.near = m.near; p.near_row = m.near_row; p.rule = m.rule;
     for (int r = 0; r < 4; ++r) { p.rule_off[r] = m.rule_off[r]; p.rule_len[r] = m.rule_len[r]; }
     p.n_near = m.n_near;
     p.kap = 0.0; p.k2n = 0.0; p.dkap = 0.0; p.dk2n = 0.0;
@@ -877,6 +879,224 @@ __global__ void kpart_reduce_kernel(const double* kpart, const double* krow, int
     for (int q = i / 256; q < njb; ++q) {
         a += kpart[((int64_t)i * njb + q) * 2];
         b += kpart[((int64_t)i * njb + q) * 2 + 1];
+    }
+    add_elem(y, dtype, i, ar * a - ai * b, ar * b + ai * a, false);
+}
+
+// ---------------------------------------------------------------------------
+// The FP64 Dirichlet pass in moment form.  Same pairs and outputs as
+// far_dirichlet_kernel (V into A[i][j] and A[j][i], K conj(g) of both rows),
+// with the test panel i in its local frame (Mesh::frame): a test point is
+// x_a - C = U + r1_a E1 + r2_a E2, and with Y = y_b - C for the trial point
+//   |x_a - y_b|^2   = S_a + Y.(Y + D) + p1_a (E1.Y) + p2_a (E2.Y)
+//   (x_a - y_b).n_j = (U - Y).n_j - (p1_a (E1.n_j) + p2_a (E2.n_j)) / 2
+//   (y_b - x_a).n_i = Y.n_i                      (x_a - C lies in panel i)
+// (p1 = -2 r1, p2 = -2 r2, S_a = |x_a - C|^2, D = -2U: Mesh::pq).  So per
+// point pair only r^2 (three operations), 1/r, exp(i kap r) and three
+// complex moments over the test points,
+//   T0 = sum_a t_a,  T1 = sum_a p1_a t_a,  T2 = sum_a p2_a t_a,
+//   t_a = w_a (1 - i kap r) e^{i kap r} / r^3,
+// are formed; both K rows follow from them per trial point: row i through
+// the affine (x_a - y_b).n_j, row j through the P1 data on panel i, which is
+// affine in (r1, r2) too (phi = 1 - r1, r1 - r2, r2).
+// Blocks take DROWS test panels (DTPB tiles of DTI) of one 256-row group
+// against 256 trial panels; the row-j terms are summed over the block's rows
+// in the lane and stored once per (row slot, trial panel), so that every sum
+// runs in a fixed order and the pass is bitwise reproducible.
+// ---------------------------------------------------------------------------
+constexpr int DTPB = 4;                // tiles per block
+constexpr int DROWS = DTI * DTPB;      // test panels per block
+constexpr int DSUB = 256 / DROWS;      // row slots per 256-row group
+
+// block b -> (group g, row slot sb, trial block jb): the blocks of group g
+// pair each of its DSUB row slots with trial blocks g .. njb-1.
+__device__ __forceinline__ void sym_block_rows(int64_t b, int njb, int ngroups, int& g, int& sb, int& jb) {
+    auto prefix = [&](int gg) -> int64_t { return (int64_t)DSUB * ((int64_t)gg * njb - (int64_t)gg * (gg - 1) / 2); };
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix(mid) <= b) lo = mid; else hi = mid - 1;
+    }
+    const int64_t local = b - prefix(lo);
+    const int cnt = njb - lo;
+    g = lo;
+    sb = (int)(local / cnt);
+    jb = lo + (int)(local % cnt);
+}
+
+template <bool HELM, int NQ2>
+__global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int njb, int ngroups, double* kpart,
+                                                                double* jpart) {
+    int g, sb, jb;
+    sym_block_rows(blockIdx.x, njb, ngroups, g, sb, jb);
+    const int nf = p.nf;
+    const int q2 = NQ2 > 0 ? NQ2 : p.q2;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int j = jb * 256 + threadIdx.x;
+    const bool jvalid = j < nf;
+    const int jj = jvalid ? j : 0;
+    const int r0 = g * 256 + sb * DROWS;
+    __shared__ double red[4][2];
+    __shared__ double2 sct[256];  // sin/cos(j pi/128)
+    if (r0 >= nf) {               // an empty row slot of the last group (uniform):
+        if (jvalid) {             // its partials are read all the same
+            jpart[((int64_t)(g * DSUB + sb) * nf + j) * 2] = 0.0;
+            jpart[((int64_t)(g * DSUB + sb) * nf + j) * 2 + 1] = 0.0;
+        }
+        return;
+    }
+    if constexpr (HELM) sincos_table_fill(sct);
+    __syncthreads();
+    const double* __restrict__ PL = p.pl;
+    const ConstPtr<double> PHI = as_const(p.phi);
+    const double kap = p.kap;
+    // trial side of this lane's panel j
+    const double nx = p.normal[3 * jj], ny = p.normal[3 * jj + 1], nz = p.normal[3 * jj + 2];
+    double gjr[3], gji[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) load_elem(p.x, p.dtype_x, p.tris[3 * jj + t], p.conj_x, gjr[t], gji[t]);
+    double kjr = 0.0, kji = 0.0;  // row-j terms of this block
+    for (int tile = 0; tile < DTPB; ++tile) {
+        const int i0 = r0 + tile * DTI;
+        if (i0 >= nf) break;  // uniform
+        double vtr[DTI], vti[DTI];  // V_ij of the tile, for A[j][i0 + ti]
+        unsigned done = 0u;
+#pragma unroll
+        for (int ti = 0; ti < DTI; ++ti) { vtr[ti] = 0.0; vti[ti] = 0.0; }
+#pragma unroll 1
+        for (int ti = 0; ti < DTI; ++ti) {
+            const int i = i0 + ti;
+            double kir = 0.0, kii = 0.0;
+            if (i < nf && jvalid && j > i && !shares_vertex(p.tris, i, j)) {
+                // test panel frame and points: wave-uniform, scalar loads
+                const ConstPtr<double> FR = as_const(p.frame + (int64_t)i * PANEL_FRAME);
+                const ConstPtr<double> PQ = as_const(p.pq + (int64_t)i * q2 * 4);
+                const double cx = FR[0], cy = FR[1], cz = FR[2];
+                const double dx = FR[15], dy = FR[16], dz = FR[17];
+                const double e1x = FR[6], e1y = FR[7], e1z = FR[8];
+                const double e2x = FR[9], e2y = FR[10], e2z = FR[11];
+                const double mix = FR[12], miy = FR[13], miz = FR[14];
+                const double un = FR[3] * nx + FR[4] * ny + FR[5] * nz;  // U.n_j
+                const double h1 = -0.5 * (e1x * nx + e1y * ny + e1z * nz);
+                const double h2 = -0.5 * (e2x * nx + e2y * ny + e2z * nz);
+                double vr = 0, vi = 0, ar = 0, ai = 0;
+                double b0r = 0, b0i = 0, b1r = 0, b1i = 0, b2r = 0, b2i = 0;
+#pragma unroll 1
+                for (int b = 0; b < q2; ++b) {
+                    const double yx = PL[((int64_t)b * 4 + 0) * nf + j] - cx;
+                    const double yy = PL[((int64_t)b * 4 + 1) * nf + j] - cy;
+                    const double yz = PL[((int64_t)b * 4 + 2) * nf + j] - cz;
+                    const double wb = PL[((int64_t)b * 4 + 3) * nf + j];
+                    const double P1 = e1x * yx + e1y * yy + e1z * yz;
+                    const double P2 = e2x * yx + e2y * yy + e2z * yz;
+                    const double Q = yx * (yx + dx) + yy * (yy + dy) + yz * (yz + dz);
+                    double pvr = 0, pvi = 0, t0r = 0, t0i = 0, t1r = 0, t1i = 0, t2r = 0, t2i = 0;
+#pragma unroll 5
+                    for (int a = 0; a < (NQ2 > 0 ? NQ2 : q2); ++a) {
+                        const double p1 = PQ[4 * a + 1], p2 = PQ[4 * a + 2], wa = PQ[4 * a + 3];
+                        const double r2 = __builtin_fma(p2, P2, __builtin_fma(p1, P1, PQ[4 * a] + Q));
+                        const double rinv = rsqrt_f64(r2);
+                        const double gv = wa * rinv;
+                        const double g3 = gv * (rinv * rinv);
+                        if constexpr (HELM) {
+                            const double th = kap * (r2 * rinv);
+                            double sn, cs;
+                            sincos_f64_tab(th, sct, &sn, &cs);
+                            pvr = __builtin_fma(gv, cs, pvr);
+                            pvi = __builtin_fma(gv, sn, pvi);
+                            // (1 - i th) e^{i th}
+                            const double tr = g3 * __builtin_fma(th, sn, cs);
+                            const double tim = g3 * __builtin_fma(-th, cs, sn);
+                            t0r += tr; t0i += tim;
+                            t1r = __builtin_fma(p1, tr, t1r); t1i = __builtin_fma(p1, tim, t1i);
+                            t2r = __builtin_fma(p2, tr, t2r); t2i = __builtin_fma(p2, tim, t2i);
+                        } else {
+                            pvr += gv;
+                            t0r += g3;
+                            t1r = __builtin_fma(p1, g3, t1r);
+                            t2r = __builtin_fma(p2, g3, t2r);
+                        }
+                    }
+                    vr = __builtin_fma(wb, pvr, vr);
+                    vi = __builtin_fma(wb, pvi, vi);
+                    // row i: sum_a t_a (x_a - y_b).n_j times the data at y_b
+                    const double c0 = un - (yx * nx + yy * ny + yz * nz);
+                    const double kr = __builtin_fma(h2, t2r, __builtin_fma(h1, t1r, c0 * t0r));
+                    const double ki = __builtin_fma(h2, t2i, __builtin_fma(h1, t1i, c0 * t0i));
+                    const double ph0 = PHI[3 * b], ph1 = PHI[3 * b + 1], ph2 = PHI[3 * b + 2];
+                    const double xr = ph0 * gjr[0] + ph1 * gjr[1] + ph2 * gjr[2];
+                    const double xi = ph0 * gji[0] + ph1 * gji[1] + ph2 * gji[2];
+                    const double wr = wb * kr, wi = wb * ki;
+                    ar = __builtin_fma(wr, xr, ar); ar = __builtin_fma(-wi, xi, ar);
+                    ai = __builtin_fma(wr, xi, ai); ai = __builtin_fma(wi, xr, ai);
+                    // row j: w_b (y_b - x_a).n_i, the same for every test point
+                    const double sw = wb * (yx * mix + yy * miy + yz * miz);
+                    b0r = __builtin_fma(sw, t0r, b0r); b0i = __builtin_fma(sw, t0i, b0i);
+                    b1r = __builtin_fma(sw, t1r, b1r); b1i = __builtin_fma(sw, t1i, b1i);
+                    b2r = __builtin_fma(sw, t2r, b2r); b2i = __builtin_fma(sw, t2i, b2i);
+                }
+                const double Vr = kInv4Pi * vr, Vi = kInv4Pi * vi;
+                add_elem(p.out, p.dtype, (int64_t)i * p.ld + j, p.ar * Vr - p.ai * Vi, p.ar * Vi + p.ai * Vr, false);
+                vtr[ti] = Vr; vti[ti] = Vi;
+                done |= 1u << ti;
+                kir = kInv4Pi * ar; kii = kInv4Pi * ai;
+                // row j: the data on panel i at its test points is
+                // g0 (1 + p1/2) + g1 (p2 - p1)/2 - g2 p2/2
+                double g0r, g0i, g1r, g1i, g2r, g2i;
+                load_elem(p.x, p.dtype_x, p.tris[3 * i], p.conj_x, g0r, g0i);
+                load_elem(p.x, p.dtype_x, p.tris[3 * i + 1], p.conj_x, g1r, g1i);
+                load_elem(p.x, p.dtype_x, p.tris[3 * i + 2], p.conj_x, g2r, g2i);
+                const double c0r = b0r + 0.5 * b1r, c0i = b0i + 0.5 * b1i;  // coefficient of g0
+                const double c1r = 0.5 * (b2r - b1r), c1i = 0.5 * (b2i - b1i);
+                const double c2r = -0.5 * b2r, c2i = -0.5 * b2i;
+                const double sr = c0r * g0r - c0i * g0i + c1r * g1r - c1i * g1i + c2r * g2r - c2i * g2i;
+                const double si = c0r * g0i + c0i * g0r + c1r * g1i + c1i * g1r + c2r * g2i + c2i * g2r;
+                kjr += kInv4Pi * sr;
+                kji += kInv4Pi * si;
+            }
+            // row i: reduce over the block into the (i, jb) partial
+            kir = wave_sum(kir);
+            kii = wave_sum(kii);
+            if (lane == 0) { red[wid][0] = kir; red[wid][1] = kii; }
+            __syncthreads();
+            if (threadIdx.x == 0 && i < nf) {
+                kpart[((int64_t)i * njb + jb) * 2] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+                kpart[((int64_t)i * njb + jb) * 2 + 1] = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+            }
+            __syncthreads();
+        }
+        if (jvalid) {
+            // the mirrored entries A[j][i0 .. i0 + DTI) of this lane (contiguous)
+#pragma unroll
+            for (int ti = 0; ti < DTI; ++ti)
+                if (done & (1u << ti))
+                    add_elem(p.out, p.dtype, (int64_t)j * p.ld + i0 + ti, p.ar * vtr[ti] - p.ai * vti[ti],
+                             p.ar * vti[ti] + p.ai * vtr[ti], false);
+        }
+    }
+    if (jvalid) {
+        // one store per (row slot, trial panel): every slot with g <= jb is written
+        double* o = jpart + ((int64_t)(g * DSUB + sb) * nf + j) * 2;
+        o[0] = kjr;
+        o[1] = kji;
+    }
+}
+
+// rhs[i] += alpha (sum_{jb >= i/256} kpart[i][jb] + sum over the row slots
+// of the groups g <= i/256 of jpart[slot][i]), in a fixed order.
+__global__ void kpart_jpart_reduce_kernel(const double* kpart, const double* jpart, int nf, int njb, void* y,
+                                          int dtype, double ar, double ai) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    double a = 0.0, b = 0.0;
+    for (int q = i / 256; q < njb; ++q) {
+        a += kpart[((int64_t)i * njb + q) * 2];
+        b += kpart[((int64_t)i * njb + q) * 2 + 1];
+    }
+    const int nslot = (i / 256 + 1) * DSUB;
+    for (int q = 0; q < nslot; ++q) {
+        a += jpart[((int64_t)q * nf + i) * 2];
+        b += jpart[((int64_t)q * nf + i) * 2 + 1];
     }
     add_elem(y, dtype, i, ar * a - ai * b, ar * b + ai * a, false);
 }
@@ -1809,7 +2029,10 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
     if (rc) return rc;
     Mesh& m = ctx->mesh;
     const int nf = m.nf, njb = (nf + 255) / 256;
-    const size_t kbytes = ((size_t)nf * njb + nf) * 2 * sizeof(double);
+    // row-i partials [nf][njb], then the row-j partials: [nf] (FP32, atomics)
+    // or [row slots][nf] (FP64, one store per slot)
+    const size_t jrows = d->precision == BEM_PREC_FP32 ? 1 : (size_t)njb * DSUB;
+    const size_t kbytes = ((size_t)nf * njb + jrows * nf) * 2 * sizeof(double);
     if (m.kpart_bytes < kbytes) {
         if (m.kpart) hipFree(m.kpart);
         m.kpart = nullptr;
@@ -1824,17 +2047,26 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
     const bool helm = d->kernel != BEM_LAPLACE;
     const int parts = d->parts ? d->parts : (BEM_PART_FAR | BEM_PART_NEAR);
     if (parts & BEM_PART_FAR) {
-        auto fk = d->precision == BEM_PREC_FP32 ? (helm ? far_dirichlet_kernel<float, true> : far_dirichlet_kernel<float, false>)
-                                                : (helm ? far_dirichlet_kernel<double, true> : far_dirichlet_kernel<double, false>);
-        double* krow = m.kpart + (size_t)nf * njb * 2;
-        hipLaunchKernelGGL(zero_kernel, dim3((2 * nf + 255) / 256), dim3(256), 0, s, krow, (int64_t)2 * nf);
         const int ngroups = (nf + 255) / 256;
-        // upper-triangle blocks: DTG tiles of group g pair with njb - g trial blocks
-        const int64_t nblocks = (int64_t)DTG * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
-        hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, njb, ngroups, m.kpart, krow);
-        hipLaunchKernelGGL(kpart_reduce_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, (const double*)m.kpart,
-                           (const double*)krow, nf,
-                           njb, rhs, dtype_rhs, alpha_k, 0.0);
+        if (d->precision == BEM_PREC_FP32) {
+            auto fk = helm ? far_dirichlet_kernel<float, true> : far_dirichlet_kernel<float, false>;
+            double* krow = m.kpart + (size_t)nf * njb * 2;
+            hipLaunchKernelGGL(zero_kernel, dim3((2 * nf + 255) / 256), dim3(256), 0, s, krow, (int64_t)2 * nf);
+            // upper-triangle blocks: DTG tiles of group g pair with njb - g trial blocks
+            const int64_t nblocks = (int64_t)DTG * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
+            hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, njb, ngroups, m.kpart, krow);
+            hipLaunchKernelGGL(kpart_reduce_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, (const double*)m.kpart,
+                               (const double*)krow, nf, njb, rhs, dtype_rhs, alpha_k, 0.0);
+        } else {
+            // FP64: moment form, fixed-order partials for both rows
+            auto fk = m.q2 == 25 ? (helm ? far_dirichlet_f64_kernel<true, 25> : far_dirichlet_f64_kernel<false, 25>)
+                                 : (helm ? far_dirichlet_f64_kernel<true, 0> : far_dirichlet_f64_kernel<false, 0>);
+            double* jpart = m.kpart + (size_t)nf * njb * 2;
+            const int64_t nblocks = (int64_t)DSUB * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
+            hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, njb, ngroups, m.kpart, jpart);
+            hipLaunchKernelGGL(kpart_jpart_reduce_kernel, dim3((nf + 255) / 256), dim3(256), 0, s,
+                               (const double*)m.kpart, (const double*)jpart, nf, njb, rhs, dtype_rhs, alpha_k, 0.0);
+        }
         rc = check_hip(hipGetLastError(), "far-field Dirichlet system launch");
         if (rc) return rc;
     }

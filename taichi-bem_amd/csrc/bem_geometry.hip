@@ -75,14 +75,55 @@ __global__ void vertex_geometry_kernel(int nv, const int32_t* __restrict__ vp_ro
 // x = (1-r1) x1 + (r1-r2) x2 + r2 x3 (BEM_manager.py:427-450), weight
 // w_a w_b xi_a (2A) (single_layer_3d.py:150-164).  Writes the FP64 AoS and
 // SoA tables and the FP32 centroid-offset tables.
+// Local frame of each panel: C, U = x1 - C, E1 = x2 - x1, E2 = x3 - x2, the
+// normal and D = -2 U (Mesh::frame).  A point of the panel is
+// x(r1, r2) - C = U + r1 E1 + r2 E2, the element map
+// (1 - r1) x1 + (r1 - r2) x2 + r2 x3 (BEM_manager.py:427-450) moved to C.
+__global__ void panel_frame_kernel(int nf, const double* __restrict__ v, const int32_t* __restrict__ t,
+                                   const double* __restrict__ normal, const double* __restrict__ cen,
+                                   double* frame) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const double* p1 = v + 3 * t[3 * i];
+    const double* p2 = v + 3 * t[3 * i + 1];
+    const double* p3 = v + 3 * t[3 * i + 2];
+    double* f = frame + (int64_t)i * PANEL_FRAME;
+    for (int d = 0; d < 3; ++d) {
+        const double c = cen[3 * i + d], u = p1[d] - c;
+        f[d] = c;
+        f[3 + d] = u;
+        f[6 + d] = p2[d] - p1[d];
+        f[9 + d] = p3[d] - p2[d];
+        f[12 + d] = normal[3 * i + d];
+        f[15 + d] = -2.0 * u;
+    }
+    f[18] = 0.0;
+    f[19] = 0.0;
+}
+
 __global__ void panel_points_kernel(int nf, int q2, const double* __restrict__ v,
                                     const int32_t* __restrict__ t, const double* __restrict__ area,
                                     const double* __restrict__ cen, const double* __restrict__ r12w,
-                                    double* pu, double* pl, float* pu32, float* pl32) {
+                                    const double* __restrict__ frame, double* pu, double* pl, float* pu32,
+                                    float* pl32, double* pq) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nf * q2) return;
     int i = idx / q2, p = idx % q2;
     double r1 = r12w[3 * p], r2 = r12w[3 * p + 1], w = r12w[3 * p + 2] * (area[i] * 2.0);
+    {
+        // |x - C|^2 with x - C = U + r1 E1 + r2 E2, as the moment-form kernels rebuild it
+        const double* f = frame + (int64_t)i * PANEL_FRAME;
+        double s2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double o = f[3 + d] + r1 * f[6 + d] + r2 * f[9 + d];
+            s2 += o * o;
+        }
+        const int64_t b = ((int64_t)i * q2 + p) * 4;
+        pq[b + 0] = s2;
+        pq[b + 1] = -2.0 * r1;
+        pq[b + 2] = -2.0 * r2;
+        pq[b + 3] = w;
+    }
     const double* p1 = v + 3 * t[3 * i];
     const double* p2 = v + 3 * t[3 * i + 1];
     const double* p3 = v + 3 * t[3 * i + 2];
@@ -237,7 +278,7 @@ int dev_alloc(T** p, size_t n) {
 
 void mesh_free(Mesh* m) {
     void* ptrs[] = {m->verts, m->tris, m->area, m->normal, m->vert_area, m->vert_normal, m->curl,
-                    m->pu, m->pl, m->cen, m->pu32, m->pl32, m->phi, m->phi32, m->mass_w,
+                    m->pu, m->pl, m->cen, m->pu32, m->pl32, m->phi, m->phi32, m->mass_w, m->frame, m->pq,
                     m->near, m->near_row, m->rule, m->near_buf, m->kpart};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -284,6 +325,8 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
     rc |= dev_alloc(&m.phi, (size_t)q2 * 3);
     rc |= dev_alloc(&m.phi32, (size_t)q2 * 3);
     rc |= dev_alloc(&m.mass_w, (size_t)q2);
+    rc |= dev_alloc(&m.frame, (size_t)nf * PANEL_FRAME);
+    rc |= dev_alloc(&m.pq, (size_t)nf * q2 * 4);
     if (rc) return BEM_ERR_HIP;
     rc = check_hip(hipMemcpyAsync(m.verts, d_verts, sizeof(double) * nv * 3, hipMemcpyDeviceToDevice, s), "copy vertices");
     if (rc) return rc;
@@ -335,8 +378,10 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
                        m.tris, m.area, m.normal, m.cen, m.curl);
     hipLaunchKernelGGL(vertex_geometry_kernel, dim3((nv + 255) / 256), dim3(256), 0, s, nv, d_vp_row,
                        d_vp_col, m.area, m.normal, m.vert_area, m.vert_normal);
+    hipLaunchKernelGGL(panel_frame_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts, m.tris,
+                       m.normal, m.cen, m.frame);
     hipLaunchKernelGGL(panel_points_kernel, dim3((nf * q2 + 255) / 256), dim3(256), 0, s, nf, q2,
-                       m.verts, m.tris, m.area, m.cen, d_r12w, m.pu, m.pl, m.pu32, m.pl32);
+                       m.verts, m.tris, m.area, m.cen, d_r12w, m.frame, m.pu, m.pl, m.pu32, m.pl32, m.pq);
     rc = check_hip(hipGetLastError(), "geometry kernels");
     if (rc) return rc;
 

@@ -24,6 +24,8 @@ struct NearPt {
     double r1x, r2x, r1y, r2y, w;
 };
 
+constexpr int PANEL_FRAME = 20;  // doubles per panel of Mesh::frame (18 used)
+
 struct Mesh {
     int nv = 0, nf = 0, q = 0, q2 = 0;
     double* verts = nullptr;      // [nv*3]
@@ -41,6 +43,12 @@ struct Mesh {
     double* cen = nullptr;        // [nf*3]
     float* pu32 = nullptr;        // [nf][q2][4]
     float* pl32 = nullptr;        // [q2][4][nf]
+    // FP64 local frame of every panel (the far field's moment form):
+    // C (centroid), U = x1 - C, E1 = x2 - x1, E2 = x3 - x2, n, D = -2 U,
+    // stride PANEL_FRAME; the points are x(r1, r2) - C = U + r1 E1 + r2 E2.
+    double* frame = nullptr;      // [nf][PANEL_FRAME]
+    // per regular point: S = |x - C|^2, -2 r1, -2 r2, w  (uniform loads)
+    double* pq = nullptr;         // [nf][q2][4]
     double* phi = nullptr;        // [q2][3] P1 shape values at the regular points
     float* phi32 = nullptr;       // [q2][3]
     double* mass_w = nullptr;     // [q2] w_xi w_eta xi (single-panel rule, no area)
@@ -56,8 +64,8 @@ struct Mesh {
     std::vector<int32_t> h_tris;  // host copy for near-list construction
     double* near_buf = nullptr;   // scratch for matrix-free near contributions
     int64_t near_buf_len = 0;
-    double* kpart = nullptr;      // per (test panel, trial block) partial sums of K conj(g)
-    size_t kpart_bytes = 0;
+    double* kpart = nullptr;      // per (test panel, trial block) partial sums of K conj(g), then
+    size_t kpart_bytes = 0;       // per (row slot, trial panel) partials of the mirrored terms
 };
 
 // Slots of Context::counters.

@@ -681,6 +681,36 @@ def test_assemble_dirichlet_fused(bem, oracle, meshes, kernel, k, prec, tol):
     assert rel(to_np(rhs), r) <= tol
 
 
+@pytest.mark.parametrize("q", [4, 5])
+@pytest.mark.parametrize("flags", [0, 3])
+@pytest.mark.parametrize("kernel,k,sqrt_n", KERNELS)
+def test_assemble_dirichlet_fp64_groups_and_reproducible(bem, oracle, kernel, k, sqrt_n, flags, q):
+    """The FP64 Dirichlet pass on 320 panels: two 256-row groups, the second
+    with empty row slots, GaussQR 4 (runtime point loop) and 5; V and
+    (1/2 M + K) conj(g) against the oracle, and two calls give bitwise equal
+    results (every partial sum runs in a fixed order)."""
+    v, f = bem.meshes.perturbed_icosphere(2, seed=6)
+    ctx = make_ctx(bem, v, f, q)
+    n = len(f)
+    rng = np.random.default_rng(12)
+    g = rng.standard_normal(len(v)) + 1j * rng.standard_normal(len(v))
+    gt = torch.tensor(g, device=DEV)
+    desc = bem.Context.op_desc(0, kernel, k, sqrt_n, 0, 0, 0, 0, flags)
+    outs = []
+    for rep in range(2):
+        A = torch.zeros(n, n, dtype=torch.complex128, device=DEV)
+        rhs = torch.zeros(n, dtype=torch.complex128, device=DEV)
+        print(f"[dirichlet fp64] kernel={kernel} flags={flags} q={q} rep={rep}", flush=True)
+        ctx.assemble_dirichlet(desc, A, gt, rhs, alpha=1.0, alpha_k=1.0, alpha_m=0.5, conj_g=True)
+        outs.append((to_np(A), to_np(rhs)))
+    V = oracle.assemble(oracle.make_op(0, kernel, k, sqrt_n, q, 0, 0, flags), v, f)
+    r = oracle.rhs_dirichlet(kernel, k, q, v, f, g, True, 1.0, sqrt_n=sqrt_n, flags=flags)
+    assert rel(outs[0][0], V) <= TOL64
+    assert rel(outs[0][1], r) <= TOL64
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    ctx.close()
+
+
 VKW_KERNELS = [(0, 0.0, 1.0), (1, 3.0, 1.0), (2, 5.0, 1.5)]
 
 
