@@ -28,20 +28,23 @@ sys.path.insert(0, REPO)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3
-# Algorithmic flops per quadrature-point pair of the fused far-field kernel,
+# Algorithmic flops per quadrature-point pair of the fused far-field pass,
 # which evaluates each unordered panel pair once (V is symmetric) and serves
 # both orderings (add/mul = 1, FMA = 2; rsqrt and sin/cos are transcendentals,
-# not counted): d = x - y 3, r^2 5, w/r 1, w/r^3 2, (x-y).n_j 6, (y-x).n_i 6,
-# kr 2, V accumulate 4, (cos + kr sin, sin - kr cos) 4, row-i K accumulate 4,
-# row-j K times the test-point data 2 + 8.
+# not counted), for the direct evaluation: d = x - y 3, r^2 5, w/r 1, w/r^3 2,
+# (x-y).n_j 6, (y-x).n_i 6, kr 2, V accumulate 4, (cos + kr sin,
+# sin - kr cos) 4, row-i K accumulate 4, row-j K times the test-point data
+# 2 + 8.  The kernel's moment form does fewer operations for the same
+# integrals; the count is kept so that kernel versions compare on one unit.
 FLOPS_PER_EVAL_FUSED = 47
-# HBM bytes per launch of far_dirichlet_kernel<double, Helmholtz> on the L5
-# workload from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the
-# same counter on a 512 MiB copy (FETCH x 2.00, WRITE x 1.00), and the
-# hardware FP64 VALU flops of that launch (SQ_INSTS_VALU_FLOPS_FP64 x 64
-# lanes).  Measured: 101.8 GB fetched + 12.3 GB written; 9.930e12 flops.
-FAR_L5_TRAFFIC_BYTES = 114_107_892_320
-FAR_L5_HW_FLOPS = 9_929_664_000_000
+# HBM bytes per launch of the FP64 Dirichlet far-field kernel
+# (far_dirichlet_f64_kernel<Helmholtz, 25>) on the L5 workload from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the same counter on
+# a 512 MiB copy (FETCH x 2.00, WRITE x 1.00), and the hardware FP64 VALU
+# flops of that launch (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes).  Measured:
+# 105.9 GB fetched + 9.1 GB written; 8.845e12 flops.
+FAR_L5_TRAFFIC_BYTES = 115_006_186_496
+FAR_L5_HW_FLOPS = 8_845_081_600_000
 
 
 def cpu_threads() -> int:
@@ -198,7 +201,7 @@ def main() -> int:
     far_pairs = counts[0]
     far_flops = far_pairs // 2 * 625 * FLOPS_PER_EVAL_FUSED  # unordered pairs
     achieved = far_flops / (mean["far_ms"] * 1e-3) / 1e12
-    roofline = {"bound": "valu", "kernel": "far_dirichlet_kernel<double,Helmholtz> (V + K conj(g))",
+    roofline = {"bound": "valu", "kernel": "far_dirichlet_f64_kernel<Helmholtz> (V + K conj(g), moment form)",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                 "traffic": FAR_L5_TRAFFIC_BYTES if args.level == 5 and FAR_L5_TRAFFIC_BYTES else None,
