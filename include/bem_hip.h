@@ -379,6 +379,20 @@ int bem_batch_condest(bem_context* ctx, int32_t count, const void* const* LU, co
 int bem_lu_solve_transpose(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
                            const int32_t* ipiv, void* b, void* stream);
 
+/* bem_lu_solve without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory or synchronise inside the call
+ * (HIP graph capture, tight serving loops).  Same arguments and result as
+ * bem_lu_solve (bitwise); the pivot permutation is built on the device, and
+ * the kernels' scratch is a fixed per-device array of the library (about
+ * 2.3 MiB) set up by bem_create, so n <= 65536 (larger n: BEM_ERR_INVALID).
+ * Every call on one device shares that array: calls that overlap on one
+ * device (several streams or host threads) race; consecutive calls are
+ * independent.  Outside stream capture a sweep's timed-out wait is reported
+ * as BEM_ERR_HIP (one stream synchronisation); under capture the NaN the
+ * sweep leaves is the only sign. */
+int bem_lu_solve_noalloc(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
+                         const int32_t* ipiv, void* b, void* stream);
+
 /* r = b - A x in FP64/complex128 (A of dtype BEM_F64/BEM_C128; x, b, r of
  * the matching FP64 type). */
 int bem_residual(bem_context* ctx, const void* A, int64_t ld, int32_t n, int32_t dtype,

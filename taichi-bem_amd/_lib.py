@@ -37,7 +37,7 @@ EXPORTS = (
     "bem_residual", "bem_refine_solve", "bem_convert", "bem_matrix_axpy", "bem_lu_solve_transpose",
     "bem_shape_gradient", "bem_shape_gradient_kt", "bem_area_gradient", "bem_assemble_combined_batched",
     "bem_lu_factor_batched",
-    "bem_batch_solve", "bem_batch_condest", "bem_set_option",
+    "bem_batch_solve", "bem_batch_condest", "bem_set_option", "bem_lu_solve_noalloc",
 )
 
 
@@ -112,6 +112,7 @@ def lib() -> ctypes.CDLL:
             "bem_lu_factor": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp]),
             "bem_lu_solve": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
             "bem_lu_solve_transpose": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
+            "bem_lu_solve_noalloc": (ctypes.c_int, [vp, vp, i64, i32, i32, vp, vp, vp]),
             "bem_shape_gradient": (ctypes.c_int, [vp, ctypes.POINTER(BemOpDesc), vp, vp, dbl, dbl, vp, vp, vp]),
             "bem_shape_gradient_kt": (ctypes.c_int,
                                       [vp, ctypes.POINTER(BemOpDesc), vp, vp, dbl, dbl, vp, vp, vp, vp]),
@@ -369,6 +370,13 @@ class Context:
         _require_cuda(LU, ipiv, b)
         check(lib().bem_lu_solve(self._h, LU.data_ptr(), int(LU.stride(0)), int(LU.shape[0]), dtype_code(LU),
                                  ipiv.data_ptr(), b.data_ptr(), current_stream()))
+
+    def lu_solve_noalloc(self, LU, ipiv, b) -> None:
+        """lu_solve without any allocation in the library (fixed per-device
+        scratch, n <= 65536): for graph capture and serving loops."""
+        _require_cuda(LU, ipiv, b)
+        check(lib().bem_lu_solve_noalloc(self._h, LU.data_ptr(), int(LU.stride(0)), int(LU.shape[0]),
+                                         dtype_code(LU), ipiv.data_ptr(), b.data_ptr(), current_stream()))
 
     def lu_solve_transpose(self, LU, ipiv, b) -> None:
         """b <- A^-T b (plain transpose) from the factors of A."""

@@ -66,9 +66,15 @@ int bem_create(bem_context** out, int device) {
     bem_context* ctx = new bem_context();
     ctx->impl.device = device;
     rc = with_device(ctx, [&]() -> int {
-        return check_hip(hipMalloc((void**)&ctx->impl.counters, 64 * sizeof(int)), "counters");
+        int r = check_hip(hipMalloc((void**)&ctx->impl.counters, 64 * sizeof(int)), "counters");
+        if (r) return r;
+        return bem::noalloc_setup(device);  // this device's address of the allocation-free scratch
     });
-    if (rc) { delete ctx; return rc; }
+    if (rc) {
+        if (ctx->impl.counters) with_device(ctx, [&]() -> int { return check_hip(hipFree(ctx->impl.counters), "free"); });
+        delete ctx;
+        return rc;
+    }
     ctx->impl.counters_len = 64;
     *out = ctx;
     return BEM_OK;
@@ -269,6 +275,13 @@ int bem_lu_solve_transpose(bem_context* ctx, const void* LU, int64_t ld, int32_t
                            const int32_t* ipiv, void* b, void* stream) {
     return with_device(ctx, [&]() -> int {
         return bem::lu_solve_transpose(&ctx->impl, LU, ld, n, dtype, ipiv, b, as_stream(stream));
+    });
+}
+
+int bem_lu_solve_noalloc(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_t dtype,
+                         const int32_t* ipiv, void* b, void* stream) {
+    return with_device(ctx, [&]() -> int {
+        return bem::lu_solve_noalloc(&ctx->impl, LU, ld, n, dtype, ipiv, b, as_stream(stream));
     });
 }
 

@@ -160,6 +160,9 @@ int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const i
              void* b, hipStream_t s);
 int lu_solve_transpose(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv,
                        void* b, hipStream_t s);
+int noalloc_setup(int device);
+int lu_solve_noalloc(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv, void* b,
+                     hipStream_t s);
 int gemv_residual(Context* ctx, const void* A, int64_t ld, int n, int dtype_a, const void* x,
                   const void* b, void* r, hipStream_t s);
 int refine_solve(Context* ctx, const void* A, int64_t lda, int dtype_a, const void* LU, int64_t ldlu,

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -2356,6 +2357,107 @@ int lu_solve(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const i
     hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
     if ((rc = check_hip(hipGetLastError(), "LU solve"))) return rc;
     return tri_fail_check(ctx, s, "LU solve");
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free solve (bem_lu_solve_noalloc).  The scratch of its kernels
+// (two complex128 work vectors, the pivot permutation, the sweeps' ticket and
+// ready words and the fail word) is the file-scope array `scratch`, of
+// NOALLOC_BYTES per device, which every call on that device shares: each call
+// rewrites what it reads (the synchronisation and fail words are cleared by
+// kernels of the call), so consecutive calls are independent, but calls that
+// overlap on one device race.  Its device address is taken once per device by
+// noalloc_setup (bem_create); host code uses that address, never the array's
+// name, which on the host is only a stand-in.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int NOALLOC_NMAX = 65536;
+constexpr size_t noalloc_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NOALLOC_W = noalloc_align((size_t)NOALLOC_NMAX * 16);  // one complex128 work vector
+constexpr size_t NOALLOC_PERM = noalloc_align((size_t)NOALLOC_NMAX * 4);
+constexpr size_t NOALLOC_SYNC = noalloc_align((size_t)2 * ((NOALLOC_NMAX + TB - 1) / TB + 1) * 4);
+constexpr size_t NOALLOC_BYTES = 2 * NOALLOC_W + NOALLOC_PERM + NOALLOC_SYNC + 256;  // + the fail word
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// LAPACK interchanges -> gather permutation on the device: the swaps are
+// applied in order by one thread (n steps), so no host round trip is needed.
+__global__ void perm_from_ipiv_kernel(const int32_t* ipiv, int n, int32_t* perm) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < n; ++c) {
+            const int p = ipiv[c];
+            if (p != c && p >= 0 && p < n) { const int t = perm[c]; perm[c] = perm[p]; perm[p] = t; }
+        }
+}
+
+}  // namespace
+
+// Called once per context with its device current: records this device's
+// address of `scratch` (each device holds its own copy).
+int noalloc_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return set_error(BEM_ERR_INVALID, "device index beyond the no-allocation scratch table");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[device]) return BEM_OK;
+    void* ptr = nullptr;
+    const int rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "no-allocation scratch address");
+    if (rc) return rc;
+    g_noalloc_ptr[device] = ptr;
+    return BEM_OK;
+}
+
+int lu_solve_noalloc(Context* ctx, const void* LU, int64_t ld, int n, int dtype, const int32_t* ipiv, void* b,
+                     hipStream_t s) {
+    if (!LU || !ipiv || !b) return set_error(BEM_ERR_INVALID, "null argument");
+    if (n <= 0 || ld < n) return set_error(BEM_ERR_INVALID, "invalid matrix size");
+    if (!valid_dtype(dtype)) return set_error(BEM_ERR_INVALID, "unknown dtype");
+    if (n > NOALLOC_NMAX)
+        return set_error(BEM_ERR_INVALID, "bem_lu_solve_noalloc: n = " + std::to_string(n) +
+                                              " exceeds the scratch limit of " + std::to_string(NOALLOC_NMAX));
+    char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) base = (char*)g_noalloc_ptr[ctx->device];
+    }
+    if (!base) return set_error(BEM_ERR_INVALID, "no-allocation scratch not set up on this device");
+    double* w0 = (double*)base;
+    double* w1 = (double*)(base + NOALLOC_W);
+    int32_t* perm = (int32_t*)(base + 2 * NOALLOC_W);
+    unsigned* sync = (unsigned*)(base + 2 * NOALLOC_W + NOALLOC_PERM);
+    int* fail = (int*)(base + 2 * NOALLOC_W + NOALLOC_PERM + NOALLOC_SYNC);
+    const dim3 g((n + 255) / 256);
+    hipLaunchKernelGGL(fail_reset_kernel, dim3(1), dim3(64), 0, s, fail);
+    hipLaunchKernelGGL(perm_from_ipiv_kernel, dim3(1), dim3(256), 0, s, ipiv, n, perm);
+    hipLaunchKernelGGL(vec_to_work, g, dim3(256), 0, s, (const void*)b, dtype, n, w0);
+    hipLaunchKernelGGL(permute_kernel, g, dim3(256), 0, s, (const double*)w0, (const int32_t*)perm, n, w1);
+    const long long wt = ctx->wait_ticks;
+    switch (dtype) {
+        case BEM_F64: tri_solve_t<double, false>((const double*)LU, ld, n, w1, sync, false, wt, fail, s); break;
+        case BEM_C128: tri_solve_t<double, true>((const double*)LU, ld, n, w1, sync, false, wt, fail, s); break;
+        case BEM_F32: tri_solve_t<float, false>((const float*)LU, ld, n, w1, sync, false, wt, fail, s); break;
+        default: tri_solve_t<float, true>((const float*)LU, ld, n, w1, sync, false, wt, fail, s); break;
+    }
+    hipLaunchKernelGGL(work_to_vec, g, dim3(256), 0, s, (const double*)w1, n, b, dtype, 0);
+    int rc = check_hip(hipGetLastError(), "allocation-free LU solve");
+    if (rc) return rc;
+    // the time-out report, as bem_lu_solve's (skipped under stream capture)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    rc = check_hip(hipStreamIsCapturing(s, &cs), "capture status");
+    if (rc || cs != hipStreamCaptureStatusNone) return rc;
+    int h = 0;
+    rc = check_hip(hipMemcpyAsync(&h, fail, sizeof(int), hipMemcpyDeviceToHost, s), "triangular solve status");
+    if (rc) return rc;
+    rc = check_hip(hipStreamSynchronize(s), "triangular solve sync");
+    if (rc) return rc;
+    if (h) return set_error(BEM_ERR_HIP, "allocation-free LU solve: a triangular sweep's wait timed out");
+    return BEM_OK;
 }
 
 // A^T x = b (plain transpose, no conjugation): with P A = L U,
