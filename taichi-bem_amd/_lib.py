@@ -226,10 +226,11 @@ class Context:
         self.close()
         return False
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter shutdown the atexit hook has already closed every
-        # context; a finaliser then makes no HIP call
-        if sys.is_finalizing():
+        # context; a finaliser then makes no HIP call (the function is bound
+        # here because module globals may be gone by then)
+        if getattr(self, "_h", None) is None or _finalizing():
             return
         try:
             self.close()

@@ -270,15 +270,16 @@ def test_config4_shape_gradient_l4(bem, kernel, k):
         d /= np.linalg.norm(d)
         fd = (J(v + h * d, k) - J(v - h * d, k)) / (2 * h)
         got = float((gx * d).sum())
-        # error relative to the directional derivative, floored at 1e-4 |g| |d|
-        # (a random unit direction in R^7686 sees about 1e-2 |g|; one seeded
-        # direction here is nearly orthogonal to g, |g.d| = 2e-5 |g|, where
-        # the FP64 noise of J and of g, about 1e-10 |g|, is 1e-5 of g.d)
+        # error relative to the directional derivative itself; beside it the
+        # error relative to 1e-4 |g| |d| (a random unit direction in R^7686
+        # sees about 1e-2 |g|).  The forward pass is bitwise reproducible
+        # (every assembly and LU sum runs in a fixed order), so J(x + h d) and
+        # J(x - h d) carry no run-to-run noise.
         worst = max(worst, abs(got - fd) / max(abs(fd), 1e-4 * gnorm))
         worst_raw = max(worst_raw, abs(got - fd) / abs(fd))
-    print(f"[config4] kernel={kernel} J={j0:.6e} worst rel err over 32 directions {worst:.3e} "
-          f"(unfloored {worst_raw:.3e})", flush=True)
-    assert worst <= 1e-5
+    print(f"[config4] kernel={kernel} J={j0:.6e} worst rel err over 32 directions {worst_raw:.3e} "
+          f"(relative to max(|fd|, 1e-4 |g|): {worst:.3e})", flush=True)
+    assert worst_raw <= 1e-5
     if kernel:
         fdk = (J(v, k + 1e-6) - J(v, k - 1e-6)) / 2e-6
         print(f"[config4] dJ/dk adjoint {gk:.12e} fd {fdk:.12e}", flush=True)
