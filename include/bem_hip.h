@@ -274,7 +274,14 @@ enum {
                                 partial-pivoting panels of single
                                 factorisations, and its element growth can be
                                 larger, so use it for batches of
-                                well-conditioned systems only */,
+                                well-conditioned systems only; 2 =
+                                block-local pivoting: rows interchanged only
+                                inside each panel's 64 x 64 diagonal block
+                                (two launches per panel, no waits between
+                                workgroups), for factors that precondition
+                                bem_refine_solve against the original matrix
+                                (the mixed-precision solves); the ipiv it
+                                returns is valid for every solve entry point */,
     BEM_OPT_LU_GEMM_TILE = 4 /* 1 (default) = trailing updates of at least
                                 256 rows use 128-row MFMA tiles: 128 x 64 for
                                 complex128, 128 x 64 with three real products

@@ -217,7 +217,7 @@ int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
     if (!ctx) return set_error(BEM_ERR_INVALID, "null context");
     switch (option) {
         case BEM_OPT_LU_PIVOTING:
-            if (value != 0 && value != 1) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_PIVOTING takes 0 or 1");
+            if (value < 0 || value > 2) return set_error(BEM_ERR_INVALID, "BEM_OPT_LU_PIVOTING takes 0, 1 or 2");
             ctx->impl.lu_pivoting = value;
             return BEM_OK;
         case BEM_OPT_LU_LOOKAHEAD:

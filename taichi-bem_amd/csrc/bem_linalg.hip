@@ -689,9 +689,14 @@ __global__ void tourn_ipiv_kernel(LuProb inl, const LuProb* tbl, int k, int leve
     }
 }
 
-// Unpivoted LU of the (permuted) top w x w block in LDS, then
-// S_U = -inv(U11) for L21 = A21 inv(U11) (GEMM mode 5 stores -(A21 S_U)).
-template <typename T, bool C, bool B>
+// LU of the top w x w block of the panel in LDS, then S_U = -inv(U11) for
+// L21 = A21 inv(U11) (GEMM mode 5 stores -(A21 S_U)).  PIV = false: the
+// block is taken as it is (tournament pivoting has already moved the pivot
+// rows up).  PIV = true (block-local pivoting): partial pivoting among the
+// block's own rows only (largest |re|+|im| at or below the diagonal, first
+// row on ties), interchanges recorded in ipiv like LAPACK's; the rows below
+// the block are not searched, so no workgroup waits for another.
+template <typename T, bool C, bool B, bool PIV>
 __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb* tbl, int k) {
     const LuProb P = get_prob<B>(inl, tbl);
     const int n = P.n;
@@ -716,7 +721,31 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
         }
     }
     __syncthreads();
+    __shared__ int s_piv;
     for (int j = 0; j < w; ++j) {
+        if constexpr (PIV) {
+            if (t < 64) {  // one wave: lane r looks at row r of the block
+                double m = (t >= j && t < w) ? fabs((double)M[t][j].re) + fabs((double)M[t][j].im) : -1.0;
+                int r = t;
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double om = __shfl_xor(m, off, 64);
+                    const int orr = __shfl_xor(r, off, 64);
+                    if (om > m || (om == m && orr < r)) { m = om; r = orr; }
+                }
+                if (t == 0) {
+                    s_piv = r;
+                    P.ipiv[k + j] = k + r;
+                }
+            }
+            __syncthreads();
+            const int p = s_piv;
+            if (p != j && t < NB) {
+                const cx<T> a = M[j][t];
+                M[j][t] = M[p][t];
+                M[p][t] = a;
+            }
+            __syncthreads();
+        }
         const cx<T> d = M[j][j];
         const bool zero = d.re == T(0) && d.im == T(0);
         if (zero && t == 0 && *P.info == 0) *P.info = k + j + 1;
@@ -1792,7 +1821,10 @@ void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, in
 
 // Factorisation of the NB-column panel at k (rows k..n of every active
 // problem): partial pivoting with one fused update/search launch per column
-// (pivoting 0, LAPACK's pivot rows), or tournament pivoting (pivoting 1).
+// (pivoting 0, LAPACK's pivot rows), tournament pivoting (pivoting 1), or
+// block-local pivoting (pivoting 2: rows interchanged only inside the
+// panel's top NB x NB block; for factors that precondition an iterative
+// refinement against the original matrix).
 // pgrid > 0 (single problems): partial pivoting runs as one persistent
 // launch of at most pgrid workgroups (one per compute unit of the stream)
 // when the panel's rows fit their LDS.
@@ -1813,6 +1845,13 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
             *pbase += (unsigned)(panel_barriers(std::min(NB, nrem)) * G);
             return;
         }
+    }
+    if (pivoting == 2) {
+        // block-local pivoting: the top block in one workgroup, L21 on MFMA
+        hipLaunchKernelGGL((topblock_kernel<T, C, B, true>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
+        const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
+        lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
+        return;
     }
     if (pivoting == 0) {
         const int kendmax = std::min(ns[0], k + NB);
@@ -1839,7 +1878,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
     hipLaunchKernelGGL((tourn_ipiv_kernel<B>), dim3(1, 1, active), dim3(64), 0, s, one, d_tbl, k, level);
     hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, k + NB, k,
                        k + NB, 0, 0);
-    hipLaunchKernelGGL((topblock_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
+    hipLaunchKernelGGL((topblock_kernel<T, C, B, false>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
     const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
     lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
 }
@@ -2181,7 +2220,10 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
         }
         int pgrid = ctx->num_cu;
         if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
-        lu_launch<T, C, false>(tbl[0], nullptr, std::vector<int>{ns[0]}, 0, gemm_opts(ctx), pgrid, s, K_switch);
+        // block-local pivoting keeps its own rule for the tail
+        const int tail_piv = ctx->lu_pivoting == 2 ? 2 : 0;
+        lu_launch<T, C, false>(tbl[0], nullptr, std::vector<int>{ns[0]}, tail_piv, gemm_opts(ctx), pgrid, s,
+                               K_switch);
     }
     return check_hip(hipGetLastError(), "batched LU launch");
 }

@@ -402,6 +402,67 @@ def test_lu_solve_many_blocks(bem, dtype, tol, n):
         assert rel(to_np(x).astype(np.complex128), ref) <= tol
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-8), (torch.complex64, 5e-3), (torch.float64, 1e-8)])
+@pytest.mark.parametrize("n,lookahead", [(700, 0), (1500, 32), (130, 0)])
+def test_lu_block_local_pivoting(bem, dtype, tol, n, lookahead):
+    """BEM_OPT_LU_PIVOTING = 2: the interchanges stay inside each 64-row
+    diagonal block, the factors solve the system directly (the bar allows the
+    larger element growth of the restricted search on these unstructured
+    random matrices) and, refined in FP64 against the matrix, to 1e-12, with
+    and without look-ahead."""
+    rng = np.random.default_rng(31)
+    a = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n) * 0.5
+    if dtype.is_complex:
+        a = a + 1j * rng.standard_normal((n, n))
+    A = torch.tensor(a, dtype=dtype, device=DEV)
+    ctx = bem.Context(0)
+    ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
+    ctx.set_option(bem._lib.OPT_LU_LOOKAHEAD, lookahead)
+    LU = A.clone()
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    print(f"[lu local] n={n} dtype={dtype} lookahead={lookahead}", flush=True)
+    ctx.lu_factor(LU, ipiv)
+    p = to_np(ipiv)
+    assert np.all(p // 64 == np.arange(n) // 64) and np.all(p >= np.arange(n))
+    b = rng.standard_normal(n) + (1j * rng.standard_normal(n) if dtype.is_complex else 0)
+    x = torch.tensor(b, dtype=dtype, device=DEV)
+    ctx.lu_solve(LU, ipiv, x)
+    ref = np.linalg.solve(a, b)
+    e_direct = rel(to_np(x).astype(ref.dtype), ref)
+    A64 = torch.tensor(a, dtype=torch.complex128 if dtype.is_complex else torch.float64, device=DEV)
+    b64 = torch.tensor(b, dtype=A64.dtype, device=DEV)
+    xr = torch.zeros_like(b64)
+    st = ctx.refine_solve(A64, LU, ipiv, b64, xr, max_iter=30, tol=1e-12)
+    e_ref = rel(to_np(xr), ref)
+    print(f"[lu local] direct err {e_direct:.2e}, refined {st.iterations} steps "
+          f"(residual {st.residual_norm / st.b_norm:.1e}) err {e_ref:.2e}", flush=True)
+    assert e_direct <= tol
+    assert st.converged and e_ref <= 1e-11
+    ctx.close()
+
+
+def test_lu_block_local_pivoting_bem_matrix(bem, oracle):
+    """Block-local pivoting on the config-2 matrix shape (Helmholtz k = 3 V,
+    perturbed icosphere L2, P0): the c64 factor refined in FP64 reaches the
+    complex128 solve."""
+    v, f = bem.meshes.perturbed_icosphere(2, seed=0)
+    V = oracle.assemble(oracle.make_op(0, 1, 3.0, 1.0, 5, 0, 0, 0), v, f)
+    n = len(f)
+    A64 = torch.tensor(V, device=DEV)
+    LU = A64.to(torch.complex64)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    ctx = bem.Context(0)
+    ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
+    ctx.lu_factor(LU, ipiv)
+    b = np.exp(1j * np.arange(n) * 0.1)
+    x = torch.zeros(n, dtype=torch.complex128, device=DEV)
+    st = ctx.refine_solve(A64, LU, ipiv, torch.tensor(b, device=DEV), x, max_iter=30, tol=1e-14)
+    e = rel(to_np(x), np.linalg.solve(V, b))
+    print(f"[lu local bem] {st.iterations} steps, err {e:.2e}", flush=True)
+    assert st.converged and e <= 1e-12
+    ctx.close()
+
+
 def test_set_option_rejects(bem):
     ctx = bem.Context(0)
     with pytest.raises(bem.BEMError):
@@ -409,7 +470,7 @@ def test_set_option_rejects(bem):
     with pytest.raises(bem.BEMError):
         ctx.set_option(99, 0)
     with pytest.raises(bem.BEMError):
-        ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
+        ctx.set_option(bem._lib.OPT_LU_PIVOTING, 3)
     with pytest.raises(bem.BEMError):
         ctx.set_option(bem._lib.OPT_LU_GEMM_TILE, 3)
     with pytest.raises(bem.BEMError):
