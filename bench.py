@@ -246,7 +246,9 @@ def main() -> int:
             ctx.assemble_dirichlet(desc(L.OP_V, 0, 0, L.PREC_FP64, L.PART_NEAR), A32, g, b, 1.0, 1.0, 0.5)
             e[2].record()
             LU32.copy_(A32)
+            ctx.set_option(L.OPT_LU_PIVOTING, 2)  # block-local pivoting (the factor preconditions refinement)
             ctx.lu_factor(LU32, ipiv)
+            ctx.set_option(L.OPT_LU_PIVOTING, 0)
             e[3].record()
             st = ctx.refine_solve(A32, LU32, ipiv, b, x, max_iter=30, tol=1e-13)
             e[4].record()
@@ -255,7 +257,7 @@ def main() -> int:
         tts["mixed_fp32asm"] = {"ms": round(e[0].elapsed_time(e[4]), 3), "far_fp32_ms": round(e[0].elapsed_time(e[1]), 3),
                                 "assembly_ms": round(e[0].elapsed_time(e[2]), 3), "lu_c64_ms": round(e[2].elapsed_time(e[3]), 3),
                                 "refine_ms": round(e[3].elapsed_time(e[4]), 3), "refine_iters": st.iterations,
-                                "rel_err_vs_fp64": err,
+                                "lu_pivoting": "block-local", "rel_err_vs_fp64": err,
                                 "assembly_entries_per_s": entries / (e[0].elapsed_time(e[2]) * 1e-3)}
         # mixed (accurate): FP64 assembly, c64 LU, FP64 refinement against A64
         for rep in range(2):
@@ -266,7 +268,9 @@ def main() -> int:
             ctx.assemble_dirichlet(desc(L.OP_V, 0, 0, L.PREC_FP64), A64, g, b64, 1.0, 1.0, 0.5)
             e[1].record()
             bem.convert(A64, LU32)
+            ctx.set_option(L.OPT_LU_PIVOTING, 2)  # block-local pivoting (the factor preconditions refinement)
             ctx.lu_factor(LU32, ipiv)
+            ctx.set_option(L.OPT_LU_PIVOTING, 0)
             e[2].record()
             st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=1e-14)
             e[3].record()
@@ -275,7 +279,7 @@ def main() -> int:
         tts["mixed_fp64asm"] = {"ms": round(e[0].elapsed_time(e[3]), 3), "assembly_ms": round(e[0].elapsed_time(e[1]), 3),
                                 "lu_c64_ms": round(e[1].elapsed_time(e[2]), 3), "refine_ms": round(e[2].elapsed_time(e[3]), 3),
                                 "refine_iters": st.iterations, "converged": st.converged,
-                                "rel_err_vs_fp64": err2}
+                                "lu_pivoting": "block-local", "rel_err_vs_fp64": err2}
         tts["speedup_mixed_fp32asm"] = round(mean["total_ms"] / tts["mixed_fp32asm"]["ms"], 3)
         tts["speedup_mixed_fp64asm"] = round(mean["total_ms"] / tts["mixed_fp64asm"]["ms"], 3)
         result["time_to_solution"] = tts
@@ -617,16 +621,17 @@ def transmission_main(args, bem) -> int:
     for rep in range(2):
         e = [ev() for _ in range(3)]
         e[0].record()
+        # partial pivoting: block-local pivoting (BEM_OPT_LU_PIVOTING = 2) does
+        # not give a converging refinement on this system (kappa about 1e6)
         L.convert(A64, LU32)
         bm.ctx.lu_factor(LU32, ipiv)
-        e[1].record()
         st = bm.ctx.refine_solve(A64, LU32, ipiv, b, x, max_iter=30, tol=1e-14)
         e[2].record()
         torch.cuda.synchronize()
-    mixA = {"lu_c64_ms": e[0].elapsed_time(e[1]), "refine_ms": e[1].elapsed_time(e[2]),
+    mixA = {"lu_and_refine_ms": e[0].elapsed_time(e[2]), "lu_pivoting": "partial",
             "refine_iters": int(st.iterations), "converged": int(st.converged),
             "rel_err_vs_fp64": torch.linalg.vector_norm(x - x64).item() / xn}
-    mixA["ms"] = asm64 + mixA["lu_c64_ms"] + mixA["refine_ms"]
+    mixA["ms"] = asm64 + mixA["lu_and_refine_ms"]
     del LU32
     # gradient with respect to the interior wave number
     rng = np.random.default_rng(9)
@@ -650,15 +655,14 @@ def transmission_main(args, bem) -> int:
         e[1].record()
         LU = b32._mat_A.clone()
         b32.ctx.lu_factor(LU, ipiv)
-        e[2].record()
         st = b32.ctx.refine_solve(b32._mat_A, LU, ipiv, b, x, max_iter=30, tol=1e-12)
         e[3].record()
         torch.cuda.synchronize()
         del LU
-    mixB = {"assembly_ms": e[0].elapsed_time(e[1]), "lu_c64_ms": e[1].elapsed_time(e[2]),
-            "refine_ms": e[2].elapsed_time(e[3]), "refine_iters": int(st.iterations),
+    mixB = {"assembly_ms": e[0].elapsed_time(e[1]), "lu_and_refine_ms": e[1].elapsed_time(e[3]),
+            "lu_pivoting": "partial", "refine_iters": int(st.iterations),
             "rel_err_vs_fp64": torch.linalg.vector_norm(x - x64).item() / xn}
-    mixB["ms"] = mixB["assembly_ms"] + mixB["lu_c64_ms"] + mixB["refine_ms"]
+    mixB["ms"] = mixB["assembly_ms"] + mixB["lu_and_refine_ms"]
     cm32.kill()
     del b32, cm32
     torch.cuda.empty_cache()

@@ -295,6 +295,104 @@ class BEMManager:
         self.compute_analytical_raw_solved()
         self.splat_u_from_raw_solved_to_vertices()
 
+    # -- norm analysis (BEM_manager.py:850-988) ---------------------------------------
+    # Blocks of an (N+M) x (N+M) matrix in the block layout of matA_add_M.
+    def get_V_part(self, torch_input: torch.Tensor):
+        N0, N = self._Neumann_offset_i, self.N_Neumann
+        return torch_input[N0: N0 + N, self._Neumann_offset_j: self._Neumann_offset_j + N]
+
+    def get_K_part(self, torch_input: torch.Tensor):
+        return torch_input[self._Neumann_offset_i: self._Neumann_offset_i + self.N_Neumann,
+                           self._Dirichlet_offset_j: self._Dirichlet_offset_j + self.M_Dirichlet]
+
+    def get_adj_K_part(self, torch_input: torch.Tensor):
+        return torch_input[self._Dirichlet_offset_i: self._Dirichlet_offset_i + self.M_Dirichlet,
+                           self._Neumann_offset_j: self._Neumann_offset_j + self.N_Neumann]
+
+    def get_W_part(self, torch_input: torch.Tensor):
+        D0, M = self._Dirichlet_offset_i, self.M_Dirichlet
+        return torch_input[D0: D0 + M, self._Dirichlet_offset_j: self._Dirichlet_offset_j + M]
+
+    def set_V_part(self, old_tensor: torch.Tensor, new_tensor: torch.Tensor):
+        self.get_V_part(old_tensor).copy_(new_tensor)
+
+    def set_K_part(self, old_tensor: torch.Tensor, new_tensor: torch.Tensor):
+        self.get_K_part(old_tensor).copy_(new_tensor)
+
+    def set_adj_K_part(self, old_tensor: torch.Tensor, new_tensor: torch.Tensor):
+        self.get_adj_K_part(old_tensor).copy_(new_tensor)
+
+    def set_W_part(self, old_tensor: torch.Tensor, new_tensor: torch.Tensor):
+        self.get_W_part(old_tensor).copy_(new_tensor)
+
+    def compute_inv_norm(self, A: torch.Tensor, A_type: int) -> float:
+        """max 1 / sigma_i of L^-1 A L^-H (BEM_manager.py:874-919): the
+        inverse's norm in the energy norm of the Cholesky factor L of
+        blkdiag(W, V) of the first matrix seen (A_type 1; (MV + MV^T)/2 as
+        there) or of blkdiag(K', K) (A_type 2), kept for later calls.  The
+        dense factorisation, inverse and singular values are the library
+        routines the reference calls (torch.linalg, here on the device);
+        the per-wave-number estimate of the batched sweep is
+        bem_batch_condest."""
+        if self._L1_inv_H is None and A_type == 1:
+            MV = torch.zeros_like(A)
+            self.set_W_part(MV, self.get_W_part(A))
+            self.set_V_part(MV, self.get_V_part(A))
+            MV = (MV + MV.T) / 2.0
+            self._L1_inv_H = torch.linalg.inv(torch.linalg.cholesky(MV, upper=False)).T.conj()
+            self._L2_inv_H = None
+        elif self._L2_inv_H is None and A_type == 2:
+            MV = torch.block_diag(self.get_adj_K_part(A), self.get_K_part(A))
+            self._L2_inv_H = torch.linalg.inv(torch.linalg.cholesky(MV, upper=False)).T.conj()
+            self._L1_inv_H = None
+        LH = self._L1_inv_H if A_type == 1 else self._L2_inv_H
+        if LH is None:
+            raise RuntimeError("A_type must be 1 or 2")
+        n = self.N_Neumann + self.M_Dirichlet
+        A = A @ LH
+        if A.shape[0] == n:
+            A = LH.T.conj() @ A
+        elif A.shape[0] == 2 * n:
+            A = torch.block_diag(LH.T.conj(), LH.T.conj()) @ A
+        else:
+            raise NotImplementedError("We only support Non-Augmentd matrix and Augmented Matrix for Norm analysis")
+        s_val = torch.linalg.svdvals(A)
+        return (1.0 / s_val).max().item()
+
+    def _assemble_norm_matrix(self, k: float, interior_multiplier: float):
+        """_mat_A = P_MINUS(exterior) + m P_PLUS(interior) at wave number k
+        (BEM_manager.py:921-927, 935-941)."""
+        self._mat_A.zero_()
+        self.matrix_layer_forward(k=k, sqrt_n=self._sqrt_no, scope_type=int(ScopeType.EXTERIOR))
+        self.assemble_matA(assemble_type=int(AssembleType.ADD_P_MINUS), multiplier=1.0)
+        self.matrix_layer_forward(k=k, sqrt_n=self._sqrt_ni, scope_type=int(ScopeType.INTERIOR))
+        self.assemble_matA(assemble_type=int(AssembleType.ADD_P_PLUS), multiplier=interior_multiplier)
+
+    def get_mat_A1_inv_norm(self, k: float) -> float:
+        """A1 = P-(exterior) - P+(interior) (BEM_manager.py:921-933)."""
+        self._assemble_norm_matrix(k, -1.0)
+        return self.compute_inv_norm(self._mat_A, A_type=1)
+
+    def get_mat_A2_inv_norm(self, k: float) -> float:
+        """A2 = P-(exterior) + P+(interior) (BEM_manager.py:935-947)."""
+        self._assemble_norm_matrix(k, 1.0)
+        return self.compute_inv_norm(self._mat_A, A_type=2)
+
+    def _augmented(self, k: float, need_recompute_matrix_layers: bool) -> torch.Tensor:
+        """[_mat_A; P+(interior)] (BEM_manager.py:949-966)."""
+        A = self._mat_A.clone()
+        self._mat_A.zero_()
+        if need_recompute_matrix_layers:
+            self.matrix_layer_forward(k=k, sqrt_n=self._sqrt_ni, scope_type=int(ScopeType.INTERIOR))
+        self.assemble_matA(assemble_type=int(AssembleType.ADD_P_PLUS), multiplier=1.0)
+        return torch.cat((A, self._mat_A.clone()), 0)
+
+    def get_augment_mat_A1_inv_norm(self, k: float, need_recompute_matrix_layers: bool) -> float:
+        return self.compute_inv_norm(self._augmented(k, need_recompute_matrix_layers), A_type=1)
+
+    def get_augment_mat_A2_inv_norm(self, k: float, need_recompute_matrix_layers: bool) -> float:
+        return self.compute_inv_norm(self._augmented(k, need_recompute_matrix_layers), A_type=2)
+
     # -- post-processing (BEM_manager.py:664-801) ---------------------------------------
     def compute_analytical_raw_solved(self):
         mm = self._core_manager._mesh_manager

@@ -31,7 +31,7 @@ class WavenumberSweep:
     def __init__(self, meshes, problems, dtype=torch.complex64, precision: int = _lib.PREC_FP32, flags: int = 0,
                  direction=None, gauss_qr: int = 5, device: int = 0, eta_factor: float = 1.0,
                  refine_iters: int = 2, condest_iters: int = 3, tuning: tuple[int, int] | None = None,
-                 pivoting: int = 1):
+                 pivoting: int = 2):
         """meshes: list of (vertices [nv,3], panels [nf,3]) numpy arrays;
         problems: list of (mesh index, k)."""
         self.dev = torch.device("cuda", device)
@@ -43,7 +43,9 @@ class WavenumberSweep:
         self.tuning = tuning
         self.batch_ctx = _lib.Context(device)
         # the combined-field operator is well conditioned (cond_1 ~ 1e2 on
-        # the sweep's meshes), so its factorisations use tournament pivoting
+        # the sweep's meshes) and its c64 factors only precondition the FP64
+        # refinement, so they use block-local pivoting (2: no waits between
+        # workgroups); 1 selects tournament, 0 partial pivoting
         self.batch_ctx.set_option(_lib.OPT_LU_PIVOTING, int(pivoting))
         self.batch = _lib.Batch(self.batch_ctx)
         self.ctxs = []

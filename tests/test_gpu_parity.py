@@ -694,6 +694,64 @@ def test_set_mesh_with_bad_index_leaves_no_mesh(bem):
     ctx.close()
 
 
+def test_norm_analysis_matches_dense_formula(bem, oracle):
+    """get_mat_A2_inv_norm and get_augment_mat_A2_inv_norm
+    (BEM_manager.py:874-919,935-947,968-983) in the layout of the reference's
+    sweep demo (P1 Neumann and P1 Dirichlet unknowns, demo_3d_plot_Sio.py:171-172):
+    the inverse's norm in the energy norm of blkdiag(K', K), against the same
+    formula evaluated with numpy on the oracle's blocks (icosphere L1, k = 0.5,
+    n_i = 2.25).  (A1's blkdiag(W, V) is indefinite on constants for k > 0, so
+    its Cholesky factor does not exist on closed surfaces; the dense library
+    raises there, in the reference as here.)"""
+    v, f = bem.meshes.icosphere(1)
+    k, ni = 0.5, 2.25
+
+    def fD(x, sqrt_n):
+        return torch.exp(1j * k * sqrt_n * x[:, 0])
+
+    def fN(x, n, sqrt_n):
+        return 1j * k * sqrt_n * n[:, 0] * torch.exp(1j * k * sqrt_n * x[:, 0])
+
+    params = {"kernel": "Helmholtz_Transmission", "k": k, "n_i": ni, "n_o": 1.0, "boundary": "Full",
+              "GaussQR": 5, "Q_Neumann": 1, "Q_Dirichlet": 1, "vertices": v, "panels": f}
+    cm = bem.CoreManager(params)
+    cm.initialization(fD, fN)
+    bm = cm._BEM_manager
+    nv = len(v)
+    n = 2 * nv
+    M11 = oracle.mass(5, 1, 1, v, f)
+
+    def blocks(sn):
+        V = oracle.assemble(oracle.make_op(0, 2, k, sn, 5, 1, 1, 0), v, f)
+        K = oracle.assemble(oracle.make_op(1, 2, k, sn, 5, 1, 1, 0), v, f)
+        W = oracle.assemble(oracle.make_op(3, 2, k, sn, 5, 1, 1, 0), v, f)
+        C = np.zeros((n, n), dtype=complex)
+        C[nv:, nv:], C[nv:, :nv], C[:nv, nv:], C[:nv, :nv] = -V, K, -K.T, -W
+        return C
+
+    Mp = np.zeros((n, n))
+    Mp[nv:, :nv], Mp[:nv, nv:] = M11, M11.T
+    Ci = blocks(np.sqrt(ni))
+    A = (0.5 * Mp - blocks(1.0)) + (0.5 * Mp + Ci)   # P-(exterior) + P+(interior)
+    MV = np.zeros_like(A)
+    MV[:nv, :nv], MV[nv:, nv:] = A[:nv, nv:], A[nv:, :nv]
+    LH = np.linalg.inv(np.linalg.cholesky(MV)).T.conj()
+
+    def inv_norm(Amat):
+        B = (LH.T.conj() if Amat.shape[0] == n else np.kron(np.eye(2), LH.T.conj())) @ Amat @ LH
+        return (1.0 / np.linalg.svd(B, compute_uv=False)).max()
+
+    got = bm.get_mat_A2_inv_norm(k)
+    assert rel(to_np(bm.get_mat_A()), A) <= TOL64
+    ref = inv_norm(A)
+    print(f"[norm analysis] A2: {got:.10e} (numpy {ref:.10e})", flush=True)
+    assert abs(got - ref) <= 1e-9 * ref
+    got_aug = bm.get_augment_mat_A2_inv_norm(k, False)
+    ref_aug = inv_norm(np.concatenate([A, 0.5 * Mp + Ci], 0))
+    assert abs(got_aug - ref_aug) <= 1e-9 * ref_aug
+    cm.kill()
+
+
 def test_transmission_solve_accuracy(bem):
     """The ill-conditioned 2x2 Calderon system (L3, n = 1922) is solved by the
     default LU to the accuracy of LAPACK on the same matrix."""

@@ -244,3 +244,24 @@ def test_autotune_cache(bem, tmp_path):
     sw.tuning = None
     r2 = bem.sweep.autotune(sw, candidates=((1, 2), (2, 4)), cache_path=path)
     assert r2["cached"] and tuple(sw.tuning) == tuple(r1["tuning"])
+
+
+def test_condest_on_l3_combined_field_matrices(bem):
+    """The batched 1-norm condition estimate on the sweep's own operator:
+    combined-field matrices of icosphere L3 (642 P1 unknowns) at k = 16.5 and
+    17.9, complex128, against the exact kappa_1 from numpy (an estimate is a
+    lower bound, and Hager-Higham is usually within a small factor)."""
+    mesh = bem.meshes.icosphere(3)
+    sw = bem.sweep.WavenumberSweep([mesh], [(0, 16.5), (0, 17.9)], dtype=torch.complex128, precision=0,
+                                   refine_iters=0, condest_iters=4, direction=bem.meshes.seeded_direction(2))
+    print("[condest L3] combined field", flush=True)
+    sw.run()
+    torch.cuda.synchronize()
+    for q, r in enumerate(sw.results()):
+        a = sw.A[q].cpu().numpy()
+        exact = np.linalg.cond(a, 1)
+        print(f"[condest L3] k={r['k']}: estimate {r['cond1_est']:.4e} exact {exact:.4e}", flush=True)
+        assert abs(r["norm1_A"] - np.linalg.norm(a, 1)) <= 1e-12 * np.linalg.norm(a, 1)
+        assert r["cond1_est"] <= exact * (1 + 1e-10)
+        assert r["cond1_est"] >= 0.3 * exact
+    sw.close()
