@@ -263,7 +263,10 @@ enum {
                                 the others (two internal streams, default
                                 priority, ordered by events against the
                                 caller's stream); -1 = the same two streams
-                                without compute-unit masks */,
+                                without compute-unit masks.  Factorisations
+                                with block-local pivoting (BEM_OPT_LU_PIVOTING
+                                = 2) always use unmasked streams when
+                                look-ahead is on */,
     BEM_OPT_LU_PIVOTING = 3  /* 0 (default) = partial pivoting with one launch
                                 per column, the pivot rows LAPACK's getrf
                                 chooses; 1 = tournament pivoting per 64-column

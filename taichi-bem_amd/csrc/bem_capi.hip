@@ -93,6 +93,8 @@ int bem_destroy(bem_context* ctx) {
             if (w) hipFree(w);
         for (hipStream_t st : ctx->impl.lu_stream)
             if (st) hipStreamDestroy(st);
+        for (hipStream_t st : ctx->impl.lu_stream_free)
+            if (st) hipStreamDestroy(st);
         for (hipEvent_t e : ctx->impl.lu_event)
             if (e) hipEventDestroy(e);
         if (ctx->impl.cf_ws) hipFree(ctx->impl.cf_ws);

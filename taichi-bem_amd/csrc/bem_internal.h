@@ -102,6 +102,7 @@ struct Context {
     int lu_gemm_tile = 1;         // LU trailing updates: 1 128-row tiles where they fit, 0 64 x 64 only
     int lu_streams_cus = 0;       // the setting the streams were created for
     hipStream_t lu_stream[2] = {nullptr, nullptr};
+    hipStream_t lu_stream_free[2] = {nullptr, nullptr};  // look-ahead without CU masks (block-local pivoting)
     hipEvent_t lu_event[3] = {nullptr, nullptr, nullptr};
 };
 

@@ -2074,6 +2074,24 @@ int ensure_lu_streams(Context* ctx) {
     return BEM_OK;
 }
 
+// The unmasked look-ahead streams of block-local pivoting (default priority):
+// its panels are two short launches, so reserving compute units for them
+// only takes them from the trailing update (measured on the L5 c64
+// factorisation: 216 ms unmasked, 226 ms with 32 masked units).
+int ensure_lu_streams_free(Context* ctx) {
+    for (int q = 0; q < 2; ++q)
+        if (!ctx->lu_stream_free[q]) {
+            int rc = check_hip(hipStreamCreateWithFlags(&ctx->lu_stream_free[q], hipStreamNonBlocking), "LU stream");
+            if (rc) return rc;
+        }
+    for (int q = 0; q < 3; ++q)
+        if (!ctx->lu_event[q]) {
+            int rc = check_hip(hipEventCreateWithFlags(&ctx->lu_event[q], hipEventDisableTiming), "LU event");
+            if (rc) return rc;
+        }
+    return BEM_OK;
+}
+
 // counter = 0 and info = 0 of every problem (a kernel, not a memset, so the
 // sequence can be captured in a graph)
 // The counters of a factorisation (the persistent panel's barrier words sit
@@ -2133,7 +2151,12 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     }
     // workgroups of the persistent panel: one per compute unit its stream may use
     int pgrid = ctx->lu_panel ? ctx->num_cu : 0;
-    if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
+    if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2 && ctx->lu_pivoting == 2) {
+        rc = ensure_lu_streams_free(ctx);
+        if (rc) return rc;
+        lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), 0, s, ctx->lu_stream_free[0],
+                                  ctx->lu_stream_free[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
+    } else if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
         rc = ensure_lu_streams(ctx);
         if (rc) return rc;
         if (pgrid > 0 && ctx->lu_lookahead_cus > 0) pgrid = panel_grid_masked(ctx);
