@@ -216,7 +216,9 @@ int bem_vkw_dk_forms(bem_context* ctx, const bem_op_desc* op, const void* aN, co
  * apply_K_dot_D_boundary (double_layer_3d.py:440-501),
  * apply_K_dot_N_boundary (adj_double_layer_3d.py:443-506) and
  * apply_W_dot_D_boundary (hypersingular_layer_3d.py:502-565); the reference
- * always conjugates complex data (conj_x = 1). */
+ * always conjugates complex data (conj_x = 1).  Sums run in a fixed order
+ * (bitwise reproducible) except the far part of a BEM_FLAG_TRANSPOSE product,
+ * whose test-panel chunks meet in atomics. */
 int bem_apply(bem_context* ctx, const bem_op_desc* op, const void* x, int32_t conj_x,
               void* y, double alpha_re, double alpha_im, void* stream);
 
@@ -427,8 +429,11 @@ int bem_refine_solve(bem_context* ctx, const void* A, int64_t lda, int32_t dtype
  * the same discrete operator bem_assemble builds (flags included).  op: V
  * (P0 x P0) or K (P0 test x P1 trial), FP64, dtype BEM_C128 (lam, u complex128
  * of nf and nf|nv entries); grad_vertices [nv*3] f64; grad_k (one f64) may be
- * NULL and is untouched for Laplace.  Accumulation uses atomics (the order of
- * the sums is not fixed).  No reference counterpart: it differentiates
+ * NULL and is untouched for Laplace.  Every sum runs in a fixed order
+ * (per-slot partials, no atomics): the result is bitwise reproducible.  The
+ * partials live in a per-mesh buffer that grows to about 1 GiB at most for
+ * the trial side (larger meshes take longer test chunks) plus 9 doubles per
+ * (256-panel trial block, panel).  No reference counterpart: it differentiates
  * SingleLayer3d/DoubleLayer3d (single_layer_3d.py:155-177,
  * double_layer_3d.py:158-180) for the shape gradient of the Dirichlet
  * problem of BEMManager.run (BEM_manager.py:990-1011). */

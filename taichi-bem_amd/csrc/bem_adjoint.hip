@@ -51,7 +51,22 @@ struct AdjParams {
     double* grad;                    // [nv*3]
     double* gk;                      // [1] or null
     double* kt;                      // K only: complex128 [nv] += (K^T lam) of the regular pairs, or null
+    // per-slot partials, each written by exactly one workgroup or wave and
+    // summed in a fixed order by the reduction kernels below
+    int ti;                          // test panels per regular-pair workgroup
+    int nyc;                         // trial-side components: 9, or 15 with the K^T lam terms
+    double* fx;                      // [njb][9][nf]   test-panel corners, per trial block
+    double* fy;                      // [nib][nyc][nf] trial-panel corners (+ K^T lam), per test chunk
+    double* gkb;                     // [nib * njb]    k-derivative, per workgroup
+    double* nb;                      // [n_near][NB]   near pairs: test corners, trial corners, k
+    double* pc;                      // [nyc][nf]      panel-corner totals
+    const int32_t* __restrict__ near_row;
+    const int32_t* __restrict__ near_tr;
+    const int32_t* __restrict__ vp_row;
+    const int32_t* __restrict__ vp_el;
 };
+
+constexpr int NB = 20;  // per near pair: 9 test-corner, 9 trial-corner, 1 k-derivative slots (+ pad)
 
 // d(2A)/dX_m / (2A) for the stored slots of panel with vertices P[3][3]:
 // 2A = |N|, N = (P1-P0) x (P2-P0): d|N|/dP1 = E2 x n, d|N|/dP2 = n x E1.
@@ -107,25 +122,43 @@ __device__ __forceinline__ void trig_lds(double kap, double r, const double2* ta
 }
 
 // ---------------------------------------------------------------------------
-// Regular pairs.  Block: 256 trial panels (one per lane) x TI test panels
+// Regular pairs.  Block: 256 trial panels (one per lane) x ti test panels
 // (wave-uniform: every lane reads the same test points).  The trial-vertex
-// gradient stays in registers over the TI test panels; the test-vertex
-// gradient is reduced over the block per test panel.  Launch bounds ask for
-// two waves per SIMD; the K Helmholtz variant then spills a few registers,
-// whose loads and stores run once per panel pair, outside the point loops.
+// gradient stays in registers over the ti test panels and is stored once per
+// (test chunk, trial panel); the test-vertex gradient is reduced over the
+// block per test panel and stored once per (trial block, test panel).  No
+// atomics: adjoint_panel_reduce_kernel sums the slots in a fixed order, so the
+// result is bitwise reproducible.  Launch bounds ask for two waves per SIMD;
+// the K Helmholtz variant then spills a few registers, whose loads and stores
+// run once per panel pair, outside the point loops.
 //
 // V is symmetric pair by pair (V_ji sums the same point pairs as V_ij with
 // the roles of the two rule copies exchanged, and G(-d) = G(d)), so V runs
 // over the pairs i < j only, contracted with lam_i u_j + lam_j u_i; blocks
-// with no such pair return at once.  K runs over all ordered pairs.
+// with no such pair write zeros to their slots and return.  K runs over all
+// ordered pairs.
 // ---------------------------------------------------------------------------
-constexpr int TI = 8;
+__device__ void adjoint_zero_slots(const AdjParams& p, int ib, int jb) {
+    const int nf = p.nf, ti = p.ti;
+    for (int e = threadIdx.x; e < ti * 9; e += 256) {
+        const int i = ib * ti + e / 9;
+        if (i < nf) p.fx[((int64_t)jb * 9 + e % 9) * nf + i] = 0.0;
+    }
+    const int j = jb * 256 + threadIdx.x;
+    if (j < nf)
+        for (int c = 0; c < p.nyc; ++c) p.fy[((int64_t)ib * p.nyc + c) * nf + j] = 0.0;
+    if (threadIdx.x == 0) p.gkb[blockIdx.x] = 0.0;
+}
 
 template <int OP, bool HELM>
 __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int njb) {
     const int ib = blockIdx.x / njb, jb = blockIdx.x % njb;
+    const int TI = p.ti;
     if constexpr (OP == BEM_OP_V)
-        if (ib * TI >= min(p.nf, (jb + 1) * 256) - 1) return;  // every i >= every j
+        if (ib * TI >= min(p.nf, (jb + 1) * 256) - 1) {  // every i >= every j
+            adjoint_zero_slots(p, ib, jb);
+            return;
+        }
     __shared__ double2 sct[256];  // sin/cos(j pi/128)
     if constexpr (HELM) {
         sincos_table_fill(sct);
@@ -393,7 +426,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
             }
             gkp = cre(cr, ci, SKr, SKi) * p.dkap;
         }
-        // test-vertex gradient of panel i: block reduction, 9 atomics
+        // test-vertex gradient of panel i: block reduction, one slot per trial block
         gk_lane += gkp;
 #pragma unroll
         for (int m = 0; m < 3; ++m)
@@ -405,31 +438,31 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
         __syncthreads();
         if (threadIdx.x < 9) {
             const double s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-            const int m = threadIdx.x / 3, d = threadIdx.x % 3;
-            if (s != 0.0) atomicAdd(p.grad + 3 * p.tris[3 * i + m] + d, s);
+            p.fx[((int64_t)jb * 9 + threadIdx.x) * nf + i] = s;
         }
         __syncthreads();
     }
     if (jvalid) {
+        double* fy = p.fy + (int64_t)ib * p.nyc * nf + j;
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-                if (gy[m][d] != 0.0) atomicAdd(p.grad + 3 * p.tris[3 * j + m] + d, gy[m][d]);
+            for (int d = 0; d < 3; ++d) fy[(int64_t)(3 * m + d) * nf] = gy[m][d];
         if constexpr (OP == BEM_OP_K) {
-            if (p.kt) {
+            if (p.nyc == 15) {
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
-                    double* o = p.kt + 2 * p.tris[3 * j + t];
-                    atomicAdd(o, kInv4Pi * ktr[t]);
-                    atomicAdd(o + 1, kInv4Pi * kti[t]);
+                    fy[(int64_t)(9 + 2 * t) * nf] = kInv4Pi * ktr[t];
+                    fy[(int64_t)(10 + 2 * t) * nf] = kInv4Pi * kti[t];
                 }
             }
         }
     }
-    if (p.gk) {
+    {
         const double s = wave_sum(gk_lane);
-        if (lane == 0 && s != 0.0) atomicAdd(p.gk, s);
+        if (lane == 0) red[wid][9] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) p.gkb[blockIdx.x] = red[0][9] + red[1][9] + red[2][9] + red[3][9];
     }
 }
 
@@ -580,23 +613,19 @@ __global__ __launch_bounds__(256) void adjoint_near_kernel(AdjParams p) {
         lr = t;
     }
     const double cr = p.ar * lr - p.ai * li, ci = p.ar * li + p.ai * lr;
-    // rotated-slot point derivatives
-    for (int m = 0; m < 3; ++m)
-        for (int d = 0; d < 3; ++d) {
-            atomicAdd(p.grad + 3 * p.tris[3 * i + px[m]] + d, cre(cr, ci, EXr[m][d], EXi[m][d]));
-            atomicAdd(p.grad + 3 * p.tris[3 * j + py[m]] + d, cre(cr, ci, EYr[m][d], EYi[m][d]));
-        }
     // area (and, for K, N_j) derivatives in stored slot order
-    double Xs[3][3], Ys[3][3], dli[3][3], dlj[3][3];
+    double Xs[3][3], Ys[3][3], gi[3][3], gj[3][3];
     load_panel(p, i, Xs);
     load_panel(p, j, Ys);
-    dlog_area(Xs, dli);
-    dlog_area(Ys, dlj);
+    dlog_area(Xs, gi);
+    dlog_area(Ys, gj);
+    const double s0 = cre(cr, ci, S0r, S0i);
+#pragma unroll
     for (int m = 0; m < 3; ++m)
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
-            atomicAdd(p.grad + 3 * p.tris[3 * i + m] + d, cre(cr, ci, S0r, S0i) * dli[m][d]);
-            if constexpr (OP == BEM_OP_V)
-                atomicAdd(p.grad + 3 * p.tris[3 * j + m] + d, cre(cr, ci, S0r, S0i) * dlj[m][d]);
+            gi[m][d] *= s0;
+            gj[m][d] = OP == BEM_OP_V ? gj[m][d] * s0 : 0.0;
         }
     if constexpr (OP == BEM_OP_K) {
         const double a2j = 2.0 * p.area[j];
@@ -608,33 +637,109 @@ __global__ __launch_bounds__(256) void adjoint_near_kernel(AdjParams p) {
         const double c1[3] = {e2[1] * t[2] - e2[2] * t[1], e2[2] * t[0] - e2[0] * t[2], e2[0] * t[1] - e2[1] * t[0]};
         const double c2[3] = {t[1] * e1[2] - t[2] * e1[1], t[2] * e1[0] - t[0] * e1[2], t[0] * e1[1] - t[1] * e1[0]};
         for (int d = 0; d < 3; ++d) {
-            atomicAdd(p.grad + 3 * p.tris[3 * j + 1] + d, c1[d]);
-            atomicAdd(p.grad + 3 * p.tris[3 * j + 2] + d, c2[d]);
-            atomicAdd(p.grad + 3 * p.tris[3 * j + 0] + d, -(c1[d] + c2[d]));
+            gj[1][d] += c1[d];
+            gj[2][d] += c2[d];
+            gj[0][d] -= c1[d] + c2[d];
         }
     }
-    if (p.gk) atomicAdd(p.gk, cre(cr, ci, SKr, SKi) * p.dkap);
+    // rotated-slot point derivatives, moved to the stored slots px[m], py[m]
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (px[m] == sl) gi[sl][d] += cre(cr, ci, EXr[m][d], EXi[m][d]);
+                if (py[m] == sl) gj[sl][d] += cre(cr, ci, EYr[m][d], EYi[m][d]);
+            }
+    double* o = p.nb + pid * NB;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            o[3 * m + d] = gi[m][d];
+            o[9 + 3 * m + d] = gj[m][d];
+        }
+    o[18] = cre(cr, ci, SKr, SKi) * p.dkap;
 }
 
-// grad += sum_i coef_i dA_i/dX, one thread per panel.
-__global__ void area_grad_kernel(int nf, const int32_t* __restrict__ tris, const double* __restrict__ verts,
+// Panel-corner totals in a fixed order: thread per panel q and component
+// c = blockIdx.y (3 slot + dim, then for K^T lam the (re, im) pairs of the
+// three slots): the trial-block slots of q as a test panel, the test-chunk
+// slots of q as a trial panel, its near pairs as test panel (row q) and as
+// trial panel (the mirrors of row q).
+__global__ void adjoint_panel_reduce_kernel(AdjParams p, int njb, int nib) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (q >= p.nf) return;
+    const int nf = p.nf;
+    double s = 0.0;
+    if (c < 9)
+        for (int jb = 0; jb < njb; ++jb) s += p.fx[((int64_t)jb * 9 + c) * nf + q];
+    for (int ib = 0; ib < nib; ++ib) s += p.fy[((int64_t)ib * p.nyc + c) * nf + q];
+    if (c < 9 && p.n_near > 0) {
+        for (int e = p.near_row[q]; e < p.near_row[q + 1]; ++e) s += p.nb[(int64_t)e * NB + c];
+        for (int e = p.near_row[q]; e < p.near_row[q + 1]; ++e) s += p.nb[(int64_t)p.near_tr[e] * NB + 9 + c];
+    }
+    p.pc[(int64_t)c * nf + q] = s;
+}
+
+// Vertex sums over the vertex's panel corners in panel order.
+__global__ void adjoint_vertex_kernel(AdjParams p) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= p.nv) return;
+    const int nf = p.nf;
+    double g[3] = {0.0, 0.0, 0.0}, kr = 0.0, ki = 0.0;
+    for (int c = p.vp_row[v]; c < p.vp_row[v + 1]; ++c) {
+        const int q = p.vp_el[c] / 3, sl = p.vp_el[c] % 3;
+        for (int d = 0; d < 3; ++d) g[d] += p.pc[(int64_t)(3 * sl + d) * nf + q];
+        if (p.nyc == 15) {
+            kr += p.pc[(int64_t)(9 + 2 * sl) * nf + q];
+            ki += p.pc[(int64_t)(10 + 2 * sl) * nf + q];
+        }
+    }
+    for (int d = 0; d < 3; ++d) p.grad[3 * v + d] += g[d];
+    if (p.nyc == 15) { p.kt[2 * v] += kr; p.kt[2 * v + 1] += ki; }
+}
+
+// k-derivative: one workgroup, a fixed assignment of slots to threads and a
+// fixed tree.
+__global__ __launch_bounds__(256) void adjoint_gk_kernel(AdjParams p, int64_t nblocks) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += 256) s += p.gkb[b];
+    for (int64_t e = threadIdx.x; e < p.n_near; e += 256) s += p.nb[e * NB + 18];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *p.gk += red[0];
+}
+
+// grad += sum_i coef_i dA_i/dX: one thread per vertex over its panel corners
+// in panel order.
+__global__ void area_grad_kernel(int nv, const int32_t* __restrict__ vp_row, const int32_t* __restrict__ vp_el,
+                                 const int32_t* __restrict__ tris, const double* __restrict__ verts,
                                  const double* __restrict__ area, const double* __restrict__ coef, double* grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nf) return;
-    const double c = coef[i];
-    if (c == 0.0) return;
-    double P[3][3], g[3][3];
-    for (int m = 0; m < 3; ++m)
-        for (int d = 0; d < 3; ++d) P[m][d] = verts[3 * tris[3 * i + m] + d];
-    dlog_area(P, g);
-    const double s = c * area[i];  // dA = A dlog(2A)
-    for (int m = 0; m < 3; ++m)
-        for (int d = 0; d < 3; ++d) atomicAdd(grad + 3 * tris[3 * i + m] + d, s * g[m][d]);
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int c = vp_row[v]; c < vp_row[v + 1]; ++c) {
+        const int i = vp_el[c] / 3, sl = vp_el[c] % 3;
+        double P[3][3], g[3][3];
+        for (int m = 0; m < 3; ++m)
+            for (int d = 0; d < 3; ++d) P[m][d] = verts[3 * tris[3 * i + m] + d];
+        dlog_area(P, g);
+        const double s = coef[i] * area[i];  // dA = A dlog(2A)
+        for (int d = 0; d < 3; ++d) acc[d] += s * g[sl][d];
+    }
+    for (int d = 0; d < 3; ++d) grad[3 * v + d] += acc[d];
 }
 
 template <int OP>
-void launch_adjoint(const AdjParams& p, bool helm, int njb, hipStream_t s) {
-    const int nib = (p.nf + TI - 1) / TI;
+void launch_adjoint(const AdjParams& p, bool helm, int njb, int nib, hipStream_t s) {
     const dim3 gf((unsigned)((int64_t)nib * njb));
     if (helm) hipLaunchKernelGGL((adjoint_far_kernel<OP, true>), gf, dim3(256), 0, s, p, njb);
     else hipLaunchKernelGGL((adjoint_far_kernel<OP, false>), gf, dim3(256), 0, s, p, njb);
@@ -643,7 +748,14 @@ void launch_adjoint(const AdjParams& p, bool helm, int njb, hipStream_t s) {
         if (helm) hipLaunchKernelGGL((adjoint_near_kernel<OP, true>), gn, dim3(256), 0, s, p);
         else hipLaunchKernelGGL((adjoint_near_kernel<OP, false>), gn, dim3(256), 0, s, p);
     }
+    hipLaunchKernelGGL(adjoint_panel_reduce_kernel, dim3((p.nf + 255) / 256, p.nyc), dim3(256), 0, s, p, njb, nib);
+    hipLaunchKernelGGL(adjoint_vertex_kernel, dim3((p.nv + 255) / 256), dim3(256), 0, s, p);
+    if (p.gk) hipLaunchKernelGGL(adjoint_gk_kernel, dim3(1), dim3(256), 0, s, p, (int64_t)nib * njb);
 }
+
+// Trial-side slots cap: test chunks grow (ti doubles from 8) until the
+// per-chunk partials fit in it.
+constexpr size_t ADJ_FY_CAP = size_t(1) << 30;
 
 }  // namespace
 
@@ -679,8 +791,25 @@ int shape_gradient(Context* ctx, const bem_op_desc* d, const void* lam, const vo
     p.grad = grad; p.gk = helm ? grad_k : nullptr;
     p.kt = (double*)kt;
     const int njb = (m.nf + 255) / 256;
-    if (v_op) launch_adjoint<BEM_OP_V>(p, helm, njb, s);
-    else launch_adjoint<BEM_OP_K>(p, helm, njb, s);
+    p.nyc = kt ? 15 : 9;
+    p.ti = 8;
+    while (p.ti < m.nf && (size_t)((m.nf + p.ti - 1) / p.ti) * p.nyc * m.nf * sizeof(double) > ADJ_FY_CAP) p.ti *= 2;
+    const int nib = (m.nf + p.ti - 1) / p.ti;
+    const size_t nfx = (size_t)njb * 9 * m.nf, nfy = (size_t)nib * p.nyc * m.nf, ngk = (size_t)nib * njb;
+    const size_t nnb = (size_t)m.n_near * NB, npc = (size_t)p.nyc * m.nf;
+    const size_t bytes = (nfx + nfy + ngk + nnb + npc) * sizeof(double);
+    if (m.adj_bytes < bytes) {
+        if (m.adj) hipFree(m.adj);
+        m.adj = nullptr;
+        m.adj_bytes = 0;
+        rc = check_hip(hipMalloc((void**)&m.adj, bytes), "adjoint partials");
+        if (rc) return rc;
+        m.adj_bytes = bytes;
+    }
+    p.fx = m.adj; p.fy = p.fx + nfx; p.gkb = p.fy + nfy; p.nb = p.gkb + ngk; p.pc = p.nb + nnb;
+    p.near_row = m.near_row; p.near_tr = m.near_tr; p.vp_row = m.vp_row; p.vp_el = m.vp_el;
+    if (v_op) launch_adjoint<BEM_OP_V>(p, helm, njb, nib, s);
+    else launch_adjoint<BEM_OP_K>(p, helm, njb, nib, s);
     rc = check_hip(hipGetLastError(), "shape gradient launch");
     if (rc || !kt) return rc;
     // the near pairs of K^T lam: the transposed near-field product
@@ -694,7 +823,8 @@ int area_gradient(Context* ctx, const double* coef, double* grad, hipStream_t s)
     Mesh& m = ctx->mesh;
     if (!m.nf) return set_error(BEM_ERR_INVALID, "no mesh: call bem_set_mesh first");
     if (!coef || !grad) return set_error(BEM_ERR_INVALID, "null argument");
-    hipLaunchKernelGGL(area_grad_kernel, dim3((m.nf + 255) / 256), dim3(256), 0, s, m.nf, (const int32_t*)m.tris,
+    hipLaunchKernelGGL(area_grad_kernel, dim3((m.nv + 255) / 256), dim3(256), 0, s, m.nv,
+                       (const int32_t*)m.vp_row, (const int32_t*)m.vp_el, (const int32_t*)m.tris,
                        (const double*)m.verts, (const double*)m.area, coef, grad);
     return check_hip(hipGetLastError(), "area gradient launch");
 }

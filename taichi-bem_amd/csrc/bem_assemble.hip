@@ -15,8 +15,10 @@
 //                of the pair's relation, FP64 always, wave-reduced sums.
 //
 // Output: out[row * ld + col] += alpha * value with row/col the test/trial
-// dof (panel for P0, vertex for P1).  P1 rows or columns collect from several
-// panels and use atomics; P0 x P0 entries have a single writer.
+// dof (panel for P0, vertex for P1).  Assembled P1 rows or columns collect
+// from several panels and use atomics; P0 x P0 entries have a single writer.
+// Matrix-free products into vertex rows store per-panel and per-pair partials
+// and add them per vertex in panel order (bitwise reproducible).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -63,6 +65,10 @@ struct OpParams {
     int dtype_x;
     void* y;
     double* near_buf;
+    double* panel_buf;                    // [nf][3][2] far-field partials of P1 test rows
+    const int32_t* __restrict__ near_tr;  // mirrored pair of each near pair
+    const int32_t* __restrict__ vp_row;   // vertex -> panel corners (CSR, panel order)
+    const int32_t* __restrict__ vp_el;
 };
 
 template <typename T> struct Ptr;
@@ -499,8 +505,13 @@ __global__ __launch_bounds__(256) void far_apply_kernel(OpParams p) {
         for (int s = 0; s < NS; ++s) {
             double a = red[0][2 * s] + red[1][2 * s] + red[2][2 * s] + red[3][2 * s];
             double b = red[0][2 * s + 1] + red[1][2 * s + 1] + red[2][2 * s + 1] + red[3][2 * s + 1];
-            const int64_t row = DT ? p.tris[3 * i + s] : i;
-            add_elem(p.y, p.dtype_x, row, p.ar * a - p.ai * b, p.ar * b + p.ai * a, DT != 0);
+            if (DT) {
+                // vertex rows collect from several panels: vertex_rows_kernel adds them in panel order
+                p.panel_buf[6 * i + 2 * s] = a;
+                p.panel_buf[6 * i + 2 * s + 1] = b;
+            } else {
+                add_elem(p.y, p.dtype_x, i, p.ar * a - p.ai * b, p.ar * b + p.ai * a, false);
+            }
         }
     }
 }
@@ -533,8 +544,9 @@ __global__ __launch_bounds__(256) void near_apply_kernel(OpParams p) {
 // from the same pair evaluations as the forward products.  Far: a block
 // holds 256 trial panels (one per lane) and a chunk of TCH test panels
 // (wave-uniform, as in far_apply_kernel); a lane sums Op[s][t] x[s] over its
-// chunk and adds its trial dofs once.  Near: one wave per pair, its trial
-// dofs added atomically.
+// chunk and adds its trial dofs once, with atomics (the blocks of the test
+// chunks meet there, in no fixed order).  Near: one wave per pair writes its
+// per-pair sums, and trial_gather_kernel adds them in a fixed order.
 // ---------------------------------------------------------------------------
 constexpr int TCH = 64;
 
@@ -588,32 +600,65 @@ __global__ __launch_bounds__(256) void near_apply_t_kernel(OpParams p) {
             yr += vr[s][t] * xr - vi[s][t] * xi;
             yi += vr[s][t] * xi + vi[s][t] * xr;
         }
-        const int64_t col = DR ? p.tris[3 * np.j + t] : np.j;
-        add_elem(p.y, p.dtype_x, col, p.ar * yr - p.ai * yi, p.ar * yi + p.ai * yr, true);
+        p.near_buf[pid * 6 + 2 * t] = yr;
+        p.near_buf[pid * 6 + 2 * t + 1] = yi;
     }
 }
 
-// Deterministic per-row sum of the near contributions (P0 test rows), or an
-// atomic scatter to vertex rows (P1 test).
-__global__ void near_reduce_kernel(OpParams p, int deg_test) {
-    if (!deg_test) {
-        const int i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= p.nf) return;
-        double a = 0.0, b = 0.0;
-        for (int e = p.near_row[i]; e < p.near_row[i + 1]; ++e) {
-            a += p.near_buf[(int64_t)e * 6];
-            b += p.near_buf[(int64_t)e * 6 + 1];
-        }
-        add_elem(p.y, p.dtype_x, i, p.ar * a - p.ai * b, p.ar * b + p.ai * a, false);
-    } else {
-        const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (e >= p.n_near) return;
-        const int i = p.near[e].i;
-        for (int s = 0; s < 3; ++s) {
-            double a = p.near_buf[e * 6 + 2 * s], b = p.near_buf[e * 6 + 2 * s + 1];
-            add_elem(p.y, p.dtype_x, p.tris[3 * i + s], p.ar * a - p.ai * b, p.ar * b + p.ai * a, true);
-        }
+// Sums of the near pairs of test panel i for its slot s, in list order.
+__device__ __forceinline__ void near_row_sum(const OpParams& p, int i, int s, double& a, double& b) {
+    for (int e = p.near_row[i]; e < p.near_row[i + 1]; ++e) {
+        a += p.near_buf[(int64_t)e * 6 + 2 * s];
+        b += p.near_buf[(int64_t)e * 6 + 2 * s + 1];
     }
+}
+
+// Deterministic per-row sum of the near contributions, P0 test rows.
+__global__ void near_reduce_kernel(OpParams p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nf) return;
+    double a = 0.0, b = 0.0;
+    near_row_sum(p, i, 0, a, b);
+    add_elem(p.y, p.dtype_x, i, p.ar * a - p.ai * b, p.ar * b + p.ai * a, false);
+}
+
+// P1 test rows: one thread per vertex adds, over the vertex's panel corners
+// (i, s) in panel order, the far-field partial of the panel (use_far) and its
+// near pairs in list order (use_near).
+__global__ void vertex_rows_kernel(OpParams p, int use_far, int use_near) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= p.nv) return;
+    double a = 0.0, b = 0.0;
+    for (int c = p.vp_row[v]; c < p.vp_row[v + 1]; ++c) {
+        const int i = p.vp_el[c] / 3, s = p.vp_el[c] % 3;
+        if (use_far) { a += p.panel_buf[6 * i + 2 * s]; b += p.panel_buf[6 * i + 2 * s + 1]; }
+        if (use_near) near_row_sum(p, i, s, a, b);
+    }
+    add_elem(p.y, p.dtype_x, v, p.ar * a - p.ai * b, p.ar * b + p.ai * a, false);
+}
+
+// Near part of a transposed product: trial dof j (P0) or v (P1) collects the
+// pairs whose trial panel is j, found as the mirrors of row j's pairs, in
+// row order (and for P1 over the vertex's panel corners in panel order).
+__device__ __forceinline__ void near_col_sum(const OpParams& p, int j, int t, double& a, double& b) {
+    for (int e = p.near_row[j]; e < p.near_row[j + 1]; ++e) {
+        const int64_t f = p.near_tr[e];
+        a += p.near_buf[f * 6 + 2 * t];
+        b += p.near_buf[f * 6 + 2 * t + 1];
+    }
+}
+
+__global__ void trial_gather_kernel(OpParams p, int deg_trial) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (!deg_trial) {
+        if (v >= p.nf) return;
+        near_col_sum(p, v, 0, a, b);
+    } else {
+        if (v >= p.nv) return;
+        for (int c = p.vp_row[v]; c < p.vp_row[v + 1]; ++c) near_col_sum(p, p.vp_el[c] / 3, p.vp_el[c] % 3, a, b);
+    }
+    add_elem(p.y, p.dtype_x, v, p.ar * a - p.ai * b, p.ar * b + p.ai * a, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -647,20 +692,37 @@ __global__ void mass_assemble_kernel(OpParams p, const double* mw, int dt, int d
         }
 }
 
-__global__ void mass_apply_kernel(OpParams p, const double* mw, int dt, int dr) {
+// P0 test rows: one thread per panel.
+__global__ void mass_apply_kernel(OpParams p, const double* mw, int dr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.nf) return;
     double m[3][3];
-    mass_block(p, mw, i, dt, dr, m);
-    for (int s = 0; s < (dt ? 3 : 1); ++s) {
-        double yr = 0.0, yi = 0.0;
+    mass_block(p, mw, i, 0, dr, m);
+    double yr = 0.0, yi = 0.0;
+    for (int t = 0; t < (dr ? 3 : 1); ++t) {
+        double xr, xi;
+        load_elem(p.x, p.dtype_x, dr ? p.tris[3 * i + t] : i, p.conj_x, xr, xi);
+        yr += m[0][t] * xr; yi += m[0][t] * xi;
+    }
+    add_elem(p.y, p.dtype_x, i, p.ar * yr, p.ar * yi, false);
+}
+
+// P1 test rows: one thread per vertex over its panel corners in panel order.
+__global__ void mass_apply_vertex_kernel(OpParams p, const double* mw, int dr) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= p.nv) return;
+    double yr = 0.0, yi = 0.0;
+    for (int c = p.vp_row[v]; c < p.vp_row[v + 1]; ++c) {
+        const int i = p.vp_el[c] / 3, s = p.vp_el[c] % 3;
+        double m[3][3];
+        mass_block(p, mw, i, 1, dr, m);
         for (int t = 0; t < (dr ? 3 : 1); ++t) {
             double xr, xi;
             load_elem(p.x, p.dtype_x, dr ? p.tris[3 * i + t] : i, p.conj_x, xr, xi);
             yr += m[s][t] * xr; yi += m[s][t] * xi;
         }
-        add_elem(p.y, p.dtype_x, dt ? p.tris[3 * i + s] : i, p.ar * yr, p.ar * yi, dt != 0);
     }
+    add_elem(p.y, p.dtype_x, v, p.ar * yr, p.ar * yi, false);
 }
 
 OpParams make_params(Context* ctx, const bem_op_desc* d) {
@@ -682,6 +744,7 @@ OpParams make_params(Context* ctx, const bem_op_desc* d) {
     }
     p.out = nullptr; p.ld = 0; p.dtype = BEM_C128; p.ar = 1.0; p.ai = 0.0;
     p.x = nullptr; p.conj_x = 0; p.dtype_x = BEM_C128; p.y = nullptr; p.near_buf = m.near_buf;
+    p.panel_buf = m.panel_buf; p.near_tr = m.near_tr; p.vp_row = m.vp_row; p.vp_el = m.vp_el;
     return p;
 }
 
@@ -1988,11 +2051,16 @@ int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void
         }
         if (p.n_near > 0 && (parts & BEM_PART_NEAR)) {
             hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
+            const int ncol = d->deg_trial ? p.nv : p.nf;
+            hipLaunchKernelGGL(trial_gather_kernel, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, p,
+                               d->deg_trial);
             rc = check_hip(hipGetLastError(), "transposed near-field apply launch");
         }
         return rc;
     }
-    if (parts & BEM_PART_FAR) {
+    const bool use_far = (parts & BEM_PART_FAR) != 0;
+    const bool use_near = p.n_near > 0 && (parts & BEM_PART_NEAR);
+    if (use_far) {
         NearFn far = dk ? far_apply_dk_any(d->op, d->deg_test, d->deg_trial)
                         : d->precision == BEM_PREC_FP32 ? far_apply_any<float>(d->op, helm, d->deg_test, d->deg_trial)
                                                         : far_apply_any<double>(d->op, helm, d->deg_test, d->deg_trial);
@@ -2000,14 +2068,19 @@ int apply_op(Context* ctx, const bem_op_desc* d, const void* x, int conj_x, void
         rc = check_hip(hipGetLastError(), "far-field apply launch");
         if (rc) return rc;
     }
-    if (p.n_near > 0 && (parts & BEM_PART_NEAR)) {
+    if (use_near) {
         NearFn nearf = dk ? near_apply_dk_any(d->op, d->deg_test, d->deg_trial)
                           : near_any(d->op, helm, d->deg_test, d->deg_trial, true);
         hipLaunchKernelGGL(nearf, dim3((unsigned)((p.n_near + 3) / 4)), dim3(256), 0, s, p);
-        const int64_t nth = d->deg_test ? p.n_near : p.nf;
-        hipLaunchKernelGGL(near_reduce_kernel, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, p,
-                           d->deg_test);
+        if (!d->deg_test)
+            hipLaunchKernelGGL(near_reduce_kernel, dim3((unsigned)((p.nf + 255) / 256)), dim3(256), 0, s, p);
         rc = check_hip(hipGetLastError(), "near-field apply launch");
+        if (rc) return rc;
+    }
+    if (d->deg_test && (use_far || use_near)) {
+        hipLaunchKernelGGL(vertex_rows_kernel, dim3((unsigned)((p.nv + 255) / 256)), dim3(256), 0, s, p,
+                           (int)use_far, (int)use_near);
+        rc = check_hip(hipGetLastError(), "vertex-row sum launch");
     }
     return rc;
 }
@@ -2233,8 +2306,12 @@ int mass_apply(Context* ctx, int deg_test, int deg_trial, int dtype, const void*
     if (dtype < BEM_F64 || dtype > BEM_C64) return set_error(BEM_ERR_INVALID, "unknown dtype");
     OpParams p = make_params(ctx, nullptr);
     p.x = x; p.conj_x = conj_x; p.dtype_x = dtype; p.y = y; p.ar = alpha;
-    hipLaunchKernelGGL(mass_apply_kernel, dim3((p.nf + 255) / 256), dim3(256), 0, s, p,
-                       ctx->mesh.mass_w, deg_test, deg_trial);
+    if (deg_test)
+        hipLaunchKernelGGL(mass_apply_vertex_kernel, dim3((p.nv + 255) / 256), dim3(256), 0, s, p,
+                           ctx->mesh.mass_w, deg_trial);
+    else
+        hipLaunchKernelGGL(mass_apply_kernel, dim3((p.nf + 255) / 256), dim3(256), 0, s, p, ctx->mesh.mass_w,
+                           deg_trial);
     return check_hip(hipGetLastError(), "mass apply launch");
 }
 

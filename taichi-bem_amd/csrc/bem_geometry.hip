@@ -55,14 +55,14 @@ __global__ void panel_geometry_kernel(int nf, const double* __restrict__ v,
 // One thread per vertex over a vertex->panel CSR, so the sums are ordered
 // (panel order) and deterministic.
 __global__ void vertex_geometry_kernel(int nv, const int32_t* __restrict__ vp_row,
-                                       const int32_t* __restrict__ vp_col,
+                                       const int32_t* __restrict__ vp_el,
                                        const double* __restrict__ area,
                                        const double* __restrict__ normal, double* va, double* vn) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     double a = 0.0, n[3] = {0.0, 0.0, 0.0};
     for (int e = vp_row[v]; e < vp_row[v + 1]; ++e) {
-        int i = vp_col[e];
+        int i = vp_el[e] / 3;
         a += area[i] / 3.0;
         for (int d = 0; d < 3; ++d) n[d] += normal[3 * i + d];
     }
@@ -279,7 +279,8 @@ int dev_alloc(T** p, size_t n) {
 void mesh_free(Mesh* m) {
     void* ptrs[] = {m->verts, m->tris, m->area, m->normal, m->vert_area, m->vert_normal, m->curl,
                     m->pu, m->pl, m->cen, m->pu32, m->pl32, m->phi, m->phi32, m->mass_w, m->frame, m->pq,
-                    m->near, m->near_row, m->rule, m->near_buf, m->kpart};
+                    m->near, m->near_row, m->rule, m->near_buf, m->kpart, m->vp_row, m->vp_el,
+                    m->near_tr, m->adj, m->panel_buf};
     for (void* p : ptrs)
         if (p) hipFree(p);
     *m = Mesh();
@@ -290,12 +291,8 @@ namespace {
 // Device buffers of one mesh build that do not outlive it.
 struct BuildTemps {
     double* r12w = nullptr;
-    int32_t* vp_row = nullptr;
-    int32_t* vp_col = nullptr;
     ~BuildTemps() {
         if (r12w) hipFree(r12w);
-        if (vp_row) hipFree(vp_row);
-        if (vp_col) hipFree(vp_col);
     }
 };
 
@@ -327,6 +324,7 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
     rc |= dev_alloc(&m.mass_w, (size_t)q2);
     rc |= dev_alloc(&m.frame, (size_t)nf * PANEL_FRAME);
     rc |= dev_alloc(&m.pq, (size_t)nf * q2 * 4);
+    rc |= dev_alloc(&m.panel_buf, (size_t)nf * 6);
     if (rc) return BEM_ERR_HIP;
     rc = check_hip(hipMemcpyAsync(m.verts, d_verts, sizeof(double) * nv * 3, hipMemcpyDeviceToDevice, s), "copy vertices");
     if (rc) return rc;
@@ -361,23 +359,22 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
     hipMemcpyAsync(m.phi32, phi32.data(), phi32.size() * sizeof(float), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.mass_w, mw.data(), mw.size() * sizeof(double), hipMemcpyHostToDevice, s);
 
-    // vertex -> panel CSR (host, deterministic panel order)
-    std::vector<int32_t> vp_row(nv + 1, 0), vp_col((size_t)nf * 3);
+    // vertex -> panel-corner CSR (host, deterministic panel order)
+    std::vector<int32_t> vp_row(nv + 1, 0), vp_el((size_t)nf * 3);
     for (int e = 0; e < nf * 3; ++e) vp_row[m.h_tris[e] + 1]++;
     for (int v = 0; v < nv; ++v) vp_row[v + 1] += vp_row[v];
     {
         std::vector<int32_t> fill(vp_row.begin(), vp_row.end() - 1);
-        for (int e = 0; e < nf * 3; ++e) vp_col[fill[m.h_tris[e]]++] = e / 3;
+        for (int e = 0; e < nf * 3; ++e) vp_el[fill[m.h_tris[e]]++] = e;
     }
-    if (dev_alloc(&tmp.vp_row, vp_row.size()) || dev_alloc(&tmp.vp_col, vp_col.size())) return BEM_ERR_HIP;
-    int32_t *d_vp_row = tmp.vp_row, *d_vp_col = tmp.vp_col;
-    hipMemcpyAsync(d_vp_row, vp_row.data(), vp_row.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(d_vp_col, vp_col.data(), vp_col.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (dev_alloc(&m.vp_row, vp_row.size()) || dev_alloc(&m.vp_el, vp_el.size())) return BEM_ERR_HIP;
+    hipMemcpyAsync(m.vp_row, vp_row.data(), vp_row.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(m.vp_el, vp_el.data(), vp_el.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
 
     hipLaunchKernelGGL(panel_geometry_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts,
                        m.tris, m.area, m.normal, m.cen, m.curl);
-    hipLaunchKernelGGL(vertex_geometry_kernel, dim3((nv + 255) / 256), dim3(256), 0, s, nv, d_vp_row,
-                       d_vp_col, m.area, m.normal, m.vert_area, m.vert_normal);
+    hipLaunchKernelGGL(vertex_geometry_kernel, dim3((nv + 255) / 256), dim3(256), 0, s, nv, m.vp_row,
+                       m.vp_el, m.area, m.normal, m.vert_area, m.vert_normal);
     hipLaunchKernelGGL(panel_frame_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts, m.tris,
                        m.normal, m.cen, m.frame);
     hipLaunchKernelGGL(panel_points_kernel, dim3((nf * q2 + 255) / 256), dim3(256), 0, s, nf, q2,
@@ -440,9 +437,24 @@ int mesh_build_near(Context* ctx, int flags, hipStream_t s) {
     }
     m.n_near = (int64_t)list.size();
     m.near_counts[BEM_SEPARATE] = (int64_t)nf * nf - m.n_near;
-    if (dev_alloc(&m.near, list.size()) || dev_alloc(&m.near_row, row.size())) return BEM_ERR_HIP;
+    // the relation is symmetric, so (j, i) is in row j, whose trial panels are sorted
+    std::vector<int32_t> tr(list.size());
+    for (size_t e = 0; e < list.size(); ++e) {
+        const int i = list[e].i, j = list[e].j;
+        int lo = row[j], hi = row[j + 1];
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (list[mid].j < i) lo = mid + 1; else hi = mid;
+        }
+        if (lo == row[j + 1] || list[lo].j != i)
+            return set_error(BEM_ERR_INVALID, "near list: a pair without its mirror (inconsistent panels)");
+        tr[e] = lo;
+    }
+    if (dev_alloc(&m.near, list.size()) || dev_alloc(&m.near_row, row.size()) || dev_alloc(&m.near_tr, tr.size()))
+        return BEM_ERR_HIP;
     hipMemcpyAsync(m.near, list.data(), list.size() * sizeof(NearPair), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.near_row, row.data(), row.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(m.near_tr, tr.data(), tr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
     if (m.near_buf_len < m.n_near * 6) {
         if (dev_alloc(&m.near_buf, (size_t)m.n_near * 6)) return BEM_ERR_HIP;
         m.near_buf_len = m.n_near * 6;

@@ -62,10 +62,19 @@ struct Mesh {
     int32_t rule_off[4] = {0, 0, 0, 0};  // offset per relation (index 0 unused)
     int32_t rule_len[4] = {0, 0, 0, 0};
     std::vector<int32_t> h_tris;  // host copy for near-list construction
+    // vertex -> panel corners, CSR in panel order: vp_el[e] = 3 * panel + slot
+    // for e in [vp_row[v], vp_row[v + 1]); the fixed order of every per-vertex
+    // sum of panel contributions
+    int32_t* vp_row = nullptr;    // [nv+1]
+    int32_t* vp_el = nullptr;     // [3 nf]
+    int32_t* near_tr = nullptr;   // [n_near] index of the mirrored pair (j, i) of near pair (i, j)
     double* near_buf = nullptr;   // scratch for matrix-free near contributions
+    double* panel_buf = nullptr;  // [nf][3][2] per-panel partials of matrix-free products into vertex rows
     int64_t near_buf_len = 0;
     double* kpart = nullptr;      // per (test panel, trial block) partial sums of K conj(g), then
     size_t kpart_bytes = 0;       // per (row slot, trial panel) partials of the mirrored terms
+    double* adj = nullptr;        // adjoint contractions: per-slot partials of the vertex gradient
+    size_t adj_bytes = 0;
 };
 
 // Slots of Context::counters.

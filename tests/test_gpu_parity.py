@@ -187,6 +187,38 @@ def test_apply_transpose_matches_assembled_l3(bem, op, dt, dr):
     assert rel(to_np(y), to_np(ref)) <= 1e-13
 
 
+@pytest.mark.parametrize("op,dt,dr", [(3, 1, 1), (2, 1, 0), (1, 1, 1)])
+def test_vertex_sums_bitwise_reproducible(bem, op, dt, dr):
+    """Products into vertex rows (per-panel far partials and per-pair near
+    sums, added per vertex in panel order), the near part of transposed
+    products and the P1-row mass product are bitwise reproducible; the
+    product in two calls (far, then near) equals the one-call product."""
+    v, f = bem.meshes.perturbed_icosphere(3, seed=2)
+    ctx = make_ctx(bem, v, f)
+    L = bem._lib
+    nx = len(v) if dr else len(f)
+    g = torch.Generator(device="cpu").manual_seed(9)
+    x = torch.randn(nx, dtype=torch.complex128, generator=g).to(DEV)
+
+    def run(flags=0, parts=0, n_out=len(v), xx=x):
+        y = torch.zeros(n_out, dtype=torch.complex128, device=DEV)
+        ctx.apply(bem.Context.op_desc(op, 1, 2.5, 1.0, dt, dr, 0, 0, flags, parts), xx, y, 0.5 + 0.25j)
+        return y
+
+    a, b = run(), run()
+    assert torch.equal(a, b)
+    split = run(parts=L.PART_FAR) + run(parts=L.PART_NEAR)
+    assert rel(to_np(split), to_np(a)) <= 1e-14
+    xt = torch.randn(len(v), dtype=torch.complex128, generator=g).to(DEV)
+    nt = len(v) if dr else len(f)
+    a, b = (run(L.FLAG_TRANSPOSE, L.PART_NEAR, nt, xt) for _ in range(2))
+    assert torch.equal(a, b) and float(a.abs().max()) > 0.0
+    ym = [torch.zeros(len(v), dtype=torch.complex128, device=DEV) for _ in range(2)]
+    for y in ym:
+        ctx.mass_apply(1, dr, x, y, 0.5, conj_x=False)
+    assert torch.equal(ym[0], ym[1]) and float(ym[0].abs().max()) > 0.0
+
+
 def test_apply_transpose_rejects(bem, meshes):
     v, f = meshes["p1"]
     ctx = make_ctx(bem, v, f)
