@@ -2118,19 +2118,34 @@ __global__ void ipiv_identity_kernel(LuProb inl, const LuProb* tbl) {
     if (i < P.n) P.ipiv[i] = i;
 }
 
-// Workgroups of a persistent panel on the CU-masked panel stream.  The mask
-// takes the first np compute units, which fill whole XCDs (num_cu / 8 each)
-// in order, and workgroups are dealt round-robin over the XCDs the mask
-// touches, so every XCD must hold its share: with a partial last XCD of r
-// units that is (XCDs touched) x r workgroups.  (Measured: a grid of 48 on
-// a 48-unit mask, 24 per XCD on 32 + 16 units, never became resident; the
-// panel's bounded waits turned that into an error.)
-int panel_grid_masked(const Context* ctx) {
-    const int ncu = ctx->num_cu;
-    const int np = std::min(ctx->lu_lookahead_cus, ncu - 1);
-    const int per = std::max(1, ncu / 8), full = np / per, rem = np % per;
-    if (rem == 0) return np;
-    return (full + 1) * rem;
+// Largest grid of the persistent panel whose workgroups are all resident at
+// once on `units` otherwise idle compute units (its waits need every one of
+// them running).  Two facts fix it:
+//  * per unit, the number of workgroups the kernel's registers and LDS allow
+//    (hipOccupancyMaxActiveBlocksPerMultiprocessor; 0 means the kernel cannot
+//    run at all, and the panels then take the per-column launches);
+//  * the dispatcher deals workgroups round-robin over the XCDs the stream may
+//    use (num_cu / 8 units each; a CU mask takes the first units, which fill
+//    whole XCDs in order), so each XCD must hold ceil(G / XCDs) of them, and
+//    the XCD with the fewest units bounds the grid.
+// Example: a 48-unit mask spans 32 + 16 units, so at one workgroup per unit
+// the grid is at most 2 x 16 = 32 (a grid of 48 puts 24 on the 16-unit XCD:
+// measured never to become resident, the bounded waits reported it).
+template <typename T, bool C>
+int panel_grid_resident(const Context* ctx, int units) {
+    static int per_unit = -1;  // one query per kernel instance and process
+    if (per_unit < 0) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(panel_persist_kernel<T, C>),
+                                                         PNT, 0) != hipSuccess)
+            occ = 0;
+        per_unit = occ;
+    }
+    if (per_unit <= 0 || units <= 0) return 0;
+    const int per_xcd = std::max(1, ctx->num_cu / 8);
+    const int full = units / per_xcd, rem = units % per_xcd;
+    const int xcds = full + (rem ? 1 : 0);
+    return xcds * (rem ? rem : per_xcd) * per_unit;
 }
 
 template <typename T, bool C>
@@ -2149,8 +2164,9 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
         if (rc) return rc;
         ctx->num_cu = prop.multiProcessorCount;
     }
-    // workgroups of the persistent panel: one per compute unit its stream may use
-    int pgrid = ctx->lu_panel ? ctx->num_cu : 0;
+    // workgroups of the persistent panel: as many as stay resident on the
+    // compute units its stream may use
+    int pgrid = ctx->lu_panel ? panel_grid_resident<T, C>(ctx, ctx->num_cu) : 0;
     if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2 && ctx->lu_pivoting == 2) {
         rc = ensure_lu_streams_free(ctx);
         if (rc) return rc;
@@ -2159,7 +2175,8 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     } else if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
         rc = ensure_lu_streams(ctx);
         if (rc) return rc;
-        if (pgrid > 0 && ctx->lu_lookahead_cus > 0) pgrid = panel_grid_masked(ctx);
+        if (pgrid > 0 && ctx->lu_lookahead_cus > 0)
+            pgrid = panel_grid_resident<T, C>(ctx, std::min(ctx->lu_lookahead_cus, ctx->num_cu - 1));
         if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
         lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s, ctx->lu_stream[0],
                                   ctx->lu_stream[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
@@ -2241,7 +2258,7 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
             if (rc) return rc;
             ctx->num_cu = prop.multiProcessorCount;
         }
-        int pgrid = ctx->num_cu;
+        int pgrid = panel_grid_resident<T, C>(ctx, ctx->num_cu);
         if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
         // block-local pivoting keeps its own rule for the tail
         const int tail_piv = ctx->lu_pivoting == 2 ? 2 : 0;

@@ -373,12 +373,14 @@ def test_lu_gemm_tiles_identical(bem, dtype, tile):
 
 @pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-11), (torch.float64, 1e-11),
                                        (torch.complex64, 2e-3), (torch.float32, 2e-3)])
-@pytest.mark.parametrize("n,lookahead", [(700, 0), (1500, 32), (1500, 0)])
+@pytest.mark.parametrize("n,lookahead", [(700, 0), (1500, 32), (1500, 48), (1500, 0)])
 def test_lu_persistent_panel_matches_per_column(bem, dtype, tol, n, lookahead):
     """The persistent panel (grid barriers, sub-panels in LDS) picks the same
     pivot rows as the one-launch-per-column panel, and its factors agree to
     rounding (the right part of each panel is updated by sub-panel instead
-    of by column)."""
+    of by column).  A 48-unit look-ahead mask spans a whole and a partial
+    XCD: the grid follows the residency rule (occupancy x the smallest XCD
+    share) and the factorisation completes."""
     rng = np.random.default_rng(n + lookahead)
     A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if dtype.is_complex else 0)
     At = torch.tensor(A, device=DEV).to(dtype).contiguous()
