@@ -20,6 +20,7 @@ import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -102,6 +103,9 @@ def main() -> int:
     ap.add_argument("--no-fd", action="store_true", help="transmission: skip the finite-difference check")
     ap.add_argument("--lu-lookahead", type=int, default=None,
                     help="compute units of the LU's panel stream (0: no look-ahead; default: the library's)")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="dirichlet: skip the one-step runs of the sweep, transmission and gradient workloads "
+                         "(child processes) whose summaries the line carries under other_configs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the reference kernel that scales "
                          "FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -317,10 +321,47 @@ def main() -> int:
         result["cpu_baseline"]["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
         log(f"cpu baseline: {rows2} rows in {dt2:.1f}s")
 
-    # release the library's streams and device memory before exit
+    # release the library's streams and device memory before the other
+    # configurations run in their own processes
     ctx.close()
+    if not args.no_other_configs:
+        result["other_configs"] = other_configs()
     print(json.dumps(result), flush=True)
     return 0
+
+
+# Keys of the other workloads' lines that the Dirichlet line repeats.
+OTHER_KEYS = ("metric", "value", "unit", "ms_per_step", "dtype", "config", "stages_ms_eager", "graph_replay_ms",
+              "roofline", "max_rel_residual", "rel_err_vs_fp64", "cond1_est_L3_L5", "fp64",
+              "mixed_c64lu_fp64asm", "mixed_fp32asm", "speedup_fp32asm", "gradient", "levels")
+
+
+def other_configs() -> dict:
+    """One warm step of each other workload (the batched sweep, the
+    transmission solve with its interior-k gradient, the adjoint shape
+    gradient), each in a child process of its own, so that one bench run
+    records every configuration; a child's failure is recorded, not raised."""
+    runs = {"sweep": ["--steps", "1", "--warmup", "1"],
+            "transmission": ["--steps", "1", "--warmup", "1", "--no-fd"],
+            "gradient": ["--steps", "1", "--warmup", "1"]}
+    out = {}
+    for w, extra in runs.items():
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", w, "--no-cpu-baseline"] + extra
+        log(f"other configuration: {w}")
+        t = time.perf_counter()
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        except subprocess.TimeoutExpired:
+            out[w] = {"error": "timed out after 300 s"}
+            continue
+        lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+        if p.returncode != 0 or not lines:
+            out[w] = {"error": f"exit {p.returncode}", "stderr_tail": p.stderr[-400:]}
+            continue
+        d = json.loads(lines[-1])
+        out[w] = {k: d[k] for k in OTHER_KEYS if k in d}
+        out[w]["wall_s"] = round(time.perf_counter() - t, 1)
+    return out
 
 
 # Algorithmic flops per point pair of the combined-field far kernel
