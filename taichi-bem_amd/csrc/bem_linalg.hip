@@ -71,6 +71,17 @@ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
     return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
 
+// 1 / b for b != 0: conj(b) / |b|^2 with one reciprocal (correctly rounded
+// division), or Smith's quotient where |b|^2 would leave the normal range.
+template <typename T>
+__device__ __forceinline__ cx<T> crecip(cx<T> b) {
+    const double m = fmax(fabs((double)b.re), fabs((double)b.im));
+    const double lo = sizeof(T) == 4 ? 1e-18 : 1e-150, hi = sizeof(T) == 4 ? 1e18 : 1e150;
+    if (m < lo || m > hi) return cdiv(cx<T>{T(1), T(0)}, b);
+    const T d = T(1) / (b.re * b.re + b.im * b.im);
+    return {b.re * d, -b.im * d};
+}
+
 // One factorisation of a (possibly batched) LU: the matrix and its
 // per-problem scratch.  Kernels take it by value (one problem) or from a
 // device table indexed by blockIdx.z (B = true: a batch of uneven sizes
@@ -182,6 +193,8 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
         diag.im = __shfl(u.im, c - 1 - k, 64);
     }
     const bool dzero = diag.re == T(0) && diag.im == T(0);
+    // LAPACK's scaling of the column: multiply by the reciprocal of the pivot
+    const cx<T> rdiag = dzero ? cx<T>{T(1), T(0)} : crecip(diag);
     double best = -1.0;
     int brow = 0x7fffffff;
     constexpr int RW = RPB / 4;  // rows per wave
@@ -200,7 +213,7 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
         if (r >= n) break;
         if (upd) {
             cx<T> l = {__shfl(x[q].re, c - 1 - k, 64), __shfl(x[q].im, c - 1 - k, 64)};
-            if (!dzero) l = cdiv(l, diag);
+            if (!dzero) l = cmul(l, rdiag);
             if (col == c - 1) x[q] = l;
             else if (col >= c && col < kend) {
                 cx<T> lu = cmul(l, u);
@@ -296,7 +309,7 @@ __global__ __launch_bounds__(256) void col_step_kernel(LuProb inl, const LuProb*
 // ---------------------------------------------------------------------------
 constexpr int SB = 8;        // sub-panel width
 constexpr int PGMAX = 64;    // largest workgroup count of a persistent panel (one wave reduces them)
-constexpr int PNT = 1024;    // threads per workgroup: sixteen waves keep many row loads in flight
+constexpr int PNT = 512;     // threads per workgroup: eight waves (measured: 1024 and 256 were slower)
 template <typename T>
 constexpr int panel_rmax() { return (128 * 1024) / (SB * (int)sizeof(cx<T>)); }
 
@@ -327,9 +340,11 @@ __device__ bool grid_barrier(unsigned* count, int* fail, unsigned target, long l
         int ok = 1;
         __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         const long long t0 = wall_clock64();
+        // one load round trip per poll; the fail word and the clock every 16th
+        unsigned it = 0;
         while ((int)(__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
-            __builtin_amdgcn_s_sleep(1);
-            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || wall_clock64() - t0 > ticks) {
+            if ((++it & 15u) == 0 &&
+                (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || wall_clock64() - t0 > ticks)) {
                 ok = 0;
                 break;
             }
@@ -340,6 +355,49 @@ __device__ bool grid_barrier(unsigned* count, int* fail, unsigned target, long l
     }
     __syncthreads();
     return s_ok != 0;
+}
+
+// (b, bi) <- (v, r) if v is the better pivot candidate: larger value, then
+// smaller row (an order-free rule, so any reduction order picks the same
+// candidate).  Bitwise logic and selects, no branches.
+__device__ __forceinline__ void take_better(double v, int r, double& b, int& bi) {
+    const bool take = (v > b) | ((v == b) & (r < bi));
+    b = take ? v : b;
+    bi = take ? r : bi;
+}
+
+// One step of a wave reduction through a DPP lane pattern (a VALU move, no
+// LDS round trip); the patterns used keep every source lane inside the row.
+template <int CTRL>
+__device__ __forceinline__ void dpp_best(double& b, int& bi) {
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(b), __double2hiint(b), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(b), __double2loint(b), CTRL, 0xF, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(bi, bi, CTRL, 0xF, 0xF, false);
+    take_better(__hiloint2double(hi, lo), oi, b, bi);
+}
+
+// Best candidate of the wave, returned in every lane (the whole wave must be
+// active): within each row of 16 lanes by xor-1 and xor-2 quad permutations,
+// the half-row mirror and the row mirror, after which every lane of a row
+// holds the row's best; then the four rows through lane reads.
+__device__ __forceinline__ void wave_best(double& b, int& bi) {
+    dpp_best<0xB1>(b, bi);   // quad_perm [1, 0, 3, 2]
+    dpp_best<0x4E>(b, bi);   // quad_perm [2, 3, 0, 1]
+    dpp_best<0x141>(b, bi);  // row_half_mirror
+    dpp_best<0x140>(b, bi);  // row_mirror
+    double rb[4];
+    int rr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        rb[q] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(b), 16 * q),
+                                 __builtin_amdgcn_readlane(__double2loint(b), 16 * q));
+        rr[q] = __builtin_amdgcn_readlane(bi, 16 * q);
+    }
+    take_better(rb[1], rr[1], rb[0], rr[0]);
+    take_better(rb[3], rr[3], rb[2], rr[2]);
+    take_better(rb[2], rr[2], rb[0], rr[0]);
+    b = rb[0];
+    bi = rr[0];
 }
 
 // Barriers one persistent panel launch of w columns takes (one per column,
@@ -356,9 +414,8 @@ __global__ __launch_bounds__(PNT) void panel_persist_kernel(LuProb P, int k, int
     __shared__ cx<T> allc[PGMAX][SB];   // every workgroup's candidate row values
     __shared__ double cv_s[PGMAX];
     __shared__ int cr_s[PGMAX];
-    __shared__ double wv[PNT / 64];
+    __shared__ double wv[PNT / 64];   // the waves' candidates
     __shared__ int wi[PNT / 64];
-    __shared__ int s_p, s_gw;
     T* A = (T*)P.A;
     const int64_t ld = P.ld;
     const int n = P.n, kend = min(n, k + NB);
@@ -389,21 +446,29 @@ __global__ __launch_bounds__(PNT) void panel_persist_kernel(LuProb P, int k, int
         for (int i = i0 + t; i < nr; i += PNT) {
             const cx<T> v = sub[i][0];
             const double m = fabs((double)v.re) + fabs((double)v.im);
-            if (m > b) { b = m; bi = rb + i; }
+            const bool tk = m > b;
+            b = tk ? m : b;
+            bi = tk ? rb + i : bi;
         }
         for (int c = cs; c < ce; ++c) {
             const int jc = c - cs;
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(b, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ob > b || (ob == b && oi < bi)) { b = ob; bi = oi; }
-            }
+            // the workgroup's candidate: a DPP reduction per wave, then every
+            // thread folds the PNT / 64 wave results as a tree
+            wave_best(b, bi);
             if (lane == 0) { wv[wave] = b; wi[wave] = bi; }
             __syncthreads();
-            b = wv[0];
-            bi = wi[0];
-            for (int q = 1; q < PNT / 64; ++q)
-                if (wv[q] > b || (wv[q] == b && wi[q] < bi)) { b = wv[q]; bi = wi[q]; }
+            {
+                double x[PNT / 64];
+                int y[PNT / 64];
+#pragma unroll
+                for (int q = 0; q < PNT / 64; ++q) { x[q] = wv[q]; y[q] = wi[q]; }
+#pragma unroll
+                for (int w = PNT / 128; w > 0; w >>= 1)
+#pragma unroll
+                    for (int q = 0; q < w; ++q) take_better(x[q + w], y[q + w], x[q], y[q]);
+                b = x[0];
+                bi = y[0];
+            }
             if (t == 0) { X->cval[par][g] = b; X->crow[par][g] = bi; }
             if (t < sw) {
                 if (bi != 0x7fffffff) X->cvals[par][g][t] = sub[bi - rb][t];
@@ -437,21 +502,23 @@ __global__ __launch_bounds__(PNT) void panel_persist_kernel(LuProb P, int k, int
                 }
             }
             __syncthreads();
-            if (wave == 0) {
-                b = lane < G ? cv_s[lane] : -1.0;
-                bi = lane < G ? cr_s[lane] : 0x7fffffff;
-                int bg = lane;
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double ob = __shfl_xor(b, off, 64);
-                    const int oi = __shfl_xor(bi, off, 64), og = __shfl_xor(bg, off, 64);
-                    if (ob > b || (ob == b && oi < bi)) { b = ob; bi = oi; bg = og; }
-                }
-                if (lane == 0) { s_p = bi; s_gw = bg; }
+            // the pivot: every wave reduces the G workgroup candidates (lane q
+            // holds workgroup q's) and finds the owner of the winner by ballot
+            int p, gw;
+            {
+                const double v0 = lane < G ? cv_s[lane] : -1.0;
+                const int r0 = lane < G ? cr_s[lane] : 0x7fffffff;
+                b = v0;
+                bi = r0;
+                wave_best(b, bi);
+                const unsigned long long win = __ballot(lane < G && v0 == b && r0 == bi);
+                p = bi;
+                gw = win ? __ffsll((unsigned long long)win) - 1 : 0;
             }
-            __syncthreads();
-            const int p = s_p, gw = s_gw;
             const cx<T> piv = allc[gw][jc];
             const bool dz = piv.re == T(0) && piv.im == T(0);
+            // LAPACK's scaling of the column: multiply by the reciprocal of the pivot
+            const cx<T> rpiv = dz ? cx<T>{T(1), T(0)} : crecip(piv);
             // workgroup 0 interchanges rows c and p in the panel columns outside
             // the sub-panel: loads now, stores after the elimination below
             const int scol = k + t;
@@ -475,16 +542,21 @@ __global__ __launch_bounds__(PNT) void panel_persist_kernel(LuProb P, int k, int
             bi = 0x7fffffff;
             for (int i = max(0, c + 1 - rb) + t; i < nr; i += PNT) {
                 cx<T> l = sub[i][jc];
-                if (!dz) l = cdiv(l, piv);
+                if (!dz) l = cmul(l, rpiv);
                 sub[i][jc] = l;
-                for (int j = jc + 1; j < sw; ++j) {
-                    const cx<T> lu = cmul(l, urow[j]);
-                    sub[i][j].re -= lu.re;
-                    sub[i][j].im -= lu.im;
+#pragma unroll
+                for (int j = 1; j < SB; ++j) {
+                    if (j > jc && j < sw) {
+                        const cx<T> lu = cmul(l, urow[j]);
+                        sub[i][j].re -= lu.re;
+                        sub[i][j].im -= lu.im;
+                    }
                 }
                 if (jc + 1 < sw) {
                     const double m = fabs((double)sub[i][jc + 1].re) + fabs((double)sub[i][jc + 1].im);
-                    if (m > b) { b = m; bi = rb + i; }
+                    const bool tk = m > b;
+                    b = tk ? m : b;
+                    bi = tk ? rb + i : bi;
                 }
             }
             if (swp) {

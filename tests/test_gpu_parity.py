@@ -343,6 +343,37 @@ def test_lu_partial_pivoting_matches_lapack(bem, n):
     assert rel(to_np(At), lu) <= 1e-12
 
 
+@pytest.mark.parametrize("panel", [1, 0])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.float64])
+def test_lu_pivot_ties_take_the_first_row(bem, dtype, panel):
+    """Exact ties of |re| + |im| in the first column (rows 41 and 37 of one
+    wave, 101 of another wave, 700 and 1400 of other workgroups): the pivot is
+    row 37, LAPACK's first row, for the persistent panel and the per-column
+    launches; the remaining pivots match scipy's."""
+    import scipy.linalg
+
+    n = 1500
+    rng = np.random.default_rng(17)
+    cplx = dtype.is_complex
+    A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    A[:, 0] = 0.25 * (rng.uniform(-1, 1, n) + (1j * rng.uniform(-1, 1, n) if cplx else 0))
+    ties = [41, 700, 101, 37, 1400]
+    vals = [3.0, -3.0, 3.0, -3.0, 3.0] if not cplx else [3.0, -3.0j, 1.0 + 2.0j, -2.0 - 1.0j, 3.0j]
+    for r, x in zip(ties, vals):
+        A[r, 0] = x
+    ctx = bem.Context(0)
+    ctx.set_option(bem._lib.OPT_LU_PIVOTING, 0)
+    ctx.set_option(bem._lib.OPT_LU_PANEL, panel)
+    At = torch.tensor(A, device=DEV).contiguous()
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    print(f"[lu ties] dtype={dtype} panel={panel}", flush=True)
+    ctx.lu_factor(At, ipiv)
+    piv = to_np(ipiv)
+    assert piv[0] == 37
+    _, ref = scipy.linalg.lu_factor(A)
+    assert np.array_equal(piv, ref)
+
+
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64, torch.float64, torch.float32])
 @pytest.mark.parametrize("tile", [1, 2])
 def test_lu_gemm_tiles_identical(bem, dtype, tile):
