@@ -46,6 +46,7 @@ struct OpParams {
     const float* __restrict__ phi32;
     const double* __restrict__ frame;  // [nf][PANEL_FRAME] local frames
     const double* __restrict__ pq;     // [nf][q2][4] (|x - C|^2, -2 r1, -2 r2, w)
+    const float* __restrict__ pq32;    // the same in FP32
     const NearPair* __restrict__ near;
     const int32_t* __restrict__ near_row;
     const NearPt* __restrict__ rule;
@@ -731,7 +732,7 @@ OpParams make_params(Context* ctx, const bem_op_desc* d) {
     p.nf = m.nf; p.nv = m.nv; p.q2 = m.q2; p.flags = d ? d->flags : 0;
     p.tris = m.tris; p.verts = m.verts; p.area = m.area; p.normal = m.normal; p.curl = m.curl;
     p.cen = m.cen; p.pu = m.pu; p.pl = m.pl; p.pu32 = m.pu32; p.pl32 = m.pl32;
-    p.phi = m.phi; p.phi32 = m.phi32; p.frame = m.frame; p.pq = m.pq; p.near = m.near; p.near_row = m.near_row; p.rule = m.rule;
+    p.phi = m.phi; p.phi32 = m.phi32; p.frame = m.frame; p.pq = m.pq; p.pq32 = m.pq32; p.near = m.near; p.near_row = m.near_row; p.rule = m.rule;
     for (int r = 0; r < 4; ++r) { p.rule_off[r] = m.rule_off[r]; p.rule_len[r] = m.rule_len[r]; }
     p.n_near = m.n_near;
     p.kap = 0.0; p.k2n = 0.0; p.dkap = 0.0; p.dk2n = 0.0;
@@ -1680,6 +1681,111 @@ __device__ __forceinline__ void far_pair_cf_sym(const OpParams& p, double eta, i
         }
 }
 
+// The same pair in FP32 with the test panel in its local frame (FP64 frame
+// table, converted once per pair): with Y = y_b - C_i the trial point
+// relative to the test centroid and x_a - C_i = U + r1 E1 + r2 E2,
+//   |x_a - y_b|^2      = S_a + Y.(Y + D) + p1 (E1.Y) + p2 (E2.Y),
+//   (x_a - y_b).n_j    = (U - Y).n_j - p1 (E1.n_j)/2 - p2 (E2.n_j)/2,
+//   (y_b - x_a).n_i    = Y.n_i                    (x_a - C_i lies in panel i),
+// with (S_a, p1, p2) = (|x_a - C_i|^2, -2 r1, -2 r2) from the per-point table
+// and D = -2U: three operations for r^2 and two for the projection per point
+// pair instead of six and three.  The test bases weight the sums directly
+// (phi from the uniform basis table), so no basis is formed as a difference
+// of moments.  NQ2 > 0 fixes the rule size for unrolling.
+template <int NQ2>
+__device__ __forceinline__ void far_pair_cf_sym_f32(const OpParams& p, double eta, int i, int j, double (&Cr)[3][3],
+                                                    double (&Ci)[3][3], double (&Dr)[3][3], double (&Di)[3][3]) {
+    const int q2 = NQ2 > 0 ? NQ2 : p.q2, nf = p.nf;
+    const ConstPtr<float> PQ = as_const_uniform(p.pq32 + (int64_t)i * q2 * 4);
+    const ConstPtr<float> PHI = as_const_uniform(p.phi32);
+    const ConstPtr<double> FR = as_const_uniform(p.frame + (int64_t)i * PANEL_FRAME);
+    const float* __restrict__ PL = p.pl32;
+    float U[3], E1[3], E2[3], D[3], ni[3], nj[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        U[d] = (float)FR[3 + d];
+        E1[d] = (float)FR[6 + d];
+        E2[d] = (float)FR[9 + d];
+        ni[d] = (float)FR[12 + d];
+        D[d] = (float)FR[15 + d];
+        nj[d] = (float)p.normal[3 * j + d];
+    }
+    // Y = (y_b - C_j) + (C_j - C_i)
+    const float dcx = (float)(p.cen[3 * j] - p.cen[3 * i]);
+    const float dcy = (float)(p.cen[3 * j + 1] - p.cen[3 * i + 1]);
+    const float dcz = (float)(p.cen[3 * j + 2] - p.cen[3 * i + 2]);
+    const float e1n = -0.5f * (E1[0] * nj[0] + E1[1] * nj[1] + E1[2] * nj[2]);
+    const float e2n = -0.5f * (E2[0] * nj[0] + E2[1] * nj[1] + E2[2] * nj[2]);
+    const float un = U[0] * nj[0] + U[1] * nj[1] + U[2] * nj[2];
+    const float kap = (float)p.kap, et = (float)eta;
+    float acr[3][3], aci[3][3], bcr[3][3], bci[3][3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int t = 0; t < 3; ++t) { acr[s][t] = aci[s][t] = bcr[s][t] = bci[s][t] = 0.f; }
+    for (int b = 0; b < q2; ++b) {
+        const float yx = PL[((int64_t)b * 4 + 0) * nf + j] + dcx;
+        const float yy = PL[((int64_t)b * 4 + 1) * nf + j] + dcy;
+        const float yz = PL[((int64_t)b * 4 + 2) * nf + j] + dcz;
+        const float wb = PL[((int64_t)b * 4 + 3) * nf + j];
+        const float yd = yx * (yx + D[0]) + yy * (yy + D[1]) + yz * (yz + D[2]);
+        const float y1 = E1[0] * yx + E1[1] * yy + E1[2] * yz;
+        const float y2 = E2[0] * yx + E2[1] * yy + E2[2] * yz;
+        const float c0 = un - (yx * nj[0] + yy * nj[1] + yz * nj[2]);
+        const float dyi = yx * ni[0] + yy * ni[1] + yz * ni[2];  // (y - x).n_i
+        float pcr[3], pci[3], qcr[3], qci[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) { pcr[s] = pci[s] = qcr[s] = qci[s] = 0.f; }
+#pragma unroll 5
+        for (int a = 0; a < q2; ++a) {
+            const float sa = PQ[4 * a + 0], p1 = PQ[4 * a + 1], p2 = PQ[4 * a + 2], wa = PQ[4 * a + 3];
+            const float r2 = __builtin_fmaf(p2, y2, __builtin_fmaf(p1, y1, sa + yd));
+            const float rinv = rsqrt_f32(r2);
+            const float g = wa * rinv;
+            const float g3 = g * (rinv * rinv);
+            const float kr = kap * (r2 * rinv);
+            float sn, cs;
+            sincos_f32(kr, &sn, &cs);
+            // K: base (cs + kr sn, sn - kr cs); -i eta V: eta g (sn, -cs)
+            const float c1 = __builtin_fmaf(kr, sn, cs), c2 = __builtin_fmaf(-kr, cs, sn);
+            const float etg = et * g;
+            const float egs = etg * sn, egc = etg * cs;
+            const float dnj = __builtin_fmaf(p2, e2n, __builtin_fmaf(p1, e1n, c0));
+            const float bj = g3 * dnj, bi_ = g3 * dyi;
+            const float fr = __builtin_fmaf(bj, c1, egs), fi = __builtin_fmaf(bj, c2, -egc);
+            const float hr = __builtin_fmaf(bi_, c1, egs), hi = __builtin_fmaf(bi_, c2, -egc);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const float ph = PHI[3 * a + s];
+                pcr[s] = __builtin_fmaf(fr, ph, pcr[s]);
+                pci[s] = __builtin_fmaf(fi, ph, pci[s]);
+                qcr[s] = __builtin_fmaf(hr, ph, qcr[s]);
+                qci[s] = __builtin_fmaf(hi, ph, qci[s]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const float wt = wb * PHI[3 * b + t];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                acr[s][t] = __builtin_fmaf(pcr[s], wt, acr[s][t]);
+                aci[s][t] = __builtin_fmaf(pci[s], wt, aci[s][t]);
+                bcr[s][t] = __builtin_fmaf(qcr[s], wt, bcr[s][t]);
+                bci[s][t] = __builtin_fmaf(qci[s], wt, bci[s][t]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            Cr[s][t] = kInv4Pi * (double)acr[s][t];
+            Ci[s][t] = kInv4Pi * (double)aci[s][t];
+            Dr[t][s] = kInv4Pi * (double)bcr[s][t];
+            Di[t][s] = kInv4Pi * (double)bci[s][t];
+        }
+}
+
 // Far item -> (test group ib, trial block jb) over the upper triangle: the
 // test groups of 256-row group gr (256 / TI of them) pair with trial blocks
 // gr .. njb-1, and a lane only evaluates pairs with j > i.
@@ -1706,7 +1812,7 @@ __device__ __forceinline__ void cf_sym_item(int64_t local, int njb, int nib, int
 // the same for every lane (the test panels are wave-uniform).
 constexpr int CF_SLOTS = 6;
 
-template <typename R, int TI>
+template <typename R, int TI, int NQ2 = 0>
 __global__ __launch_bounds__(256) void cf_far_kernel(const CfProb* tbl, int count, int64_t total) {
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
         const int q = cf_find(tbl, count, item, 0);
@@ -1791,7 +1897,7 @@ __global__ __launch_bounds__(256) void cf_far_kernel(const CfProb* tbl, int coun
                 }
                 if (!jv || j <= i || shares_vertex(p.tris, i, j)) continue;
                 double Cr[3][3], Ci[3][3], Dr[3][3], Di[3][3];
-                far_pair_cf_sym<R>(p, P.eta, i, j, Cr, Ci, Dr, Di);
+                far_pair_cf_sym_f32<NQ2>(p, P.eta, i, j, Cr, Ci, Dr, Di);
 #pragma unroll
                 for (int s = 0; s < 3; ++s)
 #pragma unroll
@@ -2390,12 +2496,17 @@ int assemble_combined_batched(Context* ctx, int count, const CfInput* probs, int
     using CfFn = void (*)(const CfProb*, int, int64_t);
     CfFn far = nullptr;
     const bool f32 = precision == BEM_PREC_FP32;
+    // FP32 with GaussQR 5 on every mesh of the batch: the unrolled 25-point loop
+    bool q25 = true;
+    for (int q = 0; q < count; ++q) q25 = q25 && probs[q].ctx->mesh.q2 == 25;
     switch (ti) {
-        case 1: far = f32 ? cf_far_kernel<float, 1> : cf_far_kernel<double, 1>; break;
-        case 2: far = f32 ? cf_far_kernel<float, 2> : cf_far_kernel<double, 2>; break;
-        case 4: far = f32 ? cf_far_kernel<float, 4> : cf_far_kernel<double, 4>; break;
-        case 8: far = f32 ? cf_far_kernel<float, 8> : cf_far_kernel<double, 8>; break;
-        default: far = f32 ? cf_far_kernel<float, 16> : cf_far_kernel<double, 16>; break;
+        case 1: far = !f32 ? cf_far_kernel<double, 1> : q25 ? cf_far_kernel<float, 1, 25> : cf_far_kernel<float, 1>; break;
+        case 2: far = !f32 ? cf_far_kernel<double, 2> : q25 ? cf_far_kernel<float, 2, 25> : cf_far_kernel<float, 2>; break;
+        case 4: far = !f32 ? cf_far_kernel<double, 4> : q25 ? cf_far_kernel<float, 4, 25> : cf_far_kernel<float, 4>; break;
+        case 8: far = !f32 ? cf_far_kernel<double, 8> : q25 ? cf_far_kernel<float, 8, 25> : cf_far_kernel<float, 8>; break;
+        default:
+            far = !f32 ? cf_far_kernel<double, 16> : q25 ? cf_far_kernel<float, 16, 25> : cf_far_kernel<float, 16>;
+            break;
     }
     hipLaunchKernelGGL(far, dim3(grid_of(nfar)), dim3(256), 0, s, d_tbl, count, nfar);
     if (nnear > 0)

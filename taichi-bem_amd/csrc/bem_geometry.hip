@@ -105,7 +105,7 @@ __global__ void panel_points_kernel(int nf, int q2, const double* __restrict__ v
                                     const int32_t* __restrict__ t, const double* __restrict__ area,
                                     const double* __restrict__ cen, const double* __restrict__ r12w,
                                     const double* __restrict__ frame, double* pu, double* pl, float* pu32,
-                                    float* pl32, double* pq) {
+                                    float* pl32, double* pq, float* pq32) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nf * q2) return;
     int i = idx / q2, p = idx % q2;
@@ -123,6 +123,10 @@ __global__ void panel_points_kernel(int nf, int q2, const double* __restrict__ v
         pq[b + 1] = -2.0 * r1;
         pq[b + 2] = -2.0 * r2;
         pq[b + 3] = w;
+        pq32[b + 0] = (float)s2;
+        pq32[b + 1] = (float)(-2.0 * r1);
+        pq32[b + 2] = (float)(-2.0 * r2);
+        pq32[b + 3] = (float)w;
     }
     const double* p1 = v + 3 * t[3 * i];
     const double* p2 = v + 3 * t[3 * i + 1];
@@ -280,7 +284,7 @@ void mesh_free(Mesh* m) {
     void* ptrs[] = {m->verts, m->tris, m->area, m->normal, m->vert_area, m->vert_normal, m->curl,
                     m->pu, m->pl, m->cen, m->pu32, m->pl32, m->phi, m->phi32, m->mass_w, m->frame, m->pq,
                     m->near, m->near_row, m->rule, m->near_buf, m->kpart, m->vp_row, m->vp_el,
-                    m->near_tr, m->adj, m->panel_buf};
+                    m->near_tr, m->adj, m->panel_buf, m->pq32};
     for (void* p : ptrs)
         if (p) hipFree(p);
     *m = Mesh();
@@ -324,6 +328,7 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
     rc |= dev_alloc(&m.mass_w, (size_t)q2);
     rc |= dev_alloc(&m.frame, (size_t)nf * PANEL_FRAME);
     rc |= dev_alloc(&m.pq, (size_t)nf * q2 * 4);
+    rc |= dev_alloc(&m.pq32, (size_t)nf * q2 * 4);
     rc |= dev_alloc(&m.panel_buf, (size_t)nf * 6);
     if (rc) return BEM_ERR_HIP;
     rc = check_hip(hipMemcpyAsync(m.verts, d_verts, sizeof(double) * nv * 3, hipMemcpyDeviceToDevice, s), "copy vertices");
@@ -378,7 +383,8 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
     hipLaunchKernelGGL(panel_frame_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts, m.tris,
                        m.normal, m.cen, m.frame);
     hipLaunchKernelGGL(panel_points_kernel, dim3((nf * q2 + 255) / 256), dim3(256), 0, s, nf, q2,
-                       m.verts, m.tris, m.area, m.cen, d_r12w, m.frame, m.pu, m.pl, m.pu32, m.pl32, m.pq);
+                       m.verts, m.tris, m.area, m.cen, d_r12w, m.frame, m.pu, m.pl, m.pu32, m.pl32, m.pq,
+                       m.pq32);
     rc = check_hip(hipGetLastError(), "geometry kernels");
     if (rc) return rc;
 

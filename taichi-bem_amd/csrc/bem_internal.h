@@ -49,6 +49,7 @@ struct Mesh {
     double* frame = nullptr;      // [nf][PANEL_FRAME]
     // per regular point: S = |x - C|^2, -2 r1, -2 r2, w  (uniform loads)
     double* pq = nullptr;         // [nf][q2][4]
+    float* pq32 = nullptr;        // [nf][q2][4] the same in FP32 (the combined-field sweep's far field)
     double* phi = nullptr;        // [q2][3] P1 shape values at the regular points
     float* phi32 = nullptr;       // [q2][3]
     double* mass_w = nullptr;     // [q2] w_xi w_eta xi (single-panel rule, no area)
