@@ -140,7 +140,8 @@ class WavenumberSweep:
 # ---------------------------------------------------------------------------
 # Autotuner of the persistent far-field launch shape
 # ---------------------------------------------------------------------------
-DEFAULT_CANDIDATES = ((1, 2), (1, 4), (2, 4), (2, 8), (4, 4), (4, 8), (8, 2), (8, 4), (16, 2), (16, 4))
+DEFAULT_CANDIDATES = ((1, 2), (1, 4), (2, 4), (2, 8), (4, 4), (4, 8), (8, 2), (8, 4), (8, 8), (16, 2), (16, 4),
+                      (16, 8))
 
 
 def default_cache_path() -> str:
