@@ -284,7 +284,7 @@ void mesh_free(Mesh* m) {
     void* ptrs[] = {m->verts, m->tris, m->area, m->normal, m->vert_area, m->vert_normal, m->curl,
                     m->pu, m->pl, m->cen, m->pu32, m->pl32, m->phi, m->phi32, m->mass_w, m->frame, m->pq,
                     m->near, m->near_row, m->rule, m->near_buf, m->kpart, m->vp_row, m->vp_el,
-                    m->near_tr, m->adj, m->panel_buf, m->pq32};
+                    m->near_tr, m->adj, m->panel_buf, m->pq32, m->r12w};
     for (void* p : ptrs)
         if (p) hipFree(p);
     *m = Mesh();
@@ -292,13 +292,21 @@ void mesh_free(Mesh* m) {
 
 namespace {
 
-// Device buffers of one mesh build that do not outlive it.
-struct BuildTemps {
-    double* r12w = nullptr;
-    ~BuildTemps() {
-        if (r12w) hipFree(r12w);
-    }
-};
+// The tables that depend on the vertex positions: panel and vertex
+// geometry, local frames, regular-rule points.
+int mesh_geometry(Mesh& m, hipStream_t s) {
+    const int nf = m.nf, nv = m.nv, q2 = m.q2;
+    hipLaunchKernelGGL(panel_geometry_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts,
+                       m.tris, m.area, m.normal, m.cen, m.curl);
+    hipLaunchKernelGGL(vertex_geometry_kernel, dim3((nv + 255) / 256), dim3(256), 0, s, nv, m.vp_row,
+                       m.vp_el, m.area, m.normal, m.vert_area, m.vert_normal);
+    hipLaunchKernelGGL(panel_frame_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts, m.tris,
+                       m.normal, m.cen, m.frame);
+    hipLaunchKernelGGL(panel_points_kernel, dim3((nf * q2 + 255) / 256), dim3(256), 0, s, nf, q2,
+                       m.verts, m.tris, m.area, m.cen, m.r12w, m.frame, m.pu, m.pl, m.pu32, m.pl32, m.pq,
+                       m.pq32);
+    return check_hip(hipGetLastError(), "geometry kernels");
+}
 
 int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, int nv, int nf, int q,
                     hipStream_t s) {
@@ -307,10 +315,30 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
     if (!gauss01(q, gp, gw))
         return set_error(BEM_ERR_UNSUPPORTED, "Please refer N as 2~8 for gaussion interation (GaussQR must be 1..8 or 12)");
     Mesh& m = ctx->mesh;
+    int rc = 0;
+    // The same panels and rule at new vertex positions (shape derivatives,
+    // finite differences, optimisation loops): the buffers, the near-field
+    // list and the rule tables stay; the positions and what depends on them
+    // are rebuilt.
+    if (m.nf == nf && m.nv == nv && m.q == q && m.verts && m.r12w) {
+        std::vector<int32_t> t((size_t)nf * 3);
+        rc = check_hip(hipMemcpyAsync(t.data(), d_tris, sizeof(int32_t) * nf * 3, hipMemcpyDeviceToHost, s),
+                       "panels to host");
+        if (rc) return rc;
+        rc = check_hip(hipStreamSynchronize(s), "sync");
+        if (rc) return rc;
+        if (t == m.h_tris) {
+            rc = check_hip(hipMemcpyAsync(m.verts, d_verts, sizeof(double) * nv * 3, hipMemcpyDeviceToDevice, s),
+                           "copy vertices");
+            if (rc) return rc;
+            rc = mesh_geometry(m, s);
+            if (rc) return rc;
+            return check_hip(hipStreamSynchronize(s), "mesh update");
+        }
+    }
     mesh_free(&m);
     m.nv = nv; m.nf = nf; m.q = q; m.q2 = q * q;
     const int q2 = m.q2;
-    int rc = 0;
     rc |= dev_alloc(&m.verts, (size_t)nv * 3);
     rc |= dev_alloc(&m.tris, (size_t)nf * 3);
     rc |= dev_alloc(&m.area, nf);
@@ -356,10 +384,8 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
             mw[p] = gw[a] * gw[b] * r1;
         }
     for (int e = 0; e < q2 * 3; ++e) phi32[e] = (float)phi[e];
-    BuildTemps tmp;
-    if (dev_alloc(&tmp.r12w, r12w.size())) return BEM_ERR_HIP;
-    double* d_r12w = tmp.r12w;
-    hipMemcpyAsync(d_r12w, r12w.data(), r12w.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (dev_alloc(&m.r12w, r12w.size())) return BEM_ERR_HIP;
+    hipMemcpyAsync(m.r12w, r12w.data(), r12w.size() * sizeof(double), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.phi, phi.data(), phi.size() * sizeof(double), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.phi32, phi32.data(), phi32.size() * sizeof(float), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.mass_w, mw.data(), mw.size() * sizeof(double), hipMemcpyHostToDevice, s);
@@ -376,16 +402,7 @@ int mesh_build_impl(Context* ctx, const double* d_verts, const int32_t* d_tris, 
     hipMemcpyAsync(m.vp_row, vp_row.data(), vp_row.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(m.vp_el, vp_el.data(), vp_el.size() * sizeof(int32_t), hipMemcpyHostToDevice, s);
 
-    hipLaunchKernelGGL(panel_geometry_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts,
-                       m.tris, m.area, m.normal, m.cen, m.curl);
-    hipLaunchKernelGGL(vertex_geometry_kernel, dim3((nv + 255) / 256), dim3(256), 0, s, nv, m.vp_row,
-                       m.vp_el, m.area, m.normal, m.vert_area, m.vert_normal);
-    hipLaunchKernelGGL(panel_frame_kernel, dim3((nf + 255) / 256), dim3(256), 0, s, nf, m.verts, m.tris,
-                       m.normal, m.cen, m.frame);
-    hipLaunchKernelGGL(panel_points_kernel, dim3((nf * q2 + 255) / 256), dim3(256), 0, s, nf, q2,
-                       m.verts, m.tris, m.area, m.cen, d_r12w, m.frame, m.pu, m.pl, m.pu32, m.pl32, m.pq,
-                       m.pq32);
-    rc = check_hip(hipGetLastError(), "geometry kernels");
+    rc = mesh_geometry(m, s);
     if (rc) return rc;
 
     // near-field rules

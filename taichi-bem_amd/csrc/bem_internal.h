@@ -53,6 +53,7 @@ struct Mesh {
     double* phi = nullptr;        // [q2][3] P1 shape values at the regular points
     float* phi32 = nullptr;       // [q2][3]
     double* mass_w = nullptr;     // [q2] w_xi w_eta xi (single-panel rule, no area)
+    double* r12w = nullptr;       // [q2][3] (r1, r2, w_xi w_eta xi) of the regular rule
     // Near field
     NearPair* near = nullptr;     // sorted by i then j
     int32_t* near_row = nullptr;  // CSR row pointer over test panels [nf+1]
