@@ -150,7 +150,9 @@ __device__ void adjoint_zero_slots(const AdjParams& p, int ib, int jb) {
     if (threadIdx.x == 0) p.gkb[blockIdx.x] = 0.0;
 }
 
-template <int OP, bool HELM>
+// NQ2 > 0 fixes the rule size so the point loops unroll (their scalar loads
+// of the test points are then issued in batches).
+template <int OP, bool HELM, int NQ2 = 0>
 __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int njb) {
     const int ib = blockIdx.x / njb, jb = blockIdx.x % njb;
     const int TI = p.ti;
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const bool jvalid = j < p.nf;
     const int jj = jvalid ? j : 0;
-    const int q2 = p.q2, nf = p.nf;
+    const int q2 = NQ2 > 0 ? NQ2 : p.q2, nf = p.nf;
     const double* __restrict__ PL = p.pl;
     const ConstPtr<double> PHI = as_const(p.phi);  // wave-uniform: scalar loads
     const double kap = p.kap;
@@ -244,6 +246,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
                     // the phi-weighted sums follow from the moments as for K below
                     double Gr = 0.0, Gi = 0.0, G1r = 0.0, G1i = 0.0, G2r = 0.0, G2i = 0.0;
                     double G11r = 0.0, G11i = 0.0, G12r = 0.0, G12i = 0.0, G22r = 0.0, G22i = 0.0;
+#pragma unroll 5
                     for (int a = 0; a < q2; ++a) {
                         const double dv[3] = {PU[4 * a + 0] - yx, PU[4 * a + 1] - yy, PU[4 * a + 2] - yz};
                         const double wa = PU[4 * a + 3];
@@ -288,6 +291,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_far_kernel(AdjParams p, int nj
                     double Hr = 0.0, Hi = 0.0, H1r = 0.0, H1i = 0.0, H2r = 0.0, H2i = 0.0;
                     double Gr = 0.0, Gi = 0.0, G1r = 0.0, G1i = 0.0, G2r = 0.0, G2i = 0.0;
                     double G11r = 0.0, G11i = 0.0, G12r = 0.0, G12i = 0.0, G22r = 0.0, G22i = 0.0;
+#pragma unroll 5
                     for (int a = 0; a < q2; ++a) {
                         const double dv[3] = {PU[4 * a + 0] - yx, PU[4 * a + 1] - yy, PU[4 * a + 2] - yz};
                         const double wa = PU[4 * a + 3];
@@ -741,8 +745,13 @@ __global__ void area_grad_kernel(int nv, const int32_t* __restrict__ vp_row, con
 template <int OP>
 void launch_adjoint(const AdjParams& p, bool helm, int njb, int nib, hipStream_t s) {
     const dim3 gf((unsigned)((int64_t)nib * njb));
-    if (helm) hipLaunchKernelGGL((adjoint_far_kernel<OP, true>), gf, dim3(256), 0, s, p, njb);
-    else hipLaunchKernelGGL((adjoint_far_kernel<OP, false>), gf, dim3(256), 0, s, p, njb);
+    if (p.q2 == 25) {
+        if (helm) hipLaunchKernelGGL((adjoint_far_kernel<OP, true, 25>), gf, dim3(256), 0, s, p, njb);
+        else hipLaunchKernelGGL((adjoint_far_kernel<OP, false, 25>), gf, dim3(256), 0, s, p, njb);
+    } else {
+        if (helm) hipLaunchKernelGGL((adjoint_far_kernel<OP, true>), gf, dim3(256), 0, s, p, njb);
+        else hipLaunchKernelGGL((adjoint_far_kernel<OP, false>), gf, dim3(256), 0, s, p, njb);
+    }
     if (p.n_near > 0) {
         const dim3 gn((unsigned)((p.n_near + 3) / 4));
         if (helm) hipLaunchKernelGGL((adjoint_near_kernel<OP, true>), gn, dim3(256), 0, s, p);
