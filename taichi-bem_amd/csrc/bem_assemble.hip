@@ -1190,11 +1190,13 @@ struct VkwOut {
     void* W; int64_t ldw;
 };
 
-template <typename R, bool HELM>
+// NQ2 > 0 fixes the rule size so the test-point loop unrolls (batched
+// scalar loads of the test points).
+template <typename R, bool HELM, int NQ2 = 0>
 __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int njb, int ngroups) {
     int tile, jb;
     sym_block(blockIdx.x, njb, ngroups, tile, jb);
-    const int nf = p.nf, q2 = p.q2;
+    const int nf = p.nf, q2 = NQ2 > 0 ? NQ2 : p.q2;
     const int i0 = tile * DTI;
     const int j = jb * 256 + threadIdx.x;
     __shared__ double2 sct[256];  // sin/cos(j pi/128) for the FP64 phase
@@ -1239,6 +1241,7 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
             R pwr[3], pwi[3], qkr[3], qki[3], pkr = 0, pki = 0;
 #pragma unroll
             for (int s = 0; s < 3; ++s) pwr[s] = pwi[s] = qkr[s] = qki[s] = 0;
+#pragma unroll 5
             for (int a = 0; a < q2; ++a) {
                 const R dx = PU[4 * a + 0] + ex;
                 const R dy = PU[4 * a + 1] + ey;
@@ -1371,10 +1374,11 @@ __device__ __forceinline__ void cmac(double& zr, double& zi, double xr, double x
     zi += xr * yi + xi * yr;
 }
 
+template <int NQ2 = 0>
 __global__ __launch_bounds__(256) void far_vkw_dk_kernel(OpParams p, DkVecs v, int njb, int ngroups, double* part) {
     int tile, jb;
     sym_block(blockIdx.x, njb, ngroups, tile, jb);
-    const int nf = p.nf, q2 = p.q2;
+    const int nf = p.nf, q2 = NQ2 > 0 ? NQ2 : p.q2;
     const int i0 = tile * DTI;
     const int j = jb * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -1424,6 +1428,7 @@ __global__ __launch_bounds__(256) void far_vkw_dk_kernel(OpParams p, DkVecs v, i
                 double pvr = 0, pvi = 0, pkr = 0, pki = 0, pwr[3], pwi[3], qkr[3], qki[3];
 #pragma unroll
                 for (int s = 0; s < 3; ++s) pwr[s] = pwi[s] = qkr[s] = qki[s] = 0;
+#pragma unroll 5
                 for (int a = 0; a < q2; ++a) {
                     const double dx = PU[4 * a + 0] + ex;
                     const double dy = PU[4 * a + 1] + ey;
@@ -2315,8 +2320,12 @@ int assemble_vkw(Context* ctx, const bem_op_desc* d, void* V, int64_t ldv, void*
     const bool helm = d->kernel != BEM_LAPLACE;
     const int nf = m.nf, njb = (nf + 255) / 256, ngroups = (nf + 255) / 256;
     if (parts & BEM_PART_FAR) {
-        auto fk = d->precision == BEM_PREC_FP32 ? (helm ? far_vkw_kernel<float, true> : far_vkw_kernel<float, false>)
-                                                : (helm ? far_vkw_kernel<double, true> : far_vkw_kernel<double, false>);
+        const bool q25 = p.q2 == 25;
+        auto fk = d->precision == BEM_PREC_FP32
+                      ? (helm ? (q25 ? far_vkw_kernel<float, true, 25> : far_vkw_kernel<float, true>)
+                              : (q25 ? far_vkw_kernel<float, false, 25> : far_vkw_kernel<float, false>))
+                      : (helm ? (q25 ? far_vkw_kernel<double, true, 25> : far_vkw_kernel<double, true>)
+                              : (q25 ? far_vkw_kernel<double, false, 25> : far_vkw_kernel<double, false>));
         const int64_t nblocks = (int64_t)DTG * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
         VkwOut o{V, ldv, K, ldk, W, ldw};
         hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, o, njb, ngroups);
@@ -2375,7 +2384,10 @@ int vkw_dk_forms(Context* ctx, const bem_op_desc* d, const void* aN, const void*
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ylen + 255) / 256)), dim3(256), 0, s, y, (int64_t)ylen);
     OpParams p = make_params(ctx, d);
     DkVecs vv{(const double*)aN, (const double*)bN, (const double*)aD, (const double*)bD};
-    hipLaunchKernelGGL(far_vkw_dk_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, p, vv, njb, ngroups, part);
+    if (p.q2 == 25)
+        hipLaunchKernelGGL(far_vkw_dk_kernel<25>, dim3((unsigned)nblocks), dim3(256), 0, s, p, vv, njb, ngroups, part);
+    else
+        hipLaunchKernelGGL(far_vkw_dk_kernel<0>, dim3((unsigned)nblocks), dim3(256), 0, s, p, vv, njb, ngroups, part);
     rc = check_hip(hipGetLastError(), "far-field k-derivative forms launch");
     if (rc) return rc;
     // near field: the matrix-free k-derivative products, then plain dots
