@@ -74,11 +74,13 @@ struct OpParams {
 
 template <typename T> struct Ptr;
 template <> struct Ptr<double> {
+    static __device__ __forceinline__ const double* pq(const OpParams& p) { return p.pq; }
     static __device__ __forceinline__ const double* pu(const OpParams& p) { return p.pu; }
     static __device__ __forceinline__ const double* pl(const OpParams& p) { return p.pl; }
     static __device__ __forceinline__ const double* phi(const OpParams& p) { return p.phi; }
 };
 template <> struct Ptr<float> {
+    static __device__ __forceinline__ const float* pq(const OpParams& p) { return p.pq32; }
     static __device__ __forceinline__ const float* pu(const OpParams& p) { return p.pu32; }
     static __device__ __forceinline__ const float* pl(const OpParams& p) { return p.pl32; }
     static __device__ __forceinline__ const float* phi(const OpParams& p) { return p.phi32; }
@@ -1214,17 +1216,29 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
     const int i_end = min(nf, i0 + DTI);
     for (int i = i0; i < i_end; ++i) {
         if (j <= i || shares_vertex(p.tris, i, j)) continue;
-        // wave-uniform test points, read with scalar loads
-        const ConstPtr<R> PU = as_const(Ptr<R>::pu(p) + (int64_t)i * q2 * 4);
-        R dcx = 0, dcy = 0, dcz = 0;
-        if constexpr (sizeof(R) == 4) {
-            dcx = (R)(p.cen[3 * i] - p.cen[3 * j]);
-            dcy = (R)(p.cen[3 * i + 1] - p.cen[3 * j + 1]);
-            dcz = (R)(p.cen[3 * i + 2] - p.cen[3 * j + 2]);
+        // Moment form (the test panel in its local frame, as in the Dirichlet
+        // pass): with Y = y_b - C_i and x_a - C_i = U + r1 E1 + r2 E2,
+        //   |x_a - y_b|^2   = S_a + Y.(Y + D) + p1 (E1.Y) + p2 (E2.Y),
+        //   (x_a - y_b).n_j = (U - Y).n_j - p1 (E1.n_j)/2 - p2 (E2.n_j)/2,
+        //   (y_b - x_a).n_i = Y.n_i,
+        // (S_a, p1, p2, w_a) from the per-point table (scalar loads).
+        const ConstPtr<R> PQ = as_const(Ptr<R>::pq(p) + (int64_t)i * q2 * 4);
+        const ConstPtr<double> FR = as_const(p.frame + (int64_t)i * PANEL_FRAME);
+        R U[3], E1[3], E2[3], D[3], ni[3], oY[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            U[d] = (R)FR[3 + d];
+            E1[d] = (R)FR[6 + d];
+            E2[d] = (R)FR[9 + d];
+            ni[d] = (R)FR[12 + d];
+            D[d] = (R)FR[15 + d];
+            // FP64 trial points are absolute (Y = y - C_i); FP32 ones are
+            // offsets from their centroid (Y = (y - C_j) + (C_j - C_i))
+            oY[d] = sizeof(R) == 8 ? (R)(-FR[d]) : (R)(p.cen[3 * j + d] - p.cen[3 * i + d]);
         }
-        const R mix = (R)p.normal[3 * i], miy = (R)p.normal[3 * i + 1], miz = (R)p.normal[3 * i + 2];
-        // (x - y).n_i = x_a.n_i + e_b.n_i with x_a.n_i the same for every test point of the flat panel i
-        const R pun = PU[0] * mix + PU[1] * miy + PU[2] * miz;
+        const R e1n = R(-0.5) * (E1[0] * nnx + E1[1] * nny + E1[2] * nnz);
+        const R e2n = R(-0.5) * (E2[0] * nnx + E2[1] * nny + E2[2] * nnz);
+        const R un = U[0] * nnx + U[1] * nny + U[2] * nnz;
         R kar[3], kai[3], kbr[3], kbi[3], wr[3][3], wi[3][3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
@@ -1233,26 +1247,27 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
             for (int t = 0; t < 3; ++t) wr[s][t] = wi[s][t] = 0;
         }
         for (int b = 0; b < q2; ++b) {
-            const R ex = dcx - PL[((int64_t)b * 4 + 0) * nf + j];
-            const R ey = dcy - PL[((int64_t)b * 4 + 1) * nf + j];
-            const R ez = dcz - PL[((int64_t)b * 4 + 2) * nf + j];
+            const R yx = PL[((int64_t)b * 4 + 0) * nf + j] + oY[0];
+            const R yy = PL[((int64_t)b * 4 + 1) * nf + j] + oY[1];
+            const R yz = PL[((int64_t)b * 4 + 2) * nf + j] + oY[2];
             const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
-            const R dni = pun + ex * mix + ey * miy + ez * miz;
+            const R yd = yx * (yx + D[0]) + yy * (yy + D[1]) + yz * (yz + D[2]);
+            const R y1 = E1[0] * yx + E1[1] * yy + E1[2] * yz;
+            const R y2 = E2[0] * yx + E2[1] * yy + E2[2] * yz;
+            const R c0 = un - (yx * nnx + yy * nny + yz * nnz);
+            const R dyi = yx * ni[0] + yy * ni[1] + yz * ni[2];  // (y - x).n_i
             R pwr[3], pwi[3], qkr[3], qki[3], pkr = 0, pki = 0;
 #pragma unroll
             for (int s = 0; s < 3; ++s) pwr[s] = pwi[s] = qkr[s] = qki[s] = 0;
 #pragma unroll 5
             for (int a = 0; a < q2; ++a) {
-                const R dx = PU[4 * a + 0] + ex;
-                const R dy = PU[4 * a + 1] + ey;
-                const R dz = PU[4 * a + 2] + ez;
-                const R wa = PU[4 * a + 3];
-                const R r2 = dx * dx + dy * dy + dz * dz;
+                const R sa = PQ[4 * a + 0], p1 = PQ[4 * a + 1], p2 = PQ[4 * a + 2], wa = PQ[4 * a + 3];
+                const R r2 = p2 * y2 + (p1 * y1 + (sa + yd));
                 const R rinv = Math<R>::rsqrt(r2);
                 const R g = wa * rinv;
                 const R g3 = g * (rinv * rinv);
-                const R bj = g3 * (dx * nnx + dy * nny + dz * nnz);  // (x - y).n_j: K row i
-                const R bi_ = -g3 * dni;                               // (y - x).n_i: K row j
+                const R bj = g3 * (p2 * e2n + (p1 * e1n + c0));  // (x - y).n_j: K row i
+                const R bi_ = g3 * dyi;                          // (y - x).n_i: K row j
                 if constexpr (HELM) {
                     const R kr = kap * (r2 * rinv);
                     R sn, cs;
