@@ -1223,6 +1223,7 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
         //   (y_b - x_a).n_i = Y.n_i,
         // (S_a, p1, p2, w_a) from the per-point table (scalar loads).
         const ConstPtr<R> PQ = as_const(Ptr<R>::pq(p) + (int64_t)i * q2 * 4);
+        const ConstPtr<R> PU = as_const(Ptr<R>::pu(p) + (int64_t)i * q2 * 4);  // FP32: offsets from C_i
         const ConstPtr<double> FR = as_const(p.frame + (int64_t)i * PANEL_FRAME);
         R U[3], E1[3], E2[3], D[3], ni[3], oY[3];
 #pragma unroll
@@ -1261,13 +1262,25 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
             for (int s = 0; s < 3; ++s) pwr[s] = pwi[s] = qkr[s] = qki[s] = 0;
 #pragma unroll 5
             for (int a = 0; a < q2; ++a) {
-                const R sa = PQ[4 * a + 0], p1 = PQ[4 * a + 1], p2 = PQ[4 * a + 2], wa = PQ[4 * a + 3];
-                const R r2 = p2 * y2 + (p1 * y1 + (sa + yd));
+                R r2, wa, dnj;
+                if constexpr (sizeof(R) == 8) {
+                    const R sa = PQ[4 * a + 0], p1 = PQ[4 * a + 1], p2 = PQ[4 * a + 2];
+                    wa = PQ[4 * a + 3];
+                    r2 = p2 * y2 + (p1 * y1 + (sa + yd));
+                    dnj = p2 * e2n + (p1 * e1n + c0);
+                } else {
+                    // FP32 keeps the direct form (its K rows lost digits in the
+                    // moment form at no gain: the FP32 pass is atomic-bound)
+                    const R dx = PU[4 * a + 0] - yx, dy = PU[4 * a + 1] - yy, dz = PU[4 * a + 2] - yz;
+                    wa = PU[4 * a + 3];
+                    r2 = dx * dx + dy * dy + dz * dz;
+                    dnj = dx * nnx + dy * nny + dz * nnz;
+                }
                 const R rinv = Math<R>::rsqrt(r2);
                 const R g = wa * rinv;
                 const R g3 = g * (rinv * rinv);
-                const R bj = g3 * (p2 * e2n + (p1 * e1n + c0));  // (x - y).n_j: K row i
-                const R bi_ = g3 * dyi;                          // (y - x).n_i: K row j
+                const R bj = g3 * dnj;   // (x - y).n_j: K row i
+                const R bi_ = g3 * dyi;  // (y - x).n_i: K row j
                 if constexpr (HELM) {
                     const R kr = kap * (r2 * rinv);
                     R sn, cs;
