@@ -376,6 +376,30 @@ __device__ __forceinline__ void dpp_best(double& b, int& bi) {
     take_better(__hiloint2double(hi, lo), oi, b, bi);
 }
 
+// Orders this wave's LDS accesses for the compiler (the hardware runs one
+// wave's LDS instructions in order): a store before it is seen by a load of
+// another lane of the wave after it.
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// x + (x of the lane the DPP pattern selects), for FP64 and FP32 values.
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xF, 0xF, false);
+    return v + __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the four lanes of a quad, in every lane of it (whole wave active).
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v) {
+    v = dpp_add<0xB1>(v);     // quad_perm [1, 0, 3, 2]
+    return dpp_add<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+}
+
 // Best candidate of the wave, returned in every lane (the whole wave must be
 // active): within each row of 16 lanes by xor-1 and xor-2 quad permutations,
 // the half-row mirror and the row mirror, after which every lane of a row
@@ -901,27 +925,30 @@ __global__ __launch_bounds__(256) void linv_kernel(LuProb inl, const LuProb* tbl
         }
     }
     __syncthreads();
-    // X = inv(L11) (unit lower) by anti-diagonals d = i - c: X[i][i-d] needs
-    // only entries of smaller d; four threads per entry split the dot product.
+    // X = inv(L11) (unit lower) by forward substitution, one column per quad
+    // of lanes: X[i][c] = -sum_{c <= j < i} M[i][j] X[j][c] for i > c.  A
+    // column only reads its own entries, so each wave runs its 16 columns on
+    // its own, ordered through its LDS accesses (one wave's LDS instructions
+    // execute in order), with no barrier between rows; the four lanes of a
+    // quad split the sum and add it by DPP.
     {
-        const int g = t >> 2, q = t & 3;
-        for (int d = 0; d < NB; ++d) {
-            const int i = g + d, c = g;
+        const int lane = t & 63, c = 16 * (t >> 6) + (lane >> 2), q = lane & 3;
+        if (q == 0) X[c][c] = cx<T>{T(1), T(0)};
+        for (int i = 1; i < NB; ++i) {
+            wave_lds_order();
             cx<T> sum = {T(0), T(0)};
-            if (i < NB)
+            if (i > c)
                 for (int j = c + q; j < i; j += 4) {
                     const cx<T> lx = cmul(M[i][j], X[j][c]);
                     sum.re += lx.re;
                     sum.im += lx.im;
                 }
-            sum.re += __shfl_xor(sum.re, 1, 64);
-            sum.im += __shfl_xor(sum.im, 1, 64);
-            sum.re += __shfl_xor(sum.re, 2, 64);
-            sum.im += __shfl_xor(sum.im, 2, 64);
-            if (i < NB && q == 0) X[i][c] = cx<T>{T(d == 0 ? 1 : 0) - sum.re, -sum.im};
-            __syncthreads();
+            sum.re = quad_sum(sum.re);
+            sum.im = quad_sum(sum.im);
+            if (i > c && q == 0) X[i][c] = cx<T>{-sum.re, -sum.im};
         }
     }
+    __syncthreads();
     for (int e = t; e < NB * NB; e += 256) {
         const int i = e / NB, c = e % NB;
         Acc<T, C>::st(SL, e, cx<T>{T(i == c ? 1 : 0) - X[i][c].re, -X[i][c].im});
