@@ -25,6 +25,8 @@ import sys
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
+# BEM_OPT_C64_LU_GEMM values the mixed pipelines take
+LU_GEMM_NAMES = {0: "FP32 MFMA", 2: "fp16 MFMA on split operand planes (three part products, FP32 accumulation)"}
 sys.path.insert(0, REPO)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
@@ -103,6 +105,9 @@ def main() -> int:
     ap.add_argument("--no-fd", action="store_true", help="transmission: skip the finite-difference check")
     ap.add_argument("--lu-lookahead", type=int, default=None,
                     help="compute units of the LU's panel stream (0: no look-ahead; default: the library's)")
+    ap.add_argument("--c64-lu-gemm", type=int, default=2, choices=(0, 2),
+                    help="trailing updates of the mixed pipelines' complex64 LU: 2 = fp16 MFMA on split operand "
+                         "planes (default), 0 = FP32 MFMA")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="dirichlet: skip the one-step runs of the sweep, transmission and gradient workloads "
                          "(child processes) whose summaries the line carries under other_configs")
@@ -233,6 +238,7 @@ def main() -> int:
 
     if not args.no_mixed:
         tts = {"fp64_ms": round(mean["total_ms"], 3)}
+        lu_gemm_name = LU_GEMM_NAMES[args.c64_lu_gemm]
         x64n = torch.linalg.vector_norm(x64).item()
         # mixed (fast): FP32 regular pairs + FP64 near field into c64, c64 LU,
         # refinement against that c64 matrix with FP64 residuals
@@ -251,8 +257,10 @@ def main() -> int:
             e[2].record()
             LU32.copy_(A32)
             ctx.set_option(L.OPT_LU_PIVOTING, 2)  # block-local pivoting (the factor preconditions refinement)
+            ctx.set_option(L.OPT_C64_LU_GEMM, args.c64_lu_gemm)
             ctx.lu_factor(LU32, ipiv)
             ctx.set_option(L.OPT_LU_PIVOTING, 0)
+            ctx.set_option(L.OPT_C64_LU_GEMM, 0)
             e[3].record()
             st = ctx.refine_solve(A32, LU32, ipiv, b, x, max_iter=30, tol=1e-13)
             e[4].record()
@@ -261,7 +269,7 @@ def main() -> int:
         tts["mixed_fp32asm"] = {"ms": round(e[0].elapsed_time(e[4]), 3), "far_fp32_ms": round(e[0].elapsed_time(e[1]), 3),
                                 "assembly_ms": round(e[0].elapsed_time(e[2]), 3), "lu_c64_ms": round(e[2].elapsed_time(e[3]), 3),
                                 "refine_ms": round(e[3].elapsed_time(e[4]), 3), "refine_iters": st.iterations,
-                                "lu_pivoting": "block-local", "rel_err_vs_fp64": err,
+                                "lu_pivoting": "block-local", "lu_gemm": lu_gemm_name, "rel_err_vs_fp64": err,
                                 "assembly_entries_per_s": entries / (e[0].elapsed_time(e[2]) * 1e-3)}
         # mixed (accurate): FP64 assembly, c64 LU, FP64 refinement against A64
         for rep in range(2):
@@ -273,8 +281,10 @@ def main() -> int:
             e[1].record()
             bem.convert(A64, LU32)
             ctx.set_option(L.OPT_LU_PIVOTING, 2)  # block-local pivoting (the factor preconditions refinement)
+            ctx.set_option(L.OPT_C64_LU_GEMM, args.c64_lu_gemm)
             ctx.lu_factor(LU32, ipiv)
             ctx.set_option(L.OPT_LU_PIVOTING, 0)
+            ctx.set_option(L.OPT_C64_LU_GEMM, 0)
             e[2].record()
             st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=1e-14)
             e[3].record()
@@ -283,7 +293,7 @@ def main() -> int:
         tts["mixed_fp64asm"] = {"ms": round(e[0].elapsed_time(e[3]), 3), "assembly_ms": round(e[0].elapsed_time(e[1]), 3),
                                 "lu_c64_ms": round(e[1].elapsed_time(e[2]), 3), "refine_ms": round(e[2].elapsed_time(e[3]), 3),
                                 "refine_iters": st.iterations, "converged": st.converged,
-                                "lu_pivoting": "block-local", "rel_err_vs_fp64": err2}
+                                "lu_pivoting": "block-local", "lu_gemm": lu_gemm_name, "rel_err_vs_fp64": err2}
         tts["speedup_mixed_fp32asm"] = round(mean["total_ms"] / tts["mixed_fp32asm"]["ms"], 3)
         tts["speedup_mixed_fp64asm"] = round(mean["total_ms"] / tts["mixed_fp64asm"]["ms"], 3)
         result["time_to_solution"] = tts
@@ -665,11 +675,14 @@ def transmission_main(args, bem) -> int:
         # partial pivoting: block-local pivoting (BEM_OPT_LU_PIVOTING = 2) does
         # not give a converging refinement on this system (kappa about 1e6)
         L.convert(A64, LU32)
+        bm.ctx.set_option(L.OPT_C64_LU_GEMM, args.c64_lu_gemm)
         bm.ctx.lu_factor(LU32, ipiv)
+        bm.ctx.set_option(L.OPT_C64_LU_GEMM, 0)
         st = bm.ctx.refine_solve(A64, LU32, ipiv, b, x, max_iter=30, tol=1e-14)
         e[2].record()
         torch.cuda.synchronize()
     mixA = {"lu_and_refine_ms": e[0].elapsed_time(e[2]), "lu_pivoting": "partial",
+            "lu_gemm": LU_GEMM_NAMES[args.c64_lu_gemm],
             "refine_iters": int(st.iterations), "converged": int(st.converged),
             "rel_err_vs_fp64": torch.linalg.vector_norm(x - x64).item() / xn}
     mixA["ms"] = asm64 + mixA["lu_and_refine_ms"]
@@ -695,13 +708,15 @@ def transmission_main(args, bem) -> int:
         assemble(b32)
         e[1].record()
         LU = b32._mat_A.clone()
+        b32.ctx.set_option(L.OPT_C64_LU_GEMM, args.c64_lu_gemm)
         b32.ctx.lu_factor(LU, ipiv)
+        b32.ctx.set_option(L.OPT_C64_LU_GEMM, 0)
         st = b32.ctx.refine_solve(b32._mat_A, LU, ipiv, b, x, max_iter=30, tol=1e-12)
         e[3].record()
         torch.cuda.synchronize()
         del LU
     mixB = {"assembly_ms": e[0].elapsed_time(e[1]), "lu_and_refine_ms": e[1].elapsed_time(e[3]),
-            "lu_pivoting": "partial", "refine_iters": int(st.iterations),
+            "lu_pivoting": "partial", "lu_gemm": LU_GEMM_NAMES[args.c64_lu_gemm], "refine_iters": int(st.iterations),
             "rel_err_vs_fp64": torch.linalg.vector_norm(x - x64).item() / xn}
     mixB["ms"] = mixB["assembly_ms"] + mixB["lu_and_refine_ms"]
     cm32.kill()

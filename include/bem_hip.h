@@ -253,6 +253,13 @@ int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_
 enum {
     BEM_OPT_C64_LU_GEMM = 1  /* arithmetic of the trailing updates of complex64
                                 LU factorisations: 0 (default) = FP32 MFMA;
+                                2 = the rank-256 updates of single
+                                factorisations on the fp16 MFMA, each operand
+                                row (column) scaled by a power of two and
+                                split into two fp16 parts, three part
+                                products per real product, FP32 accumulation
+                                (update error about 3e-7 of sum |a||b|; for
+                                factors that precondition FP64 refinement);
                                 6 = bf16 MFMA on operands split in three bf16
                                 parts, six part products (about FP32
                                 accuracy); 3 = two parts, three products

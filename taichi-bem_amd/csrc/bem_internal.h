@@ -103,7 +103,10 @@ struct Context {
     size_t cf_bytes = 0;
     std::vector<char> cf_table;
     int num_cu = 0;
-    int c64_gemm_parts = 0;       // c64 LU trailing updates: 0 FP32 MFMA, 2/3 split-bf16 parts
+    int c64_gemm_parts = 0;       // c64 LU trailing updates: 0 FP32 MFMA, 2/3 split-bf16 parts,
+                                  // C64_GEMM_H16 split-fp16 planes
+    void* lu_h16 = nullptr;       // split-fp16 planes of the c64 rank-256 updates
+    size_t lu_h16_bytes = 0;
     int lu_lookahead_cus = 32;    // LU look-ahead: 0 off, > 0 panel-stream CUs, -1 unmasked streams
     int lu_pivoting = 0;          // LU panels: 0 partial pivoting (LAPACK rows), 1 tournament
     int lu_panel = 1;             // LU partial-pivoting panels: 1 persistent launch where it fits, 0 per column
@@ -116,6 +119,9 @@ struct Context {
     hipStream_t lu_stream_free[2] = {nullptr, nullptr};  // look-ahead without CU masks (block-local pivoting)
     hipEvent_t lu_event[3] = {nullptr, nullptr, nullptr};
 };
+
+// c64_gemm_parts value of the split-fp16 rank-256 updates (a GEMM option bit)
+constexpr int C64_GEMM_H16 = 64;
 
 // error helpers (bem_capi.hip)
 int set_error(int code, const std::string& msg);

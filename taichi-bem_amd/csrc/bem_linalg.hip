@@ -36,7 +36,8 @@ bool dtype_complex(int dtype) { return dtype == BEM_C128 || dtype == BEM_C64; }
 
 namespace {
 
-constexpr int NB = 64;  // panel width
+constexpr int NB = 64;   // panel width
+constexpr int W2 = 256;  // outer block width (NB-column panels inside, one rank-W2 trailing update per block)
 
 template <typename T>
 struct cx {
@@ -785,14 +786,10 @@ __global__ void tourn_ipiv_kernel(LuProb inl, const LuProb* tbl, int k, int leve
     }
 }
 
-// LU of the top w x w block of the panel in LDS, then S_U = -inv(U11) for
-// L21 = A21 inv(U11) (GEMM mode 5 stores -(A21 S_U)).  PIV = false: the
-// block is taken as it is (tournament pivoting has already moved the pivot
-// rows up).  PIV = true (block-local pivoting): partial pivoting among the
-// block's own rows only (largest |re|+|im| at or below the diagonal, first
-// row on ties), interchanges recorded in ipiv like LAPACK's; the rows below
-// the block are not searched, so no workgroup waits for another.
-template <typename T, bool C, bool B, bool PIV>
+// LU without pivoting of the top w x w block of the panel in LDS (tournament
+// pivoting has already moved the pivot rows up), then S_U = -inv(U11) for
+// L21 = A21 inv(U11) (GEMM mode 5 stores -(A21 S_U)).
+template <typename T, bool C, bool B>
 __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb* tbl, int k) {
     const LuProb P = get_prob<B>(inl, tbl);
     const int n = P.n;
@@ -817,31 +814,7 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
         }
     }
     __syncthreads();
-    __shared__ int s_piv;
     for (int j = 0; j < w; ++j) {
-        if constexpr (PIV) {
-            if (t < 64) {  // one wave: lane r looks at row r of the block
-                double m = (t >= j && t < w) ? fabs((double)M[t][j].re) + fabs((double)M[t][j].im) : -1.0;
-                int r = t;
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double om = __shfl_xor(m, off, 64);
-                    const int orr = __shfl_xor(r, off, 64);
-                    if (om > m || (om == m && orr < r)) { m = om; r = orr; }
-                }
-                if (t == 0) {
-                    s_piv = r;
-                    P.ipiv[k + j] = k + r;
-                }
-            }
-            __syncthreads();
-            const int p = s_piv;
-            if (p != j && t < NB) {
-                const cx<T> a = M[j][t];
-                M[j][t] = M[p][t];
-                M[p][t] = a;
-            }
-            __syncthreads();
-        }
         const cx<T> d = M[j][j];
         const bool zero = d.re == T(0) && d.im == T(0);
         if (zero && t == 0 && *P.info == 0) *P.info = k + j + 1;
@@ -893,6 +866,123 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
     for (int e = t; e < NB * NB; e += 256) {
         const int i = e / NB, c = e % NB;
         Acc<T, C>::st(SU, e, cx<T>{-X[i][c].re, -X[i][c].im});
+    }
+}
+
+// Block-local pivoting panel top in one workgroup of two waves, with no
+// workgroup barrier per column:
+//  * wave 0 factors the 64 x 64 diagonal block in LDS: per column a wave
+//    argmax of |re|+|im| over the rows at or below the diagonal (first row on
+//    ties; ipiv as LAPACK's), the interchange (lane c swaps column c of the
+//    two rows), the multipliers, and the rank-1 update with lane i owning
+//    row i; one wave's LDS accesses run in order, so it needs no barrier;
+//  * the factored block goes to the matrix, then wave 0 forms SU =
+//    -inv(U11) (lane c owns column c, back substitution row by row) while
+//    wave 1 forms S_L = I - inv(L11) (forward substitution), both in place
+//    in the LDS copy: step i of a column reads row i of the factor and the
+//    column's entries already replaced, then replaces entry i;
+//  * S_L goes to slot (k mod 256) / 64 of the outer block, so the separate
+//    linv_kernel launch is not needed for this pivoting.
+// The block past w (the matrix's last partial block) is the identity.
+template <typename T, bool C, bool B>
+__global__ __launch_bounds__(128) void topblock_local_kernel(LuProb inl, const LuProb* tbl, int k) {
+    const LuProb P = get_prob<B>(inl, tbl);
+    const int n = P.n;
+    if (k >= n) return;
+    const int kend = min(n, k + NB), w = kend - k;
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    __shared__ cx<T> M[NB][NB + 1];
+    __shared__ cx<T> U[NB];  // the pivot row of the current column
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int r = wave; r < NB; r += 2)
+        M[r][lane] = (r < w && lane < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + lane)
+                                         : cx<T>{T(r == lane ? 1 : 0), T(0)};
+    __syncthreads();
+    if (wave == 0) {
+        for (int j = 0; j < NB; ++j) {
+            double b = lane >= j ? fabs((double)M[lane][j].re) + fabs((double)M[lane][j].im) : -1.0;
+            int bi = lane;
+            wave_best(b, bi);
+            const int p = bi;
+            if (lane == 0 && j < w) P.ipiv[k + j] = k + p;
+            const cx<T> xj = M[j][lane], xp = M[p][lane];
+            M[p][lane] = xj;
+            M[j][lane] = xp;
+            U[lane] = xp;
+            wave_lds_order();
+            const cx<T> d = U[j];
+            const bool zero = d.re == T(0) && d.im == T(0);
+            if (zero && lane == 0 && *P.info == 0) *P.info = k + j + 1;
+            if (lane > j) {
+                cx<T> l = M[lane][j];
+                if (!zero) {
+                    l = cdiv(l, d);
+                    M[lane][j] = l;
+                }
+                // the pivot row is read from its own array, so the compiler
+                // may keep eight read-modify-writes of this row in flight
+#pragma unroll 8
+                for (int c = j + 1; c < NB; ++c) {
+                    const cx<T> u = U[c];
+                    cx<T> m = M[lane][c];
+                    m.re -= l.re * u.re - l.im * u.im;
+                    m.im -= l.re * u.im + l.im * u.re;
+                    M[lane][c] = m;
+                }
+            }
+            wave_lds_order();
+        }
+    }
+    __syncthreads();
+    for (int r = wave; r < w; r += 2)
+        if (lane < w) Acc<T, C>::st(A, (int64_t)(k + r) * ld + k + lane, M[r][lane]);
+    __syncthreads();
+    const int c = lane;
+    if (wave == 0) {
+        // X = inv(U11): X[i][c] = (delta_ic - sum_{i < j <= c} U[i][j] X[j][c]) / U[i][i]
+        for (int i = NB - 1; i >= 0; --i) {
+            const cx<T> dii = M[i][i];
+            cx<T> s = {T(i == c ? 1 : 0), T(0)};
+            if (i <= c) {
+#pragma unroll 4
+                for (int j = i + 1; j <= c; ++j) {
+                    const cx<T> u = M[i][j], x = M[j][c];
+                    s.re -= u.re * x.re - u.im * x.im;
+                    s.im -= u.re * x.im + u.im * x.re;
+                }
+            }
+            wave_lds_order();
+            if (i <= c) M[i][c] = (dii.re == T(0) && dii.im == T(0)) ? s : cdiv(s, dii);
+            wave_lds_order();
+        }
+    } else {
+        // X = inv(L11), unit lower: X[i][c] = -sum_{c <= j < i} L[i][j] X[j][c], X[c][c] = 1
+        for (int i = 1; i < NB; ++i) {
+            cx<T> s = {T(0), T(0)};
+            if (i > c) {
+                const cx<T> l0 = M[i][c];
+                s = {-l0.re, -l0.im};
+#pragma unroll 4
+                for (int j = c + 1; j < i; ++j) {
+                    const cx<T> l = M[i][j], x = M[j][c];
+                    s.re -= l.re * x.re - l.im * x.im;
+                    s.im -= l.re * x.im + l.im * x.re;
+                }
+            }
+            wave_lds_order();
+            if (i > c) M[i][c] = s;
+            wave_lds_order();
+        }
+    }
+    __syncthreads();
+    T* SU = (T*)P.SU;
+    T* SL = (T*)P.SL + (C ? 2 : 1) * (int64_t)((k % W2) / NB) * NB * NB;
+    for (int e = t; e < NB * NB; e += 128) {
+        const int i = e / NB, cc = e % NB;
+        const cx<T> x = M[i][cc];
+        Acc<T, C>::st(SU, e, i <= cc ? cx<T>{-x.re, -x.im} : cx<T>{T(0), T(0)});
+        Acc<T, C>::st(SL, e, i > cc ? cx<T>{-x.re, -x.im} : cx<T>{T(0), T(0)});
     }
 }
 
@@ -1011,7 +1101,6 @@ struct GemmArgs {
 //           subtracting; in place: N <= 64, so every block owns whole rows)
 //           (columns c0 < 0 -> K1, c1 <= 0 -> n: the whole trailing matrix)
 // with r0 = K0 + b NB, S_L = I - inv(L_bb) of the block's unit lower factor.
-constexpr int W2 = 256;
 
 struct LuStep {
     int mode, K0, k, b;
@@ -1862,6 +1951,338 @@ __global__ __launch_bounds__(256) void gemm_sub_c64bf_kernel(GemmArgs<float> inl
             }
 }
 
+// ---------------------------------------------------------------------------
+// c64 rank-256 trailing updates on the fp16 MFMA (v_mfma_f32_32x32x16_f16;
+// measured on gfx950: 1.54 PFLOP/s sustained against 143 TFLOP/s for the
+// FP32 MFMA).  The operands are split once per outer block into fp16 planes:
+//  * each row of the left factor A (each column of the right factor B) is
+//    scaled by the power of two 2^e that puts its largest |re| or |im| into
+//    [2^14, 2^15), well inside fp16's range;
+//  * a scaled value v becomes hi = fp16(v) and lo = fp16(v - hi), so
+//    |v - hi - lo| <= 2^-22 |v| where both are normal, and below 2^-24 (an
+//    fp16 subnormal step, 2^-38 of the row's largest value) where not;
+//  * a real product keeps hi*hi + hi*lo + lo*hi (lo*lo is below 2^-22 |a b|)
+//    in one FP32 accumulator, and a complex product takes four real ones
+//    (re re - im im, re im + im re; the minus sign is a sign-bit flip of B's
+//    imaginary parts), so one complex multiply-add is twelve fp16 products
+//    against four FP32 ones; measured error of the update about 3e-7 of
+//    sum |a||b|;
+//  * C -= 2^-(e_i + f_j) (re, im), the exponents applied with ldexp.
+// Plane layout: plane p (0 re hi, 1 re lo, 2 im hi, 3 im lo), stage s (16
+// consecutive k), row r at element ((p HNS + s) cap + r) 16 + k % 16, so
+// that one stage of consecutive rows is contiguous; stages past K are
+// zero-padded up to a multiple of 16.
+// ---------------------------------------------------------------------------
+typedef _Float16 h16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 h16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+constexpr int HS = 16;          // K depth of one stage (the MFMA's)
+constexpr int HNS = W2 / HS;    // stages of a full outer block
+constexpr int HT = 128;         // C tile (rows and columns)
+
+// Split planes of one factorisation (device): the outer block's left-factor
+// rows and right-factor columns, with their exponents; cap = n.
+struct H16Ws {
+    _Float16* a;
+    int* ea;
+    _Float16* b;
+    int* eb;
+    int cap;
+};
+
+__host__ __device__ inline int64_t hoff(int p, int s, int r, int cap) {
+    return ((int64_t)(p * HNS + s) * cap + r) * HS;
+}
+
+size_t h16_ws_bytes(int n) { return 2 * (4 * (size_t)n * W2 * sizeof(_Float16) + (size_t)n * sizeof(int) + 512); }
+
+H16Ws h16_carve(char* base, int n) {
+    H16Ws w;
+    w.cap = n;
+    const size_t planes = 4 * (size_t)n * W2 * sizeof(_Float16);
+    w.a = (_Float16*)base;
+    w.ea = (int*)(base + planes);
+    char* nb = base + ((planes + (size_t)n * sizeof(int) + 255) & ~(size_t)255) + 256;
+    w.b = (_Float16*)nb;
+    w.eb = (int*)(nb + planes);
+    return w;
+}
+
+// e with m 2^e in [2^14, 2^15); 0 for m = 0 or not finite
+__device__ __forceinline__ int h16_exponent(float m) {
+    if (!(m > 0.f) || !(m < INFINITY)) return 0;
+    int ex;
+    frexpf(m, &ex);  // m in [2^(ex - 1), 2^ex)
+    return max(-120, min(120, 15 - ex));
+}
+
+__device__ __forceinline__ void h16_split2(float x, int e, _Float16& h, _Float16& l) {
+    const float v = ldexpf(x, e);
+    h = (_Float16)v;
+    l = (_Float16)(v - (float)h);
+}
+
+// Rows of the left factor (M x K, K <= 256, of a row-major c64 matrix) into
+// the planes, one wave per row (lane l: k = 4 l .. 4 l + 3).
+__global__ __launch_bounds__(256) void h16_split_rows_kernel(const float* A, int64_t lda, int M, int K, H16Ws w) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= M) return;
+    const float* a = A + 2 * (int64_t)row * lda;
+    const int ns = (K + HS - 1) / HS;
+    float re[4], im[4], m = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * lane + i;
+        re[i] = 0.f;
+        im[i] = 0.f;
+        if (k < K) {
+            const float2 v = *(const float2*)(a + 2 * k);
+            re[i] = v.x;
+            im[i] = v.y;
+        }
+        m = fmaxf(m, fmaxf(fabsf(re[i]), fabsf(im[i])));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    const int e = h16_exponent(m);
+    if (lane == 0) w.ea[row] = e;
+    const int s = lane >> 2;
+    if (s >= ns) return;
+    h16x4_t v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        _Float16 h, l;
+        h16_split2(re[i], e, h, l);
+        v[0][i] = h;
+        v[1][i] = l;
+        h16_split2(im[i], e, h, l);
+        v[2][i] = h;
+        v[3][i] = l;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) *(h16x4_t*)(w.a + hoff(p, s, row, w.cap) + 4 * (lane & 3)) = v[p];
+}
+
+// Columns of the right factor (K x N, K <= 256, row-major c64) into the
+// planes, 64 columns per workgroup: the column maxima, then one stage of a
+// column per thread.
+__global__ __launch_bounds__(256) void h16_split_cols_kernel(const float* Bm, int64_t ldb, int K, int N, H16Ws w) {
+    __shared__ float sm[4][64];
+    __shared__ int se[64];
+    const int t = threadIdx.x, c = t & 63, g = t >> 6;
+    const int col = blockIdx.x * 64 + c;
+    const int ns = (K + HS - 1) / HS;
+    float m = 0.f;
+    if (col < N)
+        for (int k = g; k < K; k += 4) {
+            const float2 v = *(const float2*)(Bm + 2 * ((int64_t)k * ldb + col));
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+    sm[g][c] = m;
+    __syncthreads();
+    if (t < 64) {
+        const int e = h16_exponent(fmaxf(fmaxf(sm[0][t], sm[1][t]), fmaxf(sm[2][t], sm[3][t])));
+        se[t] = e;
+        if (blockIdx.x * 64 + t < N) w.eb[blockIdx.x * 64 + t] = e;
+    }
+    __syncthreads();
+    if (col >= N) return;
+    const int e = se[c];
+    for (int s = g; s < ns; s += 4) {
+        h16x8_t v[4][2];
+#pragma unroll
+        for (int i = 0; i < HS; ++i) {
+            const int k = HS * s + i;
+            float2 x = {0.f, 0.f};
+            if (k < K) x = *(const float2*)(Bm + 2 * ((int64_t)k * ldb + col));
+            _Float16 h, l;
+            h16_split2(x.x, e, h, l);
+            v[0][i >> 3][i & 7] = h;
+            v[1][i >> 3][i & 7] = l;
+            h16_split2(x.y, e, h, l);
+            v[2][i >> 3][i & 7] = h;
+            v[3][i >> 3][i & 7] = l;
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            h16x8_t* o = (h16x8_t*)(w.b + hoff(p, s, col, w.cap));
+            o[0] = v[p][0];
+            o[1] = v[p][1];
+        }
+    }
+}
+
+__device__ __forceinline__ h16x8_t h16_neg(h16x8_t x) {
+    typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+    u32x4_t u = __builtin_bit_cast(u32x4_t, x);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(h16x8_t, u);
+}
+
+// C[M x N] -= A B from the planes (A rows from row 0, B columns from column
+// boff).  128 x 128 tile per workgroup of four waves (2 x 2, each 64 x 64 =
+// 2 x 2 MFMA blocks of 32 x 32), two workgroups per compute unit; stages of
+// 16 k double-buffered in LDS (2 x 32 KB), the next stage's loads in flight
+// in registers during the current one.  LDS rows hold 16 k (32 bytes); the
+// two 16-byte halves of a row are swapped when bit 2 xor bit 3 of the row
+// is set, so 16 consecutive rows read at one k offset hit distinct banks.
+// Tiles are dealt so that each XCD (workgroup id mod 8) takes a contiguous
+// run of the grouped order (4 tile rows, column-major inside a group): the
+// tiles one XCD runs at a time share their A and B planes in its L2.
+__global__ __launch_bounds__(256, 2) void gemm_h16_kernel(float* Cm, int64_t ldc, int M, int N, int K, H16Ws w,
+                                                          int boff, int tiles_m, int tiles_n) {
+    constexpr int G = 4;
+    const int T = tiles_m * tiles_n, per = (T + 7) / 8;
+    const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= T) return;
+    const int grp = t / (G * tiles_n), rem = t - grp * G * tiles_n;
+    const int rows_in = min(G, tiles_m - grp * G);
+    const int bm = (grp * G + rem % rows_in) * HT, bn = (rem / rows_in) * HT;
+    __shared__ _Float16 sm[2][2][4][HT][HS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x16_t accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                accR[a][b][j] = 0.f;
+                accI[a][b][j] = 0.f;
+            }
+    // this thread's load slots: q = 0..3 plane q of A, q = 4..7 plane q - 4
+    // of B, tile row r, half c (rows past M or N are clamped: they reach
+    // only C rows or columns that are not written)
+    h16x8_t pf[8];
+    const int r = tid >> 1, c = tid & 1;
+    const unsigned offA = (unsigned)min(bm + r, M - 1) * HS + 8 * c;
+    const unsigned offB = (unsigned)(boff + min(bn + r, N - 1)) * HS + 8 * c;
+    const int64_t sst = (int64_t)w.cap * HS, pst = HNS * sst;
+    auto load = [&](int s) {
+        const _Float16* a0 = w.a + s * sst;
+        const _Float16* b0 = w.b + s * sst;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pf[q] = *(const h16x8_t*)(a0 + q * pst + offA);
+            pf[4 + q] = *(const h16x8_t*)(b0 + q * pst + offB);
+        }
+    };
+    const int cs = 8 * (c ^ (((r >> 2) ^ (r >> 3)) & 1));
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) *(h16x8_t*)&sm[buf][q >> 2][q & 3][r][cs] = pf[q];
+    };
+    const int ns = (K + HS - 1) / HS;
+    load(0);
+    store(0);
+    __syncthreads();
+    // MFMA 32x32x16 operands (measured on gfx950): lane l supplies
+    // A[l % 32][8 (l / 32) + i] and B[8 (l / 32) + i][l % 32], i = 0..7
+    const int l32 = lane & 31, kc = 8 * ((lane >> 5) ^ (((l32 >> 2) ^ (l32 >> 3)) & 1));
+    for (int s = 0; s < ns; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < ns) load(s + 1);
+        h16x8_t af[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) af[a][p] = *(const h16x8_t*)&sm[buf][0][p][wm + 32 * a + l32][kc];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            h16x8_t bf[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) bf[p] = *(const h16x8_t*)&sm[buf][1][p][wn + 32 * b + l32][kc];
+            const h16x8_t bn2 = h16_neg(bf[2]), bn3 = h16_neg(bf[3]);
+            // (hi, lo) x (hi, lo) without lo lo; four accumulators in turn
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                accR[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][0], bf[0], accR[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][0], bf[2], accI[a][b], 0, 0, 0);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                accR[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][0], bf[1], accR[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][0], bf[3], accI[a][b], 0, 0, 0);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                accR[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][1], bf[0], accR[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][1], bf[2], accI[a][b], 0, 0, 0);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                accR[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][2], bn2, accR[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][2], bf[0], accI[a][b], 0, 0, 0);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                accR[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][2], bn3, accR[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][2], bf[1], accI[a][b], 0, 0, 0);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                accR[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][3], bn2, accR[a][b], 0, 0, 0);
+                accI[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[a][3], bf[0], accI[a][b], 0, 0, 0);
+            }
+        }
+        if (s + 1 < ns) store(buf ^ 1);
+        __syncthreads();
+    }
+    // result register j of lane l: row 8 (j / 4) + 4 (l / 32) + j % 4,
+    // column l % 32 of its 32 x 32 block (measured on gfx950)
+    int ecol[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int gc = bn + wn + 32 * b + l32;
+        ecol[b] = gc < N ? w.eb[boff + gc] : 0;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int gr = bm + wm + 32 * a + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+            if (gr >= M) continue;
+            const int erow = w.ea[gr];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int gc = bn + wn + 32 * b + l32;
+                if (gc < N) {
+                    float2* cp = (float2*)(Cm + 2 * ((int64_t)gr * ldc + gc));
+                    float2 v = *cp;
+                    const int e = -(erow + ecol[b]);
+                    v.x -= ldexpf(accR[a][b][j], e);
+                    v.y -= ldexpf(accI[a][b][j], e);
+                    *cp = v;
+                }
+            }
+        }
+        asm volatile("" ::: "memory");
+    }
+}
+
+// The outer block's left factor A[K1:n, K0:K1] and right factor
+// A[K0:K1, K1:n] into the planes (once per outer block, after its U12).
+void h16_split(const LuProb& P, int K0, const H16Ws& w, hipStream_t s) {
+    const int n = P.n, K1 = std::min(n, K0 + W2), M = n - K1;
+    if (M <= 0) return;
+    float* A = (float*)P.A;
+    hipLaunchKernelGGL(h16_split_rows_kernel, dim3((M + 3) / 4), dim3(256), 0, s,
+                       (const float*)(A + 2 * ((int64_t)K1 * P.ld + K0)), P.ld, M, K1 - K0, w);
+    hipLaunchKernelGGL(h16_split_cols_kernel, dim3((M + 63) / 64), dim3(256), 0, s,
+                       (const float*)(A + 2 * ((int64_t)K0 * P.ld + K1)), P.ld, K1 - K0, M, w);
+}
+
+// A[K1:n, c0:c1] -= A[K1:n, K0:K1] A[K0:K1, c0:c1] from the planes of h16_split.
+void h16_update(const LuProb& P, int K0, int c0, int c1, const H16Ws& w, hipStream_t s) {
+    const int n = P.n, K1 = std::min(n, K0 + W2), M = n - K1, N = c1 - c0;
+    if (M <= 0 || N <= 0) return;
+    float* C = (float*)P.A + 2 * ((int64_t)K1 * P.ld + c0);
+    const int tm = (M + HT - 1) / HT, tn = (N + HT - 1) / HT;
+    hipLaunchKernelGGL(gemm_h16_kernel, dim3(8 * ((tm * tn + 7) / 8)), dim3(256), 0, s, C, P.ld, M, N, K1 - K0, w,
+                       c0 - K1, tm, tn);
+}
+
 // gopts, the GEMM options of a factorisation: bits 0-3 = split-bf16 parts of
 // c64 trailing updates (0 = FP32 MFMA for every c64 product; 2 or 3 = the
 // split bf16 kernel for modes 1 and 4); bit 4 (GEMM_SMALL_TILES) = 64 x 64
@@ -1871,7 +2292,7 @@ constexpr int GEMM_SMALL_TILES = 16;
 constexpr int GEMM_WIDE4M = 32;
 
 int gemm_opts(const Context* ctx) {
-    return ctx->c64_gemm_parts | (ctx->lu_gemm_tile ? 0 : GEMM_SMALL_TILES) |
+    return (ctx->c64_gemm_parts & 15) | (ctx->lu_gemm_tile ? 0 : GEMM_SMALL_TILES) |
            (ctx->lu_gemm_tile == 2 ? GEMM_WIDE4M : 0);
 }
 
@@ -1947,7 +2368,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
     }
     if (pivoting == 2) {
         // block-local pivoting: the top block in one workgroup, L21 on MFMA
-        hipLaunchKernelGGL((topblock_kernel<T, C, B, true>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
+        hipLaunchKernelGGL((topblock_local_kernel<T, C, B>), dim3(1, 1, active), dim3(128), 0, s, one, d_tbl, k);
         const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
         lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
         return;
@@ -1977,7 +2398,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
     hipLaunchKernelGGL((tourn_ipiv_kernel<B>), dim3(1, 1, active), dim3(64), 0, s, one, d_tbl, k, level);
     hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, k + NB, k,
                        k + NB, 0, 0);
-    hipLaunchKernelGGL((topblock_kernel<T, C, B, false>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
+    hipLaunchKernelGGL((topblock_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
     const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
     lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
 }
@@ -1986,9 +2407,11 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
 // false, the problem by value) or a batch (B = true, table on the device,
 // problems sorted by decreasing n so that the first `active` entries are
 // the ones with columns left at a given step).
+// hw (single c64 problems, C64_GEMM_H16): the rank-256 outer updates run on
+// the split-fp16 planes.
 template <typename T, bool C, bool B>
 void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int gopts,
-               int pgrid, hipStream_t s, int K_begin = 0, int K_end = 1 << 30) {
+               int pgrid, hipStream_t s, int K_begin = 0, int K_end = 1 << 30, const H16Ws* hw = nullptr) {
     const int nmax = ns[0];
     unsigned pbase = 0;
     auto count_active = [&](int from) {
@@ -2012,7 +2435,8 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
             hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((W2 + 255) / 256, 1, active), dim3(256), 0, s, one,
                                d_tbl, k, k + NB, K0, K0 + W2, k, k + NB);
             const int slot = (k - K0) / NB;
-            hipLaunchKernelGGL((linv_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, slot);
+            if (pivoting != 2)  // block-local pivoting: the panel top wrote S_L
+                hipLaunchKernelGGL((linv_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, slot);
             const int Nin = max_over([&](int n) { return std::min(n, K0 + W2) - std::min(n, k + NB); });
             const int Min = max_over([&](int n) { return n - std::min(n, k + NB); });
             lu_gemm<T, C, B>(one, d_tbl, LuStep{0, K0, k, slot}, NB, Nin, active, gopts, s);
@@ -2029,7 +2453,12 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
                 if (b > 0) lu_gemm<T, C, B>(one, d_tbl, LuStep{2, K0, K0, b}, NB, Nout, act_outer, gopts, s);
                 lu_gemm<T, C, B>(one, d_tbl, LuStep{3, K0, K0, b}, NB, Nout, act_outer, gopts, s);
             }
-            lu_gemm<T, C, B>(one, d_tbl, LuStep{4, K0, K0, 0}, Nout, Nout, act_outer, gopts, s);
+            if (!B && hw) {
+                h16_split(one, K0, *hw, s);
+                h16_update(one, K0, std::min(nmax, K0 + W2), nmax, *hw, s);
+            } else {
+                lu_gemm<T, C, B>(one, d_tbl, LuStep{4, K0, K0, 0}, Nout, Nout, act_outer, gopts, s);
+            }
         }
     }
 }
@@ -2045,7 +2474,7 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
 //   sp: wait eg; panels of block j+1 ...
 template <typename T, bool C>
 void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hipStream_t s, hipStream_t sp,
-                         hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0) {
+                         hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0, const H16Ws* hw = nullptr) {
     const int n = P.n;
     const std::vector<int> ns1{n};
     unsigned pbase = 0;
@@ -2060,7 +2489,8 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
             hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((W2 + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
                                k + NB, K0, K0 + W2, k, k + NB);
             const int slot = (k - K0) / NB;
-            hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
+            if (pivoting != 2)  // block-local pivoting: the panel top wrote S_L
+                hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
             lu_gemm<T, C, false>(P, nullptr, LuStep{0, K0, k, slot}, NB, K1 - kend, 1, gopts, sp);
             lu_gemm<T, C, false>(P, nullptr, LuStep{1, K0, k, slot}, n - kend, K1 - kend, 1, gopts, sp);
         }
@@ -2079,13 +2509,19 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
             LuStep st4{4, K0, K0, 0};
             st4.c0 = K1;
             st4.c1 = c_next;
-            lu_gemm<T, C, false>(P, nullptr, st4, Nout, c_next - K1, 1, gopts, sg);
+            if (hw) {
+                h16_split(P, K0, *hw, sg);
+                h16_update(P, K0, K1, c_next, *hw, sg);
+            } else {
+                lu_gemm<T, C, false>(P, nullptr, st4, Nout, c_next - K1, 1, gopts, sg);
+            }
             hipEventRecord(eg, sg);
             hipStreamWaitEvent(sp, eg, 0);
             if (c_next < n) {
                 st4.c0 = c_next;
                 st4.c1 = n;
-                lu_gemm<T, C, false>(P, nullptr, st4, Nout, n - c_next, 1, gopts, sg);
+                if (hw) h16_update(P, K0, c_next, n, *hw, sg);
+                else lu_gemm<T, C, false>(P, nullptr, st4, Nout, n - c_next, 1, gopts, sg);
             }
         }
     }
@@ -2266,11 +2702,27 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     // workgroups of the persistent panel: as many as stay resident on the
     // compute units its stream may use
     int pgrid = ctx->lu_panel ? panel_grid_resident<T, C>(ctx, ctx->num_cu) : 0;
+    // split-fp16 rank-256 updates (c64, an option)
+    H16Ws hws{};
+    const H16Ws* hw = nullptr;
+    if (std::is_same<T, float>::value && C && (ctx->c64_gemm_parts & C64_GEMM_H16) && n > W2) {
+        const size_t need = h16_ws_bytes(n);
+        if (ctx->lu_h16_bytes < need) {
+            if (ctx->lu_h16) hipFree(ctx->lu_h16);
+            ctx->lu_h16 = nullptr;
+            ctx->lu_h16_bytes = 0;
+            rc = check_hip(hipMalloc(&ctx->lu_h16, need), "LU fp16 planes");
+            if (rc) return rc;
+            ctx->lu_h16_bytes = need;
+        }
+        hws = h16_carve((char*)ctx->lu_h16, n);
+        hw = &hws;
+    }
     if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2 && ctx->lu_pivoting == 2) {
         rc = ensure_lu_streams_free(ctx);
         if (rc) return rc;
         lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), 0, s, ctx->lu_stream_free[0],
-                                  ctx->lu_stream_free[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
+                                  ctx->lu_stream_free[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2], hw);
     } else if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
         rc = ensure_lu_streams(ctx);
         if (rc) return rc;
@@ -2278,10 +2730,11 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
             pgrid = panel_grid_resident<T, C>(ctx, std::min(ctx->lu_lookahead_cus, ctx->num_cu - 1));
         if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
         lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s, ctx->lu_stream[0],
-                                  ctx->lu_stream[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2]);
+                                  ctx->lu_stream[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2], hw);
     } else {
         if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
-        lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s);
+        lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s, 0,
+                               1 << 30, hw);
     }
     rc = check_hip(hipGetLastError(), "LU launch");
     if (rc) return rc;
