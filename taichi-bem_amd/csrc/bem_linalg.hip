@@ -869,120 +869,157 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
     }
 }
 
-// Block-local pivoting panel top in one workgroup of two waves, with no
-// workgroup barrier per column:
-//  * wave 0 factors the 64 x 64 diagonal block in LDS: per column a wave
-//    argmax of |re|+|im| over the rows at or below the diagonal (first row on
-//    ties; ipiv as LAPACK's), the interchange (lane c swaps column c of the
-//    two rows), the multipliers, and the rank-1 update with lane i owning
-//    row i; one wave's LDS accesses run in order, so it needs no barrier;
-//  * the factored block goes to the matrix, then wave 0 forms SU =
-//    -inv(U11) (lane c owns column c, back substitution row by row) while
-//    wave 1 forms S_L = I - inv(L11) (forward substitution), both in place
-//    in the LDS copy: step i of a column reads row i of the factor and the
-//    column's entries already replaced, then replaces entry i;
+// Block-local pivoting panel top in one workgroup of eight waves, the block
+// held in registers (wave q: columns 8q .. 8q + 7 of every row, lane r:
+// row r), rows never moved (each lane tracks its row's position):
+//  * column j: its owner wave finds the pivot (largest |re|+|im| among the
+//    rows not yet pivoted, first position on ties; ipiv as LAPACK's),
+//    forms the multipliers and publishes them with the pivot lane in LDS;
+//    after one workgroup barrier every wave updates its columns right of j
+//    with the pivot row's values read from the pivot lane (readlane);
+//  * the factored block is written in pivoted order (LDS, then the matrix);
+//  * waves 0-3 form SU = -inv(U11) and waves 4-7 S_L = I - inv(L11), 16
+//    columns each, by Gauss-Jordan elimination on the identity held in
+//    registers (row i of the inverse read from lane i), the factor's
+//    entries read from LDS: no barrier per row;
 //  * S_L goes to slot (k mod 256) / 64 of the outer block, so the separate
 //    linv_kernel launch is not needed for this pivoting.
 // The block past w (the matrix's last partial block) is the identity.
+template <typename T>
+__device__ __forceinline__ cx<T> readlane_cx(cx<T> v, int l) {
+    if constexpr (sizeof(T) == 8) {
+        const double re = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.re), l),
+                                           __builtin_amdgcn_readlane(__double2loint(v.re), l));
+        const double im = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.im), l),
+                                           __builtin_amdgcn_readlane(__double2loint(v.im), l));
+        return {re, im};
+    } else {
+        return {__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.re), l)),
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.im), l))};
+    }
+}
+
 template <typename T, bool C, bool B>
-__global__ __launch_bounds__(128) void topblock_local_kernel(LuProb inl, const LuProb* tbl, int k) {
+__global__ __launch_bounds__(512) void topblock_local_kernel(LuProb inl, const LuProb* tbl, int k) {
     const LuProb P = get_prob<B>(inl, tbl);
     const int n = P.n;
     if (k >= n) return;
     const int kend = min(n, k + NB), w = kend - k;
     T* A = (T*)P.A;
     const int64_t ld = P.ld;
-    __shared__ cx<T> M[NB][NB + 1];
-    __shared__ cx<T> U[NB];  // the pivot row of the current column
+    // the block in LDS, entry (r, c) at column (c + r) mod 64 of row r: rows
+    // and columns both read without bank conflicts, no padding, so that with
+    // the multipliers in the same space the kernel needs 32 KB (complex64)
+    // and fits beside two workgroups of the trailing-update GEMM
+    __shared__ cx<T> M[NB][NB];
+    cx<T>* const Lc = &M[0][0];           // [2][NB] multipliers of column j, by parity of j (factorisation only)
+    int* const Pv = (int*)&M[2][0];       // [2] pivot lane of column j (factorisation only)
+    auto at = [&](int r, int c) -> cx<T>& { return M[r][(c + r) & (NB - 1)]; };
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int r = wave; r < NB; r += 2)
-        M[r][lane] = (r < w && lane < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + lane)
-                                         : cx<T>{T(r == lane ? 1 : 0), T(0)};
+    for (int r = wave; r < NB; r += 8)
+        at(r, lane) = (r < w && lane < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + lane)
+                                          : cx<T>{T(r == lane ? 1 : 0), T(0)};
     __syncthreads();
-    if (wave == 0) {
-        for (int j = 0; j < NB; ++j) {
-            double b = lane >= j ? fabs((double)M[lane][j].re) + fabs((double)M[lane][j].im) : -1.0;
-            int bi = lane;
-            wave_best(b, bi);
-            const int p = bi;
-            if (lane == 0 && j < w) P.ipiv[k + j] = k + p;
-            const cx<T> xj = M[j][lane], xp = M[p][lane];
-            M[p][lane] = xj;
-            M[j][lane] = xp;
-            U[lane] = xp;
-            wave_lds_order();
-            const cx<T> d = U[j];
-            const bool zero = d.re == T(0) && d.im == T(0);
-            if (zero && lane == 0 && *P.info == 0) *P.info = k + j + 1;
-            if (lane > j) {
-                cx<T> l = M[lane][j];
-                if (!zero) {
-                    l = cdiv(l, d);
-                    M[lane][j] = l;
+    cx<T> v[8];
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) v[cc] = at(lane, 8 * wave + cc);
+    __syncthreads();  // the block is in registers: its LDS space now holds the multipliers
+    int pos = lane;
+    bool done = false;
+    for (int jq = 0; jq < 8; ++jq) {
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = 8 * jq + jj, par = j & 1;
+            if (wave == jq) {
+                double b = done ? -1.0 : fabs((double)v[jj].re) + fabs((double)v[jj].im);
+                int bi = pos;
+                wave_best(b, bi);
+                const int p = __builtin_ffsll(__ballot(pos == bi)) - 1;
+                const cx<T> d = readlane_cx(v[jj], p);
+                const bool zero = d.re == T(0) && d.im == T(0);
+                const bool upd = !done && lane != p;
+                cx<T> l = {T(0), T(0)};
+                if (upd) {
+                    l = zero ? v[jj] : cdiv(v[jj], d);
+                    v[jj] = l;
                 }
-                // the pivot row is read from its own array, so the compiler
-                // may keep eight read-modify-writes of this row in flight
-#pragma unroll 8
-                for (int c = j + 1; c < NB; ++c) {
-                    const cx<T> u = U[c];
-                    cx<T> m = M[lane][c];
-                    m.re -= l.re * u.re - l.im * u.im;
-                    m.im -= l.re * u.im + l.im * u.re;
-                    M[lane][c] = m;
+                Lc[par * NB + lane] = l;
+                if (lane == 0) {
+                    Pv[par] = p;
+                    if (j < w) P.ipiv[k + j] = k + bi;
+                    if (zero && *P.info == 0) *P.info = k + j + 1;
                 }
             }
-            wave_lds_order();
+            __syncthreads();
+            const int p = Pv[par];
+            const cx<T> l = Lc[par * NB + lane];
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc)
+                if (8 * wave + cc > j) {
+                    const cx<T> u = readlane_cx(v[cc], p);
+                    v[cc].re -= l.re * u.re - l.im * u.im;
+                    v[cc].im -= l.re * u.im + l.im * u.re;
+                }
+            const int ppos = __builtin_amdgcn_readlane(pos, p);
+            if (pos == j) pos = ppos;
+            if (lane == p) {
+                pos = j;
+                done = true;
+            }
         }
     }
+    __syncthreads();  // every wave has read the last multipliers
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) at(pos, 8 * wave + cc) = v[cc];
     __syncthreads();
-    for (int r = wave; r < w; r += 2)
-        if (lane < w) Acc<T, C>::st(A, (int64_t)(k + r) * ld + k + lane, M[r][lane]);
-    __syncthreads();
-    const int c = lane;
-    if (wave == 0) {
-        // X = inv(U11): X[i][c] = (delta_ic - sum_{i < j <= c} U[i][j] X[j][c]) / U[i][i]
+    for (int r = wave; r < w; r += 8)
+        if (lane < w) Acc<T, C>::st(A, (int64_t)(k + r) * ld + k + lane, at(r, lane));
+    // inverses: lane = row (in pivoted order), 16 columns per wave
+    const int c0 = 16 * (wave & 3);
+    cx<T> x[16];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) x[cc] = cx<T>{T(lane == c0 + cc ? 1 : 0), T(0)};
+    if (wave < 4) {
+        // X = inv(U11): for i = 63 .. 0, row i /= U[i][i], then rows above
+        // subtract U[r][i] times row i (only columns c >= i are nonzero)
         for (int i = NB - 1; i >= 0; --i) {
-            const cx<T> dii = M[i][i];
-            cx<T> s = {T(i == c ? 1 : 0), T(0)};
-            if (i <= c) {
-#pragma unroll 4
-                for (int j = i + 1; j <= c; ++j) {
-                    const cx<T> u = M[i][j], x = M[j][c];
-                    s.re -= u.re * x.re - u.im * x.im;
-                    s.im -= u.re * x.im + u.im * x.re;
+            const cx<T> d = at(i, i), m = at(lane, i);
+            const bool zero = d.re == T(0) && d.im == T(0);
+            const cx<T> rd = zero ? cx<T>{T(1), T(0)} : crecip(d);
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc)
+                if (c0 + cc >= i) {
+                    const cx<T> xi = cmul(readlane_cx(x[cc], i), rd);
+                    if (lane == i) x[cc] = xi;
+                    else if (lane < i) {
+                        x[cc].re -= m.re * xi.re - m.im * xi.im;
+                        x[cc].im -= m.re * xi.im + m.im * xi.re;
+                    }
                 }
-            }
-            wave_lds_order();
-            if (i <= c) M[i][c] = (dii.re == T(0) && dii.im == T(0)) ? s : cdiv(s, dii);
-            wave_lds_order();
         }
     } else {
-        // X = inv(L11), unit lower: X[i][c] = -sum_{c <= j < i} L[i][j] X[j][c], X[c][c] = 1
-        for (int i = 1; i < NB; ++i) {
-            cx<T> s = {T(0), T(0)};
-            if (i > c) {
-                const cx<T> l0 = M[i][c];
-                s = {-l0.re, -l0.im};
-#pragma unroll 4
-                for (int j = c + 1; j < i; ++j) {
-                    const cx<T> l = M[i][j], x = M[j][c];
-                    s.re -= l.re * x.re - l.im * x.im;
-                    s.im -= l.re * x.im + l.im * x.re;
+        // X = inv(L11), unit lower: for i = 0 .. 63, rows below subtract
+        // L[r][i] times row i (only columns c <= i are nonzero)
+        for (int i = 0; i < NB; ++i) {
+            const cx<T> m = at(lane, i);
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc)
+                if (c0 + cc <= i) {
+                    const cx<T> xi = readlane_cx(x[cc], i);
+                    if (lane > i) {
+                        x[cc].re -= m.re * xi.re - m.im * xi.im;
+                        x[cc].im -= m.re * xi.im + m.im * xi.re;
+                    }
                 }
-            }
-            wave_lds_order();
-            if (i > c) M[i][c] = s;
-            wave_lds_order();
         }
     }
-    __syncthreads();
-    T* SU = (T*)P.SU;
-    T* SL = (T*)P.SL + (C ? 2 : 1) * (int64_t)((k % W2) / NB) * NB * NB;
-    for (int e = t; e < NB * NB; e += 128) {
-        const int i = e / NB, cc = e % NB;
-        const cx<T> x = M[i][cc];
-        Acc<T, C>::st(SU, e, i <= cc ? cx<T>{-x.re, -x.im} : cx<T>{T(0), T(0)});
-        Acc<T, C>::st(SL, e, i > cc ? cx<T>{-x.re, -x.im} : cx<T>{T(0), T(0)});
+    // SU = -inv(U11) (zero below the diagonal), S_L = I - inv(L11) (zero on and above it)
+    T* S = wave < 4 ? (T*)P.SU : (T*)P.SL + (C ? 2 : 1) * (int64_t)((k % W2) / NB) * NB * NB;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) {
+        const int c = c0 + cc;
+        const bool keep = wave < 4 ? lane <= c : lane > c;
+        Acc<T, C>::st(S, lane * NB + c, keep ? cx<T>{-x[cc].re, -x[cc].im} : cx<T>{T(0), T(0)});
     }
 }
 
@@ -2368,7 +2405,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
     }
     if (pivoting == 2) {
         // block-local pivoting: the top block in one workgroup, L21 on MFMA
-        hipLaunchKernelGGL((topblock_local_kernel<T, C, B>), dim3(1, 1, active), dim3(128), 0, s, one, d_tbl, k);
+        hipLaunchKernelGGL((topblock_local_kernel<T, C, B>), dim3(1, 1, active), dim3(512), 0, s, one, d_tbl, k);
         const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
         lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
         return;
