@@ -259,12 +259,8 @@ enum {
                                 split into two fp16 parts, three part
                                 products per real product, FP32 accumulation
                                 (update error about 3e-7 of sum |a||b|; for
-                                factors that precondition FP64 refinement);
-                                6 = bf16 MFMA on operands split in three bf16
-                                parts, six part products (about FP32
-                                accuracy); 3 = two parts, three products
-                                (about 16 bits, for refinement of
-                                well-conditioned systems) */,
+                                factors that precondition FP64 refinement).
+                                Batched factorisations keep the FP32 MFMA */,
     BEM_OPT_LU_LOOKAHEAD = 2 /* bem_lu_factor outside graph capture: 0 = one
                                 stream; N > 0 (default 32) = the panels of the
                                 next 256-column block run on N compute units

@@ -358,8 +358,7 @@ class Context:
                                    int(bool(conj_x)), y.data_ptr(), float(alpha), current_stream()))
 
     def set_option(self, option: int, value: int) -> None:
-        """bem_set_option (OPT_C64_LU_GEMM: 0 FP32 MFMA (default), 2 split-fp16 planes on the fp16 MFMA,
-        6 split-bf16 x6, 3 split-bf16 x3)."""
+        """bem_set_option (OPT_C64_LU_GEMM: 0 FP32 MFMA (default), 2 split-fp16 planes on the fp16 MFMA)."""
         check(lib().bem_set_option(self._h, int(option), int(value)))
 
     def lu_factor(self, A, ipiv) -> None:

@@ -236,9 +236,8 @@ int bem_set_option(bem_context* ctx, int32_t option, int32_t value) {
             ctx->impl.lu_gemm_tile = value;
             return BEM_OK;
         case BEM_OPT_C64_LU_GEMM:
-            if (value != 0 && value != 2 && value != 3 && value != 6)
-                return set_error(BEM_ERR_INVALID, "BEM_OPT_C64_LU_GEMM takes 0, 2, 3 or 6");
-            ctx->impl.c64_gemm_parts = value == 0 ? 0 : value == 2 ? bem::C64_GEMM_H16 : (value == 3 ? 2 : 3);
+            if (value != 0 && value != 2) return set_error(BEM_ERR_INVALID, "BEM_OPT_C64_LU_GEMM takes 0 or 2");
+            ctx->impl.c64_gemm_h16 = value == 2;
             return BEM_OK;
         case BEM_OPT_WAIT_LIMIT_US:
             if (value < 0 || value > 2000000)

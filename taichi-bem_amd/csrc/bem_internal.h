@@ -103,8 +103,7 @@ struct Context {
     size_t cf_bytes = 0;
     std::vector<char> cf_table;
     int num_cu = 0;
-    int c64_gemm_parts = 0;       // c64 LU trailing updates: 0 FP32 MFMA, 2/3 split-bf16 parts,
-                                  // C64_GEMM_H16 split-fp16 planes
+    int c64_gemm_h16 = 0;         // c64 LU rank-256 updates: 0 FP32 MFMA, 1 split-fp16 planes on the fp16 MFMA
     void* lu_h16 = nullptr;       // split-fp16 planes of the c64 rank-256 updates
     size_t lu_h16_bytes = 0;
     int lu_lookahead_cus = 32;    // LU look-ahead: 0 off, > 0 panel-stream CUs, -1 unmasked streams
@@ -119,9 +118,6 @@ struct Context {
     hipStream_t lu_stream_free[2] = {nullptr, nullptr};  // look-ahead without CU masks (block-local pivoting)
     hipEvent_t lu_event[3] = {nullptr, nullptr, nullptr};
 };
-
-// c64_gemm_parts value of the split-fp16 rank-256 updates (a GEMM option bit)
-constexpr int C64_GEMM_H16 = 64;
 
 // error helpers (bem_capi.hip)
 int set_error(int code, const std::string& msg);

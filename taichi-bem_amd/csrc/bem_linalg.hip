@@ -1860,135 +1860,6 @@ __global__ __launch_bounds__(256) void axpy_kernel(const T* src, int64_t lds, T*
 }
 
 // ---------------------------------------------------------------------------
-// Complex64 C -= A B on the bf16 MFMA (v_mfma_f32_16x16x32_bf16, FP32
-// accumulation) with split operands: every FP32 value is a sum of NP bf16
-// parts (x = h + m [+ l], each part the bf16 rounding of the remainder) and a
-// real product keeps the part products of order >= 2^-16 (NP = 2: hh, hm, mh)
-// or >= 2^-24 (NP = 3: also hl, mm, lh), so the update carries about 16 or 24
-// bits.  Operand layout measured on gfx950: lane l supplies
-// A[l%16][8*(l/16)+i] and B[8*(l/16)+i][l%16] (i = 0..7); result register r
-// holds row 4*(l/16)+r, column l%16.  The parts are formed once per element
-// when a K chunk is staged in LDS (A row-major, B column-major, rows padded to
-// avoid bank conflicts); the complex product uses four real products, the
-// real part accumulated as (re re) and (im im) separately.
-// ---------------------------------------------------------------------------
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-constexpr int BKC = 32;       // K chunk (the MFMA depth)
-constexpr int BKP = BKC + 8;  // padded row length in LDS (80 bytes)
-
-template <int NP>
-__device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[NP]) {
-    const __bf16 h = (__bf16)x;
-    p[0] = h;
-    float r = x - (float)h;
-    if constexpr (NP >= 2) {
-        const __bf16 m = (__bf16)r;
-        p[1] = m;
-        r -= (float)m;
-    }
-    if constexpr (NP >= 3) p[2] = (__bf16)r;
-}
-
-template <int NP, bool B>
-__global__ __launch_bounds__(256) void gemm_sub_c64bf_kernel(GemmArgs<float> inl, const LuProb* tbl, LuStep st) {
-    GemmArgs<float> ga = inl;
-    if constexpr (B) ga = lu_gemm_args<float, true>(tbl[blockIdx.z], st);
-    const int M = ga.M, N = ga.N, K = ga.K;
-    if (M <= 0 || N <= 0 || (int)blockIdx.y * GT >= M || (int)blockIdx.x * GT >= N) return;
-    __shared__ __bf16 sA[2][NP][GT][BKP];
-    __shared__ __bf16 sB[2][NP][GT][BKP];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int bm = blockIdx.y * GT, bn = blockIdx.x * GT;
-    const int wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
-    f32x4_t accR[2][2], accII[2][2], accI[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            accR[a][b] = f32x4_t{0, 0, 0, 0};
-            accII[a][b] = f32x4_t{0, 0, 0, 0};
-            accI[a][b] = f32x4_t{0, 0, 0, 0};
-        }
-    const float* Am = ga.Am;
-    const float* Bm = ga.Bm;
-    for (int k0 = 0; k0 < K; k0 += BKC) {
-        // stage A (GT rows x BKC) and B (BKC x GT, stored column-major)
-        for (int e = tid; e < GT * BKC; e += 256) {
-            const int r = e / BKC, c = e % BKC;
-            const int gr = bm + r, gc = k0 + c;
-            float re = 0.f, im = 0.f;
-            if (gr < M && gc < K) {
-                re = Am[2 * ((int64_t)gr * ga.lda + gc)];
-                im = Am[2 * ((int64_t)gr * ga.lda + gc) + 1];
-            }
-            __bf16 pr[NP], pi[NP];
-            split_bf16<NP>(re, pr);
-            split_bf16<NP>(im, pi);
-#pragma unroll
-            for (int q = 0; q < NP; ++q) { sA[0][q][r][c] = pr[q]; sA[1][q][r][c] = pi[q]; }
-        }
-        for (int e = tid; e < BKC * GT; e += 256) {
-            const int r = e / GT, c = e % GT;
-            const int gr = k0 + r, gc = bn + c;
-            float re = 0.f, im = 0.f;
-            if (gr < K && gc < N) {
-                re = Bm[2 * ((int64_t)gr * ga.ldb + gc)];
-                im = Bm[2 * ((int64_t)gr * ga.ldb + gc) + 1];
-            }
-            __bf16 pr[NP], pi[NP];
-            split_bf16<NP>(re, pr);
-            split_bf16<NP>(im, pi);
-#pragma unroll
-            for (int q = 0; q < NP; ++q) { sB[0][q][c][r] = pr[q]; sB[1][q][c][r] = pi[q]; }
-        }
-        __syncthreads();
-        const int kof = 8 * (lane >> 4);
-        bf16x8_t aR[2][NP], aI[2][NP], bR[2][NP], bI[2][NP];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                const int row = wm + 16 * a + (lane & 15);
-                aR[a][q] = *(const bf16x8_t*)&sA[0][q][row][kof];
-                aI[a][q] = *(const bf16x8_t*)&sA[1][q][row][kof];
-                const int col = wn + 16 * a + (lane & 15);
-                bR[a][q] = *(const bf16x8_t*)&sB[0][q][col][kof];
-                bI[a][q] = *(const bf16x8_t*)&sB[1][q][col][kof];
-            }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int pa = 0; pa < NP; ++pa)
-#pragma unroll
-                    for (int pb = 0; pb < NP; ++pb) {
-                        if (pa + pb >= NP) continue;  // part products below 2^-(8 NP)
-                        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aR[a][pa], bR[b][pb], accR[a][b], 0, 0, 0);
-                        accII[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aI[a][pa], bI[b][pb], accII[a][b], 0, 0, 0);
-                        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aR[a][pa], bI[b][pb], accI[a][b], 0, 0, 0);
-                        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aI[a][pa], bR[b][pb], accI[a][b], 0, 0, 0);
-                    }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gr = bm + wm + 16 * a + 4 * (lane >> 4) + r;
-                const int gc = bn + wn + 16 * b + (lane & 15);
-                if (gr < M && gc < N) {
-                    float* c = ga.Cm + 2 * ((int64_t)gr * ga.ldc + gc);
-                    c[0] -= accR[a][b][r] - accII[a][b][r];
-                    c[1] -= accI[a][b][r];
-                }
-            }
-}
-
-// ---------------------------------------------------------------------------
 // c64 rank-256 trailing updates on the fp16 MFMA (v_mfma_f32_32x32x16_f16;
 // measured on gfx950: 1.54 PFLOP/s sustained against 143 TFLOP/s for the
 // FP32 MFMA).  The operands are split once per outer block into fp16 planes:
@@ -2320,16 +2191,15 @@ void h16_update(const LuProb& P, int K0, int c0, int c1, const H16Ws& w, hipStre
                        c0 - K1, tm, tn);
 }
 
-// gopts, the GEMM options of a factorisation: bits 0-3 = split-bf16 parts of
-// c64 trailing updates (0 = FP32 MFMA for every c64 product; 2 or 3 = the
-// split bf16 kernel for modes 1 and 4); bit 4 (GEMM_SMALL_TILES) = 64 x 64
-// tiles for every update; bit 5 (GEMM_WIDE4M) = complex trailing updates on
-// 128 x 128 tiles with four real products per complex product.
+// gopts, the GEMM options of a factorisation: bit 4 (GEMM_SMALL_TILES) = 64 x
+// 64 tiles for every update; bit 5 (GEMM_WIDE4M) = complex trailing updates on
+// 128 x 128 tiles with four real products per complex product.  (The
+// split-fp16 rank-256 updates are chosen by the caller of lu_launch.)
 constexpr int GEMM_SMALL_TILES = 16;
 constexpr int GEMM_WIDE4M = 32;
 
 int gemm_opts(const Context* ctx) {
-    return (ctx->c64_gemm_parts & 15) | (ctx->lu_gemm_tile ? 0 : GEMM_SMALL_TILES) |
+    return (ctx->lu_gemm_tile ? 0 : GEMM_SMALL_TILES) |
            (ctx->lu_gemm_tile == 2 ? GEMM_WIDE4M : 0);
 }
 
@@ -2340,14 +2210,6 @@ void lu_gemm(const LuProb& one, const LuProb* d_tbl, const LuStep& st, int M, in
     GemmArgs<T> g{};
     if constexpr (!B) g = lu_gemm_args<T, C>(one, st);
     const dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, active);
-    const int bf16_parts = gopts & 15;
-    if constexpr (std::is_same<T, float>::value && C) {
-        if (bf16_parts && (st.mode == 1 || st.mode == 4)) {
-            if (bf16_parts == 2) hipLaunchKernelGGL((gemm_sub_c64bf_kernel<2, B>), grid, dim3(256), 0, s, g, d_tbl, st);
-            else hipLaunchKernelGGL((gemm_sub_c64bf_kernel<3, B>), grid, dim3(256), 0, s, g, d_tbl, st);
-            return;
-        }
-    }
     // trailing updates with room for 128-row tiles take the large-tile kernel
     // (64-wide column tiles when the update is narrower than 128 or would
     // leave a half-empty last 128-column tile)
@@ -2444,7 +2306,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
 // false, the problem by value) or a batch (B = true, table on the device,
 // problems sorted by decreasing n so that the first `active` entries are
 // the ones with columns left at a given step).
-// hw (single c64 problems, C64_GEMM_H16): the rank-256 outer updates run on
+// hw (single c64 problems, BEM_OPT_C64_LU_GEMM = 2): the rank-256 outer updates run on
 // the split-fp16 planes.
 template <typename T, bool C, bool B>
 void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int gopts,
@@ -2742,7 +2604,7 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     // split-fp16 rank-256 updates (c64, an option)
     H16Ws hws{};
     const H16Ws* hw = nullptr;
-    if (std::is_same<T, float>::value && C && (ctx->c64_gemm_parts & C64_GEMM_H16) && n > W2) {
+    if (std::is_same<T, float>::value && C && ctx->c64_gemm_h16 && n > W2) {
         const size_t need = h16_ws_bytes(n);
         if (ctx->lu_h16_bytes < need) {
             if (ctx->lu_h16) hipFree(ctx->lu_h16);

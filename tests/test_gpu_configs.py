@@ -177,6 +177,30 @@ def test_config2_mixed_fp64_assembly(bem, l3_dirichlet):
     assert st.converged and e <= 1e-8
 
 
+def test_config2_mixed_as_benched(bem, l3_dirichlet):
+    """The bench's compliant mixed pipeline at L3: c64 LU of the FP64-assembled
+    matrix with block-local pivoting and the split-fp16 trailing updates
+    (BEM_OPT_C64_LU_GEMM = 2), FP64 refinement against the complex128 matrix:
+    the 1e-8 target against the complex128 solve."""
+    L = bem._lib
+    ctx, desc, g, A64, b64, x64 = l3_dirichlet
+    LU32 = torch.empty(A64.shape, dtype=torch.complex64, device=DEV)
+    bem.convert(A64, LU32)
+    ipiv = torch.empty(A64.shape[0], dtype=torch.int32, device=DEV)
+    ctx.set_option(L.OPT_LU_PIVOTING, 2)
+    ctx.set_option(L.OPT_C64_LU_GEMM, 2)
+    try:
+        ctx.lu_factor(LU32, ipiv)
+    finally:
+        ctx.set_option(L.OPT_LU_PIVOTING, 0)
+        ctx.set_option(L.OPT_C64_LU_GEMM, 0)
+    x = torch.zeros_like(x64)
+    st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=1e-14)
+    e = rel(to_np(x), to_np(x64))
+    print(f"[config2 mixed as benched] {st.iterations} steps, rel err {e:.3e}", flush=True)
+    assert st.converged and st.iterations <= 4 and e <= 1e-8
+
+
 def test_config2_mixed_fp32_assembly(bem, l3_dirichlet):
     """FP32 regular pairs + FP64 near field into c64, c64 LU, FP64 residuals
     against that c64 matrix: refinement converges to the solution of the
