@@ -915,7 +915,8 @@ __global__ __launch_bounds__(512) void topblock_local_kernel(LuProb inl, const L
     cx<T>* const Lc = &M[0][0];           // [2][NB] multipliers of column j, by parity of j (factorisation only)
     int* const Pv = (int*)&M[2][0];       // [2] pivot lane of column j (factorisation only)
     auto at = [&](int r, int c) -> cx<T>& { return M[r][(c + r) & (NB - 1)]; };
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // the wave index as a scalar, so that every per-wave choice below is a scalar branch
+    const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
     for (int r = wave; r < NB; r += 8)
         at(r, lane) = (r < w && lane < w) ? Acc<T, C>::ld(A, (int64_t)(k + r) * ld + k + lane)
                                           : cx<T>{T(r == lane ? 1 : 0), T(0)};
@@ -938,9 +939,11 @@ __global__ __launch_bounds__(512) void topblock_local_kernel(LuProb inl, const L
                 const cx<T> d = readlane_cx(v[jj], p);
                 const bool zero = d.re == T(0) && d.im == T(0);
                 const bool upd = !done && lane != p;
+                // the column times the pivot's reciprocal (as LAPACK scales)
+                const cx<T> rd = zero ? cx<T>{T(1), T(0)} : crecip(d);
                 cx<T> l = {T(0), T(0)};
                 if (upd) {
-                    l = zero ? v[jj] : cdiv(v[jj], d);
+                    l = cmul(v[jj], rd);
                     v[jj] = l;
                 }
                 Lc[par * NB + lane] = l;
@@ -986,30 +989,30 @@ __global__ __launch_bounds__(512) void topblock_local_kernel(LuProb inl, const L
             const cx<T> d = at(i, i), m = at(lane, i);
             const bool zero = d.re == T(0) && d.im == T(0);
             const cx<T> rd = zero ? cx<T>{T(1), T(0)} : crecip(d);
+            // rows above i subtract U[r][i] row i; row i takes the scaled
+            // row; rows below keep theirs (a zero multiplier, no branch)
+            const cx<T> mm = lane < i ? m : cx<T>{T(0), T(0)};
 #pragma unroll
             for (int cc = 0; cc < 16; ++cc)
                 if (c0 + cc >= i) {
                     const cx<T> xi = cmul(readlane_cx(x[cc], i), rd);
-                    if (lane == i) x[cc] = xi;
-                    else if (lane < i) {
-                        x[cc].re -= m.re * xi.re - m.im * xi.im;
-                        x[cc].im -= m.re * xi.im + m.im * xi.re;
-                    }
+                    const cx<T> y = {x[cc].re - (mm.re * xi.re - mm.im * xi.im),
+                                     x[cc].im - (mm.re * xi.im + mm.im * xi.re)};
+                    x[cc] = lane == i ? xi : y;
                 }
         }
     } else {
         // X = inv(L11), unit lower: for i = 0 .. 63, rows below subtract
         // L[r][i] times row i (only columns c <= i are nonzero)
         for (int i = 0; i < NB; ++i) {
-            const cx<T> m = at(lane, i);
+            const cx<T> m0 = at(lane, i);
+            const cx<T> m = lane > i ? m0 : cx<T>{T(0), T(0)};
 #pragma unroll
             for (int cc = 0; cc < 16; ++cc)
                 if (c0 + cc <= i) {
                     const cx<T> xi = readlane_cx(x[cc], i);
-                    if (lane > i) {
-                        x[cc].re -= m.re * xi.re - m.im * xi.im;
-                        x[cc].im -= m.re * xi.im + m.im * xi.re;
-                    }
+                    x[cc].re -= m.re * xi.re - m.im * xi.im;
+                    x[cc].im -= m.re * xi.im + m.im * xi.re;
                 }
         }
     }
