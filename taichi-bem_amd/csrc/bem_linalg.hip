@@ -1717,18 +1717,28 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(SolveProb inl, const So
         // each solved value reaches the other lanes through scalar registers.
         cx<double> rd = {1.0, 0.0};
         if (!UNIT && lane < nb) rd = cdiv(cx<double>{1.0, 0.0}, cx<double>{dr[lane][lane], C ? di[lane][lane] : 0.0});
-        for (int s = 0; s < nb; ++s) {
-            const int c = LOWER ? s : nb - 1 - s;
-            if (!UNIT && lane == c) {
+        // branch-free steps (selects instead of lane branches), so that the
+        // LDS reads of the diagonal block do not wait for the solved values
+        auto step = [&](int c) {
+            if (!UNIT) {
                 const cx<double> q = cmul(cx<double>{acc_r, acc_i}, rd);
-                acc_r = q.re; acc_i = q.im;
+                acc_r = lane == c ? q.re : acc_r;
+                acc_i = lane == c ? q.im : acc_i;
             }
             const double vr = readlane_f64(acc_r, c), vi = readlane_f64(acc_i, c);
-            if (lane < nb && (LOWER ? lane > c : lane < c)) {
-                const double mr = dr[lane][c], mi = C ? di[lane][c] : 0.0;
-                acc_r -= mr * vr - mi * vi;
-                acc_i -= mr * vi + mi * vr;
-            }
+            const bool act = lane < nb && (LOWER ? lane > c : lane < c);
+            const double mr0 = dr[lane][c], mi0 = C ? di[lane][c] : 0.0;
+            const double mr = act ? mr0 : 0.0, mi = act ? mi0 : 0.0;
+            acc_r -= mr * vr - mi * vi;
+            acc_i -= mr * vi + mi * vr;
+        };
+        if (nb == TB) {
+            // a full block: unrolled, so that the diagonal block's LDS reads
+            // (which do not depend on the solved values) are issued ahead
+#pragma unroll
+            for (int s = 0; s < TB; ++s) step(LOWER ? s : TB - 1 - s);
+        } else {
+            for (int s = 0; s < nb; ++s) step(LOWER ? s : nb - 1 - s);
         }
         if (lane < nb) { x[2 * (r0 + lane)] = acc_r; x[2 * (r0 + lane) + 1] = acc_i; }
         __threadfence();
