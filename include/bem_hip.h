@@ -253,7 +253,7 @@ int bem_lu_solve(bem_context* ctx, const void* LU, int64_t ld, int32_t n, int32_
 enum {
     BEM_OPT_C64_LU_GEMM = 1  /* arithmetic of the trailing updates of complex64
                                 LU factorisations: 0 (default) = FP32 MFMA;
-                                2 = the rank-256 updates of single
+                                2 = the rank-512 updates of single
                                 factorisations on the fp16 MFMA, each operand
                                 row (column) scaled by a power of two and
                                 split into two fp16 parts, three part
@@ -263,7 +263,8 @@ enum {
                                 Batched factorisations keep the FP32 MFMA */,
     BEM_OPT_LU_LOOKAHEAD = 2 /* bem_lu_factor outside graph capture: 0 = one
                                 stream; N > 0 (default 32) = the panels of the
-                                next 256-column block run on N compute units
+                                next outer block (256 columns, 512 for FP32
+                                and complex64 factors) run on N compute units
                                 while the rest of the trailing update runs on
                                 the others (two internal streams, default
                                 priority, ordered by events against the

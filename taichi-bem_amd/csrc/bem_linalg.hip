@@ -37,7 +37,13 @@ bool dtype_complex(int dtype) { return dtype == BEM_C128 || dtype == BEM_C64; }
 namespace {
 
 constexpr int NB = 64;   // panel width
-constexpr int W2 = 256;  // outer block width (NB-column panels inside, one rank-W2 trailing update per block)
+// Outer block width: NB-column panels inside, one rank-W trailing update per
+// block.  512 columns for FP32 / complex64 factors, 256 for FP64 / complex128
+// (measured at n = 20,480: the c64 factorisation with split-fp16 updates 160
+// -> 146 ms at 512, whose rank-512 updates read and write C half as often;
+// the c128 one 471 -> 474 ms, and at n = 5120 slower, so it keeps 256).
+template <typename T>
+__host__ __device__ constexpr int outer_w() { return sizeof(T) == 4 ? 512 : 256; }
 
 template <typename T>
 struct cx {
@@ -882,7 +888,7 @@ __global__ __launch_bounds__(256) void topblock_kernel(LuProb inl, const LuProb*
 //    columns each, by Gauss-Jordan elimination on the identity held in
 //    registers (row i of the inverse read from lane i), the factor's
 //    entries read from LDS: no barrier per row;
-//  * S_L goes to slot (k mod 256) / 64 of the outer block, so the separate
+//  * S_L goes to slot (k mod W) / 64 of the outer block, so the separate
 //    linv_kernel launch is not needed for this pivoting.
 // The block past w (the matrix's last partial block) is the identity.
 template <typename T>
@@ -1017,7 +1023,7 @@ __global__ __launch_bounds__(512) void topblock_local_kernel(LuProb inl, const L
         }
     }
     // SU = -inv(U11) (zero below the diagonal), S_L = I - inv(L11) (zero on and above it)
-    T* S = wave < 4 ? (T*)P.SU : (T*)P.SL + (C ? 2 : 1) * (int64_t)((k % W2) / NB) * NB * NB;
+    T* S = wave < 4 ? (T*)P.SU : (T*)P.SL + (C ? 2 : 1) * (int64_t)((k % outer_w<T>()) / NB) * NB * NB;
 #pragma unroll
     for (int cc = 0; cc < 16; ++cc) {
         const int c = c0 + cc;
@@ -1129,7 +1135,7 @@ struct GemmArgs {
 
 // The LU products of one step for problem P (bounds clamped to its n).  The
 // factorisation is blocked twice: inner panels of NB columns inside outer
-// blocks [K0, K1) of W2 columns.
+// blocks [K0, K1) of outer_w<T>() columns.
 //   mode 0  inner U12:      A[k:kend, kend:K1] -= S_L A[k:kend, kend:K1]   (in place)
 //   mode 1  inner trailing: A[kend:n, kend:K1] -= A[kend:n, k:kend] A[k:kend, kend:K1]
 //   mode 2  outer U12, block b, elimination of the solved blocks above it:
@@ -1153,7 +1159,7 @@ __host__ __device__ inline GemmArgs<T> lu_gemm_args(const LuProb& P, const LuSte
     T* A = (T*)P.A;
     const int64_t ld = P.ld;
     const int n = P.n;
-    const int K1 = n < st.K0 + W2 ? n : st.K0 + W2;
+    const int K1 = n < st.K0 + outer_w<T>() ? n : st.K0 + outer_w<T>();
     auto at = [&](int r, int c) { return A + E * ((int64_t)r * ld + c); };
     GemmArgs<T> g{A, ld, A, ld, A, ld, 0, 0, 0};
     if (st.K0 >= n) return g;
@@ -1873,7 +1879,7 @@ __global__ __launch_bounds__(256) void axpy_kernel(const T* src, int64_t lds, T*
 }
 
 // ---------------------------------------------------------------------------
-// c64 rank-256 trailing updates on the fp16 MFMA (v_mfma_f32_32x32x16_f16;
+// c64 rank-512 trailing updates on the fp16 MFMA (v_mfma_f32_32x32x16_f16;
 // measured on gfx950: 1.54 PFLOP/s sustained against 143 TFLOP/s for the
 // FP32 MFMA).  The operands are split once per outer block into fp16 planes:
 //  * each row of the left factor A (each column of the right factor B) is
@@ -1898,7 +1904,7 @@ typedef _Float16 h16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 h16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 constexpr int HS = 16;          // K depth of one stage (the MFMA's)
-constexpr int HNS = W2 / HS;    // stages of a full outer block
+constexpr int HNS = outer_w<float>() / HS;  // stages of a full outer block
 constexpr int HT = 128;         // C tile (rows and columns)
 
 // Split planes of one factorisation (device): the outer block's left-factor
@@ -1915,12 +1921,12 @@ __host__ __device__ inline int64_t hoff(int p, int s, int r, int cap) {
     return ((int64_t)(p * HNS + s) * cap + r) * HS;
 }
 
-size_t h16_ws_bytes(int n) { return 2 * (4 * (size_t)n * W2 * sizeof(_Float16) + (size_t)n * sizeof(int) + 512); }
+size_t h16_ws_bytes(int n) { return 2 * (4 * (size_t)n * outer_w<float>() * sizeof(_Float16) + (size_t)n * sizeof(int) + 512); }
 
 H16Ws h16_carve(char* base, int n) {
     H16Ws w;
     w.cap = n;
-    const size_t planes = 4 * (size_t)n * W2 * sizeof(_Float16);
+    const size_t planes = 4 * (size_t)n * outer_w<float>() * sizeof(_Float16);
     w.a = (_Float16*)base;
     w.ea = (int*)(base + planes);
     char* nb = base + ((planes + (size_t)n * sizeof(int) + 255) & ~(size_t)255) + 256;
@@ -1943,18 +1949,19 @@ __device__ __forceinline__ void h16_split2(float x, int e, _Float16& h, _Float16
     l = (_Float16)(v - (float)h);
 }
 
-// Rows of the left factor (M x K, K <= 256, of a row-major c64 matrix) into
-// the planes, one wave per row (lane l: k = 4 l .. 4 l + 3).
+// Rows of the left factor (M x K, K <= 512, of a row-major c64 matrix) into
+// the planes, one wave per row (lane l: k = 8 l .. 8 l + 7).
 __global__ __launch_bounds__(256) void h16_split_rows_kernel(const float* A, int64_t lda, int M, int K, H16Ws w) {
+    constexpr int KPL = outer_w<float>() / 64;  // consecutive k per lane
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= M) return;
     const float* a = A + 2 * (int64_t)row * lda;
     const int ns = (K + HS - 1) / HS;
-    float re[4], im[4], m = 0.f;
+    float re[KPL], im[KPL], m = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int k = 4 * lane + i;
+    for (int i = 0; i < KPL; ++i) {
+        const int k = KPL * lane + i;
         re[i] = 0.f;
         im[i] = 0.f;
         if (k < K) {
@@ -1967,24 +1974,22 @@ __global__ __launch_bounds__(256) void h16_split_rows_kernel(const float* A, int
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
     const int e = h16_exponent(m);
     if (lane == 0) w.ea[row] = e;
-    const int s = lane >> 2;
+    const int s = (KPL * lane) / HS;
     if (s >= ns) return;
-    h16x4_t v[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < KPL; ++i) {
         _Float16 h, l;
+        const int k = KPL * lane + i;
         h16_split2(re[i], e, h, l);
-        v[0][i] = h;
-        v[1][i] = l;
+        w.a[hoff(0, s, row, w.cap) + k % HS] = h;
+        w.a[hoff(1, s, row, w.cap) + k % HS] = l;
         h16_split2(im[i], e, h, l);
-        v[2][i] = h;
-        v[3][i] = l;
+        w.a[hoff(2, s, row, w.cap) + k % HS] = h;
+        w.a[hoff(3, s, row, w.cap) + k % HS] = l;
     }
-#pragma unroll
-    for (int p = 0; p < 4; ++p) *(h16x4_t*)(w.a + hoff(p, s, row, w.cap) + 4 * (lane & 3)) = v[p];
 }
 
-// Columns of the right factor (K x N, K <= 256, row-major c64) into the
+// Columns of the right factor (K x N, K <= 512, row-major c64) into the
 // planes, 64 columns per workgroup: the column maxima, then one stage of a
 // column per thread.
 __global__ __launch_bounds__(256) void h16_split_cols_kernel(const float* Bm, int64_t ldb, int K, int N, H16Ws w) {
@@ -2185,7 +2190,7 @@ __global__ __launch_bounds__(256, 2) void gemm_h16_kernel(float* Cm, int64_t ldc
 // The outer block's left factor A[K1:n, K0:K1] and right factor
 // A[K0:K1, K1:n] into the planes (once per outer block, after its U12).
 void h16_split(const LuProb& P, int K0, const H16Ws& w, hipStream_t s) {
-    const int n = P.n, K1 = std::min(n, K0 + W2), M = n - K1;
+    const int n = P.n, K1 = std::min(n, K0 + outer_w<float>()), M = n - K1;
     if (M <= 0) return;
     float* A = (float*)P.A;
     hipLaunchKernelGGL(h16_split_rows_kernel, dim3((M + 3) / 4), dim3(256), 0, s,
@@ -2196,7 +2201,7 @@ void h16_split(const LuProb& P, int K0, const H16Ws& w, hipStream_t s) {
 
 // A[K1:n, c0:c1] -= A[K1:n, K0:K1] A[K0:K1, c0:c1] from the planes of h16_split.
 void h16_update(const LuProb& P, int K0, int c0, int c1, const H16Ws& w, hipStream_t s) {
-    const int n = P.n, K1 = std::min(n, K0 + W2), M = n - K1, N = c1 - c0;
+    const int n = P.n, K1 = std::min(n, K0 + outer_w<float>()), M = n - K1, N = c1 - c0;
     if (M <= 0 || N <= 0) return;
     float* C = (float*)P.A + 2 * ((int64_t)K1 * P.ld + c0);
     const int tm = (M + HT - 1) / HT, tn = (N + HT - 1) / HT;
@@ -2207,7 +2212,7 @@ void h16_update(const LuProb& P, int K0, int c0, int c1, const H16Ws& w, hipStre
 // gopts, the GEMM options of a factorisation: bit 4 (GEMM_SMALL_TILES) = 64 x
 // 64 tiles for every update; bit 5 (GEMM_WIDE4M) = complex trailing updates on
 // 128 x 128 tiles with four real products per complex product.  (The
-// split-fp16 rank-256 updates are chosen by the caller of lu_launch.)
+// split-fp16 rank-512 updates are chosen by the caller of lu_launch.)
 constexpr int GEMM_SMALL_TILES = 16;
 constexpr int GEMM_WIDE4M = 32;
 
@@ -2282,7 +2287,7 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
         // block-local pivoting: the top block in one workgroup, L21 on MFMA
         hipLaunchKernelGGL((topblock_local_kernel<T, C, B>), dim3(1, 1, active), dim3(512), 0, s, one, d_tbl, k);
         const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
-        lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
+        lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % outer_w<T>(), k, 0}, M, NB, active, gopts, s);
         return;
     }
     if (pivoting == 0) {
@@ -2312,14 +2317,14 @@ void panel_factor(const LuProb& one, const LuProb* d_tbl, const std::vector<int>
                        k + NB, 0, 0);
     hipLaunchKernelGGL((topblock_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k);
     const int M = max_over([&](int n) { return n - std::min(n, k + NB); });
-    lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % W2, k, 0}, M, NB, active, gopts, s);
+    lu_gemm<T, C, B>(one, d_tbl, LuStep{5, k - k % outer_w<T>(), k, 0}, M, NB, active, gopts, s);
 }
 
 // Launch sequence of the two-level right-looking LU for one problem (B =
 // false, the problem by value) or a batch (B = true, table on the device,
 // problems sorted by decreasing n so that the first `active` entries are
 // the ones with columns left at a given step).
-// hw (single c64 problems, BEM_OPT_C64_LU_GEMM = 2): the rank-256 outer updates run on
+// hw (single c64 problems, BEM_OPT_C64_LU_GEMM = 2): the rank-512 outer updates run on
 // the split-fp16 planes.
 template <typename T, bool C, bool B>
 void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& ns, int pivoting, int gopts,
@@ -2331,9 +2336,9 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
         while (a < (int)ns.size() && ns[a] > from) ++a;
         return a;
     };
-    // outer blocks [K_begin, min(nmax, K_end)), K_begin a multiple of W2
-    for (int K0 = K_begin; K0 < std::min(nmax, K_end); K0 += W2) {
-        const int K1max = std::min(nmax, K0 + W2);
+    // outer blocks [K_begin, min(nmax, K_end)), K_begin a multiple of outer_w<T>()
+    for (int K0 = K_begin; K0 < std::min(nmax, K_end); K0 += outer_w<T>()) {
+        const int K1max = std::min(nmax, K0 + outer_w<T>());
         const int act_outer = count_active(K0);
         for (int k = K0; k < K1max; k += NB) {
             const int active = count_active(k);
@@ -2344,21 +2349,21 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
             };
             panel_factor<T, C, B>(one, d_tbl, ns, k, active, pivoting, gopts, pgrid, &pbase, s);
             // this panel's interchanges inside the outer block
-            hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((W2 + 255) / 256, 1, active), dim3(256), 0, s, one,
-                               d_tbl, k, k + NB, K0, K0 + W2, k, k + NB);
+            hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((outer_w<T>() + 255) / 256, 1, active), dim3(256), 0, s, one,
+                               d_tbl, k, k + NB, K0, K0 + outer_w<T>(), k, k + NB);
             const int slot = (k - K0) / NB;
             if (pivoting != 2)  // block-local pivoting: the panel top wrote S_L
                 hipLaunchKernelGGL((linv_kernel<T, C, B>), dim3(1, 1, active), dim3(256), 0, s, one, d_tbl, k, slot);
-            const int Nin = max_over([&](int n) { return std::min(n, K0 + W2) - std::min(n, k + NB); });
+            const int Nin = max_over([&](int n) { return std::min(n, K0 + outer_w<T>()) - std::min(n, k + NB); });
             const int Min = max_over([&](int n) { return n - std::min(n, k + NB); });
             lu_gemm<T, C, B>(one, d_tbl, LuStep{0, K0, k, slot}, NB, Nin, active, gopts, s);
             lu_gemm<T, C, B>(one, d_tbl, LuStep{1, K0, k, slot}, Min, Nin, active, gopts, s);
         }
         // the outer block's interchanges on the columns left and right of it
         hipLaunchKernelGGL((laswp_kernel<T, C, B>), dim3((nmax + 255) / 256, 1, act_outer), dim3(256), 0, s, one,
-                           d_tbl, K0, K0 + W2, 0, nmax, K0, K0 + W2);
+                           d_tbl, K0, K0 + outer_w<T>(), 0, nmax, K0, K0 + outer_w<T>());
         int Nout = 0;
-        for (int q = 0; q < act_outer; ++q) Nout = std::max(Nout, ns[q] - std::min(ns[q], K0 + W2));
+        for (int q = 0; q < act_outer; ++q) Nout = std::max(Nout, ns[q] - std::min(ns[q], K0 + outer_w<T>()));
         if (Nout > 0) {
             const int nb_blocks = (K1max - K0 + NB - 1) / NB;
             for (int b = 0; b < nb_blocks; ++b) {
@@ -2367,7 +2372,7 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
             }
             if (!B && hw) {
                 h16_split(one, K0, *hw, s);
-                h16_update(one, K0, std::min(nmax, K0 + W2), nmax, *hw, s);
+                h16_update(one, K0, std::min(nmax, K0 + outer_w<T>()), nmax, *hw, s);
             } else {
                 lu_gemm<T, C, B>(one, d_tbl, LuStep{4, K0, K0, 0}, Nout, Nout, act_outer, gopts, s);
             }
@@ -2393,13 +2398,13 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
     hipEventRecord(e0, s);
     hipStreamWaitEvent(sp, e0, 0);
     hipStreamWaitEvent(sg, e0, 0);
-    for (int K0 = 0; K0 < n; K0 += W2) {
-        const int K1 = std::min(n, K0 + W2);
+    for (int K0 = 0; K0 < n; K0 += outer_w<T>()) {
+        const int K1 = std::min(n, K0 + outer_w<T>());
         for (int k = K0; k < K1; k += NB) {
             const int kend = std::min(n, k + NB);
             panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, gopts, pgrid, &pbase, sp);
-            hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((W2 + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
-                               k + NB, K0, K0 + W2, k, k + NB);
+            hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((outer_w<T>() + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
+                               k + NB, K0, K0 + outer_w<T>(), k, k + NB);
             const int slot = (k - K0) / NB;
             if (pivoting != 2)  // block-local pivoting: the panel top wrote S_L
                 hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
@@ -2409,7 +2414,7 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
         hipEventRecord(ep, sp);
         hipStreamWaitEvent(sg, ep, 0);
         hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((n + 255) / 256), dim3(256), 0, sg, P, nullptr, K0,
-                           K0 + W2, 0, n, K0, K0 + W2);
+                           K0 + outer_w<T>(), 0, n, K0, K0 + outer_w<T>());
         const int Nout = n - K1;
         if (Nout > 0) {
             const int nb_blocks = (K1 - K0 + NB - 1) / NB;
@@ -2417,7 +2422,7 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
                 if (b > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, b}, NB, Nout, 1, gopts, sg);
                 lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, b}, NB, Nout, 1, gopts, sg);
             }
-            const int c_next = std::min(n, K1 + W2);
+            const int c_next = std::min(n, K1 + outer_w<T>());
             LuStep st4{4, K0, K0, 0};
             st4.c0 = K1;
             st4.c1 = c_next;
@@ -2444,13 +2449,13 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
 }
 
 // Per-problem scratch: counter and info words, the per-block pivot
-// partials and S_L of each NB block of an outer block (W2 / NB of them).
+// partials and S_L of each NB block of an outer block (outer_w / NB of them).
 template <typename T, bool C>
 size_t lu_scratch_bytes(int n) {
     const int gmax = (n + RPB - 1) / RPB;
     const size_t tids = 2 * (size_t)((n + TRW - 1) / TRW) * NB * sizeof(int);
     return 512 + (((size_t)gmax * (sizeof(double) + sizeof(int)) + 255) & ~(size_t)255) +
-           (W2 / NB) * NB * NB * sizeof(cx<T>) + 256 + ((tids + 255) & ~(size_t)255) + NB * NB * sizeof(cx<T>) + 256 +
+           (outer_w<T>() / NB) * NB * NB * sizeof(cx<T>) + 256 + ((tids + 255) & ~(size_t)255) + NB * NB * sizeof(cx<T>) + 256 +
            sizeof(PanelX<T>) + 256;
 }
 
@@ -2464,7 +2469,7 @@ LuProb lu_carve(char* base, void* A, int64_t ld, int n, int32_t* ipiv, int* info
     P.pval = (double*)(base + 512);
     P.pidx = (int*)(P.pval + gmax);
     P.SL = (void*)(((uintptr_t)(P.pidx + gmax) + 255) & ~(uintptr_t)255);
-    P.tids = (int*)(((uintptr_t)((cx<T>*)P.SL + (W2 / NB) * NB * NB) + 255) & ~(uintptr_t)255);
+    P.tids = (int*)(((uintptr_t)((cx<T>*)P.SL + (outer_w<T>() / NB) * NB * NB) + 255) & ~(uintptr_t)255);
     P.SU = (void*)(((uintptr_t)(P.tids + 2 * ((n + TRW - 1) / TRW) * NB) + 255) & ~(uintptr_t)255);
     P.pan = (void*)(((uintptr_t)((cx<T>*)P.SU + NB * NB) + 255) & ~(uintptr_t)255);
     return P;
@@ -2614,10 +2619,10 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
     // workgroups of the persistent panel: as many as stay resident on the
     // compute units its stream may use
     int pgrid = ctx->lu_panel ? panel_grid_resident<T, C>(ctx, ctx->num_cu) : 0;
-    // split-fp16 rank-256 updates (c64, an option)
+    // split-fp16 rank-512 updates (c64, an option)
     H16Ws hws{};
     const H16Ws* hw = nullptr;
-    if (std::is_same<T, float>::value && C && ctx->c64_gemm_h16 && n > W2) {
+    if (std::is_same<T, float>::value && C && ctx->c64_gemm_h16 && n > outer_w<T>()) {
         const size_t need = h16_ws_bytes(n);
         if (ctx->lu_h16_bytes < need) {
             if (ctx->lu_h16) hipFree(ctx->lu_h16);
@@ -2630,12 +2635,12 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
         hws = h16_carve((char*)ctx->lu_h16, n);
         hw = &hws;
     }
-    if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2 && ctx->lu_pivoting == 2) {
+    if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > outer_w<T>() && ctx->lu_pivoting == 2) {
         rc = ensure_lu_streams_free(ctx);
         if (rc) return rc;
         lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), 0, s, ctx->lu_stream_free[0],
                                   ctx->lu_stream_free[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2], hw);
-    } else if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > W2) {
+    } else if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > outer_w<T>()) {
         rc = ensure_lu_streams(ctx);
         if (rc) return rc;
         if (pgrid > 0 && ctx->lu_lookahead_cus > 0)
@@ -2711,7 +2716,7 @@ int lu_factor_batched_t(Context* ctx, int count, void* const* A, const int64_t* 
     int K_switch = 1 << 30;
     if (ctx->lu_panel != 0) {
         const int second = count > 1 ? ns[1] : 0;
-        const int ks = (second + W2 - 1) / W2 * W2;
+        const int ks = (second + outer_w<T>() - 1) / outer_w<T>() * outer_w<T>();
         if (ns[0] > ks) K_switch = ks;
     }
     if (K_switch > 0) lu_launch<T, C, true>(LuProb{}, d_tbl, ns, ctx->lu_pivoting, gemm_opts(ctx), 0, s, 0, K_switch);
