@@ -962,7 +962,13 @@ __global__ void kpart_reduce_kernel(const double* kpart, const double* krow, int
 // complex moments over the test points,
 //   T0 = sum_a t_a,  T1 = sum_a p1_a t_a,  T2 = sum_a p2_a t_a,
 //   t_a = w_a (1 - i kap r) e^{i kap r} / r^3,
-// are formed; both K rows follow from them per trial point: row i through
+// are formed.  For Helmholtz the pair works in units of 1/|kap|: r^2 is
+// formed as (kap r)^2 (S_a, Q, P1, P2 scaled by kap^2), so kap r is
+// (kap r)^2 / (kap r) with no multiply by kap, and V and the moments come
+// out divided by |kap| and |kap|^3, restored once per panel pair; kap < 0
+// conjugates the point sums (exp(i kap r) = conj(exp(i |kap| r))), and
+// kap = 0 takes the Laplace instance.  Both K rows follow from the moments
+// per trial point: row i through
 // the affine (x_a - y_b).n_j, row j through the P1 data on panel i, which is
 // affine in (r1, r2) too (phi = 1 - r1, r1 - r2, r2).
 // Blocks take DROWS test panels (DTPB tiles of DTI) of one 256-row group
@@ -1003,7 +1009,7 @@ __global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int 
     const int jj = jvalid ? j : 0;
     const int r0 = g * 256 + sb * DROWS;
     __shared__ double red[4][2];
-    __shared__ double2 sct[256];  // sin/cos(j pi/128)
+    __shared__ double2 sct[HELM ? kSinCosTab2048 : 1];  // {sin, cos}(2 pi j / 2048)
     if (r0 >= nf) {               // an empty row slot of the last group (uniform):
         if (jvalid) {             // its partials are read all the same
             jpart[((int64_t)(g * DSUB + sb) * nf + j) * 2] = 0.0;
@@ -1011,11 +1017,13 @@ __global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int 
         }
         return;
     }
-    if constexpr (HELM) sincos_table_fill(sct);
+    if constexpr (HELM) sincos_table2048_fill(sct);
     __syncthreads();
     const double* __restrict__ PL = p.pl;
     const ConstPtr<double> PHI = as_const(p.phi);
     const double kap = p.kap;
+    const double k2 = HELM ? kap * kap : 1.0;            // scale of r^2
+    const double ka = HELM ? __builtin_fabs(kap) : 1.0;  // V carries 1/ka, the moments 1/ka^3
     // trial side of this lane's panel j
     const double nx = p.normal[3 * jj], ny = p.normal[3 * jj + 1], nz = p.normal[3 * jj + 2];
     double gjr[3], gji[3];
@@ -1053,21 +1061,22 @@ __global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int 
                     const double yy = PL[((int64_t)b * 4 + 1) * nf + j] - cy;
                     const double yz = PL[((int64_t)b * 4 + 2) * nf + j] - cz;
                     const double wb = PL[((int64_t)b * 4 + 3) * nf + j];
-                    const double P1 = e1x * yx + e1y * yy + e1z * yz;
-                    const double P2 = e2x * yx + e2y * yy + e2z * yz;
-                    const double Q = yx * (yx + dx) + yy * (yy + dy) + yz * (yz + dz);
+                    const double P1 = k2 * (e1x * yx + e1y * yy + e1z * yz);
+                    const double P2 = k2 * (e2x * yx + e2y * yy + e2z * yz);
+                    const double Q = k2 * (yx * (yx + dx) + yy * (yy + dy) + yz * (yz + dz));
                     double pvr = 0, pvi = 0, t0r = 0, t0i = 0, t1r = 0, t1i = 0, t2r = 0, t2i = 0;
 #pragma unroll 5
                     for (int a = 0; a < (NQ2 > 0 ? NQ2 : q2); ++a) {
                         const double p1 = PQ[4 * a + 1], p2 = PQ[4 * a + 2], wa = PQ[4 * a + 3];
-                        const double r2 = __builtin_fma(p2, P2, __builtin_fma(p1, P1, PQ[4 * a] + Q));
+                        const double r2 = HELM ? __builtin_fma(p2, P2, __builtin_fma(p1, P1, __builtin_fma(PQ[4 * a], k2, Q)))
+                                               : __builtin_fma(p2, P2, __builtin_fma(p1, P1, PQ[4 * a] + Q));
                         const double rinv = rsqrt_f64(r2);
                         const double gv = wa * rinv;
                         const double g3 = gv * (rinv * rinv);
                         if constexpr (HELM) {
-                            const double th = kap * (r2 * rinv);
+                            const double th = r2 * rinv;  // |kap| r
                             double sn, cs;
-                            sincos_f64_tab(th, sct, &sn, &cs);
+                            sincos_f64_t2048(th, sct, &sn, &cs);
                             pvr = __builtin_fma(gv, cs, pvr);
                             pvi = __builtin_fma(gv, sn, pvi);
                             // (1 - i th) e^{i th}
@@ -1082,6 +1091,9 @@ __global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int 
                             t1r = __builtin_fma(p1, g3, t1r);
                             t2r = __builtin_fma(p2, g3, t2r);
                         }
+                    }
+                    if (HELM && kap < 0.0) {  // uniform
+                        pvi = -pvi; t0i = -t0i; t1i = -t1i; t2i = -t2i;
                     }
                     vr = __builtin_fma(wb, pvr, vr);
                     vi = __builtin_fma(wb, pvi, vi);
@@ -1101,11 +1113,12 @@ __global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int 
                     b1r = __builtin_fma(sw, t1r, b1r); b1i = __builtin_fma(sw, t1i, b1i);
                     b2r = __builtin_fma(sw, t2r, b2r); b2i = __builtin_fma(sw, t2i, b2i);
                 }
-                const double Vr = kInv4Pi * vr, Vi = kInv4Pi * vi;
+                const double sv = kInv4Pi * ka, sk = kInv4Pi * (ka * ka * ka);
+                const double Vr = sv * vr, Vi = sv * vi;
                 add_elem(p.out, p.dtype, (int64_t)i * p.ld + j, p.ar * Vr - p.ai * Vi, p.ar * Vi + p.ai * Vr, false);
                 vtr[ti] = Vr; vti[ti] = Vi;
                 done |= 1u << ti;
-                kir = kInv4Pi * ar; kii = kInv4Pi * ai;
+                kir = sk * ar; kii = sk * ai;
                 // row j: the data on panel i at its test points is
                 // g0 (1 + p1/2) + g1 (p2 - p1)/2 - g2 p2/2
                 double g0r, g0i, g1r, g1i, g2r, g2i;
@@ -1117,8 +1130,8 @@ __global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int 
                 const double c2r = -0.5 * b2r, c2i = -0.5 * b2i;
                 const double sr = c0r * g0r - c0i * g0i + c1r * g1r - c1i * g1i + c2r * g2r - c2i * g2i;
                 const double si = c0r * g0i + c0i * g0r + c1r * g1i + c1i * g1r + c2r * g2i + c2i * g2r;
-                kjr += kInv4Pi * sr;
-                kji += kInv4Pi * si;
+                kjr += sk * sr;
+                kji += sk * si;
             }
             // row i: reduce over the block into the (i, jb) partial
             kir = wave_sum(kir);
@@ -2275,8 +2288,10 @@ int assemble_dirichlet(Context* ctx, const bem_op_desc* d, void* A, int64_t ld, 
                                (const double*)krow, nf, njb, rhs, dtype_rhs, alpha_k, 0.0);
         } else {
             // FP64: moment form, fixed-order partials for both rows
-            auto fk = m.q2 == 25 ? (helm ? far_dirichlet_f64_kernel<true, 25> : far_dirichlet_f64_kernel<false, 25>)
-                                 : (helm ? far_dirichlet_f64_kernel<true, 0> : far_dirichlet_f64_kernel<false, 0>);
+            // Helmholtz at kap = 0 is the Laplace kernel (the Helmholtz instance scales by 1/|kap|)
+            const bool hk = helm && p.kap != 0.0;
+            auto fk = m.q2 == 25 ? (hk ? far_dirichlet_f64_kernel<true, 25> : far_dirichlet_f64_kernel<false, 25>)
+                                 : (hk ? far_dirichlet_f64_kernel<true, 0> : far_dirichlet_f64_kernel<false, 0>);
             double* jpart = m.kpart + (size_t)nf * njb * 2;
             const int64_t nblocks = (int64_t)DSUB * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
             hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, njb, ngroups, m.kpart, jpart);

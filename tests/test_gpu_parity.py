@@ -910,9 +910,11 @@ def test_transmission_solve_accuracy(bem):
 
 
 @pytest.mark.parametrize("prec,tol", [(0, TOL64), (1, 5e-6)])
-@pytest.mark.parametrize("kernel,k", [(0, 0.0), (1, 3.0)])
+@pytest.mark.parametrize("kernel,k", [(0, 0.0), (1, 3.0), (1, -2.0), (1, 0.0)])
 def test_assemble_dirichlet_fused(bem, oracle, meshes, kernel, k, prec, tol):
-    """One-pass V + (1/2 M + K) conj(g) equals the separate oracle pieces."""
+    """One-pass V + (1/2 M + K) conj(g) equals the separate oracle pieces;
+    Helmholtz also at k < 0 (the FP64 pass conjugates its point sums) and at
+    k = 0 (it takes the Laplace instance)."""
     v, f = meshes["p1"]
     ctx = make_ctx(bem, v, f)
     n = len(f)
