@@ -45,9 +45,9 @@ FLOPS_PER_EVAL_FUSED = 47
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by the same counter on
 # a 512 MiB copy (FETCH x 2.00, WRITE x 1.00), and the hardware FP64 VALU
 # flops of that launch (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes).  Measured:
-# 105.9 GB fetched + 9.1 GB written; 8.845e12 flops.
-FAR_L5_TRAFFIC_BYTES = 115_006_186_496
-FAR_L5_HW_FLOPS = 8_845_081_600_000
+# 105.2 GB fetched + 8.4 GB written; 8.335e12 flops.
+FAR_L5_TRAFFIC_BYTES = 113_612_336_928
+FAR_L5_HW_FLOPS = 8_335_159_869_440
 
 
 def cpu_threads() -> int:

@@ -98,6 +98,8 @@ int bem_destroy(bem_context* ctx) {
             if (st) hipStreamDestroy(st);
         for (hipEvent_t e : ctx->impl.lu_event)
             if (e) hipEventDestroy(e);
+        for (hipEvent_t e : ctx->impl.lu_pevent)
+            if (e) hipEventDestroy(e);
         if (ctx->impl.cf_ws) hipFree(ctx->impl.cf_ws);
         return BEM_OK;
     });

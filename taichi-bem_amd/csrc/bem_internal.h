@@ -117,6 +117,7 @@ struct Context {
     hipStream_t lu_stream[2] = {nullptr, nullptr};
     hipStream_t lu_stream_free[2] = {nullptr, nullptr};  // look-ahead without CU masks (block-local pivoting)
     hipEvent_t lu_event[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t lu_pevent[8] = {};  // per-panel events of an outer block (look-ahead)
 };
 
 // error helpers (bem_capi.hip)

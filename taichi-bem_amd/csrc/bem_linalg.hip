@@ -2391,7 +2391,8 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
 //   sp: wait eg; panels of block j+1 ...
 template <typename T, bool C>
 void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hipStream_t s, hipStream_t sp,
-                         hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0, const H16Ws* hw = nullptr) {
+                         hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0, const hipEvent_t* pev,
+                         const H16Ws* hw = nullptr) {
     const int n = P.n;
     const std::vector<int> ns1{n};
     unsigned pbase = 0;
@@ -2400,6 +2401,7 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
     hipStreamWaitEvent(sg, e0, 0);
     for (int K0 = 0; K0 < n; K0 += outer_w<T>()) {
         const int K1 = std::min(n, K0 + outer_w<T>());
+        const int Nout = n - K1;
         for (int k = K0; k < K1; k += NB) {
             const int kend = std::min(n, k + NB);
             panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, gopts, pgrid, &pbase, sp);
@@ -2408,20 +2410,23 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
             const int slot = (k - K0) / NB;
             if (pivoting != 2)  // block-local pivoting: the panel top wrote S_L
                 hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
+            // The panel's rows of the outer U12 (columns K1..n) need only this
+            // panel (its pivots, S_L and its rows of L) and the U12 rows above
+            // it: the update stream takes them as each panel completes, beside
+            // the rest of the block's panels (a later panel's interchanges never
+            // reach the rows of an earlier one).
+            hipEventRecord(pev[slot], sp);
+            hipStreamWaitEvent(sg, pev[slot], 0);
+            hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((n + 255) / 256), dim3(256), 0, sg, P, nullptr, k,
+                               k + NB, 0, n, K0, K0 + outer_w<T>());
+            if (Nout > 0) {
+                if (slot > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, slot}, NB, Nout, 1, gopts, sg);
+                lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, slot}, NB, Nout, 1, gopts, sg);
+            }
             lu_gemm<T, C, false>(P, nullptr, LuStep{0, K0, k, slot}, NB, K1 - kend, 1, gopts, sp);
             lu_gemm<T, C, false>(P, nullptr, LuStep{1, K0, k, slot}, n - kend, K1 - kend, 1, gopts, sp);
         }
-        hipEventRecord(ep, sp);
-        hipStreamWaitEvent(sg, ep, 0);
-        hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((n + 255) / 256), dim3(256), 0, sg, P, nullptr, K0,
-                           K0 + outer_w<T>(), 0, n, K0, K0 + outer_w<T>());
-        const int Nout = n - K1;
         if (Nout > 0) {
-            const int nb_blocks = (K1 - K0 + NB - 1) / NB;
-            for (int b = 0; b < nb_blocks; ++b) {
-                if (b > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, b}, NB, Nout, 1, gopts, sg);
-                lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, b}, NB, Nout, 1, gopts, sg);
-            }
             const int c_next = std::min(n, K1 + outer_w<T>());
             LuStep st4{4, K0, K0, 0};
             st4.c0 = K1;
@@ -2522,6 +2527,11 @@ int ensure_lu_streams(Context* ctx) {
             rc = check_hip(hipEventCreateWithFlags(&ctx->lu_event[q], hipEventDisableTiming), "LU event");
             if (rc) return rc;
         }
+    for (hipEvent_t& e : ctx->lu_pevent)
+        if (!e) {
+            rc = check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "LU event");
+            if (rc) return rc;
+        }
     ctx->lu_streams_cus = ctx->lu_lookahead_cus;
     return BEM_OK;
 }
@@ -2539,6 +2549,11 @@ int ensure_lu_streams_free(Context* ctx) {
     for (int q = 0; q < 3; ++q)
         if (!ctx->lu_event[q]) {
             int rc = check_hip(hipEventCreateWithFlags(&ctx->lu_event[q], hipEventDisableTiming), "LU event");
+            if (rc) return rc;
+        }
+    for (hipEvent_t& e : ctx->lu_pevent)
+        if (!e) {
+            int rc = check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "LU event");
             if (rc) return rc;
         }
     return BEM_OK;
@@ -2639,7 +2654,8 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
         rc = ensure_lu_streams_free(ctx);
         if (rc) return rc;
         lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), 0, s, ctx->lu_stream_free[0],
-                                  ctx->lu_stream_free[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2], hw);
+                                  ctx->lu_stream_free[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2],
+                                  ctx->lu_pevent, hw);
     } else if (ctx->lu_lookahead_cus != 0 && cap == hipStreamCaptureStatusNone && n > outer_w<T>()) {
         rc = ensure_lu_streams(ctx);
         if (rc) return rc;
@@ -2647,7 +2663,8 @@ int lu_factor_t(Context* ctx, T* A, int64_t ld, int n, int32_t* ipiv, hipStream_
             pgrid = panel_grid_resident<T, C>(ctx, std::min(ctx->lu_lookahead_cus, ctx->num_cu - 1));
         if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
         lu_launch_lookahead<T, C>(P, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s, ctx->lu_stream[0],
-                                  ctx->lu_stream[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2], hw);
+                                  ctx->lu_stream[1], ctx->lu_event[0], ctx->lu_event[1], ctx->lu_event[2],
+                                  ctx->lu_pevent, hw);
     } else {
         if (ctx->lu_panel > 1) pgrid = std::min(pgrid, ctx->lu_panel);
         lu_launch<T, C, false>(P, nullptr, std::vector<int>{n}, ctx->lu_pivoting, gemm_opts(ctx), pgrid, s, 0,

@@ -528,11 +528,11 @@ def _lu_block_local_ref(a, nb=64):
     return a, ipiv
 
 
-@pytest.mark.parametrize("n", [300, 130])
+@pytest.mark.parametrize("n", [300, 130, 700])
 def test_lu_block_local_pivoting_matches_restatement(bem, n):
     """Block-local pivoting (complex128) picks the same pivot rows as a numpy
     restatement of the rule and gives the same factors, including the last
-    partial block."""
+    partial block (n = 700: three outer blocks of the look-ahead schedule)."""
     rng = np.random.default_rng(41)
     a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     ctx = bem.Context(0)
