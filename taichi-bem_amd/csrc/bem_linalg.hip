@@ -989,23 +989,31 @@ __global__ __launch_bounds__(512) void topblock_local_kernel(LuProb inl, const L
 #pragma unroll
     for (int cc = 0; cc < 16; ++cc) x[cc] = cx<T>{T(lane == c0 + cc ? 1 : 0), T(0)};
     if (wave < 4) {
-        // X = inv(U11): for i = 63 .. 0, row i /= U[i][i], then rows above
-        // subtract U[r][i] times row i (only columns c >= i are nonzero)
+        // X = inv(U11) = inv(D^-1 U11) D^-1, D = diag(U11): the unit upper
+        // factor's inverse by back substitution (for i = 63 .. 0, rows above
+        // subtract U[r][i] / U[r][r] times row i, which is final; only
+        // columns c >= i are nonzero), then column c times 1 / U[c][c].  The
+        // same work per step as inv(L11) on the other four waves.
+        const cx<T> dl = at(lane, lane);
+        const cx<T> rdl = dl.re == T(0) && dl.im == T(0) ? cx<T>{T(1), T(0)} : crecip(dl);
         for (int i = NB - 1; i >= 0; --i) {
-            const cx<T> d = at(i, i), m = at(lane, i);
-            const bool zero = d.re == T(0) && d.im == T(0);
-            const cx<T> rd = zero ? cx<T>{T(1), T(0)} : crecip(d);
-            // rows above i subtract U[r][i] row i; row i takes the scaled
-            // row; rows below keep theirs (a zero multiplier, no branch)
+            // rows above i subtract; row i and the rows below keep theirs
+            // (a zero multiplier, no branch)
+            const cx<T> m = cmul(at(lane, i), rdl);
             const cx<T> mm = lane < i ? m : cx<T>{T(0), T(0)};
 #pragma unroll
             for (int cc = 0; cc < 16; ++cc)
                 if (c0 + cc >= i) {
-                    const cx<T> xi = cmul(readlane_cx(x[cc], i), rd);
-                    const cx<T> y = {x[cc].re - (mm.re * xi.re - mm.im * xi.im),
-                                     x[cc].im - (mm.re * xi.im + mm.im * xi.re)};
-                    x[cc] = lane == i ? xi : y;
+                    const cx<T> xi = readlane_cx(x[cc], i);
+                    x[cc].re -= mm.re * xi.re - mm.im * xi.im;
+                    x[cc].im -= mm.re * xi.im + mm.im * xi.re;
                 }
+        }
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const cx<T> d = at(c0 + cc, c0 + cc);
+            const cx<T> rd = d.re == T(0) && d.im == T(0) ? cx<T>{T(1), T(0)} : crecip(d);
+            x[cc] = cmul(x[cc], rd);
         }
     } else {
         // X = inv(L11), unit lower: for i = 0 .. 63, rows below subtract
