@@ -141,6 +141,51 @@ __global__ void laswp_kernel(LuProb inl, const LuProb* tbl, int p_lo, int p_hi, 
     }
 }
 
+// The same for block-local pivoting, where every pivot of [p_lo, p_hi)
+// (at most NB of them) lies inside those rows: each wave composes the
+// interchanges into one permutation of the w rows (lane i ends holding the
+// row that lands at p_lo + i), then every column moves with w independent
+// loads followed by w stores instead of w dependent swaps, and rows that keep
+// their place are not touched.  Pivots outside the rows (a failed panel's)
+// are skipped.  FP32 / complex64 (the column's w entries stay in registers).
+template <bool C, bool B>
+__global__ __launch_bounds__(256) void laswp_local_kernel(LuProb inl, const LuProb* tbl, int p_lo, int p_hi, int c_lo,
+                                                          int c_hi, int x_lo, int x_hi) {
+    using T = float;
+    const LuProb P = get_prob<B>(inl, tbl);
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    const int n = P.n;
+    p_hi = min(n, min(p_hi, p_lo + NB));
+    c_hi = min(n, c_hi);
+    x_hi = min(n, x_hi);
+    const int w = p_hi - p_lo;
+    if (w <= 0) return;
+    const int lane = threadIdx.x & 63;
+    const int pv = lane < w ? P.ipiv[p_lo + lane] - p_lo : lane;
+    int src = lane;
+    for (int c = 0; c < w; ++c) {
+        const int p = __builtin_amdgcn_readlane(pv, c);
+        if (p != c && p >= 0 && p < w) {  // uniform
+            const int a = __builtin_amdgcn_readlane(src, c), b = __builtin_amdgcn_readlane(src, p);
+            src = lane == c ? b : (lane == p ? a : src);
+        }
+    }
+    const int col = c_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= c_hi || (col >= x_lo && col < x_hi)) return;
+    // rows that keep their place (most of them, for diagonally dominant
+    // blocks) are neither read nor written
+    cx<T> v[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int si = __builtin_amdgcn_readlane(src, i);
+        if (i < w && si != i) v[i] = Acc<T, C>::ld(A, (int64_t)(p_lo + si) * ld + col);
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (i < w && __builtin_amdgcn_readlane(src, i) != i) Acc<T, C>::st(A, (int64_t)(p_lo + i) * ld + col, v[i]);
+}
+
 // ---------------------------------------------------------------------------
 // Panel column step c (one launch per column of the NB-wide panel [k, kend)):
 //   - rows r > c-1 get the rank-1 update of column c-1 (multiplier
@@ -1102,7 +1147,7 @@ __global__ __launch_bounds__(256) void linv_kernel(LuProb inl, const LuProb* tbl
 // ---------------------------------------------------------------------------
 // C[M x N] -= A[M x K] B[K x N] on MFMA.  Block tile 64 x 64, four waves in
 // a 2 x 2 arrangement, each wave 32 x 32 = 2 x 2 MFMA 16x16 blocks, K staged
-// through LDS in chunks of KC.  MFMA 16x16x4 operand layout: lane l supplies
+// through LDS in chunks of KC (32 for FP32, 16 for FP64).  MFMA 16x16x4 operand layout: lane l supplies
 // A[l%16][l/16] and B[l/16][l%16].  Result register r of lane l holds column
 // l%16 of row 4*(l/16)+r for v_mfma_f32_16x16x4f32 but of row l/16+4*r for
 // v_mfma_f64_16x16x4f64 (both measured on gfx950).  Complex products use four
@@ -1127,7 +1172,6 @@ template <> struct Mfma<float> {
 
 constexpr int GT = 64;   // block tile
 constexpr int WT = 32;   // wave tile
-constexpr int KC = 16;   // K chunk
 
 // Operands of one C -= A B.
 template <typename T>
@@ -1209,6 +1253,10 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const Lu
     if (M <= 0 || N <= 0 || (int)blockIdx.y * GT >= M || (int)blockIdx.x * GT >= N) return;
     using V4 = typename Mfma<T>::v4;
     constexpr int NBLK = WT / 16;
+    // K chunks of 32 (FP32) or 16 (FP64): these products are short (K = 64 ..
+    // 448 over one or a few tile rows) and latency-bound on operands fresh
+    // from HBM, so fewer, larger chunks mean fewer dependent load rounds
+    constexpr int KC = sizeof(T) == 4 ? 32 : 16;
     __shared__ T sAr[GT][KC + 1], sAi[C ? GT : 1][KC + 1];
     __shared__ T sBr[KC][GT + 1], sBi[C ? KC : 1][GT + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1222,6 +1270,20 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const Lu
             accR[a][b] = V4{0, 0, 0, 0};
             if constexpr (C) accI[a][b] = V4{0, 0, 0, 0};
         }
+    // C's entries of this thread are loaded before the K loop, so their
+    // latency is hidden behind it
+    cx<T> cpre[NBLK][NBLK][4];
+#pragma unroll
+    for (int a = 0; a < NBLK; ++a)
+#pragma unroll
+        for (int b = 0; b < NBLK; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = bm + wm + 16 * a + Mfma<T>::row(lane, r);
+                const int gc = bn + wn + 16 * b + (lane & 15);
+                cpre[a][b][r] = (st.mode != 5 && gr < M && gc < N) ? Acc<T, C>::ld(Cm, (int64_t)gr * ldc + gc)
+                                                                    : cx<T>{T(0), T(0)};
+            }
     // register prefetch: the next K chunk is loaded while the current one is
     // multiplied out of LDS
     constexpr int PE = GT * KC / 256;
@@ -1286,8 +1348,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const Lu
                 const int gr = bm + wm + 16 * a + Mfma<T>::row(lane, r);
                 const int gc = bn + wn + 16 * b + (lane & 15);
                 if (gr < M && gc < N) {
-                    cx<T> v = {T(0), T(0)};
-                    if (st.mode != 5) v = Acc<T, C>::ld(Cm, (int64_t)gr * ldc + gc);
+                    cx<T> v = cpre[a][b][r];
                     v.re -= accR[a][b][r];
                     if constexpr (C) v.im -= accI[a][b][r];
                     Acc<T, C>::st(Cm, (int64_t)gr * ldc + gc, v);
@@ -2397,6 +2458,25 @@ void lu_launch(const LuProb& one, const LuProb* d_tbl, const std::vector<int>& n
 //   sg: wait ep; interchanges outside block j; U12; update of block j+1's
 //       columns -> event eg; update of the remaining columns
 //   sp: wait eg; panels of block j+1 ...
+// Interchanges of pivots [p_lo, p_hi) over columns [c_lo, c_hi) outside
+// [x_lo, x_hi) of one problem: the composed permutation for block-local
+// pivoting of FP32 factors, the swap sequence otherwise.
+template <typename T, bool C>
+void laswp_launch(const LuProb& P, int pivoting, int p_lo, int p_hi, int c_lo, int c_hi, int x_lo, int x_hi,
+                  hipStream_t s) {
+    const dim3 grid((std::max(0, c_hi - c_lo) + 255) / 256);
+    if (grid.x == 0) return;
+    if constexpr (std::is_same<T, float>::value) {
+        if (pivoting == 2) {
+            hipLaunchKernelGGL((laswp_local_kernel<C, false>), grid, dim3(256), 0, s, P, nullptr, p_lo, p_hi, c_lo,
+                               c_hi, x_lo, x_hi);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((laswp_kernel<T, C, false>), grid, dim3(256), 0, s, P, nullptr, p_lo, p_hi, c_lo, c_hi, x_lo,
+                       x_hi);
+}
+
 template <typename T, bool C>
 void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hipStream_t s, hipStream_t sp,
                          hipStream_t sg, hipEvent_t ep, hipEvent_t eg, hipEvent_t e0, const hipEvent_t* pev,
@@ -2413,8 +2493,7 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
         for (int k = K0; k < K1; k += NB) {
             const int kend = std::min(n, k + NB);
             panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, gopts, pgrid, &pbase, sp);
-            hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((outer_w<T>() + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
-                               k + NB, K0, K0 + outer_w<T>(), k, k + NB);
+            laswp_launch<T, C>(P, pivoting, k, k + NB, K0, K0 + outer_w<T>(), k, k + NB, sp);
             const int slot = (k - K0) / NB;
             if (pivoting != 2)  // block-local pivoting: the panel top wrote S_L
                 hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
@@ -2425,8 +2504,7 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
             // reach the rows of an earlier one).
             hipEventRecord(pev[slot], sp);
             hipStreamWaitEvent(sg, pev[slot], 0);
-            hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((n + 255) / 256), dim3(256), 0, sg, P, nullptr, k,
-                               k + NB, 0, n, K0, K0 + outer_w<T>());
+            laswp_launch<T, C>(P, pivoting, k, k + NB, 0, n, K0, K0 + outer_w<T>(), sg);
             if (Nout > 0) {
                 if (slot > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, slot}, NB, Nout, 1, gopts, sg);
                 lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, slot}, NB, Nout, 1, gopts, sg);

@@ -299,6 +299,35 @@ def test_lu_factors_reconstruct(bem, n, dtype, tol):
     assert np.abs(L).max() < 20.0
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("n", [600, 1100])
+def test_lu_block_local_c64_reconstruct(bem, n, mode):
+    """complex64 factors with block-local pivoting on the look-ahead schedule
+    (n > 512: per-panel events, the composed row permutations of the
+    interchange kernel, FP32-MFMA or split-fp16 outer updates): every pivot
+    stays inside its 64-row panel and P A = L U to FP32 accuracy."""
+    rng = np.random.default_rng(17)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + np.eye(n) * np.sqrt(n)
+    ctx = bem.Context(0)
+    ctx.set_option(bem._lib.OPT_LU_PIVOTING, 2)
+    ctx.set_option(bem._lib.OPT_C64_LU_GEMM, mode)
+    At = torch.tensor(A, device=DEV).to(torch.complex64).contiguous()
+    A = At.cpu().numpy().astype(np.complex128)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    print(f"[lu local c64] n={n} mode={mode}", flush=True)
+    ctx.lu_factor(At, ipiv)
+    lu, piv = to_np(At).astype(np.complex128), to_np(ipiv)
+    c = np.arange(n)
+    assert np.all(piv >= c) and np.all(piv < np.minimum(n, (c // 64 + 1) * 64))
+    perm = np.arange(n)
+    for c, p in enumerate(piv):
+        perm[[c, p]] = perm[[p, c]]
+    L = np.tril(lu, -1) + np.eye(n)
+    U = np.triu(lu)
+    assert rel(L @ U, A[perm]) <= 5e-5
+    ctx.close()
+
+
 @pytest.mark.parametrize("mode,tol,n", [(0, 2e-3, 600), (2, 2e-3, 600), (2, 2e-3, 1100)])
 def test_lu_c64_gemm_modes(bem, mode, tol, n):
     """The c64 trailing-update arithmetic (FP32 MFMA, split-fp16 planes with
