@@ -2493,7 +2493,10 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
         for (int k = K0; k < K1; k += NB) {
             const int kend = std::min(n, k + NB);
             panel_factor<T, C, false>(P, nullptr, ns1, k, 1, pivoting, gopts, pgrid, &pbase, sp);
-            laswp_launch<T, C>(P, pivoting, k, k + NB, K0, K0 + outer_w<T>(), k, k + NB, sp);
+            // inside the block (a few hundred columns, most rows in place for
+            // these matrices) the plain swap sequence is the shorter launch
+            hipLaunchKernelGGL((laswp_kernel<T, C, false>), dim3((outer_w<T>() + 255) / 256), dim3(256), 0, sp, P, nullptr, k,
+                               k + NB, K0, K0 + outer_w<T>(), k, k + NB);
             const int slot = (k - K0) / NB;
             if (pivoting != 2)  // block-local pivoting: the panel top wrote S_L
                 hipLaunchKernelGGL((linv_kernel<T, C, false>), dim3(1), dim3(256), 0, sp, P, nullptr, k, slot);
