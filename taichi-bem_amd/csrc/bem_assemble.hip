@@ -1214,16 +1214,22 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
     const int nf = p.nf, q2 = NQ2 > 0 ? NQ2 : p.q2;
     const int i0 = tile * DTI;
     const int j = jb * 256 + threadIdx.x;
-    __shared__ double2 sct[256];  // sin/cos(j pi/128) for the FP64 phase
+    // FP64 Helmholtz: the pair sums in units of 1/|kap| (r^2 formed as
+    // (kap r)^2, so kap r needs no multiply by kap; V-type sums rescaled by
+    // |kap| and K-type ones by |kap|^3 once per panel pair; kap < 0
+    // conjugates the point sums) and sin/cos from the 2048-entry table
+    constexpr bool KS = HELM && sizeof(R) == 8;
+    __shared__ double2 sct[KS ? kSinCosTab2048 : 1];  // {sin, cos}(2 pi j / 2048)
     if (i0 >= nf) return;  // the last group's empty tiles (uniform)
-    if constexpr (HELM && sizeof(R) == 8) {
-        sincos_table_fill(sct);
+    if constexpr (KS) {
+        sincos_table2048_fill(sct);
         __syncthreads();
     }
     if (j >= nf) return;  // no barrier below
     const R* __restrict__ PL = Ptr<R>::pl(p);
     const ConstPtr<R> PHI = as_const(Ptr<R>::phi(p));  // scalar loads
     const R kap = (R)p.kap;
+    const R k2 = KS ? kap * kap : R(1), ka = KS ? (R)fabs((double)kap) : R(1);
     const R nnx = (R)p.normal[3 * j], nny = (R)p.normal[3 * j + 1], nnz = (R)p.normal[3 * j + 2];
     const int64_t vj[3] = {p.tris[3 * j], p.tris[3 * j + 1], p.tris[3 * j + 2]};
     const int i_end = min(nf, i0 + DTI);
@@ -1265,9 +1271,9 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
             const R yy = PL[((int64_t)b * 4 + 1) * nf + j] + oY[1];
             const R yz = PL[((int64_t)b * 4 + 2) * nf + j] + oY[2];
             const R wb = PL[((int64_t)b * 4 + 3) * nf + j];
-            const R yd = yx * (yx + D[0]) + yy * (yy + D[1]) + yz * (yz + D[2]);
-            const R y1 = E1[0] * yx + E1[1] * yy + E1[2] * yz;
-            const R y2 = E2[0] * yx + E2[1] * yy + E2[2] * yz;
+            const R yd = k2 * (yx * (yx + D[0]) + yy * (yy + D[1]) + yz * (yz + D[2]));
+            const R y1 = k2 * (E1[0] * yx + E1[1] * yy + E1[2] * yz);
+            const R y2 = k2 * (E2[0] * yx + E2[1] * yy + E2[2] * yz);
             const R c0 = un - (yx * nnx + yy * nny + yz * nnz);
             const R dyi = yx * ni[0] + yy * ni[1] + yz * ni[2];  // (y - x).n_i
             R pwr[3], pwi[3], qkr[3], qki[3], pkr = 0, pki = 0;
@@ -1279,7 +1285,7 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
                 if constexpr (sizeof(R) == 8) {
                     const R sa = PQ[4 * a + 0], p1 = PQ[4 * a + 1], p2 = PQ[4 * a + 2];
                     wa = PQ[4 * a + 3];
-                    r2 = p2 * y2 + (p1 * y1 + (sa + yd));
+                    r2 = p2 * y2 + (p1 * y1 + (KS ? __builtin_fma(sa, k2, yd) : sa + yd));
                     dnj = p2 * e2n + (p1 * e1n + c0);
                 } else {
                     // FP32 keeps the direct form (its K rows lost digits in the
@@ -1295,9 +1301,9 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
                 const R bj = g3 * dnj;   // (x - y).n_j: K row i
                 const R bi_ = g3 * dyi;  // (y - x).n_i: K row j
                 if constexpr (HELM) {
-                    const R kr = kap * (r2 * rinv);
+                    const R kr = KS ? r2 * rinv : kap * (r2 * rinv);  // |kap| r (KS) or kap r
                     R sn, cs;
-                    if constexpr (sizeof(R) == 8) sincos_f64_tab(kr, sct, &sn, &cs);
+                    if constexpr (KS) sincos_f64_t2048(kr, sct, &sn, &cs);
                     else Math<R>::sincos(kr, &sn, &cs);
                     const R gr = g * cs, gi = g * sn;
                     const R c1 = cs + kr * sn, c2 = sn - kr * cs;
@@ -1319,6 +1325,11 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
                     }
                 }
             }
+            if (KS && kap < R(0)) {  // uniform
+                pki = -pki;
+#pragma unroll
+                for (int s = 0; s < 3; ++s) { pwi[s] = -pwi[s]; qki[s] = -qki[s]; }
+            }
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
                 const R wt = wb * PHI[3 * b + t];
@@ -1328,6 +1339,15 @@ __global__ __launch_bounds__(256) void far_vkw_kernel(OpParams p, VkwOut o, int 
             }
 #pragma unroll
             for (int s = 0; s < 3; ++s) { kbr[s] += qkr[s] * wb; kbi[s] += qki[s] * wb; }
+        }
+        if constexpr (KS) {
+            const R ka3 = ka * ka * ka;
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                kar[s] *= ka3; kai[s] *= ka3; kbr[s] *= ka3; kbi[s] *= ka3;
+#pragma unroll
+                for (int t = 0; t < 3; ++t) { wr[s][t] *= ka; wi[s][t] *= ka; }
+            }
         }
         // the V integral: the P1 bases sum to one on either panel
         double s0r = 0.0, s0i = 0.0;
@@ -2364,11 +2384,13 @@ int assemble_vkw(Context* ctx, const bem_op_desc* d, void* V, int64_t ldv, void*
     const int nf = m.nf, njb = (nf + 255) / 256, ngroups = (nf + 255) / 256;
     if (parts & BEM_PART_FAR) {
         const bool q25 = p.q2 == 25;
+        // kap = 0 takes the Laplace instance (the FP64 Helmholtz one scales by 1/|kap|)
+        const bool hk = helm && p.kap != 0.0;
         auto fk = d->precision == BEM_PREC_FP32
                       ? (helm ? (q25 ? far_vkw_kernel<float, true, 25> : far_vkw_kernel<float, true>)
                               : (q25 ? far_vkw_kernel<float, false, 25> : far_vkw_kernel<float, false>))
-                      : (helm ? (q25 ? far_vkw_kernel<double, true, 25> : far_vkw_kernel<double, true>)
-                              : (q25 ? far_vkw_kernel<double, false, 25> : far_vkw_kernel<double, false>));
+                      : (hk ? (q25 ? far_vkw_kernel<double, true, 25> : far_vkw_kernel<double, true>)
+                            : (q25 ? far_vkw_kernel<double, false, 25> : far_vkw_kernel<double, false>));
         const int64_t nblocks = (int64_t)DTG * ((int64_t)ngroups * njb - (int64_t)ngroups * (ngroups - 1) / 2);
         VkwOut o{V, ldv, K, ldk, W, ldw};
         hipLaunchKernelGGL(fk, dim3((unsigned)nblocks), dim3(256), 0, s, p, o, njb, ngroups);

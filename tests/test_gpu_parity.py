@@ -994,14 +994,16 @@ def test_assemble_dirichlet_fp64_groups_and_reproducible(bem, oracle, kernel, k,
     ctx.close()
 
 
-VKW_KERNELS = [(0, 0.0, 1.0), (1, 3.0, 1.0), (2, 5.0, 1.5)]
+VKW_KERNELS = [(0, 0.0, 1.0), (1, 3.0, 1.0), (2, 5.0, 1.5), (1, -2.0, 1.0), (1, 0.0, 1.0)]
 
 
 @pytest.mark.parametrize("flags", [0, 3])
 @pytest.mark.parametrize("kernel,k,sqrt_n", VKW_KERNELS)
 def test_assemble_vkw_matches_oracle(bem, oracle, meshes, kernel, k, sqrt_n, flags):
     """The fused V/K/W pass (one evaluation per unordered point pair, W added
-    once at (min, max) and mirrored) equals the oracle's separate operators."""
+    once at (min, max) and mirrored) equals the oracle's separate operators;
+    Helmholtz also at k < 0 (the FP64 pass conjugates its point sums) and at
+    k = 0 (it takes the Laplace instance)."""
     v, f = bem.meshes.perturbed_icosphere(2, seed=1)
     ctx = make_ctx(bem, v, f)
     nv, nf = len(v), len(f)
