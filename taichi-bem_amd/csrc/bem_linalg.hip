@@ -2585,10 +2585,27 @@ int ensure_lu_scratch(Context* ctx, size_t need) {
 // lu_lookahead_cus the panel stream gets the first that many compute units
 // (bits of the CU mask) and the update stream the others; -1 leaves both
 // unrestricted.
+// Only one look-ahead pair exists at a time: HIP maps a process's streams
+// onto a few hardware queues (GPU_MAX_HW_QUEUES, 4 by default), so with
+// both pairs and the null stream alive the pair created second shares
+// queues, and its panel and update launches no longer overlap (measured on
+// L5: the c64 block-local LU 120 -> 133 ms after a c128 LU had created the
+// masked pair, or the c128 LU 480 -> 516 ms the other way round).  Switching
+// between partial and block-local pivoting therefore drops the other pair
+// (its work is finished first).
+void drop_lu_pair(hipStream_t (&st)[2]) {
+    for (hipStream_t& q : st)
+        if (q) {
+            (void)hipStreamSynchronize(q);
+            (void)hipStreamDestroy(q);
+            q = nullptr;
+        }
+}
+
 int ensure_lu_streams(Context* ctx) {
     if (ctx->lu_stream[0] && ctx->lu_streams_cus == ctx->lu_lookahead_cus) return BEM_OK;
-    for (int q = 0; q < 2; ++q)
-        if (ctx->lu_stream[q]) { hipStreamDestroy(ctx->lu_stream[q]); ctx->lu_stream[q] = nullptr; }
+    drop_lu_pair(ctx->lu_stream_free);
+    drop_lu_pair(ctx->lu_stream);
     int rc;
     if (ctx->lu_lookahead_cus > 0) {
         if (ctx->num_cu <= 0) {
@@ -2630,6 +2647,7 @@ int ensure_lu_streams(Context* ctx) {
 // only takes them from the trailing update (measured on the L5 c64
 // factorisation: 216 ms unmasked, 226 ms with 32 masked units).
 int ensure_lu_streams_free(Context* ctx) {
+    drop_lu_pair(ctx->lu_stream);
     for (int q = 0; q < 2; ++q)
         if (!ctx->lu_stream_free[q]) {
             int rc = check_hip(hipStreamCreateWithFlags(&ctx->lu_stream_free[q], hipStreamNonBlocking), "LU stream");
