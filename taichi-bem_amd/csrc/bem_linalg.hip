@@ -1357,6 +1357,174 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(GemmArgs<T> inl, const Lu
 }
 
 // ---------------------------------------------------------------------------
+// The outer U12 rows of one panel under block-local pivoting (FP32 factors),
+// one launch per panel instead of an interchange, a product with the solved
+// rows above and a product with inv(L_bb):
+//   U_b = inv(L_bb) (P_b A[r0:r0+64, K1:n] - L[b, 0:b] U[0:b]),  r0 = K0 + 64 b,
+// with P_b the panel's interchanges composed into one permutation (all of its
+// pivots lie in its 64 rows) and inv(L_bb) = I - S_L,b.  Each workgroup owns
+// all 64 rows of a 64-column tile: it reads the tile's rows through P_b
+// before any store (so the tile is updated in place), forms the first
+// product over K = 64 b on MFMA, stages the difference in LDS as the second
+// product's operand (K = 64, S_L from the panel's slot) and stores the rows
+// in order.  The columns left of the outer block are interchanged by a
+// launch of their own.  Only full panels of blocks with columns right of
+// them exist here (K1 < n).
+// ---------------------------------------------------------------------------
+template <bool C>
+__global__ __launch_bounds__(256) void u12_panel_kernel(LuProb P, int K0, int b) {
+    using T = float;
+    using V4 = typename Mfma<T>::v4;
+    constexpr int NBLK = WT / 16, KC = 32, PE = GT * KC / 256;
+    const int n = P.n, K1 = min(n, K0 + outer_w<T>()), r0 = K0 + b * NB, N = n - K1;
+    const int bn = blockIdx.x * GT;
+    if (N <= 0 || bn >= N || r0 + NB > K1) return;  // uniform
+    T* A = (T*)P.A;
+    const int64_t ld = P.ld;
+    __shared__ T sAr[GT][KC + 1], sAi[C ? GT : 1][KC + 1];
+    // the first product's B chunks (rows 0 .. KC), then the tile
+    __shared__ T sTr[GT][GT + 1], sTi[C ? GT : 1][GT + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
+    // the composed permutation: lane i ends holding the row that lands at r0 + i
+    const int pv = P.ipiv[r0 + lane] - r0;
+    int src = lane;
+    for (int c = 0; c < NB; ++c) {
+        const int p = __builtin_amdgcn_readlane(pv, c);
+        if (p != c && p >= 0 && p < NB) {  // uniform
+            const int x = __builtin_amdgcn_readlane(src, c), y = __builtin_amdgcn_readlane(src, p);
+            src = lane == c ? y : (lane == p ? x : src);
+        }
+    }
+    // this thread's tile entries, read through the permutation
+    cx<T> tv[NBLK][NBLK][4];
+#pragma unroll
+    for (int a = 0; a < NBLK; ++a)
+#pragma unroll
+        for (int bb = 0; bb < NBLK; ++bb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wm + 16 * a + Mfma<T>::row(lane, r);
+                const int sr = __shfl(src, row, 64);
+                const int gc = bn + wn + 16 * bb + (lane & 15);
+                tv[a][bb][r] = gc < N ? Acc<T, C>::ld(A, (int64_t)(r0 + sr) * ld + K1 + gc) : cx<T>{T(0), T(0)};
+            }
+    V4 accR[NBLK][NBLK], accI[C ? NBLK : 1][C ? NBLK : 1];
+    auto zero_acc = [&]() {
+#pragma unroll
+        for (int a = 0; a < NBLK; ++a)
+#pragma unroll
+            for (int bb = 0; bb < NBLK; ++bb) {
+                accR[a][bb] = V4{0, 0, 0, 0};
+                if constexpr (C) accI[a][bb] = V4{0, 0, 0, 0};
+            }
+    };
+    // accR/accI += (sA chunk) (B chunk in sT rows 0 .. KC, or the tile rows k0 .. k0 + KC)
+    auto mma = [&](int brow0) {
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+            const int ka = kk + (lane >> 4);
+            T ar[NBLK], ai[NBLK], br[NBLK], bi[NBLK];
+#pragma unroll
+            for (int a = 0; a < NBLK; ++a) {
+                ar[a] = sAr[wm + 16 * a + (lane & 15)][ka];
+                if constexpr (C) ai[a] = sAi[wm + 16 * a + (lane & 15)][ka];
+            }
+#pragma unroll
+            for (int bb = 0; bb < NBLK; ++bb) {
+                br[bb] = sTr[brow0 + ka][wn + 16 * bb + (lane & 15)];
+                if constexpr (C) bi[bb] = sTi[brow0 + ka][wn + 16 * bb + (lane & 15)];
+            }
+#pragma unroll
+            for (int a = 0; a < NBLK; ++a)
+#pragma unroll
+                for (int bb = 0; bb < NBLK; ++bb) {
+                    accR[a][bb] = Mfma<T>::op(ar[a], br[bb], accR[a][bb]);
+                    if constexpr (C) {
+                        accR[a][bb] = Mfma<T>::op(-ai[a], bi[bb], accR[a][bb]);
+                        accI[a][bb] = Mfma<T>::op(ar[a], bi[bb], accI[a][bb]);
+                        accI[a][bb] = Mfma<T>::op(ai[a], br[bb], accI[a][bb]);
+                    }
+                }
+        }
+    };
+    zero_acc();
+    const int K = b * NB;  // the solved rows above
+    if (K > 0) {
+        cx<T> pa[PE], pb[PE];
+        auto fetch = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < PE; ++q) {
+                const int e = tid + 256 * q;
+                const int ra = e / KC, ca = e % KC, rb = e / GT, cb = e % GT;
+                pa[q] = Acc<T, C>::ld(A, (int64_t)(r0 + ra) * ld + K0 + k0 + ca);
+                pb[q] = bn + cb < N ? Acc<T, C>::ld(A, (int64_t)(K0 + k0 + rb) * ld + K1 + bn + cb) : cx<T>{T(0), T(0)};
+            }
+        };
+        fetch(0);
+        for (int k0 = 0; k0 < K; k0 += KC) {
+#pragma unroll
+            for (int q = 0; q < PE; ++q) {
+                const int e = tid + 256 * q;
+                const int ra = e / KC, ca = e % KC, rb = e / GT, cb = e % GT;
+                sAr[ra][ca] = pa[q].re;
+                if constexpr (C) sAi[ra][ca] = pa[q].im;
+                sTr[rb][cb] = pb[q].re;
+                if constexpr (C) sTi[rb][cb] = pb[q].im;
+            }
+            __syncthreads();
+            if (k0 + KC < K) fetch(k0 + KC);
+            mma(0);
+            __syncthreads();
+        }
+    }
+    // the difference, kept in registers and staged in LDS as the tile
+#pragma unroll
+    for (int a = 0; a < NBLK; ++a)
+#pragma unroll
+        for (int bb = 0; bb < NBLK; ++bb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                tv[a][bb][r].re -= accR[a][bb][r];
+                if constexpr (C) tv[a][bb][r].im -= accI[a][bb][r];
+                const int row = wm + 16 * a + Mfma<T>::row(lane, r), col = wn + 16 * bb + (lane & 15);
+                sTr[row][col] = tv[a][bb][r].re;
+                if constexpr (C) sTi[row][col] = tv[a][bb][r].im;
+            }
+    // second product: S_L,b (tile), K = 64 in chunks of KC
+    const T* SL = (const T*)P.SL + (C ? 2 : 1) * (int64_t)b * NB * NB;
+    zero_acc();
+    for (int k0 = 0; k0 < NB; k0 += KC) {
+#pragma unroll
+        for (int q = 0; q < PE; ++q) {
+            const int e = tid + 256 * q;
+            const int ra = e / KC, ca = e % KC;
+            const cx<T> v = Acc<T, C>::ld(SL, (int64_t)ra * NB + k0 + ca);
+            sAr[ra][ca] = v.re;
+            if constexpr (C) sAi[ra][ca] = v.im;
+        }
+        __syncthreads();  // (the first pass: also the tile's stores)
+        mma(k0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < NBLK; ++a)
+#pragma unroll
+        for (int bb = 0; bb < NBLK; ++bb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wm + 16 * a + Mfma<T>::row(lane, r);
+                const int gc = bn + wn + 16 * bb + (lane & 15);
+                if (gc < N) {
+                    cx<T> v = tv[a][bb][r];
+                    v.re -= accR[a][bb][r];
+                    if constexpr (C) v.im -= accI[a][bb][r];
+                    Acc<T, C>::st(A, (int64_t)(r0 + row) * ld + K1 + gc, v);
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
 // Trailing updates of large factorisations (modes 1 and 4): C -= A B with a
 // 128 x BN block tile (BN = 128 or 64), eight waves of 64 x BN/4 (4 x BN/64
 // MFMA 16x16 blocks each), so every A and B element staged in LDS feeds
@@ -2507,10 +2675,19 @@ void lu_launch_lookahead(const LuProb& P, int pivoting, int gopts, int pgrid, hi
             // reach the rows of an earlier one).
             hipEventRecord(pev[slot], sp);
             hipStreamWaitEvent(sg, pev[slot], 0);
-            laswp_launch<T, C>(P, pivoting, k, k + NB, 0, n, K0, K0 + outer_w<T>(), sg);
-            if (Nout > 0) {
-                if (slot > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, slot}, NB, Nout, 1, gopts, sg);
-                lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, slot}, NB, Nout, 1, gopts, sg);
+            if (std::is_same<T, float>::value && pivoting == 2) {
+                // FP32 block-local: the panel's U12 rows in one launch (reading
+                // its rows of the right columns through its interchanges), then
+                // the interchanges of the columns left of the block
+                if (Nout > 0)
+                    hipLaunchKernelGGL((u12_panel_kernel<C>), dim3((Nout + GT - 1) / GT), dim3(256), 0, sg, P, K0, slot);
+                laswp_launch<T, C>(P, pivoting, k, k + NB, 0, K0, 0, 0, sg);
+            } else {
+                laswp_launch<T, C>(P, pivoting, k, k + NB, 0, n, K0, K0 + outer_w<T>(), sg);
+                if (Nout > 0) {
+                    if (slot > 0) lu_gemm<T, C, false>(P, nullptr, LuStep{2, K0, K0, slot}, NB, Nout, 1, gopts, sg);
+                    lu_gemm<T, C, false>(P, nullptr, LuStep{3, K0, K0, slot}, NB, Nout, 1, gopts, sg);
+                }
             }
             lu_gemm<T, C, false>(P, nullptr, LuStep{0, K0, k, slot}, NB, K1 - kend, 1, gopts, sp);
             lu_gemm<T, C, false>(P, nullptr, LuStep{1, K0, k, slot}, n - kend, K1 - kend, 1, gopts, sp);
