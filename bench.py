@@ -46,6 +46,9 @@ FLOPS_PER_EVAL_FUSED = 47
 # a 512 MiB copy (FETCH x 2.00, WRITE x 1.00), and the hardware FP64 VALU
 # flops of that launch (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes).  Measured:
 # 105.2 GB fetched + 8.4 GB written; 8.335e12 flops.
+# Residual tolerance (relative to |b|) of the compliant mixed pipeline's
+# FP64 refinement: two orders under the 1e-8 error bar of the metric.
+REFINE_TOL = 1e-10
 FAR_L5_TRAFFIC_BYTES = 113_612_336_928
 FAR_L5_HW_FLOPS = 8_335_159_869_440
 
@@ -286,13 +289,17 @@ def main() -> int:
             ctx.set_option(L.OPT_LU_PIVOTING, 0)
             ctx.set_option(L.OPT_C64_LU_GEMM, 0)
             e[2].record()
-            st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=1e-14)
+            # the metric's bar is a 1e-8 relative error: refinement stops at a
+            # residual of 1e-10 |b| (the error/residual ratio of this system
+            # measures about 34; one step reaches a 2.5e-13 residual and a
+            # 8.6e-12 error, a second 4e-14); rel_err_vs_fp64 reports the error
+            st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=REFINE_TOL)
             e[3].record()
             torch.cuda.synchronize()
         err2 = (torch.linalg.vector_norm(x - x64).item()) / x64n
         tts["mixed_fp64asm"] = {"ms": round(e[0].elapsed_time(e[3]), 3), "assembly_ms": round(e[0].elapsed_time(e[1]), 3),
                                 "lu_c64_ms": round(e[1].elapsed_time(e[2]), 3), "refine_ms": round(e[2].elapsed_time(e[3]), 3),
-                                "refine_iters": st.iterations, "converged": st.converged,
+                                "refine_iters": st.iterations, "converged": st.converged, "refine_tol": REFINE_TOL,
                                 "lu_pivoting": "block-local", "lu_gemm": lu_gemm_name, "rel_err_vs_fp64": err2}
         tts["speedup_mixed_fp32asm"] = round(mean["total_ms"] / tts["mixed_fp32asm"]["ms"], 3)
         tts["speedup_mixed_fp64asm"] = round(mean["total_ms"] / tts["mixed_fp64asm"]["ms"], 3)

@@ -194,11 +194,13 @@ def test_config2_mixed_as_benched(bem, l3_dirichlet):
     finally:
         ctx.set_option(L.OPT_LU_PIVOTING, 0)
         ctx.set_option(L.OPT_C64_LU_GEMM, 0)
-    x = torch.zeros_like(x64)
-    st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=1e-14)
-    e = rel(to_np(x), to_np(x64))
-    print(f"[config2 mixed as benched] {st.iterations} steps, rel err {e:.3e}", flush=True)
-    assert st.converged and st.iterations <= 4 and e <= 1e-8
+    # the bench's refinement tolerance (a residual of 1e-10 |b|) and a tight one
+    for tol, steps in ((1e-10, 2), (1e-14, 4)):
+        x = torch.zeros_like(x64)
+        st = ctx.refine_solve(A64, LU32, ipiv, b64, x, max_iter=30, tol=tol)
+        e = rel(to_np(x), to_np(x64))
+        print(f"[config2 mixed as benched] tol {tol:g}: {st.iterations} steps, rel err {e:.3e}", flush=True)
+        assert st.converged and st.iterations <= steps and e <= 1e-8
 
 
 def test_config2_mixed_fp32_assembly(bem, l3_dirichlet):
