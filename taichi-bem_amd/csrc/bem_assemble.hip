@@ -1072,20 +1072,22 @@ __global__ __launch_bounds__(256) void far_dirichlet_f64_kernel(OpParams p, int 
                                                : __builtin_fma(p2, P2, __builtin_fma(p1, P1, PQ[4 * a] + Q));
                         const double rinv = rsqrt_f64(r2);
                         const double gv = wa * rinv;
-                        const double g3 = gv * (rinv * rinv);
                         if constexpr (HELM) {
                             const double th = r2 * rinv;  // |kap| r
                             double sn, cs;
                             sincos_f64_t2048(th, sct, &sn, &cs);
                             pvr = __builtin_fma(gv, cs, pvr);
                             pvi = __builtin_fma(gv, sn, pvi);
-                            // (1 - i th) e^{i th}
-                            const double tr = g3 * __builtin_fma(th, sn, cs);
-                            const double tim = g3 * __builtin_fma(-th, cs, sn);
+                            // (1 - i th) e^{i th} / th^3 with th rinv = 1:
+                            // (sin th + cos th / th - i (cos th - sin th / th)) / th^2
+                            const double g2 = gv * rinv;
+                            const double tr = g2 * __builtin_fma(cs, rinv, sn);
+                            const double tim = g2 * __builtin_fma(sn, rinv, -cs);
                             t0r += tr; t0i += tim;
                             t1r = __builtin_fma(p1, tr, t1r); t1i = __builtin_fma(p1, tim, t1i);
                             t2r = __builtin_fma(p2, tr, t2r); t2i = __builtin_fma(p2, tim, t2i);
                         } else {
+                            const double g3 = gv * (rinv * rinv);
                             pvr += gv;
                             t0r += g3;
                             t1r = __builtin_fma(p1, g3, t1r);
