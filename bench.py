@@ -50,7 +50,7 @@ FLOPS_PER_EVAL_FUSED = 47
 # FP64 refinement: two orders under the 1e-8 error bar of the metric.
 REFINE_TOL = 1e-10
 FAR_L5_TRAFFIC_BYTES = 113_612_336_928
-FAR_L5_HW_FLOPS = 8_335_159_869_440
+FAR_L5_HW_FLOPS = 8_203_729_469_440
 
 
 def cpu_threads() -> int:
